@@ -1,0 +1,190 @@
+/*
+ * hotcross.h - C-ABI of the MI355X-native HotCross LES timestep.
+ *
+ * The reference (diioriod/PlumeModel-Hotcross) is a Fortran program whose
+ * timestep is a fixed sequence of argument-less subroutines acting on module
+ * state (hotcross80-revised2014.f:264-358).  This library keeps that state in
+ * HBM and exports one entry point per reference subroutine, so that Fortran
+ * host code (through ISO_C_BINDING, see plumemodel-hotcross_amd/fortran/) or
+ * any other host language can call them in the reference's own order.
+ *
+ * Conventions
+ *  - Every entry returns 0 on success and a negative HC_E* code on failure;
+ *    hc_last_error() gives the message.  The reference's equivalent is a
+ *    Fortran `stop 'message'` (hotcross80-revised2014.f:187-192,413,1314).
+ *  - Host arrays are column-major (x fastest) with explicit Fortran bounds
+ *    lo[3]..hi[3], exactly the module arrays of modules80-revised2014.f.
+ *  - One context = one device stream.  Entries that return host-visible
+ *    scalars (hc_cfl_values, hc_tracer31 budgets, hc_download) synchronise;
+ *    the rest are asynchronous on the context stream.
+ *  - Single caller thread per context (the reference calls these from the
+ *    master thread between OpenMP regions).
+ */
+#ifndef HOTCROSS_H
+#define HOTCROSS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HC_OK 0
+#define HC_EINVAL (-1)   /* bad argument / unsupported configuration      */
+#define HC_EDEVICE (-2)  /* HIP runtime error                             */
+#define HC_ENOMEM (-3)   /* device allocation failed                      */
+#define HC_ECFL (-4)     /* "stop on impending cfl violation"             */
+
+#define HC_FP64 0        /* plain form: every field and kernel in FP64    */
+#define HC_FP32 1        /* chosen form: FP32 fields, FP64 islands        */
+
+/* Case parameters: the variables the reference reads from its case file
+ * (case.inc -> DanteExp1.case) and module constants
+ * (modules80-revised2014.f:49-62,77-109,256-275). */
+typedef struct hc_params {
+  int32_t nx, ny, nz, nt;                 /* array_size.inc:6, grid_module nt */
+  double dx, dy, dz, dt;                  /* grid_module                    */
+  double zstart, zend;                    /* grid_module                    */
+  /* signals_module switches */
+  int32_t bit2, bit3, bit4, bit5, bit7, bit8, bit9, bit10, bit2k, bit5k;
+  int32_t ismol, ismag_h, ismag_v;
+  int32_t bcondx, bcondy, bottom_slip, side_slip, open_box;
+  int32_t bdbit[4];
+  int32_t modulo_x, modulo_y;
+  int32_t toggle_t, toggle_s, toggle_c;   /* toggle(6), toggle(7), toggle(8) */
+  int32_t pad0;
+  /* viscosity_module */
+  double cs_h, cs_v, prandtl, axmin, aymin, azmin, azmax, mixing_height_scale;
+  double sponge_factor, lat;
+  /* constants_module */
+  double g, rho0, assel, eps, omega;
+  /* vent source: source_module/tracer_module + source80.inc constants */
+  double source_length, source_width;
+  double dqtdt, dqsdt, dqcdt;
+  double tsource, ssource, wsource;
+  double source_ramp_hrs;   /* source80.inc:8  (reference: 2 h)            */
+  double noise_amp;         /* source80.inc:32 wee (reference: 0.025)      */
+  double wramp_hrs;         /* hotcross80-revised2014.f:835 (6 h)          */
+  /* crossflow series, site_forcing.inc:51-72:
+   * ut(l) = ramp(l) * (flow_u0 + flow_u1*sin(flow_omega*l*dt)),
+   * vt(l) = ramp(l) * flow_v1*cos(flow_omega*l*dt),
+   * ramp(l) = min(1, l*dt/(flow_ramp_hrs*3600)), or 1 if flow_ramp_hrs<=0 */
+  double flow_u0, flow_u1, flow_v1, flow_omega, flow_ramp_hrs;
+  /* equation of state: 0 = UNESCO one-atmosphere polynomial of
+   * potential_density80.f:94-101; 1 = linear
+   * rho = eos_rho_ref*(1 - eos_alpha*(T-eos_t0) + eos_beta*(S-eos_s0)) */
+  int32_t eos, pad1;
+  double eos_rho_ref, eos_alpha, eos_beta, eos_t0, eos_s0;
+} hc_params;
+
+/* Device-resident fields (module arrays of modules80-revised2014.f).
+ * Fortran bounds: u-class (-1:nx+1,0:ny+1,0:nz+1), v-class
+ * (0:nx+1,-1:ny+1,0:nz+1), w-class (0:nx+1,0:ny+1,-1:nz+1), scalar-class
+ * (0:nx+1,0:ny+1,0:nz+1). */
+enum hc_field {
+  HC_U = 0, HC_V, HC_W, HC_UP, HC_VP, HC_WP, HC_UM, HC_VM, HC_WM,
+  HC_P, HC_T, HC_S, HC_C, HC_SS, HC_SSV, HC_RHOP, HC_FX, HC_FY, HC_FZ,
+  HC_NFIELDS
+};
+
+typedef struct hc_ctx hc_ctx;
+
+/* Create a context and allocate the state in HBM (all zero).
+ * precision: HC_FP64 or HC_FP32.  Replaces the module allocations of
+ * hotcross80-revised2014.f:120-135 and the HS3CRI set-up call (:1309). */
+int hc_create(const hc_params *prm, int precision, hc_ctx **out);
+int hc_destroy(hc_ctx *ctx);
+const char *hc_last_error(const hc_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that record events. */
+void *hc_stream(hc_ctx *ctx);
+int hc_synchronize(hc_ctx *ctx);
+
+/* Host <-> device copies of one field over the Fortran box lo..hi, which must
+ * lie inside the field's bounds; host is column-major over that box. */
+int hc_upload(hc_ctx *ctx, int field, const double *host, const int32_t lo[3],
+              const int32_t hi[3]);
+int hc_download(hc_ctx *ctx, int field, double *host, const int32_t lo[3],
+                const int32_t hi[3]);
+
+/* Background profiles, each of length nz+2 (index 0..nz+1): t_bkg, s_bkg,
+ * rho_bkg (tracer_module), azzmin (viscosity_module).  Set by the
+ * reference's INITIALIZE_PROFILES (initialize_profiles80.f:46-159). */
+int hc_set_profiles(hc_ctx *ctx, const double *t_bkg, const double *s_bkg,
+                    const double *rho_bkg, const double *azzmin);
+/* Sponge masks from SPONGE_MAKER (sponge80.f:3-142): alphau(-1:nx+1,0:ny+1),
+ * alphav(0:nx+1,-1:ny+1), alphae(0:nx+1,0:ny+1), alphaw(-1:nz+1). */
+int hc_set_sponge(hc_ctx *ctx, const double *alphau, const double *alphav,
+                  const double *alphae, const double *alphaw);
+/* Host-side case set-up in the library: profiles by LOCATOR interpolation of
+ * a (z, T, S) table (initialize_profiles80.f:46-64,164-197), sponge masks
+ * (sponge80.f), and the initial state of INITIALIZE
+ * (hotcross80-revised2014.f:444-560): zero velocities with the -1e34 edge
+ * fill, tracers at background, plus optional uniform T noise of amplitude
+ * t_noise drawn from a counter-based generator seeded with `seed`. */
+int hc_init_case(hc_ctx *ctx, const double *zdata, const double *tdata,
+                 const double *sdata, int32_t ndata, double t_noise,
+                 uint64_t seed);
+
+/* ---- one entry per reference subroutine of the timestep ---------------- */
+/* Time-loop control, hotcross80-revised2014.f:266-275: time, mtime, leap,
+ * delta for step m. */
+int hc_begin_step(hc_ctx *ctx, int32_t mtime, double time);
+/* BODY_FORCE, body_force80.f:4-22 + site_forcing.inc:81-171. */
+int hc_body_force(hc_ctx *ctx);
+/* FORCES, hotcross80-revised2014.f:890-1139. */
+int hc_forces(hc_ctx *ctx);
+/* PRESSURE (+ HS3CRT solve), hotcross80-revised2014.f:1146-1405,
+ * hs3crt_composite80.f:4-143. */
+int hc_pressure(hc_ctx *ctx);
+/* MOMENTUM_U/V/W, hotcross80-revised2014.f:671-887. */
+int hc_momentum_u(hc_ctx *ctx);
+int hc_momentum_v(hc_ctx *ctx);
+int hc_momentum_w(hc_ctx *ctx);
+/* VISCOSITY3D, smagorinsky_split_mixing-revised2014.f:5-150. */
+int hc_viscosity3d(hc_ctx *ctx);
+/* SOURCE_terms, source80.f:8-32 + source80.inc.  `noise` holds the three
+ * uniform [0,1) draws per patch cell in the reference's order (j outer, i
+ * inner; r1 r2 r3), or NULL to use the library's counter-based generator. */
+int hc_source_terms(hc_ctx *ctx, const double *noise);
+/* MIXING, mixing80-revised2014.f:1-99.  The diffusivities kxt/kyt/kzt are
+ * formed inside the TRACER31 kernels from ss/ss_v with the same expressions,
+ * so this entry only validates state (the chi diagnostics are output-only). */
+int hc_mixing(hc_ctx *ctx);
+/* Work velocities 0.5*(up+u) of hotcross80-revised2014.f:303-305, turned
+ * into the Courant numbers of mpdata-hotcross80.f:94-104 with tau3. */
+int hc_work_velocity(hc_ctx *ctx, double tau3);
+/* TRACER31(c, q_c, c_bkg, cmin, cmax, c_total0, ...), mpdata-hotcross80.f:16
+ * for field HC_T, HC_S or HC_C.  c_bkg has nz+2 entries.  If budgets is not
+ * NULL and mod(mtime,100)==0 it receives c_total, sum_sponge, sum_source
+ * (mpdata-hotcross80.f:384-423) and the call synchronises. */
+int hc_tracer31(hc_ctx *ctx, int field, const double *c_bkg, double cmin,
+                double cmax, double tau3, double *budgets);
+/* Vent ghost cells after TRACER31, hotcross80-revised2014.f:311-315:
+ * c(i,j,nz+1) = value on source cells. */
+int hc_vent_ghost(hc_ctx *ctx, int field, double value);
+/* DENSITY, potential_density80.f:3-71. */
+int hc_density(hc_ctx *ctx);
+/* REFRESH, hotcross80-revised2014.f:1411-1493. */
+int hc_refresh(hc_ctx *ctx);
+/* CFL_VALUES, hotcross80-revised2014.f:374-438: out = maxu, minu, maxv,
+ * minv, maxw, minw (Courant numbers).  Returns HC_ECFL when the largest
+ * magnitude exceeds 0.75. */
+int hc_cfl_values(hc_ctx *ctx, double out[6]);
+
+/* The whole reference time loop body (hotcross80-revised2014.f:264-350) for
+ * nsteps steps starting at the context's next step, with the source noise
+ * from the counter-based generator.  CFL every 10 steps as in the
+ * reference; returns HC_ECFL on the bailout condition. */
+int hc_step(hc_ctx *ctx, int32_t nsteps);
+
+/* Per-phase device time accumulated since the last reset (ms), measured with
+ * HIP events on the context stream when enabled.  names/ms have room for
+ * `cap` entries; returns the number of phases. */
+int hc_phase_timing(hc_ctx *ctx, int enable);
+int hc_phase_times(hc_ctx *ctx, const char **names, double *ms, int32_t *counts,
+                   int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
