@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Headline measurement: Mcell-steps/s of the HotCross LES timestep on one
+MI355X (FP64 plain form, 256^3 workload), with the dominant kernel's HBM
+roofline and the reference's OpenMP CPU path timed beside it.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload les256]
+                [--precision fp64|fp32] [--no-cpu]
+
+A step is one pass of the reference's time loop body
+(hotcross80-revised2014.f:264-350: body force, forces, pressure solve,
+momentum, Smagorinsky viscosity, source, three MPDATA tracers, density,
+Asselin/leapfrog refresh, CFL every 10 steps) over the whole grid, with the
+state resident in HBM.  Mcell-steps/s = nx*ny*nz*K / seconds.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "plumemodel-hotcross_amd"))
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+# algorithmic HBM bytes per cell-step: 130.3 field moves (compulsory reads and
+# writes of every reference subroutine, counted once each) x 8 B in FP64
+FIELD_MOVES = 130.3
+# MPDATA corrective pass (k_antidiff + k_flags + k_limited of one tracer pass):
+# compulsory reads of c and the three face velocities, writes of the three
+# antidiffusive velocities and the result: 8 field moves per cell
+TRACER_PASS_MOVES = 8.0
+
+
+def init_context(H, name, precision):
+    p = H.workload_params(name)
+    ctx = H.Context(p, precision)
+    z, t, s = H.linear_stratification(p)
+    ctx.init_case(z, t, s, t_noise=1.0e-4, seed=2024)
+    return p, ctx
+
+
+def gpu_leg(args):
+    import hotcross as H
+    prec = H.FP64 if args.precision == "fp64" else H.FP32
+    p, ctx = init_context(H, args.workload, prec)
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    cells = p.nx * p.ny * p.nz
+    # phase and kernel timing with HIP events on the context stream (separate
+    # run so the events do not perturb the timed region)
+    ctx.phase_timing(True)
+    nprof = min(args.steps, 10)
+    ctx.step(nprof)
+    ctx.synchronize()
+    phases = ctx.phase_times()
+    ctx.phase_timing(False)
+    ctx.close()
+    return p, sec, cells, phases, nprof
+
+
+def cpu_leg(nx, ny, nz, steps, threads):
+    """The reference compiled in place (oracle/_ref) on this host's cores."""
+    lib = os.path.join(ROOT, "oracle", "_ref", f"g{nx}x{ny}x{nz}", "libhcref.so")
+    if not os.path.exists(lib):
+        return None
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_STACKSIZE", "512M")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", lib, str(nx), str(ny), str(nz), str(steps)]
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run(cmd, cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                           timeout=900)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr[-2000:])
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_worker(lib, nx, ny, nz, steps):
+    """Runs inside a fresh process: set up a case in the reference's module
+    state and time `steps` steps of its time loop body."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import hc_testlib as T
+    prm = T.reference_params(nx, ny, nz, dx=0.5, dy=0.5, dz=0.5, dt=0.2, bit7=0)
+    ref = T.RefLib(lib, prm)
+    z, t, s = T.linear_profile_table(zmin=prm.zstart - 10, zmax=prm.zend + 10, dzt=1.0)
+    T.write_profile_file("profile.txt", z, t, s)
+    ref.init_profiles(os.path.abspath("profile.txt"))
+    ref.call("density"); ref.call("initialize"); ref.call("sponge_maker")
+    ref.call("step", 0)  # first step (allocates the forcing series)
+    t0 = time.perf_counter()
+    for m in range(1, steps + 1):
+        ref.call("step", m)
+    sec = time.perf_counter() - t0
+    print(json.dumps({"sec": sec, "steps": steps}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="les256")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-worker", nargs=5)
+    args = ap.parse_args()
+    if args.cpu_worker:
+        lib, nx, ny, nz, steps = args.cpu_worker
+        cpu_worker(lib, int(nx), int(ny), int(nz), int(steps))
+        return
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    p, sec, cells, phases, nprof = gpu_leg(args)
+    value = cells * args.steps / sec / 1e6
+    ms_step = sec * 1e3 / args.steps
+    esz = 8 if args.precision == "fp64" else 4
+    # roofline of the dominant kernel group: the MPDATA tracer transport
+    tr_ms, tr_n = phases.get("tracer31", (0.0, 0))
+    roof = None
+    if tr_n:
+        per_call = tr_ms / tr_n * 1e-3
+        # one TRACER31 call: pass 1 (7 moves) + two corrective passes (8 + 6)
+        # = 21 field moves per cell (reads of c, ss, ss_v, the Courant numbers,
+        # antidiffusive velocities; writes of the pass results)
+        alg = 21.0 * esz * cells
+        achieved = alg / per_call / 1e9
+        roof = {"bound": "hbm", "kernel": "tracer31 (MPDATA, one tracer)", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
+                "traffic": None, "avg_ms": round(per_call * 1e3, 4)}
+    step_bytes = FIELD_MOVES * esz * cells
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+        r = cpu_leg(p.nx, p.ny, p.nz, args.cpu_steps, threads)
+        if r:
+            cpu = {"value": round(cells * r["steps"] / r["sec"] / 1e6, 4), "unit": "Mcell-steps/s",
+                   "cores": threads, "kind": "reference",
+                   "sample": "%d steps of the reference's time loop body at %dx%dx%d (amdflang -O2 OpenMP, "
+                             "reference constants), after one untimed step" % (r["steps"], p.nx, p.ny, p.nz)}
+    out = {
+        "metric": "Mcell-steps/s FP64 vs FP32 at stated err vs FP64; adjoint/forward time ratio",
+        "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if esz == 8 else "f32",
+        "data": "synthetic (linear stratification N=1e-3 1/s, seeded T noise, 50 MW vent)",
+        "config": {"workload": args.workload, "grid": [p.nx, p.ny, p.nz], "dx": p.dx, "dt": p.dt,
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "step_roofline": {"bytes_per_cell_step": FIELD_MOVES * esz,
+                          "achieved_GBps": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
+                          "frac": round(step_bytes / (ms_step * 1e-3) / HBM_PEAK, 4)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "phases_ms_per_step": {k: round(v[0] / nprof, 4) for k, v in phases.items()},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

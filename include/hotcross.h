@@ -177,6 +177,11 @@ int hc_cfl_values(hc_ctx *ctx, double out[6]);
  * reference; returns HC_ECFL on the bailout condition. */
 int hc_step(hc_ctx *ctx, int32_t nsteps);
 
+/* The three uniform draws per source-patch cell that hc_step's generator
+ * uses at step mtime (patch order as hc_source_terms); returns the number of
+ * patch cells.  For checkers that replay a native run. */
+int hc_source_noise(hc_ctx *ctx, int32_t mtime, double *out);
+
 /* Per-phase device time accumulated since the last reset (ms), measured with
  * HIP events on the context stream when enabled.  names/ms have room for
  * `cap` entries; returns the number of phases. */

@@ -1,0 +1,854 @@
+// hc_api.hip - the C-ABI of include/hotcross.h: context, HBM state, host-side
+// case set-up and the per-subroutine entry points in the reference's order.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hotcross.h"
+#include "hc_common.h"
+#include "hc_kernels.h"
+#include "hc_rng.h"
+
+using namespace hc;
+
+struct Phase {
+  const char *name;
+  double ms;
+  int count;
+  hipEvent_t a, b;
+  bool pending;
+};
+
+struct hc_ctx {
+  hc_params P;
+  int prec;
+  size_t esz;
+  Layout L;
+  hipStream_t st = nullptr;
+  std::string err;
+  void *f[HC_NFIELDS] = {};
+  // case arrays (device + host copies)
+  Aux aux{};
+  double *d_tbkg = nullptr, *d_sbkg = nullptr, *d_zero = nullptr, *d_cbkg = nullptr;
+  std::vector<double> h_tbkg, h_sbkg, h_rho_bkg, h_azzmin, h_alphae, h_alphae0, h_alphau, h_alphav, h_alphaw;
+  TracerWork tw{};
+  PoissonWork pw{};
+  double *d_tw = nullptr;
+  double *d_q[3] = {};       // q_t, q_s, q_c on the patch
+  double *d_noise = nullptr; // 3 draws per patch cell
+  double *d_stage = nullptr; // host<->device staging
+  size_t stage_n = 0;
+  double *d_cflpart = nullptr, *d_cflout = nullptr;
+  double *h_pin = nullptr;   // pinned host scratch for small results
+  // step state
+  int mtime = -1, leap = 1, delta = 0;
+  double time = 0, tau = 0;
+  double ubkg_m = 0, ubkg = 0, ubkg_p = 0, vbkg_m = 0, vbkg = 0, vbkg_p = 0, fbx = 0, fby = 0;
+  double fc = 0, fc_y = 0;
+  double acc[8] = {};
+  int nxs = 0, nxe = 0, nys = 0, nye = 0, npatch = 0;
+  bool map_valid = false;
+  double sumup = 0, sumdown = 0;
+  uint64_t seed = 12345;
+  int next_step = 0;
+  double last_budget[3] = {};
+  // timing
+  bool timing = false;
+  std::vector<Phase> phases;
+};
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) {                                                          \
+      ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);                     \
+      return HC_EDEVICE;                                                             \
+    }                                                                                \
+  } while (0)
+
+static int fail(hc_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// phase timing with HIP events on the context stream
+static int phase_begin(hc_ctx *ctx, const char *name) {
+  if (!ctx->timing) return -1;
+  for (size_t q = 0; q < ctx->phases.size(); ++q)
+    if (!strcmp(ctx->phases[q].name, name)) {
+      Phase &p = ctx->phases[q];
+      if (p.pending) {
+        float ms = 0;
+        hipEventSynchronize(p.b);
+        hipEventElapsedTime(&ms, p.a, p.b);
+        p.ms += ms;
+        p.count++;
+        p.pending = false;
+      }
+      hipEventRecord(p.a, ctx->st);
+      return (int)q;
+    }
+  Phase p{name, 0.0, 0, nullptr, nullptr, false};
+  hipEventCreate(&p.a);
+  hipEventCreate(&p.b);
+  hipEventRecord(p.a, ctx->st);
+  ctx->phases.push_back(p);
+  return (int)ctx->phases.size() - 1;
+}
+static void phase_end(hc_ctx *ctx, int q) {
+  if (q < 0) return;
+  hipEventRecord(ctx->phases[q].b, ctx->st);
+  ctx->phases[q].pending = true;
+}
+struct PhaseScope {
+  hc_ctx *c;
+  int q;
+  PhaseScope(hc_ctx *c_, const char *n) : c(c_), q(phase_begin(c_, n)) {}
+  ~PhaseScope() { phase_end(c, q); }
+};
+
+// ---------------------------------------------------------------------------
+template <typename T>
+static FieldPtrs<T> ptrs(hc_ctx *ctx) {
+  FieldPtrs<T> p;
+  T **a = (T **)&p;
+  for (int q = 0; q < HC_NFIELDS; ++q) a[q] = (T *)ctx->f[q];
+  return p;
+}
+
+static StepArgs step_args(hc_ctx *ctx) {
+  StepArgs s{};
+  s.dx = ctx->P.dx; s.dy = ctx->P.dy; s.dz = ctx->P.dz; s.dt = ctx->P.dt; s.tau3 = ctx->P.dt;
+  s.leap = ctx->leap; s.delta = ctx->delta; s.mtime = ctx->mtime;
+  s.ubkg_m = ctx->ubkg_m; s.ubkg = ctx->ubkg; s.vbkg_m = ctx->vbkg_m; s.vbkg = ctx->vbkg;
+  s.fbx = ctx->fbx; s.fby = ctx->fby; s.fc = ctx->fc; s.fc_y = ctx->fc_y;
+  for (int q = 0; q < 8; ++q) s.acc[q] = ctx->acc[q];
+  return s;
+}
+
+static VentArgs vent_args(hc_ctx *ctx) {
+  VentArgs v{};
+  v.map_valid = ctx->map_valid;
+  v.nxs = ctx->nxs; v.nxe = ctx->nxe; v.nys = ctx->nys; v.nye = ctx->nye;
+  double wup = ctx->P.wsource * fmin(1.0, (ctx->mtime * ctx->P.dt) / (ctx->P.wramp_hrs * 3600.0));
+  v.wup = wup;
+  v.sumup = ctx->sumup;
+  v.sumdown = ctx->sumdown;
+  return v;
+}
+
+// sumup/sumdown of MOMENTUM_W (hotcross80-revised2014.f:823-833)
+static void update_sums(hc_ctx *ctx) {
+  int nx = ctx->P.nx, ny = ctx->P.ny;
+  double su = 0, sd = 0;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      if (ctx->h_alphae[i + (nx + 2) * j] == 0) {
+        bool src = ctx->map_valid && i >= ctx->nxs && i <= ctx->nxe && j >= ctx->nys && j <= ctx->nye;
+        if (src) su = su + 1.0; else sd = sd + 1.0;
+      }
+  ctx->sumup = su;
+  ctx->sumdown = sd;
+}
+
+// ---------------------------------------------------------------------------
+// pack/unpack between a Fortran box (column-major, FP64) and the padded field
+template <typename T>
+__global__ void k_scatter(Layout L, int l0, int l1, int l2, int n0, int n1, int n2, const double *__restrict__ src,
+                          T *__restrict__ dst) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)n0 * n1 * n2;
+  if (q >= n) return;
+  int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
+  dst[IX(L, l0 + i, l1 + j, l2 + k)] = (T)src[q];
+}
+template <typename T>
+__global__ void k_gather(Layout L, int l0, int l1, int l2, int n0, int n1, int n2, const T *__restrict__ src,
+                         double *__restrict__ dst) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)n0 * n1 * n2;
+  if (q >= n) return;
+  int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
+  dst[q] = (double)src[IX(L, l0 + i, l1 + j, l2 + k)];
+}
+template <typename T>
+__global__ void k_fill(size_t n, T *__restrict__ p, double v) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = (T)v;
+}
+
+// q patch of SOURCE_terms (source80.inc:40-49) from the noise draws
+__global__ void k_source(int npatch, double fq_t, double fq_s, double wee, const double *__restrict__ noise,
+                         int use_hash, uint64_t seed, int mtime, double *__restrict__ qt, double *__restrict__ qs,
+                         double *__restrict__ qc) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= npatch) return;
+  double r1, r2;
+  if (use_hash) {
+    r1 = hash_uniform(seed, (uint64_t)mtime, (uint64_t)n, 0);
+    r2 = hash_uniform(seed, (uint64_t)mtime, (uint64_t)n, 1);
+  } else {
+    r1 = noise[3 * n];
+    r2 = noise[3 * n + 1];
+  }
+  qt[n] = fq_t * (1.0 + wee * (2.0 * (r1 - .5)));
+  qs[n] = fq_s * (1.0 + wee * (2.0 * (r2 - .5)));
+  qc[n] = 1.0;
+}
+
+// ---------------------------------------------------------------------------
+static void field_bounds(int field, const hc_params &P, int lo[3], int hi[3]) {
+  lo[0] = 0; lo[1] = 0; lo[2] = 0;
+  hi[0] = P.nx + 1; hi[1] = P.ny + 1; hi[2] = P.nz + 1;
+  switch (field) {
+    case HC_U: case HC_UP: case HC_UM: case HC_FX: lo[0] = -1; break;
+    case HC_V: case HC_VP: case HC_VM: case HC_FY: lo[1] = -1; break;
+    case HC_W: case HC_WP: case HC_WM: case HC_FZ: lo[2] = -1; break;
+    default: break;
+  }
+}
+
+template <typename T>
+static void launch_fill(hc_ctx *ctx, void *p, double v) {
+  size_t n = ctx->L.n;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->st, n, (T *)p, v);
+}
+static void fill(hc_ctx *ctx, void *p, double v) {
+  if (ctx->prec == HC_FP64) launch_fill<double>(ctx, p, v);
+  else launch_fill<float>(ctx, p, v);
+}
+
+static void *dalloc(hc_ctx *ctx, size_t bytes, int *rc) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    *rc = HC_ENOMEM;
+    ctx->err = "hipMalloc failed for " + std::to_string(bytes) + " bytes";
+    return nullptr;
+  }
+  hipMemsetAsync(p, 0, bytes, ctx->st);
+  return p;
+}
+
+extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
+  if (!prm || !out) return HC_EINVAL;
+  *out = nullptr;
+  hc_ctx *ctx = new hc_ctx();
+  ctx->P = *prm;
+  const hc_params &P = ctx->P;
+  ctx->prec = precision;
+  if (precision != HC_FP64 && precision != HC_FP32) {
+    int r = fail(ctx, HC_EINVAL, "precision must be HC_FP64 or HC_FP32");
+    *out = ctx;
+    return r;
+  }
+  auto fft_ok = [](int n) {
+    if (n % 2) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+  };
+  if (P.nx < 8 || P.ny < 4 || P.nz < 3 || !fft_ok(P.nx) || !fft_ok(P.ny)) {
+    // nx >= 8: the entrance ramp of FORCES touches i = 0..7
+    // (hotcross80-revised2014.f:953-956); nx, ny = 2^a 3^b, even, for the
+    // LDS transforms
+    int r = fail(ctx, HC_EINVAL, "grid must have nx>=8, ny>=4, nz>=3 and nx, ny even products of 2 and 3");
+    *out = ctx;
+    return r;
+  }
+  ctx->esz = precision == HC_FP64 ? 8 : 4;
+  ctx->L = make_layout(P.nx, P.ny, P.nz);
+  *out = ctx;
+  HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+  int rc = 0;
+  size_t fb = ctx->L.n * ctx->esz;
+  for (int q = 0; q < HC_NFIELDS; ++q)
+    if (!(ctx->f[q] = dalloc(ctx, fb, &rc))) return rc;
+  fill(ctx, ctx->f[HC_FX], MISSING);
+  fill(ctx, ctx->f[HC_FY], MISSING);
+  // tracer scratch
+  void **tws[] = {&ctx->tw.uc, &ctx->tw.vc, &ctx->tw.wc, &ctx->tw.ud, &ctx->tw.vd, &ctx->tw.wd,
+                  &ctx->tw.ud2, &ctx->tw.vd2, &ctx->tw.wd2, &ctx->tw.c1, &ctx->tw.c2};
+  for (auto p : tws)
+    if (!(*p = dalloc(ctx, fb, &rc))) return rc;
+  if (!(ctx->tw.flag = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
+  if (!(ctx->tw.budget_part = (double *)dalloc(ctx, (size_t)3 * P.ny * P.nz * 8, &rc))) return rc;
+  if (!(ctx->tw.budget = (double *)dalloc(ctx, 64, &rc))) return rc;
+  // case arrays
+  int nz2 = P.nz + 2;
+  double **a1[] = {&ctx->aux.azzmin, &ctx->aux.rho_bkg, &ctx->aux.pgradx, &ctx->aux.pgrady, &ctx->d_tbkg,
+                   &ctx->d_sbkg, &ctx->d_zero, &ctx->d_cbkg};
+  for (auto p : a1)
+    if (!(*p = (double *)dalloc(ctx, (size_t)nz2 * 8, &rc))) return rc;
+  if (!(ctx->aux.alphaw = (double *)dalloc(ctx, (size_t)(P.nz + 3) * 8, &rc))) return rc;
+  size_t n2 = (size_t)(P.nx + 3) * (P.ny + 3) * 8;
+  double **a2[] = {&ctx->aux.alphau, &ctx->aux.alphav, &ctx->aux.alphae, &ctx->aux.alphae0};
+  for (auto p : a2)
+    if (!(*p = (double *)dalloc(ctx, n2, &rc))) return rc;
+  ctx->h_alphae.assign((size_t)(P.nx + 2) * (P.ny + 2), 0.0);
+  // source patch
+  {
+    int nx = P.nx, ny = P.ny;
+    int nnx = (int)lround(P.source_length / P.dx), nny = (int)lround(P.source_width / P.dy);
+    if (nnx % 2 == 0) { ctx->nxs = nx / 3 - (nnx / 2 - 1); ctx->nxe = nx / 3 + (nnx / 2 - 1) + 1; }
+    else { ctx->nxs = nx / 3 - (nnx / 2 - 1) - 1; ctx->nxe = nx / 3 + (nnx / 2 - 1) + 1; }
+    if (nny % 2 == 0) { ctx->nys = ny / 2 - (nny / 2 - 1); ctx->nye = ny / 2 + (nny / 2 - 1) + 1; }
+    else { ctx->nys = ny / 2 - (nny / 2 - 1) - 1; ctx->nye = ny / 2 + (nny / 2 - 1) + 1; }
+    if (ctx->nxs < 1 || ctx->nxe > nx || ctx->nys < 1 || ctx->nye > ny)
+      return fail(ctx, HC_EINVAL, "source patch does not fit the grid");
+    ctx->npatch = (ctx->nxe - ctx->nxs + 1) * (ctx->nye - ctx->nys + 1);
+  }
+  for (int q = 0; q < 3; ++q)
+    if (!(ctx->d_q[q] = (double *)dalloc(ctx, (size_t)ctx->npatch * 8, &rc))) return rc;
+  if (!(ctx->d_noise = (double *)dalloc(ctx, (size_t)ctx->npatch * 3 * 8, &rc))) return rc;
+  ctx->stage_n = (size_t)(P.nx + 3) * (P.ny + 3) * (P.nz + 3);
+  if (!(ctx->d_stage = (double *)dalloc(ctx, ctx->stage_n * 8, &rc))) return rc;
+  int ncfl = (P.ny + 1) * (P.nz + 1);
+  if (!(ctx->d_cflpart = (double *)dalloc(ctx, (size_t)6 * ncfl * 8, &rc))) return rc;
+  if (!(ctx->d_cflout = (double *)dalloc(ctx, 64, &rc))) return rc;
+  HIPCHK(hipHostMalloc((void **)&ctx->h_pin, 4096, hipHostMallocDefault));
+  // pressure solve
+  size_t nc = (size_t)P.nx * P.ny * P.nz;
+  if (!(ctx->pw.r = dalloc(ctx, nc * ctx->esz, &rc))) return rc;
+  if (!(ctx->pw.piv = (double *)dalloc(ctx, nc * 8, &rc))) return rc;
+  if (!(ctx->pw.part = (double *)dalloc(ctx, (size_t)P.ny * P.nz * 8, &rc))) return rc;
+  if (!(ctx->pw.delta = (double *)dalloc(ctx, 64, &rc))) return rc;
+  if (!(ctx->d_tw = (double *)dalloc(ctx, (size_t)4 * (P.nx + P.ny) * 8, &rc))) return rc;
+  {
+    // twiddles exp(-2 pi i m/N) and DCT rotations (cos, sin)(pi k/(2N))
+    std::vector<double> tw((size_t)4 * (P.nx + P.ny));
+    double pi = 4.0 * atan(1.0);
+    size_t o = 0;
+    for (int N : {P.nx, P.ny}) {
+      for (int m = 0; m < N; ++m) { tw[o++] = cos(2 * pi * m / N); tw[o++] = -sin(2 * pi * m / N); }
+      for (int k = 0; k < N; ++k) { tw[o++] = cos(pi * k / (2.0 * N)); tw[o++] = sin(pi * k / (2.0 * N)); }
+    }
+    HIPCHK(hipMemcpy(ctx->d_tw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
+    ctx->pw.tw_x = ctx->d_tw;
+    ctx->pw.tw_y = ctx->d_tw + 4 * P.nx;
+    // pivots of a*x(k-1) + (b_k + lambda)*x(k) + a*x(k+1) = g(k), b = -2a, b_1 = b_nz = -a,
+    // lambda = -4/dx^2 sin^2(pi kx/(2nx)) - 4/dy^2 sin^2(pi ky/(2ny)), the DCT-II
+    // eigenvalues of the x and y operators (hs3crt_composite80.f:200-225, 604-611)
+    std::vector<double> piv(nc);
+    double a = 1.0 / (P.dz * P.dz);
+    double cx = 1.0 / (P.dx * P.dx), cy = 1.0 / (P.dy * P.dy);
+    for (int ky = 0; ky < P.ny; ++ky)
+      for (int kx = 0; kx < P.nx; ++kx) {
+        double sx = sin(kx * pi / (2.0 * P.nx)), sy = sin(ky * pi / (2.0 * P.ny));
+        double lam = -4.0 * cx * (sx * sx) + -4.0 * cy * (sy * sy);
+        double d = 0;
+        for (int k = 0; k < P.nz; ++k) {
+          double b = (k == 0 || k == P.nz - 1) ? -a : -2.0 * a;
+          double den = (k == 0) ? (b + lam) : (b + lam - a * a * d);
+          if (kx == 0 && ky == 0 && k == P.nz - 1) den = 1.0;  // singular mode, hs3crt :627-629
+          d = 1.0 / den;
+          piv[((size_t)k * P.ny + ky) * P.nx + kx] = d;
+        }
+      }
+    HIPCHK(hipMemcpy(ctx->pw.piv, piv.data(), nc * 8, hipMemcpyHostToDevice));
+  }
+  double pi = 4.0 * atan2(1.0, 1.0);
+  ctx->fc = 2.0 * P.omega * sin(pi * P.lat / 180.0);
+  ctx->fc_y = (double)P.bit9 * 2.0 * P.omega * cos(pi * P.lat / 180.0);
+  ctx->time = -P.dt;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
+}
+
+extern "C" int hc_destroy(hc_ctx *ctx) {
+  if (!ctx) return HC_OK;
+  if (ctx->st) hipStreamSynchronize(ctx->st);
+  for (int q = 0; q < HC_NFIELDS; ++q) hipFree(ctx->f[q]);
+  void *ps[] = {ctx->tw.uc, ctx->tw.vc, ctx->tw.wc, ctx->tw.ud, ctx->tw.vd, ctx->tw.wd, ctx->tw.ud2,
+                ctx->tw.vd2, ctx->tw.wd2, ctx->tw.c1, ctx->tw.c2, ctx->tw.flag, ctx->tw.budget_part,
+                ctx->tw.budget, ctx->aux.azzmin, ctx->aux.rho_bkg, ctx->aux.pgradx, ctx->aux.pgrady,
+                ctx->aux.alphaw, ctx->aux.alphau, ctx->aux.alphav, ctx->aux.alphae, ctx->aux.alphae0,
+                ctx->d_tbkg, ctx->d_sbkg, ctx->d_zero, ctx->d_cbkg, ctx->d_q[0], ctx->d_q[1], ctx->d_q[2],
+                ctx->d_noise, ctx->d_stage, ctx->d_cflpart, ctx->d_cflout, ctx->pw.r, ctx->pw.piv, ctx->pw.part,
+                ctx->pw.delta, ctx->d_tw};
+  for (void *p : ps)
+    if (p) hipFree(p);
+  if (ctx->h_pin) hipHostFree(ctx->h_pin);
+  for (auto &p : ctx->phases) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+  if (ctx->st) hipStreamDestroy(ctx->st);
+  delete ctx;
+  return HC_OK;
+}
+
+extern "C" const char *hc_last_error(const hc_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+extern "C" void *hc_stream(hc_ctx *ctx) { return ctx ? (void *)ctx->st : nullptr; }
+extern "C" int hc_synchronize(hc_ctx *ctx) {
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
+}
+
+static int check_box(hc_ctx *ctx, int field, const int32_t lo[3], const int32_t hi[3], int n[3]) {
+  if (field < 0 || field >= HC_NFIELDS) return fail(ctx, HC_EINVAL, "bad field id");
+  int fl[3], fh[3];
+  field_bounds(field, ctx->P, fl, fh);
+  for (int q = 0; q < 3; ++q) {
+    if (lo[q] < fl[q] || hi[q] > fh[q] || hi[q] < lo[q]) return fail(ctx, HC_EINVAL, "box outside field bounds");
+    n[q] = hi[q] - lo[q] + 1;
+  }
+  return HC_OK;
+}
+
+extern "C" int hc_upload(hc_ctx *ctx, int field, const double *host, const int32_t lo[3], const int32_t hi[3]) {
+  int n[3];
+  int r = check_box(ctx, field, lo, hi, n);
+  if (r) return r;
+  size_t cnt = (size_t)n[0] * n[1] * n[2];
+  HIPCHK(hipMemcpyAsync(ctx->d_stage, host, cnt * 8, hipMemcpyHostToDevice, ctx->st));
+  dim3 g((unsigned)((cnt + 255) / 256));
+  if (ctx->prec == HC_FP64)
+    hipLaunchKernelGGL(k_scatter<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                       ctx->d_stage, (double *)ctx->f[field]);
+  else
+    hipLaunchKernelGGL(k_scatter<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                       ctx->d_stage, (float *)ctx->f[field]);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
+}
+
+extern "C" int hc_download(hc_ctx *ctx, int field, double *host, const int32_t lo[3], const int32_t hi[3]) {
+  int n[3];
+  int r = check_box(ctx, field, lo, hi, n);
+  if (r) return r;
+  size_t cnt = (size_t)n[0] * n[1] * n[2];
+  dim3 g((unsigned)((cnt + 255) / 256));
+  if (ctx->prec == HC_FP64)
+    hipLaunchKernelGGL(k_gather<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                       (const double *)ctx->f[field], ctx->d_stage);
+  else
+    hipLaunchKernelGGL(k_gather<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                       (const float *)ctx->f[field], ctx->d_stage);
+  HIPCHK(hipMemcpyAsync(host, ctx->d_stage, cnt * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
+}
+
+extern "C" int hc_set_profiles(hc_ctx *ctx, const double *t_bkg, const double *s_bkg, const double *rho_bkg,
+                               const double *azzmin) {
+  int nz2 = ctx->P.nz + 2;
+  ctx->h_tbkg.assign(t_bkg, t_bkg + nz2);
+  ctx->h_sbkg.assign(s_bkg, s_bkg + nz2);
+  ctx->h_rho_bkg.assign(rho_bkg, rho_bkg + nz2);
+  ctx->h_azzmin.assign(azzmin, azzmin + nz2);
+  HIPCHK(hipMemcpy(ctx->d_tbkg, t_bkg, nz2 * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_sbkg, s_bkg, nz2 * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->aux.rho_bkg, rho_bkg, nz2 * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->aux.azzmin, azzmin, nz2 * 8, hipMemcpyHostToDevice));
+  return HC_OK;
+}
+
+extern "C" int hc_set_sponge(hc_ctx *ctx, const double *alphau, const double *alphav, const double *alphae,
+                             const double *alphaw) {
+  const hc_params &P = ctx->P;
+  int nx = P.nx, ny = P.ny, nz = P.nz;
+  size_t nu = (size_t)(nx + 3) * (ny + 2), nv = (size_t)(nx + 2) * (ny + 3), ne = (size_t)(nx + 2) * (ny + 2);
+  ctx->h_alphau.assign(alphau, alphau + nu);
+  ctx->h_alphav.assign(alphav, alphav + nv);
+  ctx->h_alphae.assign(alphae, alphae + ne);
+  ctx->h_alphaw.assign(alphaw, alphaw + nz + 3);
+  // alphaee (sponge80.f:115-119) is formed from alphae before its periodic
+  // copies (:128-138); rebuild that mask
+  ctx->h_alphae0.assign(ne, 0.0);
+  for (int j = 0; j <= ny + 1; ++j)
+    for (int i = 0; i <= nx + 1; ++i)
+      ctx->h_alphae0[i + (nx + 2) * j] =
+          fmax((alphau[(i + 1) + (nx + 3) * j] + alphau[i + (nx + 3) * j]),
+               (alphav[i + (nx + 2) * (j + 1)] + alphav[i + (nx + 2) * j])) / 2.0;
+  HIPCHK(hipMemcpy(ctx->aux.alphau, alphau, nu * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->aux.alphav, alphav, nv * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->aux.alphae, alphae, ne * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->aux.alphae0, ctx->h_alphae0.data(), ne * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->aux.alphaw, alphaw, (nz + 3) * 8, hipMemcpyHostToDevice));
+  update_sums(ctx);
+  return HC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-side set-up of a case (not on the timed path)
+static double rho_rho_host(const hc_params &P, double s, double t) {
+  if (P.eos == 1) return P.eos_rho_ref * (1.0 - P.eos_alpha * (t - P.eos_t0) + P.eos_beta * (s - P.eos_s0));
+  double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t4 * t;
+  double r0 = 999.842594 + 6.793952e-2 * t - 9.095290e-3 * t * t + 1.001685e-4 * t3 - 1.120083e-6 * t4 +
+              6.536332e-9 * t5;
+  return r0 + s * (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t * t - 8.2467e-7 * t3 + 5.3875e-9 * t4) +
+         pow(s, 1.5) * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t * t) + 4.8314e-4 * s * s;
+}
+
+extern "C" int hc_init_case(hc_ctx *ctx, const double *zd, const double *td, const double *sd, int32_t nd,
+                            double t_noise, uint64_t seed) {
+  const hc_params &P = ctx->P;
+  int nx = P.nx, ny = P.ny, nz = P.nz;
+  if (nd < 2) return fail(ctx, HC_EINVAL, "profile table needs two rows");
+  ctx->seed = seed;
+  // INITIALIZE_PROFILES, initialize_profiles80.f:46-64 with LOCATOR :164-197
+  std::vector<double> tb(nz + 2), sb(nz + 2), az(nz + 2), rb(nz + 2);
+  auto locate = [&](double x) {
+    int ilo = 1, ihi = nd;
+    while (ihi - ilo > 1) {
+      int im = (ihi + ilo) / 2;
+      if (x >= zd[im - 1]) ilo = im; else ihi = im;
+    }
+    if (x == zd[0]) return 1;
+    if (x == zd[nd - 1]) return nd - 1;
+    return ilo;
+  };
+  for (int k = 0; k <= nz; ++k) {
+    double zz = (P.zstart - P.dz / 2 + k * P.dz);
+    int kup = locate(zz);
+    double z0 = zd[kup - 1], z1 = zd[kup];
+    tb[k] = td[kup - 1] + (td[kup] - td[kup - 1]) / (z1 - z0) * (zz - z0);
+    sb[k] = sd[kup - 1] + (sd[kup] - sd[kup - 1]) / (z1 - z0) * (zz - z0);
+    az[k] = (P.azmin + (P.azmax - P.azmin) * exp(-((P.zend - P.zstart) - k * P.dz) / P.mixing_height_scale));
+  }
+  tb[0] = tb[1]; tb[nz + 1] = tb[nz];
+  sb[0] = sb[1]; sb[nz + 1] = sb[nz];
+  for (int k = 0; k <= nz + 1; ++k) rb[k] = rho_rho_host(P, sb[k], tb[k]);
+  int r = hc_set_profiles(ctx, tb.data(), sb.data(), rb.data(), az.data());
+  if (r) return r;
+  // SPONGE_MAKER, sponge80.f:20-138
+  std::vector<double> au((size_t)(nx + 3) * (ny + 2), 0.0), av((size_t)(nx + 2) * (ny + 3), 0.0),
+      ae((size_t)(nx + 2) * (ny + 2), 0.0), aw(nz + 3, 0.0);
+  double beta = 0.95;
+#define AU(i, j) au[(size_t)((i) + 1) + (size_t)(nx + 3) * (j)]
+#define AV(i, j) av[(size_t)(i) + (size_t)(nx + 2) * ((j) + 1)]
+#define AE(i, j) ae[(size_t)(i) + (size_t)(nx + 2) * (j)]
+  int ncut = (int)fmin(8.0, (double)lround(.10 * (double)nx));
+  for (int j = 0; j <= ny + 1; ++j) {
+    for (int i = -1; i <= ncut; ++i) AU(i, j) = beta * (double)(ncut - i) / (double)(ncut + 1);
+    for (int i = nx + 1 - ncut - 1; i <= nx + 1; ++i) AU(i, j) = beta * (double)(i - (nx + 1 - ncut - 1)) / (double)(ncut + 1);
+  }
+  if (P.bcondx == 0)
+    for (int j = 0; j <= ny + 1; ++j) { AU(-1, j) = AU(nx - 1, j); AU(nx + 1, j) = AU(1, j); }
+  for (auto &x : au) x = x * x;
+  ncut = (int)fmin(8.0, (double)lround(.10 * (double)ny));
+  for (int i = 0; i <= nx + 1; ++i) {
+    for (int j = -1; j <= ncut; ++j) AV(i, j) = beta * (double)(ncut - j) / (double)(ncut + 1);
+    for (int j = ny + 1 - ncut - 1; j <= ny + 1; ++j) AV(i, j) = beta * (double)(j - (ny + 1 - ncut - 1)) / (double)(ncut + 1);
+  }
+  if (P.bcondy == 0)
+    for (int i = 0; i <= nx + 1; ++i) { AV(i, -1) = AV(i, ny - 1); AV(i, ny + 1) = AV(i, 1); }
+  for (auto &x : av) x = x * x;
+  ncut = (int)fmin(10.0, (double)lround(.20 * (double)nz));
+  for (int k = -1; k <= ncut; ++k) aw[k + 1] = beta * (double)(ncut - k) / (double)ncut;
+  for (auto &x : aw) x = 1.0 * (x * x);
+  double sf = P.sponge_factor / P.dt;
+  for (auto &x : au) x = sf * x;
+  for (auto &x : av) x = sf * x;
+  for (auto &x : aw) x = sf * x;
+  for (int j = 0; j <= ny + 1; ++j)
+    for (int i = 0; i <= nx + 1; ++i) AE(i, j) = fmax((AU(i, j) + AU(i - 1, j)), (AV(i, j) + AV(i, j - 1))) / 2.0;
+  if (P.bcondx == 0)
+    for (int j = 0; j <= ny + 1; ++j) { AE(0, j) = AE(nx, j); AE(nx + 1, j) = AE(1, j); }
+  if (P.bcondy == 0)
+    for (int i = 0; i <= nx + 1; ++i) { AE(i, 0) = AE(i, ny); AE(i, ny + 1) = AE(i, 1); }
+#undef AU
+#undef AV
+#undef AE
+  r = hc_set_sponge(ctx, au.data(), av.data(), ae.data(), aw.data());
+  if (r) return r;
+  // INITIALIZE (hotcross80-revised2014.f:444-560): velocities and pressure
+  // zero with the -1e34 fill on the 12 box edges of u, v, w, up, vp, wp, p;
+  // tracers at background (initialize_profiles80.f:141-153), T noise
+  for (int fld = 0; fld < HC_NFIELDS; ++fld) {
+    if (fld == HC_FX || fld == HC_FY || fld == HC_FZ || fld == HC_SS || fld == HC_SSV || fld == HC_RHOP) continue;
+    int lo[3], hi[3];
+    field_bounds(fld, P, lo, hi);
+    int n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1, n2 = hi[2] - lo[2] + 1;
+    std::vector<double> a((size_t)n0 * n1 * n2, 0.0);
+    for (int k = 0; k < n2; ++k)
+      for (int j = 0; j < n1; ++j)
+        for (int i = 0; i < n0; ++i) {
+          size_t q = (size_t)i + (size_t)n0 * (j + (size_t)n1 * k);
+          int kk = lo[2] + k;
+          if (fld == HC_T) {
+            a[q] = tb[kk];
+            if (t_noise != 0.0)
+              a[q] += t_noise * (2.0 * hash_uniform(seed ^ 0x7E57ull, (uint64_t)i, (uint64_t)j, (uint64_t)k) - 1.0);
+          } else if (fld == HC_S) {
+            a[q] = sb[kk];
+          }
+          bool edge = ((i == 0 || i == n0 - 1) + (j == 0 || j == n1 - 1) + (k == 0 || k == n2 - 1)) >= 2;
+          bool filled = fld == HC_U || fld == HC_V || fld == HC_W || fld == HC_UP || fld == HC_VP ||
+                        fld == HC_WP || fld == HC_P;
+          if (edge && filled) a[q] = MISSING;
+        }
+    r = hc_upload(ctx, fld, a.data(), lo, hi);
+    if (r) return r;
+  }
+  // DENSITY of the initial state (hotcross80-revised2014.f:213)
+  return hc_density(ctx);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int hc_begin_step(hc_ctx *ctx, int32_t mtime, double time) {
+  ctx->time = time;
+  ctx->mtime = mtime;
+  ctx->leap = 2;
+  ctx->delta = 1;
+  if (mtime == 0) { ctx->leap = 1; ctx->delta = 0; }
+  return HC_OK;
+}
+
+// crossflow series, site_forcing.inc:47-72 (ut(-1) = vt(-1) = 0)
+static void flow_series(const hc_params &P, int l, double *ut, double *vt) {
+  if (l < 0) { *ut = 0; *vt = 0; return; }
+  double time_ = P.dt * l + 0.0;
+  double u = P.flow_u0 + P.flow_u1 * sin(P.flow_omega * time_);
+  double v = P.flow_v1 * cos(P.flow_omega * time_);
+  double fac = 1.0;
+  if (P.flow_ramp_hrs > 0) fac = fmin(1.0, (l * P.dt) / (P.flow_ramp_hrs * 3600.0));
+  *ut = fac * u;
+  *vt = fac * v;
+}
+
+extern "C" int hc_body_force(hc_ctx *ctx) {
+  const hc_params &P = ctx->P;
+  ctx->tau = ctx->leap * P.dt;
+  int l = (int)lround(ctx->time / P.dt);
+  double d;
+  flow_series(P, l - 1, &ctx->ubkg_m, &d); flow_series(P, l - 1, &d, &ctx->vbkg_m);
+  flow_series(P, l, &ctx->ubkg, &d); flow_series(P, l, &d, &ctx->vbkg);
+  flow_series(P, l + 1, &ctx->ubkg_p, &d); flow_series(P, l + 1, &d, &ctx->vbkg_p);
+  double v4 = ((ctx->vbkg + ctx->vbkg) + ctx->vbkg) + ctx->vbkg;
+  double u4 = ((ctx->ubkg + ctx->ubkg) + ctx->ubkg) + ctx->ubkg;
+  ctx->fbx = +(ctx->ubkg_p - (1 - ctx->delta) * ctx->ubkg - ctx->delta * ctx->ubkg_m) / ctx->tau -
+             P.bit7 * ctx->fc * v4 / 4.0;
+  ctx->fby = +(ctx->vbkg_p - (1 - ctx->delta) * ctx->vbkg - ctx->delta * ctx->vbkg_m) / ctx->tau +
+             P.bit7 * ctx->fc * u4 / 4.0;
+  return HC_OK;
+}
+
+extern "C" int hc_forces(hc_ctx *ctx) {
+  PhaseScope ps(ctx, "forces");
+  // the entrance ramp (hotcross80-revised2014.f:953-956) accumulates on the
+  // boundary entries that :937-944 never recompute
+  for (int i = 0; i <= 7; ++i) ctx->acc[i] = ctx->acc[i] + (double)(7 - i) * .01 / 7.0;
+  StepArgs s = step_args(ctx);
+  if (ctx->prec == HC_FP64) launch_forces<double>(ctx->st, ctx->L, s, ctx->P, ptrs<double>(ctx), ctx->aux);
+  else launch_forces<float>(ctx->st, ctx->L, s, ctx->P, ptrs<float>(ctx), ctx->aux);
+  return HC_OK;
+}
+
+extern "C" int hc_pressure(hc_ctx *ctx) {
+  PhaseScope ps(ctx, "pressure");
+  StepArgs s = step_args(ctx);
+  if (ctx->prec == HC_FP64) launch_pressure<double>(ctx->st, ctx->L, s, ctx->P, ptrs<double>(ctx), ctx->aux, ctx->pw);
+  else launch_pressure<float>(ctx->st, ctx->L, s, ctx->P, ptrs<float>(ctx), ctx->aux, ctx->pw);
+  return HC_OK;
+}
+
+static int momentum(hc_ctx *ctx, int comp) {
+  PhaseScope ps(ctx, "momentum");
+  StepArgs s = step_args(ctx);
+  VentArgs va = vent_args(ctx);
+  if (ctx->prec == HC_FP64) launch_momentum<double>(ctx->st, comp, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux);
+  else launch_momentum<float>(ctx->st, comp, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux);
+  return HC_OK;
+}
+extern "C" int hc_momentum_u(hc_ctx *ctx) { return momentum(ctx, 0); }
+extern "C" int hc_momentum_v(hc_ctx *ctx) { return momentum(ctx, 1); }
+extern "C" int hc_momentum_w(hc_ctx *ctx) { return momentum(ctx, 2); }
+
+extern "C" int hc_viscosity3d(hc_ctx *ctx) {
+  PhaseScope ps(ctx, "viscosity3d");
+  StepArgs s = step_args(ctx);
+  if (ctx->prec == HC_FP64) launch_viscosity<double>(ctx->st, ctx->L, s, ctx->P, ptrs<double>(ctx), ctx->aux);
+  else launch_viscosity<float>(ctx->st, ctx->L, s, ctx->P, ptrs<float>(ctx), ctx->aux);
+  return HC_OK;
+}
+
+extern "C" int hc_source_terms(hc_ctx *ctx, const double *noise) {
+  const hc_params &P = ctx->P;
+  double f = fmin(1.0, ctx->mtime * P.dt / (P.source_ramp_hrs * 3600.0));
+  if (noise) HIPCHK(hipMemcpyAsync(ctx->d_noise, noise, (size_t)ctx->npatch * 3 * 8, hipMemcpyHostToDevice, ctx->st));
+  hipLaunchKernelGGL(k_source, dim3((ctx->npatch + 255) / 256), dim3(256), 0, ctx->st, ctx->npatch, P.dqtdt * f,
+                     P.dqsdt * f, P.noise_amp, ctx->d_noise, noise ? 0 : 1, ctx->seed, ctx->mtime, ctx->d_q[0],
+                     ctx->d_q[1], ctx->d_q[2]);
+  if (noise) HIPCHK(hipStreamSynchronize(ctx->st));
+  if (!ctx->map_valid) {
+    ctx->map_valid = true;
+    update_sums(ctx);
+  }
+  return HC_OK;
+}
+
+extern "C" int hc_mixing(hc_ctx *ctx) {
+  (void)ctx;
+  return HC_OK;
+}
+
+extern "C" int hc_work_velocity(hc_ctx *ctx, double tau3) {
+  PhaseScope ps(ctx, "work_velocity");
+  StepArgs s = step_args(ctx);
+  s.tau3 = tau3;
+  if (ctx->prec == HC_FP64) launch_courant<double>(ctx->st, ctx->L, s, ptrs<double>(ctx), ctx->tw);
+  else launch_courant<float>(ctx->st, ctx->L, s, ptrs<float>(ctx), ctx->tw);
+  return HC_OK;
+}
+
+static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3, int vent,
+                  double vent_value, int want_budget) {
+  PhaseScope ps(ctx, "tracer31");
+  int which = field == HC_T ? 0 : field == HC_S ? 1 : 2;
+  StepArgs s = step_args(ctx);
+  VentArgs va = vent_args(ctx);
+  TracerArgs ta{cmin, cmax, tau3, vent, vent_value, want_budget};
+  if (ctx->prec == HC_FP64)
+    launch_tracer<double>(ctx->st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, ctx->tw,
+                          (double *)ctx->f[field], d_cbkg, ctx->d_q[which], ta);
+  else
+    launch_tracer<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, ctx->tw,
+                         (float *)ctx->f[field], d_cbkg, ctx->d_q[which], ta);
+  return HC_OK;
+}
+
+extern "C" int hc_tracer31(hc_ctx *ctx, int field, const double *c_bkg, double cmin, double cmax, double tau3,
+                           double *budgets) {
+  if (field != HC_T && field != HC_S && field != HC_C) return fail(ctx, HC_EINVAL, "tracer field must be T, S or C");
+  if (ctx->P.ismol != 1) return fail(ctx, HC_EINVAL, "only ismol = 1 (two corrective passes) is supported");
+  HIPCHK(hipMemcpyAsync(ctx->d_cbkg, c_bkg, (size_t)(ctx->P.nz + 2) * 8, hipMemcpyHostToDevice, ctx->st));
+  int want = (budgets != nullptr) && (ctx->mtime % 100 == 0);
+  int r = tracer(ctx, field, ctx->d_cbkg, cmin, cmax, tau3, 0, 0.0, want);
+  if (r) return r;
+  if (want) {
+    HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->tw.budget, 3 * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int q = 0; q < 3; ++q) budgets[q] = ctx->h_pin[q];
+  } else {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+  }
+  return HC_OK;
+}
+
+extern "C" int hc_vent_ghost(hc_ctx *ctx, int field, double value) {
+  VentArgs va = vent_args(ctx);
+  if (ctx->prec == HC_FP64) launch_vent_ghost<double>(ctx->st, ctx->L, va, (double *)ctx->f[field], value);
+  else launch_vent_ghost<float>(ctx->st, ctx->L, va, (float *)ctx->f[field], value);
+  return HC_OK;
+}
+
+extern "C" int hc_density(hc_ctx *ctx) {
+  PhaseScope ps(ctx, "density");
+  if (ctx->prec == HC_FP64) launch_density<double>(ctx->st, ctx->L, ctx->P, ptrs<double>(ctx), ctx->aux);
+  else launch_density<float>(ctx->st, ctx->L, ctx->P, ptrs<float>(ctx), ctx->aux);
+  return HC_OK;
+}
+
+extern "C" int hc_refresh(hc_ctx *ctx) {
+  PhaseScope ps(ctx, "refresh");
+  int tri[3][3] = {{HC_U, HC_UP, HC_UM}, {HC_V, HC_VP, HC_VM}, {HC_W, HC_WP, HC_WM}};
+  for (auto &t : tri) {
+    if (ctx->prec == HC_FP64)
+      launch_refresh<double>(ctx->st, ctx->L, ctx->leap, ctx->P.assel, (double *)ctx->f[t[0]], (double *)ctx->f[t[1]],
+                             (double *)ctx->f[t[2]]);
+    else
+      launch_refresh<float>(ctx->st, ctx->L, ctx->leap, ctx->P.assel, (float *)ctx->f[t[0]], (float *)ctx->f[t[1]],
+                            (float *)ctx->f[t[2]]);
+    // u = up: the up buffer becomes u; the old u buffer is overwritten by the
+    // next MOMENTUM call on every cell MOMENTUM writes
+    void *tmp = ctx->f[t[0]];
+    ctx->f[t[0]] = ctx->f[t[1]];
+    ctx->f[t[1]] = tmp;
+  }
+  return HC_OK;
+}
+
+extern "C" int hc_cfl_values(hc_ctx *ctx, double out[6]) {
+  if (ctx->prec == HC_FP64) launch_cfl<double>(ctx->st, ctx->L, ptrs<double>(ctx), ctx->d_cflpart, ctx->d_cflout);
+  else launch_cfl<float>(ctx->st, ctx->L, ptrs<float>(ctx), ctx->d_cflpart, ctx->d_cflout);
+  HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_cflout, 6 * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const hc_params &P = ctx->P;
+  for (int q = 0; q < 6; ++q) out[q] = ctx->h_pin[q] * P.dt / (q < 2 ? P.dx : q < 4 ? P.dy : P.dz);
+  double m = fmax(fmax(fmax(out[0], out[2]), fmax(-out[1], -out[3])), fmax(out[4], -out[5]));
+  if (m > 0.75) return fail(ctx, HC_ECFL, "stop on impending cfl violation");
+  return HC_OK;
+}
+
+// the reference's time loop body, hotcross80-revised2014.f:264-350
+extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
+  const hc_params &P = ctx->P;
+  int nz = P.nz;
+  if (ctx->h_tbkg.empty()) return fail(ctx, HC_EINVAL, "profiles not set (hc_init_case or hc_set_profiles)");
+  double tmin = 1e300, smax = -1e300;
+  for (int k = 0; k <= nz + 1; ++k) { tmin = fmin(tmin, ctx->h_tbkg[k]); smax = fmax(smax, ctx->h_sbkg[k]); }
+  for (int n = 0; n < nsteps; ++n) {
+    int m = ctx->next_step++;
+    hc_begin_step(ctx, m, ctx->time + P.dt);
+    hc_body_force(ctx);
+    hc_forces(ctx);
+    hc_pressure(ctx);
+    momentum(ctx, 0);
+    momentum(ctx, 1);
+    momentum(ctx, 2);
+    hc_viscosity3d(ctx);
+    {
+      PhaseScope ps(ctx, "source");
+      hc_source_terms(ctx, nullptr);
+    }
+    int budgets = (m % 100 == 0);
+    if (P.toggle_t == 1) {
+      hc_work_velocity(ctx, P.dt);
+      tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], budgets);
+    }
+    if (P.toggle_s == 1) tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], budgets);
+    if (P.bit3 == 1) hc_density(ctx);
+    if (P.toggle_c == 1) tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, budgets);
+    hc_refresh(ctx);
+    if (m % 10 == 0) {
+      double cfl[6];
+      int r = hc_cfl_values(ctx, cfl);
+      if (r) return r;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, HC_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  }
+  return HC_OK;
+}
+
+// noise draws of the native loop's generator (for checkers replaying it)
+extern "C" int hc_source_noise(hc_ctx *ctx, int32_t mtime, double *out) {
+  for (int n = 0; n < ctx->npatch; ++n) {
+    out[3 * n] = hash_uniform(ctx->seed, (uint64_t)mtime, (uint64_t)n, 0);
+    out[3 * n + 1] = hash_uniform(ctx->seed, (uint64_t)mtime, (uint64_t)n, 1);
+    out[3 * n + 2] = hash_uniform(ctx->seed, (uint64_t)mtime, (uint64_t)n, 2);
+  }
+  return ctx->npatch;
+}
+
+extern "C" int hc_phase_timing(hc_ctx *ctx, int enable) {
+  hipStreamSynchronize(ctx->st);
+  for (auto &p : ctx->phases) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+  ctx->phases.clear();
+  ctx->timing = enable != 0;
+  return HC_OK;
+}
+
+extern "C" int hc_phase_times(hc_ctx *ctx, const char **names, double *ms, int32_t *counts, int32_t cap) {
+  hipStreamSynchronize(ctx->st);
+  int n = 0;
+  for (auto &p : ctx->phases) {
+    if (p.pending) {
+      float t = 0;
+      hipEventElapsedTime(&t, p.a, p.b);
+      p.ms += t;
+      p.count++;
+      p.pending = false;
+    }
+    if (n < cap) {
+      names[n] = p.name;
+      ms[n] = p.ms;
+      counts[n] = p.count;
+    }
+    n++;
+  }
+  return n;
+}

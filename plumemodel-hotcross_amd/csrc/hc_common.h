@@ -1,0 +1,68 @@
+// hc_common.h - device layout, launch helpers and per-step scalars shared by
+// the HotCross HIP kernels and the C-ABI implementation.
+//
+// HBM layout: every 3-D field (whatever its staggering) is stored in one
+// padded box that covers i=-1..nx+1, j=-1..ny+1, k=-1..nz+1, x fastest.  The
+// storage offset puts i=0 on a 16-element boundary (128 B for FP64) so that
+// interior rows start cache-line aligned; the row pitch px is a multiple of 16
+// elements.  A Fortran module array of any class (u: -1:nx+1, v: -1:ny+1,
+// w: -1:nz+1, scalar) maps onto the same index function, so kernels that mix
+// staggered fields share one address computation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace hc {
+
+constexpr int OXP = 16;  // storage index of i = 0 is OXP
+
+struct Layout {
+  int nx, ny, nz;
+  int px, py, pz;  // padded extents
+  size_t n;        // elements per field
+};
+
+__host__ __device__ inline size_t IX(const Layout &L, int i, int j, int k) {
+  return (size_t)(i + OXP) + (size_t)L.px * ((size_t)(j + 1) + (size_t)L.py * (size_t)(k + 1));
+}
+
+inline Layout make_layout(int nx, int ny, int nz) {
+  Layout L;
+  L.nx = nx; L.ny = ny; L.nz = nz;
+  L.px = ((OXP + nx + 2) + 15) / 16 * 16;
+  L.py = ny + 3;
+  L.pz = nz + 3;
+  L.n = (size_t)L.px * L.py * L.pz;
+  return L;
+}
+
+// Scalars of one timestep that kernels take by value.
+struct StepArgs {
+  double dx, dy, dz, dt, tau3;
+  int leap, delta, mtime;
+  double ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby;
+  double fc, fc_y;
+  double acc[8];  // persistent azu/azv boundary accumulators
+};
+
+// Box-wise index for 3-D launches: block (64,4,1) over (i, j), grid.z = k.
+struct Box {
+  int i0, i1, j0, j1, k0, k1;  // inclusive Fortran ranges
+};
+
+inline dim3 box_grid(const Box &b) {
+  int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+  return dim3((ni + 63) / 64, (nj + 3) / 4, nk);
+}
+inline dim3 box_block() { return dim3(64, 4, 1); }
+
+#define HC_BOX_IJK(b)                                  \
+  const int i = (b).i0 + (int)(blockIdx.x * 64 + threadIdx.x); \
+  const int j = (b).j0 + (int)(blockIdx.y * 4 + threadIdx.y);  \
+  const int k = (b).k0 + (int)blockIdx.z;              \
+  if (i > (b).i1 || j > (b).j1) return;
+
+constexpr double MISSING = -1.0e+34;
+
+}  // namespace hc

@@ -1,0 +1,89 @@
+// hc_kernels.h - kernel launcher declarations of the HotCross timestep.
+#pragma once
+#include "../../include/hotcross.h"
+#include "hc_common.h"
+
+namespace hc {
+
+typedef hc_params Phys;
+
+// vent / source patch state used by MOMENTUM_W and the tracer kernels
+struct VentArgs {
+  int map_valid;            // isource_map set by a SOURCE_terms call
+  int nxs, nxe, nys, nye;   // patch bounds, source80.inc:10-27
+  double wup, sumup, sumdown;
+};
+
+template <typename T>
+struct FieldPtrs {
+  T *u, *v, *w, *up, *vp, *wp, *um, *vm, *wm, *p, *t, *s, *c, *ss, *ssv, *rhop, *fx, *fy, *fz;
+};
+
+// 1-D / 2-D case arrays, FP64, Fortran order with the reference's bounds
+struct Aux {
+  double *azzmin, *rho_bkg, *pgradx, *pgrady;  // (0:nz+1)
+  double *alphau;   // (-1:nx+1, 0:ny+1)
+  double *alphav;   // (0:nx+1, -1:ny+1)
+  double *alphae;   // (0:nx+1, 0:ny+1) after the periodic copies
+  double *alphae0;  // (0:nx+1, 0:ny+1) as used to form alphaee
+  double *alphaw;   // (-1:nz+1)
+};
+
+template <typename T>
+void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                   const Aux &aux);
+template <typename T>
+void launch_momentum(hipStream_t st, int comp, const Layout &L, const StepArgs &s, const Phys &ph,
+                     const VentArgs &va, const FieldPtrs<T> &f, const Aux &aux);
+template <typename T>
+void launch_viscosity(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                      const Aux &aux);
+template <typename T>
+void launch_density(hipStream_t st, const Layout &L, const Phys &ph, const FieldPtrs<T> &f, const Aux &aux);
+template <typename T>
+void launch_refresh(hipStream_t st, const Layout &L, int leap, double assel, const T *u, const T *up, T *um);
+
+// tracer transport (TRACER31 / MPDATA)
+struct TracerWork {
+  void *uc, *vc, *wc;      // Courant numbers of the work velocities
+  void *ud, *vd, *wd;      // antidiffusive velocities of the current pass
+  void *ud2, *vd2, *wd2;   // ... of the previous pass
+  void *c1, *c2;           // pass results
+  uint8_t *flag;           // limiter flags per cell
+  double *budget_part;     // per-block partial sums (FP64)
+  double *budget;          // 3 sums
+};
+struct TracerArgs {
+  double cmin, cmax, tau3;
+  int vent;          // apply the driver's vent ghost override after the call
+  double vent_value;
+  int budgets;       // mod(mtime,100) == 0
+};
+template <typename T>
+void launch_courant(hipStream_t st, const Layout &L, const StepArgs &s, const FieldPtrs<T> &f, TracerWork &tw);
+template <typename T>
+void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const VentArgs &va,
+                   const FieldPtrs<T> &f, const Aux &aux, TracerWork &tw, T *c, const double *c_bkg,
+                   const double *qpatch, const TracerArgs &ta);
+template <typename T>
+void launch_vent_ghost(hipStream_t st, const Layout &L, const VentArgs &va, T *c, double value);
+template <typename T>
+void launch_cfl(hipStream_t st, const Layout &L, const FieldPtrs<T> &f, double *part, double *out);
+
+// pressure solve (PRESSURE + HS3CRT)
+struct PoissonWork {
+  void *r;         // compact (nx,ny,nz) right-hand side / transform work
+  void *rh;        // second compact buffer
+  double *piv;     // pivots of the z tridiagonal systems, (nx,ny,nz)
+  double *lam;     // lambda_x(kx) + lambda_y(ky), (nx,ny)
+  const double *tw_x, *tw_y;  // FFT twiddles exp(-2 pi i m/N), complex (2N)
+  double *part;    // per-block partial sums of the right-hand side
+  double *delta;   // its total
+  int nparts;
+};
+template <typename T>
+void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                     const Aux &aux, PoissonWork &pw);
+void poisson_setup(const Layout &L, const Phys &ph, PoissonWork &pw, hipStream_t st);
+
+}  // namespace hc

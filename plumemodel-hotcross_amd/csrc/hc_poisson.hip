@@ -1,0 +1,461 @@
+// hc_poisson.hip - PRESSURE and its HS3CRT solve on gfx950.
+//
+// The reference (hotcross80-revised2014.f:1146-1405, hs3crt_composite80.f)
+// solves the 7-point staggered Laplacian with Neumann data on all six faces
+// by cosine transforms in y and z and a pre-factored Thomas solve in x whose
+// singular pivot (zero wavenumber in y and z) is replaced by 1 (:627-629).
+// That replacement solves the compatible system obtained by removing the
+// right-hand side's total delta from the plane i = nx, delta/(ny*nz) per cell;
+// the null-space constant is then removed by PRESSURE's mean subtraction.
+//
+// This build keeps the operator and that compatibility rule but transforms in
+// the two directions that map onto LDS best and solves along z:
+//   k_rhs       RHS + boundary data (HS3CRT :38-106), FP64 block sums of it
+//   k_sum1      delta = total of the RHS (FP64)
+//   k_dct_x<0>  rows along x: delta correction at i=nx, DCT-II   (LDS FFT)
+//   k_dct_y<0>  16-column tiles along y: DCT-II                  (LDS FFT)
+//   k_tridiag   Thomas solve per (kx,ky) column in z with pre-factored
+//               pivots; the singular (0,0) column in FP64 with its last pivot
+//               set to 1, then its k-mean removed (PRESSURE :1325-1345)
+//   k_dct_y<1>  inverse
+//   k_dct_x<1>  inverse, writes p and its x and z ghost faces (:1348-1372)
+//   k_pyghost   y ghost faces (:1374-1381)
+// The transforms are orthonormal DCT-II/III computed as one complex FFT per
+// pair of real sequences (Makhoul's reordering), mixed radix 4/2/3 Stockham
+// in LDS with an FP64 twiddle table.
+#include "hc_common.h"
+#include "hc_kernels.h"
+
+namespace hc {
+
+#define F3(a, i, j, k) (a)[IX(L, (i), (j), (k))]
+
+template <typename T>
+struct cpx {
+  T x, y;
+};
+template <typename T>
+__device__ inline cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ inline cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ inline cpx<T> cmul(cpx<T> a, cpx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+struct FftPlan {
+  int N;
+  int nst;
+  int radix[16];
+};
+
+inline FftPlan make_plan(int N) {
+  FftPlan p{};
+  p.N = N;
+  int n = N;
+  while (n % 4 == 0) { p.radix[p.nst++] = 4; n /= 4; }
+  while (n % 2 == 0) { p.radix[p.nst++] = 2; n /= 2; }
+  while (n % 3 == 0) { p.radix[p.nst++] = 3; n /= 3; }
+  if (n != 1) p.nst = -1;
+  return p;
+}
+
+// One complex FFT of length N on each of nseq sequences held in LDS (stride
+// N); sign -1 forward, +1 inverse (unnormalised).  Returns the buffer that
+// holds the result.
+template <typename T>
+__device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftPlan &pl, int nseq, const double *tw, int sign) {
+  const int N = pl.N;
+  int Ns = 1;
+  const int nthr = blockDim.x;
+  for (int st = 0; st < pl.nst; ++st) {
+    const int R = pl.radix[st];
+    const int nb = N / R;
+    const int step = N / (Ns * R);
+    for (int t = threadIdx.x; t < nb * nseq; t += nthr) {
+      int s = t / nb, j = t - s * nb;
+      const cpx<T> *in = a + (size_t)s * N;
+      cpx<T> *out = b + (size_t)s * N;
+      int jm = j % Ns;
+      cpx<T> v[4];
+      for (int r = 0; r < R; ++r) v[r] = in[j + r * nb];
+      for (int r = 1; r < R; ++r) {
+        int m = (r * jm * step) % N;
+        cpx<T> w{(T)tw[2 * m], (T)(sign > 0 ? -tw[2 * m + 1] : tw[2 * m + 1])};
+        v[r] = cmul(v[r], w);
+      }
+      cpx<T> y[4];
+      if (R == 2) {
+        y[0] = cadd(v[0], v[1]);
+        y[1] = csub(v[0], v[1]);
+      } else if (R == 4) {
+        cpx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+        // d * (sign * i)
+        cpx<T> t3 = sign < 0 ? cpx<T>{d.y, -d.x} : cpx<T>{-d.y, d.x};
+        y[0] = cadd(t0, t2);
+        y[2] = csub(t0, t2);
+        y[1] = cadd(t1, t3);
+        y[3] = csub(t1, t3);
+      } else {  // R == 3
+        const T c = (T)-0.5, sn = (T)(sign * 0.86602540378443864676);
+        cpx<T> s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
+        cpx<T> t1{v[0].x + c * s12.x, v[0].y + c * s12.y};
+        cpx<T> rot{-sn * d12.y, sn * d12.x};  // i * sn * d12
+        y[0] = cadd(v[0], s12);
+        y[1] = cadd(t1, rot);
+        y[2] = csub(t1, rot);
+      }
+      int idxD = (j / Ns) * Ns * R + jm;
+      for (int r = 0; r < R; ++r) out[idxD + r * Ns] = y[r];
+    }
+    __syncthreads();
+    cpx<T> *tmp = a; a = b; b = tmp;
+    Ns *= R;
+  }
+  return a;
+}
+
+// x index of the Makhoul reordering: v[n] = x[2n] (n < N/2), v[N-1-n] = x[2n+1]
+__device__ inline int mperm(int n, int N) { return n < N / 2 ? 2 * n : 2 * (N - 1 - n) + 1; }
+
+// DCT post-processing of a packed pair: Z = FFT(a + i b)  ->  orthonormal
+// DCT-II of a and b at index k
+template <typename T>
+__device__ inline void dct_post(const cpx<T> *Z, int N, int k, const double *ct, T &xa, T &xb) {
+  cpx<T> zk = Z[k], zc = Z[(N - k) % N];
+  zc.y = -zc.y;
+  cpx<T> A{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
+  cpx<T> D{zk.x - zc.x, zk.y - zc.y};
+  cpx<T> B{D.y * (T)0.5, -D.x * (T)0.5};  // D * (-i/2)
+  T c = (T)ct[2 * k], s = (T)ct[2 * k + 1];
+  T sk = (T)(k == 0 ? sqrt(1.0 / N) : sqrt(2.0 / N));
+  xa = sk * (A.x * c + A.y * s);
+  xb = sk * (B.x * c + B.y * s);
+}
+
+// inverse pre-processing: orthonormal coefficients X -> V[k] of one sequence
+template <typename T>
+__device__ inline cpx<T> dct_pre_inv(T xk, T xnk, int k, int N, const double *ct) {
+  T sk = (T)(k == 0 ? sqrt((double)N) : sqrt(N / 2.0));
+  T snk = (T)sqrt(N / 2.0);
+  T p = xk * sk, q = (k == 0) ? (T)0 : xnk * snk;
+  T c = (T)ct[2 * k], s = (T)ct[2 * k + 1];
+  return {p * c + q * s, p * s - q * c};
+}
+
+// ---------------------------------------------------------------------------
+// right-hand side (hotcross80-revised2014.f:1191-1257 + hs3crt :38-106)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rhs(Layout L, StepArgs s, Phys ph, const T *__restrict__ um,
+                                             const T *__restrict__ vm, const T *__restrict__ wm,
+                                             const T *__restrict__ w, const T *__restrict__ fx,
+                                             const T *__restrict__ fy, const T *__restrict__ fz,
+                                             const double *__restrict__ pgradx, const double *__restrict__ pgrady,
+                                             T *__restrict__ r, double *__restrict__ part) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const int j = 1 + (int)blockIdx.x % ny, k = 1 + (int)blockIdx.x / ny;
+  const double dx = s.dx, dy = s.dy, dz = s.dz, dt = s.dt;
+  const double dlx = 1.0 / ((nx * dx - 0.0) / nx), dly = 1.0 / ((ny * dy - 0.0) / ny), dlz = 1.0 / ((nz * dz - 0.0) / nz);
+  double acc = 0.0;
+  for (int i = 1 + (int)threadIdx.x; i <= nx; i += blockDim.x) {
+    T v = ((F3(um, i, j, k) - F3(um, i - 1, j, k)) / (T)dx + (F3(vm, i, j, k) - F3(vm, i, j - 1, k)) / (T)dy +
+           (F3(wm, i, j, k) - F3(wm, i, j, k - 1)) / (T)dz) / (T)dt / (T)s.leap +
+          ((F3(fx, i, j, k) - F3(fx, i - 1, j, k)) / (T)dx + (F3(fy, i, j, k) - F3(fy, i, j - 1, k)) / (T)dy +
+           (F3(fz, i, j, k) - F3(fz, i, j, k - 1)) / (T)dz);
+    if (i == 1) {
+      T bd = (T)(-(s.ubkg - s.ubkg_m) / dt - pgradx[k]) + F3(fx, 0, j, k);
+      v = v + bd * (T)dlx;
+    }
+    if (i == nx) {
+      T bd = (T)(-(s.ubkg - s.ubkg_m) / dt - pgradx[k]) + F3(fx, nx, j, k);
+      v = v - bd * (T)dlx;
+    }
+    if (j == 1) {
+      T bd = (T)(-(s.vbkg - s.vbkg_m) / dt - pgrady[k]) + F3(fy, i, 0, k);
+      v = v + bd * (T)dly;
+    }
+    if (j == ny) {
+      T bd = (T)(-(s.vbkg - s.vbkg_m) / dt - pgrady[k]) + F3(fy, i, ny, k);
+      v = v - bd * (T)dly;
+    }
+    if (k == 1) {
+      T bd = -(F3(w, i, j, 0) - F3(wm, i, j, 0)) / (T)dt + F3(fz, i, j, 0);
+      v = v + bd * (T)dlz;
+    }
+    if (k == nz) {
+      T bd = -(F3(w, i, j, nz) - F3(wm, i, j, nz)) / (T)dt + F3(fz, i, j, nz);
+      v = v - bd * (T)dlz;
+    }
+    r[((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1)] = v;
+    acc += (double)v;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum1(int n, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[256];
+  double a = 0;
+  for (int q = threadIdx.x; q < n; q += 256) a += part[q];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// DCT along x over row pairs.  INV=0: r -> rh (delta correction first);
+// INV=1: rh -> p (padded field) with the x and z ghost faces.
+template <typename T, int INV>
+__global__ void __launch_bounds__(256) k_dct_x(Layout L, FftPlan pl, int npair, const double *__restrict__ tw,
+                                               const double *__restrict__ ct, const T *__restrict__ in,
+                                               T *__restrict__ out, const double *__restrict__ delta, StepArgs s,
+                                               Phys ph, T *__restrict__ p, const T *__restrict__ fx,
+                                               const T *__restrict__ w, const T *__restrict__ wm,
+                                               const T *__restrict__ fz, const double *__restrict__ pgradx) {
+  extern __shared__ char smem_raw[];
+  const int N = L.nx, ny = L.ny, nz = L.nz;
+  const int nrows = ny * nz;
+  cpx<T> *A = (cpx<T> *)smem_raw, *B = A + (size_t)npair * N;
+  const int row0 = blockIdx.x * 2 * npair;
+  const double corr = INV ? 0.0 : delta[0] / ((double)ny * nz);
+  // load
+  for (int t = threadIdx.x; t < npair * N; t += blockDim.x) {
+    int pq = t / N, n = t - pq * N;
+    int ra = row0 + 2 * pq, rb = ra + 1;
+    if (!INV) {
+      int x = mperm(n, N);
+      T a = ra < nrows ? in[(size_t)ra * N + x] : (T)0;
+      T b = rb < nrows ? in[(size_t)rb * N + x] : (T)0;
+      if (x == N - 1) { a = a - (T)corr; b = b - (T)corr; }
+      A[(size_t)pq * N + n] = {a, b};
+    } else {
+      int k = n, nk = (N - k) % N;
+      T xa = ra < nrows ? in[(size_t)ra * N + k] : (T)0, xb = rb < nrows ? in[(size_t)rb * N + k] : (T)0;
+      T xna = ra < nrows ? in[(size_t)ra * N + nk] : (T)0, xnb = rb < nrows ? in[(size_t)rb * N + nk] : (T)0;
+      cpx<T> va = dct_pre_inv(xa, xna, k, N, ct), vb = dct_pre_inv(xb, xnb, k, N, ct);
+      A[(size_t)pq * N + n] = {va.x - vb.y, va.y + vb.x};  // va + i vb
+    }
+  }
+  __syncthreads();
+  cpx<T> *Z = lds_fft<T>(A, B, pl, npair, tw, INV ? +1 : -1);
+  if (!INV) {
+    for (int t = threadIdx.x; t < npair * N; t += blockDim.x) {
+      int pq = t / N, k = t - pq * N;
+      int ra = row0 + 2 * pq, rb = ra + 1;
+      T xa, xb;
+      dct_post(Z + (size_t)pq * N, N, k, ct, xa, xb);
+      if (ra < nrows) out[(size_t)ra * N + k] = xa;
+      if (rb < nrows) out[(size_t)rb * N + k] = xb;
+    }
+  } else {
+    const T invN = (T)(1.0 / N);
+    for (int t = threadIdx.x; t < npair * N; t += blockDim.x) {
+      int pq = t / N, n = t - pq * N;
+      int x = mperm(n, N);
+      cpx<T> z = Z[(size_t)pq * N + n];
+      for (int h = 0; h < 2; ++h) {
+        int rr = row0 + 2 * pq + h;
+        if (rr >= nrows) continue;
+        int j = 1 + rr % ny, k = 1 + rr / ny;
+        T val = (h == 0 ? z.x : z.y) * invN;
+        int i = x + 1;
+        F3(p, i, j, k) = val;
+        // z ghost faces, :1348-1350
+        if (k == 1) {
+          T bd = -(F3(w, i, j, 0) - F3(wm, i, j, 0)) / (T)s.dt + F3(fz, i, j, 0);
+          F3(p, i, j, 0) = val - bd * (T)s.dz;
+        }
+        if (k == nz) {
+          T bd = -(F3(w, i, j, nz) - F3(wm, i, j, nz)) / (T)s.dt + F3(fz, i, j, nz);
+          F3(p, i, j, nz + 1) = val + bd * (T)s.dz;
+        }
+      }
+    }
+    __syncthreads();
+    // x ghost faces, :1352-1372 (need the values at i = 1 and i = nx)
+    for (int t = threadIdx.x; t < 2 * npair; t += blockDim.x) {
+      int rr = row0 + t;
+      if (rr >= nrows) continue;
+      int j = 1 + rr % ny, k = 1 + rr / ny;
+      T p1 = F3(p, 1, j, k), pn = F3(p, N, j, k);
+      T bxs = (T)(-(s.ubkg - s.ubkg_m) / s.dt - pgradx[k]) + F3(fx, 0, j, k);
+      T bxf = (T)(-(s.ubkg - s.ubkg_m) / s.dt - pgradx[k]) + F3(fx, N, j, k);
+      if (ph.open_box == 1) {
+        F3(p, 0, j, k) = (p1 - bxs * (T)s.dx);
+        F3(p, N + 1, j, k) = (pn + bxf * (T)s.dx);
+      } else {
+        T bx = (T)ph.bcondx;
+        F3(p, 0, j, k) = (p1 - bxs * (T)s.dx) * bx + (T)(1 - ph.bcondx) * pn;
+        F3(p, N + 1, j, k) = (pn + bxf * (T)s.dx) * bx + (T)(1 - ph.bcondx) * p1;
+      }
+    }
+  }
+}
+
+// DCT along y on tiles of 2*npair x-columns of one z-plane, in place on buf
+template <typename T, int INV>
+__global__ void __launch_bounds__(256) k_dct_y(Layout L, FftPlan pl, int npair, const double *__restrict__ tw,
+                                               const double *__restrict__ ct, T *__restrict__ buf) {
+  extern __shared__ char smem_raw[];
+  const int nx = L.nx, N = L.ny;
+  cpx<T> *A = (cpx<T> *)smem_raw, *B = A + (size_t)npair * N;
+  const int ntile = (nx + 2 * npair - 1) / (2 * npair);
+  const int k = blockIdx.x / ntile, x0 = (blockIdx.x % ntile) * 2 * npair;
+  T *plane = buf + (size_t)k * N * nx;
+  const int TX = 2 * npair;
+  for (int t = threadIdx.x; t < TX * N; t += blockDim.x) {
+    int c = t % TX, n = t / TX;  // coalesced over c
+    int x = x0 + c;
+    int pq = c >> 1, h = c & 1;
+    if (!INV) {
+      // element y = n goes to position m with mperm(m) = n
+      int m = (n % 2 == 0) ? n / 2 : N - 1 - (n - 1) / 2;
+      T v = x < nx ? plane[(size_t)n * nx + x] : (T)0;
+      T *dst = (T *)&A[(size_t)pq * N + m];
+      dst[h] = v;
+    } else {
+      T *dst = (T *)&B[(size_t)pq * N + n];  // stage raw coefficients in B
+      dst[h] = x < nx ? plane[(size_t)n * nx + x] : (T)0;
+    }
+  }
+  __syncthreads();
+  if (INV) {
+    for (int t = threadIdx.x; t < npair * N; t += blockDim.x) {
+      int pq = t / N, k2 = t - pq * N, nk = (N - k2) % N;
+      cpx<T> xr = B[(size_t)pq * N + k2], xn = B[(size_t)pq * N + nk];
+      cpx<T> va = dct_pre_inv(xr.x, xn.x, k2, N, ct), vb = dct_pre_inv(xr.y, xn.y, k2, N, ct);
+      A[(size_t)pq * N + k2] = {va.x - vb.y, va.y + vb.x};
+    }
+    __syncthreads();
+  }
+  cpx<T> *Z = lds_fft<T>(A, B, pl, npair, tw, INV ? +1 : -1);
+  if (!INV) {
+    for (int t = threadIdx.x; t < TX * N; t += blockDim.x) {
+      int c = t % TX, k2 = t / TX, x = x0 + c;
+      int pq = c >> 1, h = c & 1;
+      T xa, xb;
+      dct_post(Z + (size_t)pq * N, N, k2, ct, xa, xb);
+      if (x < nx) plane[(size_t)k2 * nx + x] = h ? xb : xa;
+    }
+  } else {
+    const T invN = (T)(1.0 / N);
+    for (int t = threadIdx.x; t < TX * N; t += blockDim.x) {
+      int c = t % TX, y = t / TX, x = x0 + c;
+      int pq = c >> 1, h = c & 1;
+      int m = (y % 2 == 0) ? y / 2 : N - 1 - (y - 1) / 2;
+      cpx<T> z = Z[(size_t)pq * N + m];
+      if (x < nx) plane[(size_t)y * nx + x] = (h ? z.y : z.x) * invN;
+    }
+  }
+}
+
+// Thomas solve along z per (kx,ky), hs3crt_composite80.f:408-431 in the z
+// direction; the (0,0) column in FP64 with the k-mean removed
+template <typename T>
+__global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const double *__restrict__ piv,
+                                                 T *__restrict__ buf) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= plane) return;
+  const double a = 1.0 / (dz * dz);
+  if (q == 0) {
+    double y = (double)buf[0] * piv[0];
+    buf[0] = (T)y;
+    for (int k = 1; k < nz; ++k) {
+      y = ((double)buf[k * plane] - a * y) * piv[k * plane];
+      buf[k * plane] = (T)y;
+    }
+    // back substitution, keeping FP64 values for the mean
+    double xk = y, sum = xk;
+    double *tmp = nullptr;
+    (void)tmp;
+    for (int k = nz - 2; k >= 0; --k) {
+      double yk = (double)buf[k * plane];
+      xk = yk - a * piv[k * plane] * xk;
+      buf[k * plane] = (T)xk;
+      sum += xk;
+    }
+    // second pass for exact FP64 mean removal (values re-read as stored)
+    double mean = sum / nz;
+    for (int k = 0; k < nz; ++k) buf[k * plane] = (T)((double)buf[k * plane] - mean);
+    return;
+  }
+  T y = buf[q] * (T)piv[q];
+  buf[q] = y;
+  for (int k = 1; k < nz; ++k) {
+    y = (buf[k * plane + q] - (T)a * y) * (T)piv[k * plane + q];
+    buf[k * plane + q] = y;
+  }
+  T x = y;
+  for (int k = nz - 2; k >= 0; --k) {
+    x = buf[k * plane + q] - (T)a * (T)piv[k * plane + q] * x;
+    buf[k * plane + q] = x;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pyghost(Layout L, StepArgs s, Phys ph, const T *__restrict__ fy,
+                                                 const double *__restrict__ pgrady, T *__restrict__ p) {
+  int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x), k = 1 + (int)blockIdx.y;
+  if (i > L.nx) return;
+  const int ny = L.ny;
+  T by = (T)ph.bcondy, oby = (T)(1 - ph.bcondy);
+  T bys = (T)(-(s.vbkg - s.vbkg_m) / s.dt - pgrady[k]) + F3(fy, i, 0, k);
+  T byf = (T)(-(s.vbkg - s.vbkg_m) / s.dt - pgrady[k]) + F3(fy, i, ny, k);
+  T p1 = F3(p, i, 1, k), pn = F3(p, i, ny, k);
+  F3(p, i, 0, k) = (p1 - bys * (T)s.dy) * by + oby * pn;
+  F3(p, i, ny + 1, k) = (pn + byf * (T)s.dy) * by + oby * p1;
+}
+
+// ---------------------------------------------------------------------------
+static int pick_npair(int N, size_t elem) {
+  // LDS: 2 buffers x npair x N complex elements, at most 64 KiB
+  int np = 8;
+  while (np > 1 && (size_t)2 * np * N * 2 * elem > 65536) np >>= 1;
+  return np;
+}
+
+template <typename T>
+void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                     const Aux &aux, PoissonWork &pw) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  T *r = (T *)pw.r;
+  hipLaunchKernelGGL(k_rhs<T>, dim3(ny * nz), dim3(256), 0, st, L, s, ph, f.um, f.vm, f.wm, f.w, f.fx, f.fy, f.fz,
+                     aux.pgradx, aux.pgrady, r, pw.part);
+  hipLaunchKernelGGL(k_sum1, dim3(1), dim3(256), 0, st, ny * nz, pw.part, pw.delta);
+  FftPlan px = make_plan(nx), py = make_plan(ny);
+  int npx = pick_npair(nx, sizeof(T)), npy = pick_npair(ny, sizeof(T));
+  size_t shx = (size_t)2 * npx * nx * 2 * sizeof(T), shy = (size_t)2 * npy * ny * 2 * sizeof(T);
+  int nrows = ny * nz;
+  int gx = (nrows + 2 * npx - 1) / (2 * npx);
+  const double *twx = pw.tw_x, *ctx = pw.tw_x + 2 * nx;
+  const double *twy = pw.tw_y, *cty = pw.tw_y + 2 * ny;
+  hipLaunchKernelGGL((k_dct_x<T, 0>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, r, r, pw.delta, s, ph,
+                     (T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                     (const double *)nullptr);
+  int gy = nz * ((nx + 2 * npy - 1) / (2 * npy));
+  hipLaunchKernelGGL((k_dct_y<T, 0>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
+  hipLaunchKernelGGL(k_tridiag<T>, dim3((unsigned)(((size_t)nx * ny + 255) / 256)), dim3(256), 0, st, L, s.dz,
+                     pw.piv, r);
+  hipLaunchKernelGGL((k_dct_y<T, 1>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
+  hipLaunchKernelGGL((k_dct_x<T, 1>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, r, (T *)nullptr,
+                     pw.delta, s, ph, f.p, f.fx, f.w, f.wm, f.fz, aux.pgradx);
+  hipLaunchKernelGGL(k_pyghost<T>, dim3((nx + 255) / 256, nz), dim3(256), 0, st, L, s, ph, f.fy, aux.pgrady, f.p);
+}
+
+#define HC_INST_P(T)                                                                                          \
+  template void launch_pressure<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,               \
+                                   const FieldPtrs<T> &, const Aux &, PoissonWork &);
+HC_INST_P(double)
+HC_INST_P(float)
+
+}  // namespace hc

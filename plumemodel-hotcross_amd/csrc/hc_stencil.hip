@@ -1,0 +1,468 @@
+// hc_stencil.hip - stencil kernels of the HotCross timestep for gfx950.
+//
+// One thread per cell of a box, block (64,4): a wavefront sweeps 64
+// consecutive i of one (j,k) row, so every load and store of a field is a
+// coalesced x-fastest sweep.  All kernels are HBM-bandwidth bound (about
+// 1.5 flop per byte, far below the FP64 ridge), so they are written for the
+// fewest passes over HBM, not for arithmetic.  Expressions keep the
+// reference's evaluation order term by term (and the library is built with
+// -ffp-contract=off) so the FP64 form reproduces the reference's rounding.
+#include "hc_common.h"
+#include "hc_kernels.h"
+
+namespace hc {
+
+#define F3(a, i, j, k) (a)[IX(L, (i), (j), (k))]
+
+// ---------------------------------------------------------------------------
+// FORCES, hotcross80-revised2014.f:890-1139.  azu/azv are formed on the fly;
+// their never-recomputed boundary entries are the persistent accumulators
+// acc[i] (the entrance ramp of :953-956 adds (7-i)*0.01/7 to them per call).
+template <typename T>
+__device__ inline T azu_at(const Layout &L, const T *ssv, const double *azzmin,
+                           const StepArgs &s, int i, int j, int k) {
+  // azu(i,j,k) for i=0..nx, j=1..ny, k=0..nz
+  if (i == 0) return k >= 1 ? (T)s.acc[0] : (T)0;
+  if (i == L.nx) return (T)0;
+  T v = (T).25 * (F3(ssv, i, j, k) + F3(ssv, i + 1, j, k) + F3(ssv, i, j, k + 1) + F3(ssv, i + 1, j, k + 1)) +
+        (T)azzmin[k];
+  if (i <= 7 && k >= 1) v = v + (T)((double)(7 - i) * .01 / 7.0);
+  return v;
+}
+
+template <typename T>
+__device__ inline T azv_at(const Layout &L, const T *ssv, const double *azzmin,
+                           const StepArgs &s, int i, int j, int k) {
+  // azv(i,j,k) for i=1..nx, j=0..ny, k=0..nz
+  if (j == 0 || j == L.ny) {
+    if (k == 0 || i > 7) return (T)0;
+    return (T)s.acc[i];
+  }
+  T v = (T).25 * (F3(ssv, i, j, k) + F3(ssv, i, j + 1, k) + F3(ssv, i, j, k + 1) + F3(ssv, i, j + 1, k + 1)) +
+        (T)azzmin[k];
+  if (i <= 7 && k >= 1) v = v + (T)((double)(7 - i) * .01 / 7.0);
+  return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_forces_x(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+                                                  const T *__restrict__ v, const T *__restrict__ w,
+                                                  const T *__restrict__ um, const T *__restrict__ vm,
+                                                  const T *__restrict__ wm, const T *__restrict__ ss,
+                                                  const T *__restrict__ ssv, const double *__restrict__ azzmin,
+                                                  const double *__restrict__ alphau, T *__restrict__ fx) {
+  HC_BOX_IJK(b);
+  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
+  const T axmin = (T)ph.axmin, aymin = (T)ph.aymin;
+  const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
+          coe_4z = (T)((double)ph.bit4 / 4.0 / s.dz);
+  const T coe_7 = (T)(ph.bit7 * s.fc / 4.0), coe_9 = (T)(ph.bit9 * s.fc_y / 4.0), coe_10 = (T)ph.bit10;
+  T axu_p = F3(ss, i + 1, j, k) + axmin, axu_c = F3(ss, i, j, k) + axmin;
+  T ayu_c = (T).25 * (F3(ss, i, j, k) + F3(ss, i + 1, j, k) + F3(ss, i, j + 1, k) + F3(ss, i + 1, j + 1, k)) + aymin;
+  T ayu_m = (T).25 * (F3(ss, i, j - 1, k) + F3(ss, i + 1, j - 1, k) + F3(ss, i, j, k) + F3(ss, i + 1, j, k)) + aymin;
+  T azu_c = azu_at(L, ssv, azzmin, s, i, j, k), azu_m = azu_at(L, ssv, azzmin, s, i, j, k - 1);
+  T umc = F3(um, i, j, k);
+  T dis = (T)2.0 * bit5 * (axu_p * (F3(um, i + 1, j, k) - umc) - axu_c * (umc - F3(um, i - 1, j, k))) / dx / dx +
+          bit5 * (ayu_c * (F3(um, i, j + 1, k) - umc) - ayu_m * (umc - F3(um, i, j - 1, k))) / dy / dy +
+          bit5 * (ayu_c * (F3(vm, i + 1, j, k) - F3(vm, i, j, k)) - ayu_m * (F3(vm, i + 1, j - 1, k) - F3(vm, i, j - 1, k))) / dx / dy +
+          bit2 * (azu_c * (F3(um, i, j, k + 1) - umc) - azu_m * (umc - F3(um, i, j, k - 1))) / dz / dz +
+          bit2 * (azu_c * (F3(wm, i + 1, j, k) - F3(wm, i, j, k)) - azu_m * (F3(wm, i + 1, j, k - 1) - F3(wm, i, j, k - 1))) / dx / dz;
+  T uc_ = F3(u, i, j, k);
+  T a1 = F3(u, i + 1, j, k) + uc_, a2 = uc_ + F3(u, i - 1, j, k);
+  T r = dis + (T)ph.bit3 * (T)0.0 - (T)0.5 * (T)alphau[(i + 1) + (L.nx + 3) * j] * (umc - (T)s.ubkg_m) -
+        coe_4x * (a1 * a1 - a2 * a2) -
+        coe_4z * ((F3(w, i, j, k) + F3(w, i + 1, j, k)) * (F3(u, i, j, k + 1) + uc_) -
+                  (F3(w, i, j, k - 1) + F3(w, i + 1, j, k - 1)) * (uc_ + F3(u, i, j, k - 1))) -
+        coe_4y * ((F3(v, i, j, k) + F3(v, i + 1, j, k)) * (F3(u, i, j + 1, k) + uc_) -
+                  (F3(v, i, j - 1, k) + F3(v, i + 1, j - 1, k)) * (uc_ + F3(u, i, j - 1, k))) +
+        coe_7 * (F3(v, i + 1, j, k) + F3(v, i, j, k) + F3(v, i + 1, j - 1, k) + F3(v, i, j - 1, k)) -
+        coe_9 * (F3(w, i + 1, j, k) + F3(w, i + 1, j, k - 1) + F3(w, i, j, k) + F3(w, i, j, k - 1)) +
+        coe_10 * (T)s.fbx;
+  F3(fx, i, j, k) = r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_forces_y(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+                                                  const T *__restrict__ v, const T *__restrict__ w,
+                                                  const T *__restrict__ um, const T *__restrict__ vm,
+                                                  const T *__restrict__ wm, const T *__restrict__ ss,
+                                                  const T *__restrict__ ssv, const double *__restrict__ azzmin,
+                                                  const double *__restrict__ alphav, T *__restrict__ fy) {
+  HC_BOX_IJK(b);
+  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
+  const T axmin = (T)ph.axmin, aymin = (T)ph.aymin;
+  const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
+          coe_4z = (T)((double)ph.bit4 / 4.0 / s.dz);
+  const T coe_7 = (T)(ph.bit7 * s.fc / 4.0), coe_10 = (T)ph.bit10;
+  T axv_c = (T).25 * (F3(ss, i, j, k) + F3(ss, i + 1, j, k) + F3(ss, i, j + 1, k) + F3(ss, i + 1, j + 1, k)) + axmin;
+  T axv_m = (T).25 * (F3(ss, i - 1, j, k) + F3(ss, i, j, k) + F3(ss, i - 1, j + 1, k) + F3(ss, i, j + 1, k)) + axmin;
+  T ayv_p = F3(ss, i, j + 1, k) + aymin, ayv_c = F3(ss, i, j, k) + aymin;
+  T azv_c = azv_at(L, ssv, azzmin, s, i, j, k), azv_m = azv_at(L, ssv, azzmin, s, i, j, k - 1);
+  T vmc = F3(vm, i, j, k);
+  T dis = +bit5 * (axv_c * (F3(vm, i + 1, j, k) - vmc) - axv_m * (vmc - F3(vm, i - 1, j, k))) / dx / dx +
+          bit5 * (axv_c * (F3(um, i, j + 1, k) - F3(um, i, j, k)) - axv_m * (F3(um, i - 1, j + 1, k) - F3(um, i - 1, j, k))) / dx / dy +
+          (T)2.0 * bit5 * (ayv_p * (F3(vm, i, j + 1, k) - vmc) - ayv_c * (vmc - F3(vm, i, j - 1, k))) / dy / dy +
+          bit2 * (azv_c * (F3(vm, i, j, k + 1) - vmc) - azv_m * (vmc - F3(vm, i, j, k - 1))) / dz / dz +
+          bit2 * (azv_c * (F3(wm, i, j + 1, k) - F3(wm, i, j, k)) - azv_m * (F3(wm, i, j + 1, k - 1) - F3(wm, i, j, k - 1))) / dy / dz;
+  T vc_ = F3(v, i, j, k);
+  T b1 = F3(v, i, j + 1, k) + vc_, b2 = vc_ + F3(v, i, j - 1, k);
+  T r = dis + (T)ph.bit3 * (T)0.0 - (T)0.5 * (T)alphav[i + (L.nx + 2) * (j + 1)] * (vmc - (T)s.vbkg_m) -
+        coe_4y * (b1 * b1 - b2 * b2) -
+        coe_4z * ((F3(v, i, j, k + 1) + vc_) * (F3(w, i, j + 1, k) + F3(w, i, j, k)) -
+                  (vc_ + F3(v, i, j, k - 1)) * (F3(w, i, j + 1, k - 1) + F3(w, i, j, k - 1))) -
+        coe_4x * ((F3(v, i + 1, j, k) + vc_) * (F3(u, i, j, k) + F3(u, i, j + 1, k)) -
+                  (vc_ + F3(v, i - 1, j, k)) * (F3(u, i - 1, j, k) + F3(u, i - 1, j + 1, k))) -
+        coe_7 * (F3(u, i, j, k) + F3(u, i - 1, j, k) + F3(u, i, j + 1, k) + F3(u, i - 1, j + 1, k)) +
+        coe_10 * (T)s.fby;
+  F3(fy, i, j, k) = r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_forces_z(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+                                                  const T *__restrict__ v, const T *__restrict__ w,
+                                                  const T *__restrict__ um, const T *__restrict__ vm,
+                                                  const T *__restrict__ wm, const T *__restrict__ ss,
+                                                  const T *__restrict__ ssv, const T *__restrict__ rhop,
+                                                  const double *__restrict__ azzmin, const double *__restrict__ alphae,
+                                                  const double *__restrict__ alphaw, T *__restrict__ fz) {
+  HC_BOX_IJK(b);
+  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
+  const T axmin = (T)ph.axmin;
+  const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
+          coe_4z = (T)((double)ph.bit4 / 4.0 / s.dz);
+  const T coe_9 = (T)(ph.bit9 * s.fc_y / 4.0), coe_10 = (T)ph.bit10;
+  T ahp = F3(ss, i, j, k + 1) + axmin, ahm = F3(ss, i, j, k) + axmin;
+  T zk = (T)azzmin[k];
+  T s0 = F3(ssv, i, j, k + 1), s1 = F3(ssv, i, j, k);
+  T avxp = (T)0.25 * (s0 + s1 + F3(ssv, i + 1, j, k + 1) + F3(ssv, i + 1, j, k)) + zk;
+  T avxm = (T)0.25 * (s0 + s1 + F3(ssv, i - 1, j, k + 1) + F3(ssv, i - 1, j, k)) + zk;
+  T avyp = (T)0.25 * (s0 + s1 + F3(ssv, i, j + 1, k + 1) + F3(ssv, i, j + 1, k)) + zk;
+  T avym = (T)0.25 * (s0 + s1 + F3(ssv, i, j - 1, k + 1) + F3(ssv, i, j - 1, k)) + zk;
+  T wmc = F3(wm, i, j, k);
+  T dis = (T)2.0 * bit5 * (ahp * (F3(wm, i, j, k + 1) - wmc) - ahm * (wmc - F3(wm, i, j, k - 1))) / dz / dz +
+          bit2 * (avxp * (F3(wm, i + 1, j, k) - wmc) - avxm * (wmc - F3(wm, i - 1, j, k))) / dx / dx +
+          bit2 * (avxp * (F3(um, i, j, k) - F3(um, i, j, k + 1)) - avxm * (F3(um, i - 1, j, k) - F3(um, i - 1, j, k + 1))) / dz / dx +
+          bit2 * (avyp * (F3(wm, i, j + 1, k) - wmc) - avym * (wmc - F3(wm, i, j - 1, k))) / dy / dy +
+          bit2 * (avyp * (F3(vm, i, j, k) - F3(vm, i, j, k + 1)) - avym * (F3(vm, i, j - 1, k) - F3(vm, i, j - 1, k + 1))) / dz / dy;
+  T wc_ = F3(w, i, j, k);
+  T c1 = F3(w, i, j, k + 1) + wc_, c2 = wc_ + F3(w, i, j, k - 1);
+  // alphaee(i,j,k) = max(alphae(i,j), (alphaw(k)+alphaw(k-1))/2), sponge80.f:115-119
+  // (alphae here is the mask before the periodic copies of :128-138)
+  double ae = fmax(alphae[i + (L.nx + 2) * j], (alphaw[k + 1] + alphaw[k]) / 2.0);
+  T r = dis + (T)ph.bit3 * (T)ph.g * (F3(rhop, i, j, k) + F3(rhop, i, j, k + 1)) / (T)ph.rho0 / (T)2.0 -
+        (T)0.0 * (T)alphaw[k + 1] * wmc - (T)0.5 * (T)ae * wmc - coe_4z * (c1 * c1 - c2 * c2) -
+        coe_4x * ((F3(w, i + 1, j, k) + wc_) * (F3(u, i, j, k) + F3(u, i, j, k + 1)) -
+                  (wc_ + F3(w, i - 1, j, k)) * (F3(u, i - 1, j, k) + F3(u, i - 1, j, k + 1))) -
+        coe_4y * ((F3(w, i, j + 1, k) + wc_) * (F3(v, i, j, k) + F3(v, i, j, k + 1)) -
+                  (wc_ + F3(w, i, j - 1, k)) * (F3(v, i, j - 1, k) + F3(v, i, j - 1, k + 1))) +
+        coe_9 * (F3(u, i, j, k + 1) + F3(u, i - 1, j, k + 1) + F3(u, i, j, k) + F3(u, i - 1, j, k)) +
+        coe_10 * (T)0.0;
+  F3(fz, i, j, k) = r;
+}
+
+// ---------------------------------------------------------------------------
+// MOMENTUM_U/V/W, hotcross80-revised2014.f:671-887.  Each thread computes the
+// value of one cell of the written set, re-evaluating the interior formula at
+// the source cell of a boundary copy (a pure function of the inputs), so one
+// pass writes interior and ghosts with the reference's values.
+template <typename T>
+__device__ inline T mom_u_val(const Layout &L, const StepArgs &s, const T *u, const T *um, const T *p,
+                              const T *fx, const double *pgradx, int i, int j, int k) {
+  return (T)(1 - s.delta) * F3(u, i, j, k) + (T)s.delta * F3(um, i, j, k) +
+         (T)s.dt * (T)s.leap * (-(F3(p, i + 1, j, k) - F3(p, i, j, k)) / (T)s.dx - (T)pgradx[k] + F3(fx, i, j, k));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_momentum_u(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+                                                    const T *__restrict__ um, const T *__restrict__ p,
+                                                    const T *__restrict__ fx, const double *__restrict__ pgradx,
+                                                    T *__restrict__ up) {
+  HC_BOX_IJK(b);  // box: i=-1..nx+1, j=0..ny+1, k=0..nz+1
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  bool gi = (i == -1 || i == nx + 1), gj = (j == 0 || j == ny + 1), gk = (k == 0 || k == nz + 1);
+  if ((int)gi + (int)gj + (int)gk > 1) return;  // edges keep -1e34
+  T val;
+  if (!gi && !gj && !gk) {
+    val = mom_u_val(L, s, u, um, p, fx, pgradx, i, j, k);
+  } else if (gk) {
+    if (k == 0) val = mom_u_val(L, s, u, um, p, fx, pgradx, i, j, 1);
+    else val = (T)ph.bottom_slip * mom_u_val(L, s, u, um, p, fx, pgradx, i, j, nz);
+  } else if (gj) {
+    val = mom_u_val(L, s, u, um, p, fx, pgradx, i, j == 0 ? 1 : ny, k);
+  } else {
+    val = mom_u_val(L, s, u, um, p, fx, pgradx, i == -1 ? 0 : nx, j, k);
+  }
+  F3(up, i, j, k) = val;
+}
+
+template <typename T>
+__device__ inline T mom_v_val(const Layout &L, const StepArgs &s, const T *v, const T *vm, const T *p,
+                              const T *fy, const double *pgrady, int i, int j, int k) {
+  return (T)(1 - s.delta) * F3(v, i, j, k) + (T)s.delta * F3(vm, i, j, k) +
+         (T)s.dt * (T)s.leap * (-(F3(p, i, j + 1, k) - F3(p, i, j, k)) / (T)s.dy - (T)pgrady[k] + F3(fy, i, j, k));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_momentum_v(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ v,
+                                                    const T *__restrict__ vm, const T *__restrict__ p,
+                                                    const T *__restrict__ fy, const double *__restrict__ pgrady,
+                                                    T *__restrict__ vp) {
+  HC_BOX_IJK(b);  // box: i=0..nx+1, j=-1..ny+1, k=0..nz+1
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  bool gi = (i == 0 || i == nx + 1), gj = (j == -1 || j == ny + 1), gk = (k == 0 || k == nz + 1);
+  if ((int)gi + (int)gj + (int)gk > 1) return;
+  T val;
+  if (!gi && !gj && !gk) {
+    val = mom_v_val(L, s, v, vm, p, fy, pgrady, i, j, k);
+  } else if (gk) {
+    if (k == 0) val = mom_v_val(L, s, v, vm, p, fy, pgrady, i, j, 1);
+    else val = (T)ph.bottom_slip * mom_v_val(L, s, v, vm, p, fy, pgrady, i, j, nz);
+  } else if (gi) {
+    val = mom_v_val(L, s, v, vm, p, fy, pgrady, i == 0 ? 1 : nx, j, k);
+  } else {
+    val = mom_v_val(L, s, v, vm, p, fy, pgrady, i, j == -1 ? 0 : ny, k);
+  }
+  F3(vp, i, j, k) = val;
+}
+
+template <typename T>
+__device__ inline T mom_w_val(const Layout &L, const StepArgs &s, const T *w, const T *wm, const T *p,
+                              const T *fz, int i, int j, int k) {
+  return (T)(1 - s.delta) * F3(w, i, j, k) + (T)s.delta * F3(wm, i, j, k) +
+         (T)s.leap * (T)s.dt * (-(F3(p, i, j, k + 1) - F3(p, i, j, k)) / (T)s.dz + F3(fz, i, j, k));
+}
+
+// vent value of wp at the bottom (k = nz, nz+1), :821-857
+template <typename T>
+__device__ inline T w_vent(const VentArgs &va, const double *alphae, const Layout &L, int i, int j) {
+  if (alphae[i + (L.nx + 2) * j] == 0) {
+    bool src = va.map_valid && i >= va.nxs && i <= va.nxe && j >= va.nys && j <= va.nye;
+    if (src) return (T)(-va.wup);
+    return (T)(va.wup * va.sumup / va.sumdown);
+  }
+  return (T)0.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s, Phys ph, VentArgs va,
+                                                    const T *__restrict__ w, const T *__restrict__ wm,
+                                                    const T *__restrict__ p, const T *__restrict__ fz,
+                                                    const double *__restrict__ alphae, T *__restrict__ wp) {
+  HC_BOX_IJK(b);  // box: i=0..nx+1, j=0..ny+1, k=-1..nz+1
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  bool ii = (i >= 1 && i <= nx), jj = (j >= 1 && j <= ny);
+  auto inner = [&](int i2, int j2, int k2) -> T {
+    // wp(i2,j2,k2) for i2,j2 interior after the interior update and bottom/top rows
+    if (k2 >= 1 && k2 <= nz - 1) return mom_w_val(L, s, w, wm, p, fz, i2, j2, k2);
+    if (k2 == nz) return w_vent<T>(va, alphae, L, i2, j2);
+    return (T)0.0;  // k2 = 0
+  };
+  T val;
+  if (ii && jj) {
+    if (k >= 1 && k <= nz - 1) val = mom_w_val(L, s, w, wm, p, fz, i, j, k);
+    else if (k == nz || k == nz + 1) val = w_vent<T>(va, alphae, L, i, j);
+    else val = (T)0.0;  // k = -1, 0
+  } else if (ii && (j == 0 || j == ny + 1) && k >= 0 && k <= nz) {
+    // y ends, :863-875 (0 * wp(i,1,k) / 0 * wp(i,ny,k))
+    val = (T)0 * inner(i, j == 0 ? 1 : ny, k);
+  } else if ((i == 0 || i == nx + 1) && j >= 0 && j <= ny && k >= 1 && k <= nz) {
+    // x ends, :877-884 (0 * wp(1,j,k) / 0 * wp(nx,j,k)); wp(1,0,k) is the y-end value
+    int i2 = i == 0 ? 1 : nx;
+    T src = (j >= 1) ? inner(i2, j, k) : (T)0 * inner(i2, 1, k);
+    val = (T)0 * src;
+  } else {
+    return;  // not written by MOMENTUM_W
+  }
+  F3(wp, i, j, k) = val;
+}
+
+// ---------------------------------------------------------------------------
+// VISCOSITY3D, smagorinsky_split_mixing-revised2014.f:5-150.  ss is rewritten
+// over its whole box (the reference zeroes it first, :24); ss_v over its
+// interior and z ghosts.  The complex root at :112 keeps its real part,
+// sqrt(max(0,x)).
+template <typename T>
+__device__ inline T ss_val(const Layout &L, const Phys &ph, const StepArgs &s, const T *um, const T *vm,
+                           const T *wm, int i, int j, int k) {
+  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  double ls = sqrt(s.dx * s.dy);
+  T coef = (T)(ph.ismag_h * ((ph.cs_h * ls) * (ph.cs_h * ls)));
+  T s11 = (F3(um, i, j, k) - F3(um, i - 1, j, k)) / dx;
+  T s22 = (F3(vm, i, j, k) - F3(vm, i, j - 1, k)) / dy;
+  T s33 = (F3(wm, i, j, k) - F3(wm, i, j, k - 1)) / dz;
+  T s12 = (T)0.25 *
+          ((F3(um, i, j + 1, k) + F3(um, i - 1, j + 1, k) - F3(um, i, j - 1, k) - F3(um, i - 1, j - 1, k)) / dy +
+           (F3(vm, i + 1, j, k) + F3(vm, i + 1, j - 1, k) - F3(vm, i - 1, j, k) - F3(vm, i - 1, j - 1, k)) / dx) /
+          (T)2.0;
+  T q = (T)2.0 * (s11 * s11) + (T)2.0 * (s22 * s22) + (T)4.0 * (s12 * s12) + (T)2.0 * (s33 * s33);
+  return coef * sqrt(q + (T)1.0e-30);
+}
+
+template <typename T>
+__device__ inline T ssv_val(const Layout &L, const Phys &ph, const StepArgs &s, const T *um, const T *vm,
+                            const T *wm, const T *rhop, const double *rho_bkg, int i, int j, int k) {
+  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  T coef = (T)(ph.ismag_v * ((ph.cs_v * s.dz) * (ph.cs_v * s.dz)));
+  T s13 = (T)0.25 *
+          ((F3(um, i, j, k - 1) + F3(um, i - 1, j, k - 1) - F3(um, i, j, k + 1) - F3(um, i - 1, j, k + 1)) / dz +
+           (F3(wm, i + 1, j, k) + F3(wm, i + 1, j, k - 1) - F3(wm, i - 1, j, k) - F3(wm, i - 1, j, k - 1)) / dx) /
+          (T)2.0;
+  T s23 = (T)0.25 *
+          ((F3(vm, i, j, k - 1) + F3(vm, i, j - 1, k - 1) - F3(vm, i, j, k + 1) - F3(vm, i, j - 1, k + 1)) / dz +
+           (F3(wm, i, j + 1, k) + F3(wm, i, j + 1, k - 1) - F3(wm, i, j - 1, k) - F3(wm, i, j - 1, k - 1)) / dy) /
+          (T)2.0;
+  T sv = (T)4.0 * (s13 * s13) + (T)4.0 * (s23 * s23);
+  // Richardson number in FP64: rho' + rho_bkg is ~1027 and its 1-m change is
+  // ~1e-4 of that
+  double rkm = (double)F3(rhop, i, j, k - 1) + rho_bkg[k - 1];
+  double rkp = (double)F3(rhop, i, j, k + 1) + rho_bkg[k + 1];
+  double ri = -(ph.g * (rkm - rkp) / 2.0 / s.dz / ph.rho0 / ((double)sv + 1.0e-30));
+  double arg = 1.0 + ph.eps - ri / 3.0;
+  return coef * sqrt(sv + (T)1.0e-30) * (T)sqrt(fmax(0.0, arg));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_viscosity(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ um,
+                                                   const T *__restrict__ vm, const T *__restrict__ wm,
+                                                   const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
+                                                   T *__restrict__ ss, T *__restrict__ ssv) {
+  HC_BOX_IJK(b);  // box: i=0..nx+1, j=0..ny+1, k=0..nz+1
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  bool gi = (i == 0 || i == nx + 1), gj = (j == 0 || j == ny + 1), gk = (k == 0 || k == nz + 1);
+  int ng = (int)gi + (int)gj + (int)gk;
+  // ss
+  T sval = (T)0;
+  if (ng == 0) {
+    sval = ss_val(L, ph, s, um, vm, wm, i, j, k);
+  } else if (ng == 1 && gi) {
+    const T bx = (T)ph.bcondx;
+    T a = ss_val(L, ph, s, um, vm, wm, i == 0 ? 1 : nx, j, k);
+    T c = ss_val(L, ph, s, um, vm, wm, i == 0 ? nx : 1, j, k);
+    sval = a * bx + (T)(1 - ph.bcondx) * c;
+  } else if (ng == 1 && gj) {
+    const T by = (T)ph.bcondy;
+    T a = ss_val(L, ph, s, um, vm, wm, i, j == 0 ? 1 : ny, k);
+    T c = ss_val(L, ph, s, um, vm, wm, i, j == 0 ? ny : 1, k);
+    sval = a * by + (T)(1 - ph.bcondy) * c;
+  }
+  F3(ss, i, j, k) = sval;
+  // ss_v
+  if (ng == 0) {
+    F3(ssv, i, j, k) = ssv_val(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k);
+  } else if (ng == 1 && gk) {
+    F3(ssv, i, j, k) = ssv_val(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k == 0 ? 1 : nz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DENSITY, potential_density80.f:3-71: interior and the six faces.
+template <typename T>
+__device__ inline double rho_rho(const Phys &ph, double s, double t) {
+  if (ph.eos == 1) return ph.eos_rho_ref * (1.0 - ph.eos_alpha * (t - ph.eos_t0) + ph.eos_beta * (s - ph.eos_s0));
+  double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t4 * t;
+  double r0 = 999.842594 + 6.793952e-2 * t - 9.095290e-3 * t * t + 1.001685e-4 * t3 - 1.120083e-6 * t4 +
+              6.536332e-9 * t5;
+  return r0 + s * (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t * t - 8.2467e-7 * t3 + 5.3875e-9 * t4) +
+         pow(s, 1.5) * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t * t) + 4.8314e-4 * s * s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_density(Layout L, Box b, Phys ph, const T *__restrict__ t,
+                                                 const T *__restrict__ sal, const double *__restrict__ rho_bkg,
+                                                 T *__restrict__ rhop) {
+  HC_BOX_IJK(b);  // box 0..nx+1, 0..ny+1, 0..nz+1
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  int ng = (int)(i == 0 || i == nx + 1) + (int)(j == 0 || j == ny + 1) + (int)(k == 0 || k == nz + 1);
+  if (ng > 1) return;
+  // equation of state evaluated in FP64 from the stored T and S
+  double r = rho_rho<T>(ph, (double)F3(sal, i, j, k), (double)F3(t, i, j, k)) - rho_bkg[k];
+  F3(rhop, i, j, k) = (T)r;
+}
+
+// ---------------------------------------------------------------------------
+// REFRESH, hotcross80-revised2014.f:1411-1493, as "um <- filtered u" over the
+// whole box; the caller then swaps the u and up buffers (u = up).
+template <typename T>
+__global__ void __launch_bounds__(256) k_refresh(size_t n, int leap, double assel, const T *__restrict__ u,
+                                                 const T *__restrict__ up, T *__restrict__ um) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  if (leap == 1) {
+    um[q] = up[q];
+  } else {
+    T a = u[q];
+    um[q] = a + (T)assel * (up[q] - (T)2 * a + um[q]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+template <typename T>
+void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                   const Aux &aux) {
+  Box bx{0, L.nx, 1, L.ny, 1, L.nz};
+  hipLaunchKernelGGL(k_forces_x<T>, box_grid(bx), box_block(), 0, st, L, bx, s, ph, f.u, f.v, f.w, f.um, f.vm,
+                     f.wm, f.ss, f.ssv, aux.azzmin, aux.alphau, f.fx);
+  Box by{1, L.nx, 0, L.ny, 1, L.nz};
+  hipLaunchKernelGGL(k_forces_y<T>, box_grid(by), box_block(), 0, st, L, by, s, ph, f.u, f.v, f.w, f.um, f.vm,
+                     f.wm, f.ss, f.ssv, aux.azzmin, aux.alphav, f.fy);
+  Box bz{1, L.nx, 1, L.ny, 0, L.nz};
+  hipLaunchKernelGGL(k_forces_z<T>, box_grid(bz), box_block(), 0, st, L, bz, s, ph, f.u, f.v, f.w, f.um, f.vm,
+                     f.wm, f.ss, f.ssv, f.rhop, aux.azzmin, aux.alphae0, aux.alphaw, f.fz);
+}
+
+template <typename T>
+void launch_momentum(hipStream_t st, int comp, const Layout &L, const StepArgs &s, const Phys &ph,
+                     const VentArgs &va, const FieldPtrs<T> &f, const Aux &aux) {
+  if (comp == 0) {
+    Box b{-1, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+    hipLaunchKernelGGL(k_momentum_u<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.um, f.p, f.fx,
+                       aux.pgradx, f.up);
+  } else if (comp == 1) {
+    Box b{0, L.nx + 1, -1, L.ny + 1, 0, L.nz + 1};
+    hipLaunchKernelGGL(k_momentum_v<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.v, f.vm, f.p, f.fy,
+                       aux.pgrady, f.vp);
+  } else {
+    Box b{0, L.nx + 1, 0, L.ny + 1, -1, L.nz + 1};
+    hipLaunchKernelGGL(k_momentum_w<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, va, f.w, f.wm, f.p, f.fz,
+                       aux.alphae, f.wp);
+  }
+}
+
+template <typename T>
+void launch_viscosity(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                      const Aux &aux) {
+  Box b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+  hipLaunchKernelGGL(k_viscosity<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.um, f.vm, f.wm, f.rhop,
+                     aux.rho_bkg, f.ss, f.ssv);
+}
+
+template <typename T>
+void launch_density(hipStream_t st, const Layout &L, const Phys &ph, const FieldPtrs<T> &f, const Aux &aux) {
+  Box b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+  hipLaunchKernelGGL(k_density<T>, box_grid(b), box_block(), 0, st, L, b, ph, f.t, f.s, aux.rho_bkg, f.rhop);
+}
+
+template <typename T>
+void launch_refresh(hipStream_t st, const Layout &L, int leap, double assel, const T *u, const T *up, T *um) {
+  size_t n = L.n;
+  hipLaunchKernelGGL(k_refresh<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, leap, assel, u, up, um);
+}
+
+#define HC_INST(T)                                                                                            \
+  template void launch_forces<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,                 \
+                                 const FieldPtrs<T> &, const Aux &);                                          \
+  template void launch_momentum<T>(hipStream_t, int, const Layout &, const StepArgs &, const Phys &,          \
+                                   const VentArgs &, const FieldPtrs<T> &, const Aux &);                      \
+  template void launch_viscosity<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,              \
+                                    const FieldPtrs<T> &, const Aux &);                                       \
+  template void launch_density<T>(hipStream_t, const Layout &, const Phys &, const FieldPtrs<T> &,            \
+                                  const Aux &);                                                               \
+  template void launch_refresh<T>(hipStream_t, const Layout &, int, double, const T *, const T *, T *);
+HC_INST(double)
+HC_INST(float)
+
+}  // namespace hc

@@ -1,0 +1,361 @@
+"""Python host harness over the HotCross C-ABI (libhotcross.so, include/hotcross.h).
+
+This mirrors the reference's Fortran interface for the timestep: one method per
+reference subroutine (hotcross80-revised2014.f:264-350), module arrays as
+Fortran-ordered numpy arrays with the reference's lower bounds, and errors
+raised where the reference would `stop`.  The HIP library must be built
+(plumemodel-hotcross_amd/build.py); there is no fallback path.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhotcross.so")
+
+FIELDS = ("u", "v", "w", "up", "vp", "wp", "um", "vm", "wm", "p", "t", "s", "c", "ss",
+          "ss_v", "rhop", "fx", "fy", "fz")
+FIELD_ID = {n: q for q, n in enumerate(FIELDS)}
+FP64, FP32 = 0, 1
+
+
+class HcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class HcParams(ctypes.Structure):
+    """hc_params of include/hotcross.h."""
+    _fields_ = (
+        [(n, ctypes.c_int32) for n in ("nx", "ny", "nz", "nt")]
+        + [(n, ctypes.c_double) for n in ("dx", "dy", "dz", "dt", "zstart", "zend")]
+        + [(n, ctypes.c_int32) for n in (
+            "bit2", "bit3", "bit4", "bit5", "bit7", "bit8", "bit9", "bit10",
+            "bit2k", "bit5k", "ismol", "ismag_h", "ismag_v", "bcondx", "bcondy",
+            "bottom_slip", "side_slip", "open_box")]
+        + [("bdbit", ctypes.c_int32 * 4)]
+        + [(n, ctypes.c_int32) for n in ("modulo_x", "modulo_y", "toggle_t",
+                                         "toggle_s", "toggle_c", "pad0")]
+        + [(n, ctypes.c_double) for n in (
+            "cs_h", "cs_v", "prandtl", "axmin", "aymin", "azmin", "azmax",
+            "mixing_height_scale", "sponge_factor", "lat", "g", "rho0", "assel",
+            "eps", "omega", "source_length", "source_width", "dqtdt", "dqsdt",
+            "dqcdt", "tsource", "ssource", "wsource", "source_ramp_hrs",
+            "noise_amp", "wramp_hrs", "flow_u0", "flow_u1", "flow_v1",
+            "flow_omega", "flow_ramp_hrs")]
+        + [(n, ctypes.c_int32) for n in ("eos", "pad1")]
+        + [(n, ctypes.c_double) for n in ("eos_rho_ref", "eos_alpha", "eos_beta",
+                                          "eos_t0", "eos_s0")]
+    )
+
+
+def field_bounds(name, nx, ny, nz):
+    lo = [0, 0, 0]
+    hi = [nx + 1, ny + 1, nz + 1]
+    if name in ("u", "up", "um", "fx"):
+        lo[0] = -1
+    elif name in ("v", "vp", "vm", "fy"):
+        lo[1] = -1
+    elif name in ("w", "wp", "wm", "fz"):
+        lo[2] = -1
+    return tuple(lo), tuple(hi)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libhotcross.so is not built: run plumemodel-hotcross_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, dbl, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
+        sig = {
+            "hc_create": [ctypes.POINTER(HcParams), ctypes.c_int, ctypes.POINTER(vp)],
+            "hc_destroy": [vp], "hc_synchronize": [vp], "hc_stream": [vp],
+            "hc_upload": [vp, ctypes.c_int, dp, dp, dp],
+            "hc_download": [vp, ctypes.c_int, dp, dp, dp],
+            "hc_set_profiles": [vp, dp, dp, dp, dp],
+            "hc_set_sponge": [vp, dp, dp, dp, dp],
+            "hc_init_case": [vp, dp, dp, dp, i32, dbl, ctypes.c_uint64],
+            "hc_begin_step": [vp, i32, dbl],
+            "hc_body_force": [vp], "hc_forces": [vp], "hc_pressure": [vp],
+            "hc_momentum_u": [vp], "hc_momentum_v": [vp], "hc_momentum_w": [vp],
+            "hc_viscosity3d": [vp], "hc_source_terms": [vp, dp], "hc_mixing": [vp],
+            "hc_work_velocity": [vp, dbl],
+            "hc_tracer31": [vp, ctypes.c_int, dp, dbl, dbl, dbl, dp],
+            "hc_vent_ghost": [vp, ctypes.c_int, dbl],
+            "hc_density": [vp], "hc_refresh": [vp], "hc_cfl_values": [vp, dp],
+            "hc_step": [vp, i32], "hc_source_noise": [vp, i32, dp],
+            "hc_phase_timing": [vp, ctypes.c_int],
+            "hc_phase_times": [vp, ctypes.POINTER(ctypes.c_char_p), dp, dp, i32],
+        }
+        for n, a in sig.items():
+            f = getattr(L, n)
+            f.argtypes = a
+            f.restype = ctypes.c_int
+        L.hc_stream.restype = ctypes.c_void_p
+        L.hc_last_error.argtypes = [vp]
+        L.hc_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _arr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One device state of the timestep (a context of the C-ABI)."""
+
+    def __init__(self, prm, precision=FP64):
+        self.L = lib()
+        self.prm = prm
+        self.nx, self.ny, self.nz = prm.nx, prm.ny, prm.nz
+        self.h = ctypes.c_void_p()
+        r = self.L.hc_create(ctypes.byref(prm), precision, ctypes.byref(self.h))
+        self._check(r)
+        self.time = -prm.dt
+        self.m = 0
+
+    def _check(self, r):
+        if r != 0:
+            msg = self.L.hc_last_error(self.h).decode() if self.h else "hc_create failed"
+            raise HcError(r, msg)
+        return r
+
+    def close(self):
+        if self.h:
+            self.L.hc_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state transfer ----------------------------------------------------
+    def bounds(self, name):
+        return field_bounds(name, self.nx, self.ny, self.nz)
+
+    def upload(self, name, arr):
+        lo, hi = self.bounds(name)
+        shape = tuple(h - l + 1 for l, h in zip(lo, hi))
+        a = np.asfortranarray(np.asarray(arr, dtype=np.float64).reshape(shape, order="F"))
+        lo_a = np.array(lo, dtype=np.int32)
+        hi_a = np.array(hi, dtype=np.int32)
+        self._check(self.L.hc_upload(self.h, FIELD_ID[name], _arr(a), _arr(lo_a), _arr(hi_a)))
+
+    def download(self, name):
+        lo, hi = self.bounds(name)
+        shape = tuple(h - l + 1 for l, h in zip(lo, hi))
+        a = np.zeros(shape, dtype=np.float64, order="F")
+        lo_a = np.array(lo, dtype=np.int32)
+        hi_a = np.array(hi, dtype=np.int32)
+        self._check(self.L.hc_download(self.h, FIELD_ID[name], _arr(a), _arr(lo_a), _arr(hi_a)))
+        return a
+
+    def set_profiles(self, t_bkg, s_bkg, rho_bkg, azzmin):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (t_bkg, s_bkg, rho_bkg, azzmin)]
+        self._t_bkg, self._s_bkg = arrs[0].copy(), arrs[1].copy()
+        self._check(self.L.hc_set_profiles(self.h, *[_arr(x) for x in arrs]))
+
+    def set_sponge(self, alphau, alphav, alphae, alphaw):
+        arrs = [np.asfortranarray(x, dtype=np.float64) for x in (alphau, alphav, alphae, alphaw)]
+        self._check(self.L.hc_set_sponge(self.h, *[_arr(x) for x in arrs]))
+
+    def init_case(self, z, t, s, t_noise=0.0, seed=1):
+        z, t, s = (np.ascontiguousarray(x, dtype=np.float64) for x in (z, t, s))
+        self._check(self.L.hc_init_case(self.h, _arr(z), _arr(t), _arr(s), len(z), t_noise, seed))
+
+    # ---- the reference's subroutines --------------------------------------
+    def begin_step(self, m, time):
+        self._check(self.L.hc_begin_step(self.h, m, time))
+
+    def body_force(self):
+        self._check(self.L.hc_body_force(self.h))
+
+    def forces(self):
+        self._check(self.L.hc_forces(self.h))
+
+    def pressure(self):
+        self._check(self.L.hc_pressure(self.h))
+
+    def momentum_u(self):
+        self._check(self.L.hc_momentum_u(self.h))
+
+    def momentum_v(self):
+        self._check(self.L.hc_momentum_v(self.h))
+
+    def momentum_w(self):
+        self._check(self.L.hc_momentum_w(self.h))
+
+    def viscosity3d(self):
+        self._check(self.L.hc_viscosity3d(self.h))
+
+    def source_terms(self, noise=None):
+        if noise is None:
+            self._check(self.L.hc_source_terms(self.h, None))
+        else:
+            n = np.ascontiguousarray(noise, dtype=np.float64)
+            self._check(self.L.hc_source_terms(self.h, _arr(n)))
+
+    def mixing(self):
+        self._check(self.L.hc_mixing(self.h))
+
+    def work_velocity(self, tau3):
+        self._check(self.L.hc_work_velocity(self.h, tau3))
+
+    def tracer31(self, name, c_bkg, cmin, cmax, tau3):
+        b = np.ascontiguousarray(c_bkg, dtype=np.float64)
+        out = np.zeros(3)
+        self._check(self.L.hc_tracer31(self.h, FIELD_ID[name], _arr(b), cmin, cmax, tau3, _arr(out)))
+        return out
+
+    def vent_ghost(self, name, value):
+        self._check(self.L.hc_vent_ghost(self.h, FIELD_ID[name], value))
+
+    def density(self):
+        self._check(self.L.hc_density(self.h))
+
+    def refresh(self):
+        self._check(self.L.hc_refresh(self.h))
+
+    def cfl_values(self):
+        out = np.zeros(6)
+        r = self.L.hc_cfl_values(self.h, _arr(out))
+        self._check(r)
+        return out
+
+    def driver_step(self, m, noise=None):
+        """The time loop body (hotcross80-revised2014.f:264-350) through the
+        per-subroutine entries, in the reference's order."""
+        p = self.prm
+        self.time = self.time + p.dt
+        self.begin_step(m, self.time)
+        self.body_force()
+        self.forces()
+        self.pressure()
+        self.momentum_u()
+        self.momentum_v()
+        self.momentum_w()
+        self.viscosity3d()
+        self.source_terms(noise)
+        self.mixing()
+        if p.toggle_t == 1:
+            self.work_velocity(p.dt)
+            self.tracer31("t", self._t_bkg, self._t_bkg.min(), 1.0e10, p.dt)
+            self.vent_ghost("t", p.tsource + self._t_bkg[p.nz])
+        if p.toggle_s == 1:
+            self.tracer31("s", self._s_bkg, 0.0, self._s_bkg.max(), p.dt)
+            self.vent_ghost("s", p.ssource + self._s_bkg[p.nz])
+        if p.bit3 == 1:
+            self.density()
+        if p.toggle_c == 1:
+            self.tracer31("c", np.zeros(p.nz + 2), 0.0, 1.0e10, p.dt)
+        self.refresh()
+        if m % 10 == 0:
+            return self.cfl_values()
+        return None
+
+    def step(self, n):
+        """n steps of the native loop (hc_step)."""
+        self._check(self.L.hc_step(self.h, n))
+
+    def synchronize(self):
+        self._check(self.L.hc_synchronize(self.h))
+
+    def stream(self):
+        return self.L.hc_stream(self.h)
+
+    def source_noise(self, m):
+        out = np.zeros(3 * 4096)
+        n = self.L.hc_source_noise(self.h, m, _arr(out))
+        return out[: 3 * n].copy()
+
+    def phase_timing(self, enable=True):
+        self._check(self.L.hc_phase_timing(self.h, 1 if enable else 0))
+
+    def phase_times(self):
+        cap = 32
+        names = (ctypes.c_char_p * cap)()
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int32)
+        n = self.L.hc_phase_times(self.h, names, _arr(ms), _arr(cnt), cap)
+        return {names[q].decode(): (float(ms[q]), int(cnt[q])) for q in range(min(n, cap))}
+
+
+# --------------------------------------------------------------------------
+# workloads of the headline measurement
+PI = 4.0 * math.atan2(1.0, 1.0)
+
+
+def workload_params(name="les256"):
+    """Case parameters of the named workload.
+
+    les64:   64^3, dx=dy=dz=1 m, dt=0.25 s, uniform 0.05 m/s crossflow.
+    les256:  256^3, dx=dy=dz=0.5 m, dt=0.2 s, tidal crossflow
+             U(t)=0.03+0.05 sin(2 pi t/44712 s), 50 MW over 4 x 4.5 m with
+             +-5% per-cell heat-flux noise, passive tracer.
+    les384:  384 x 384 x 320, dx=0.5 m, same physics as les256.
+    Shared physics: linear T(z) with N = 1e-3 1/s, S = 34.6, linear equation
+    of state, Smagorinsky Cs = 0.2, closed-box y ends with sponge.
+    """
+    grids = {"les64": (64, 64, 64, 1.0, 0.25), "les256": (256, 256, 256, 0.5, 0.2),
+             "les384": (384, 384, 320, 0.5, 0.2), "les128": (128, 128, 128, 1.0, 0.25)}
+    nx, ny, nz, d, dt = grids[name]
+    p = HcParams()
+    p.nx, p.ny, p.nz, p.nt = nx, ny, nz, 3600
+    p.dx = p.dy = p.dz = d
+    p.dt = dt
+    p.zstart = 1750.0
+    p.zend = p.zstart + nz * d
+    for n in ("bit2", "bit3", "bit4", "bit5", "bit10", "bit2k", "bit5k", "ismol", "ismag_h",
+              "ismag_v", "bcondx", "bcondy", "bottom_slip", "side_slip", "open_box",
+              "toggle_t", "toggle_s", "toggle_c"):
+        setattr(p, n, 1)
+    p.bit7, p.bit8, p.bit9 = 0, 0, 0
+    p.bdbit[:] = [1, 1, 1, 1]
+    p.cs_h = p.cs_v = 0.2
+    p.prandtl = 1.0
+    p.axmin = p.aymin = 1.0e-4
+    p.azmin, p.azmax, p.mixing_height_scale = 1.0e-5, 1.0e-4, 5.0
+    p.sponge_factor = 0.05
+    p.lat = 47.95
+    p.g, p.rho0, p.assel, p.eps = 9.81, 1027.65, 0.01, 1.0e-10
+    p.omega = 2.0 * PI / 24.0 / 3600.0
+    p.source_length, p.source_width = 4.0, 4.5
+    # 50 MW: dQtdt per source cell in K m^3/s (q/(rho0 Cp)/ncells)
+    nnx = int(math.floor(4.0 / d + 0.5))
+    nny = int(math.floor(4.5 / d + 0.5))
+    p.dqtdt = 50.0e6 / (1027.65 * 4200.0) / (nnx * nny)
+    p.dqsdt, p.dqcdt = 0.0, 1.0
+    p.tsource, p.ssource, p.wsource = 0.0, 0.0, 0.0
+    p.source_ramp_hrs, p.noise_amp, p.wramp_hrs = 1.0e-9, 0.05, 6.0
+    if name == "les64":
+        p.flow_u0, p.flow_u1, p.flow_v1 = 0.05, 0.0, 0.0
+        p.noise_amp = 0.0
+    else:
+        p.flow_u0, p.flow_u1, p.flow_v1 = 0.03, 0.05, 0.0
+    p.flow_omega = 2.0 * PI / 44712.0
+    p.flow_ramp_hrs = 0.0
+    p.eos = 1
+    N2 = 1.0e-6
+    p.eos_rho_ref, p.eos_alpha, p.eos_beta, p.eos_t0, p.eos_s0 = 1027.65, 2.0e-4, 7.6e-4, 2.0, 34.6
+    return p
+
+
+def linear_stratification(p, n2=1.0e-6, t0=2.0, s0=34.6):
+    """(z, T, S) table for INITIALIZE_PROFILES: linear T(z) with buoyancy
+    frequency sqrt(n2) under the linear equation of state, constant S; z is
+    depth, increasing downward as in the reference's tables."""
+    dtdz = n2 / (p.g * p.eos_alpha)  # K per m, warmer upward
+    z = np.arange(p.zstart - 4.0, p.zend + 4.0 + 1e-9, 1.0)
+    t = t0 + dtdz * (p.zend - z)
+    s = np.full_like(z, s0)
+    return z, t, s

@@ -1,0 +1,102 @@
+"""Parity of the HIP library (through the C-ABI) with the reference's golden
+runs and with the oracle.  FP64 plain form: the reference's arithmetic
+evaluated in the same order, so the tolerance is the reference-vs-oracle one
+(1e-9 relative after a whole run, north_star's bar for 50 steps)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import hc_testlib as T
+import gpu_common as G
+from test_oracle_golden import load_case
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = sorted(glob.glob(os.path.join(T.ROOT, "tests", "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_gpu_replays_reference_run(path):
+    prm, before, after, noise, n = load_case(path)
+    print("case", os.path.basename(path), "steps", n, flush=True)
+    ctx = G.context_from_state(prm, before)
+    for m in range(n):
+        ctx.driver_step(m, noise[m])
+    ctx.synchronize()
+    for k in T.STATE_FIELDS:
+        a = ctx.download(k)
+        b = after[k]
+        if not np.isfinite(b).any():
+            continue  # a run the reference itself does not carry
+        err = G.compare(a, b)
+        assert err <= 1e-9, (k, err)
+
+
+def _random_state(prm, seed):
+    z = np.load(GOLDEN[0])
+    return {k[7:]: z[k] for k in z.files if k.startswith("before_")}
+
+
+SEQS = {
+    "forces": (["body_force", "forces"], ["fx", "fy", "fz"]),
+    "pressure": (["body_force", "forces", "pressure"], ["p"]),
+    "momentum": (["body_force", "forces", "pressure", "momentum_u", "momentum_v", "momentum_w"],
+                 ["up", "vp", "wp"]),
+    "viscosity": (["viscosity3d"], ["ss", "ss_v"]),
+    "density": (["density"], ["rhop"]),
+    "refresh": (["refresh"], ["u", "um", "v", "vm", "w", "wm"]),
+}
+
+
+@pytest.mark.parametrize("name", list(SEQS))
+@pytest.mark.parametrize("m", [0, 1])
+def test_gpu_routine_matches_oracle(name, m):
+    calls, outs = SEQS[name]
+    path = [p for p in GOLDEN if "16x12x10_50" in p][0]
+    prm, before, _, noise, _ = load_case(path)
+    rng = np.random.default_rng(5 + m)
+    state = dict(before)
+    for f in ("u", "v", "w", "um", "vm", "wm", "up", "vp", "wp"):
+        a = state[f].copy()
+        core = a != -1.0e34
+        a[core] += 0.01 * (rng.random(core.sum()) - 0.5)
+        state[f] = a
+    for f in ("ss", "ss_v"):
+        state[f] = 1e-4 * (1.0 + rng.random(state[f].shape))
+    print("routine", name, "step", m, flush=True)
+    o = T.Oracle(prm)
+    for k, v in state.items():
+        o.set(k, v)
+    ctx = G.context_from_state(prm, state)
+    time = -prm.dt
+    for mm in range(m + 1):
+        time = time + prm.dt
+        o.begin_step(mm)
+        ctx.begin_step(mm, time)
+        seq = calls if mm == m else ["body_force"]
+        for c in seq:
+            o.call(c)
+            getattr(ctx, c)()
+    for f in outs:
+        err = G.compare(ctx.download(f), o.get(f))
+        assert err <= 1e-13, (name, f, err)
+
+
+@pytest.mark.parametrize("m", [0, 1, 2])
+def test_gpu_tracers_match_oracle(m):
+    path = [p for p in GOLDEN if "16x12x10_50" in p][0]
+    prm, before, _, noise, _ = load_case(path)
+    print("tracers step", m, flush=True)
+    o = T.Oracle(prm)
+    for k, v in before.items():
+        o.set(k, v)
+    o.set_time(-prm.dt, 0)
+    ctx = G.context_from_state(prm, before)
+    for mm in range(m + 1):
+        o.step(mm, noise[mm])
+        ctx.driver_step(mm, noise[mm])
+    for f in ("t", "s", "c", "rhop", "ss", "ss_v", "u", "v", "w", "p"):
+        err = G.compare(ctx.download(f), o.get(f))
+        assert err <= 1e-12, (f, err)
