@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -261,6 +262,10 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
     return r;
   }
   ctx->esz = precision == HC_FP64 ? 8 : 4;
+  {
+    const char *e = getenv("HC_XCD_REMAP");
+    box_remap_flag() = (e && e[0] == '0') ? 0 : 1;
+  }
   ctx->L = make_layout(P.nx, P.ny, P.nz);
   *out = ctx;
   HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
@@ -275,7 +280,6 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
                   &ctx->tw.ud2, &ctx->tw.vd2, &ctx->tw.wd2, &ctx->tw.c1, &ctx->tw.c2};
   for (auto p : tws)
     if (!(*p = dalloc(ctx, fb, &rc))) return rc;
-  if (!(ctx->tw.flag = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
   if (!(ctx->tw.budget_part = (double *)dalloc(ctx, (size_t)3 * P.ny * P.nz * 8, &rc))) return rc;
   if (!(ctx->tw.budget = (double *)dalloc(ctx, 64, &rc))) return rc;
   // case arrays
@@ -364,7 +368,7 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
   if (ctx->st) hipStreamSynchronize(ctx->st);
   for (int q = 0; q < HC_NFIELDS; ++q) hipFree(ctx->f[q]);
   void *ps[] = {ctx->tw.uc, ctx->tw.vc, ctx->tw.wc, ctx->tw.ud, ctx->tw.vd, ctx->tw.wd, ctx->tw.ud2,
-                ctx->tw.vd2, ctx->tw.wd2, ctx->tw.c1, ctx->tw.c2, ctx->tw.flag, ctx->tw.budget_part,
+                ctx->tw.vd2, ctx->tw.wd2, ctx->tw.c1, ctx->tw.c2, ctx->tw.budget_part,
                 ctx->tw.budget, ctx->aux.azzmin, ctx->aux.rho_bkg, ctx->aux.pgradx, ctx->aux.pgrady,
                 ctx->aux.alphaw, ctx->aux.alphau, ctx->aux.alphav, ctx->aux.alphae, ctx->aux.alphae0,
                 ctx->d_tbkg, ctx->d_sbkg, ctx->d_zero, ctx->d_cbkg, ctx->d_q[0], ctx->d_q[1], ctx->d_q[2],

@@ -46,10 +46,35 @@ struct StepArgs {
   double acc[8];  // persistent azu/azv boundary accumulators
 };
 
+// XCD-aware block order on/off for the box launches (on unless the
+// environment sets HC_XCD_REMAP=0 when a context is created)
+inline int &box_remap_flag() {
+  static int flag = 1;
+  return flag;
+}
+
 // Box-wise index for 3-D launches: block (64,4,1) over (i, j), grid.z = k.
 struct Box {
-  int i0, i1, j0, j1, k0, k1;  // inclusive Fortran ranges
+  int i0, i1, j0, j1, k0, k1;            // inclusive Fortran ranges
+  int remap = box_remap_flag();          // XCD-aware block order (box_block_coords)
 };
+
+// XCD-aware block order.  The dispatcher deals workgroups to the 8 XCDs
+// round-robin by linear id; remapping id -> logical block so that XCD x gets
+// the x-th contiguous slice of the (i,j,k) block space keeps the j/k halo
+// rows a block re-reads inside its own XCD's L2.  The map is a bijection on
+// [0, nblocks), so every cell is still covered exactly once.
+__device__ inline void box_block_coords(int remap, unsigned &bx, unsigned &by, unsigned &bz) {
+  bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z;
+  if (!remap) return;
+  const unsigned gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+  const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned xcd = id % 8u, slot = id / 8u, per = n / 8u, rem = n % 8u;
+  const unsigned lid = (xcd < rem) ? xcd * (per + 1u) + slot : rem * (per + 1u) + (xcd - rem) * per + slot;
+  bx = lid % gx;
+  by = (lid / gx) % gy;
+  bz = lid / (gx * gy);
+}
 
 inline dim3 box_grid(const Box &b) {
   int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
@@ -57,10 +82,12 @@ inline dim3 box_grid(const Box &b) {
 }
 inline dim3 box_block() { return dim3(64, 4, 1); }
 
-#define HC_BOX_IJK(b)                                  \
-  const int i = (b).i0 + (int)(blockIdx.x * 64 + threadIdx.x); \
-  const int j = (b).j0 + (int)(blockIdx.y * 4 + threadIdx.y);  \
-  const int k = (b).k0 + (int)blockIdx.z;              \
+#define HC_BOX_IJK(b)                                                 \
+  unsigned bx_, by_, bz_;                                             \
+  box_block_coords((b).remap, bx_, by_, bz_);                         \
+  const int i = (b).i0 + (int)(bx_ * 64 + threadIdx.x);               \
+  const int j = (b).j0 + (int)(by_ * 4 + threadIdx.y);                \
+  const int k = (b).k0 + (int)bz_;                                    \
   if (i > (b).i1 || j > (b).j1) return;
 
 constexpr double MISSING = -1.0e+34;

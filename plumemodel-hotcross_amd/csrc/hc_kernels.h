@@ -49,7 +49,6 @@ struct TracerWork {
   void *ud, *vd, *wd;      // antidiffusive velocities of the current pass
   void *ud2, *vd2, *wd2;   // ... of the previous pass
   void *c1, *c2;           // pass results
-  uint8_t *flag;           // limiter flags per cell
   double *budget_part;     // per-block partial sums (FP64)
   double *budget;          // 3 sums
 };

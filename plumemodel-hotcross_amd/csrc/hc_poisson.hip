@@ -361,44 +361,62 @@ __global__ void __launch_bounds__(256) k_dct_y(Layout L, FftPlan pl, int npair, 
 // direction; the (0,0) column in FP64 with the k-mean removed
 template <typename T>
 __global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const double *__restrict__ piv,
-                                                 T *__restrict__ buf) {
+                                                 T *__restrict__ buf, double *__restrict__ y00) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= plane) return;
   const double a = 1.0 / (dz * dz);
   if (q == 0) {
+    // the singular (0,0) column: FP64 throughout (y00 holds the sweep), last
+    // pivot 1 (set with the table), then the k-mean removed
     double y = (double)buf[0] * piv[0];
-    buf[0] = (T)y;
+    y00[0] = y;
     for (int k = 1; k < nz; ++k) {
       y = ((double)buf[k * plane] - a * y) * piv[k * plane];
-      buf[k * plane] = (T)y;
+      y00[k] = y;
     }
-    // back substitution, keeping FP64 values for the mean
-    double xk = y, sum = xk;
-    double *tmp = nullptr;
-    (void)tmp;
+    double x = y, sum = x;
     for (int k = nz - 2; k >= 0; --k) {
-      double yk = (double)buf[k * plane];
-      xk = yk - a * piv[k * plane] * xk;
-      buf[k * plane] = (T)xk;
-      sum += xk;
+      x = y00[k] - a * piv[k * plane] * x;
+      y00[k] = x;
+      sum += x;
     }
-    // second pass for exact FP64 mean removal (values re-read as stored)
+    y00[nz - 1] = y;
     double mean = sum / nz;
-    for (int k = 0; k < nz; ++k) buf[k * plane] = (T)((double)buf[k * plane] - mean);
+    for (int k = 0; k < nz; ++k) buf[k * plane] = (T)(y00[k] - mean);
     return;
   }
-  T y = buf[q] * (T)piv[q];
-  buf[q] = y;
-  for (int k = 1; k < nz; ++k) {
-    y = (buf[k * plane + q] - (T)a * y) * (T)piv[k * plane + q];
-    buf[k * plane + q] = y;
+  // the sweeps are sequential in k; loads are issued in batches of KB levels
+  // ahead of the recurrence so their latency overlaps
+  constexpr int KB = 8;
+  T y = (T)0;
+  for (int k0 = 0; k0 < nz; k0 += KB) {
+    T g[KB], d[KB];
+    const int kn = min(KB, nz - k0);
+#pragma unroll
+    for (int t = 0; t < KB; ++t)
+      if (t < kn) { g[t] = buf[(size_t)(k0 + t) * plane + q]; d[t] = (T)piv[(size_t)(k0 + t) * plane + q]; }
+#pragma unroll
+    for (int t = 0; t < KB; ++t)
+      if (t < kn) {
+        y = (k0 + t == 0) ? g[t] * d[t] : (g[t] - (T)a * y) * d[t];
+        buf[(size_t)(k0 + t) * plane + q] = y;
+      }
   }
   T x = y;
-  for (int k = nz - 2; k >= 0; --k) {
-    x = buf[k * plane + q] - (T)a * (T)piv[k * plane + q] * x;
-    buf[k * plane + q] = x;
+  for (int k1 = nz - 2; k1 >= 0; k1 -= KB) {
+    T g[KB], d[KB];
+    const int kn = min(KB, k1 + 1);
+#pragma unroll
+    for (int t = 0; t < KB; ++t)
+      if (t < kn) { g[t] = buf[(size_t)(k1 - t) * plane + q]; d[t] = (T)piv[(size_t)(k1 - t) * plane + q]; }
+#pragma unroll
+    for (int t = 0; t < KB; ++t)
+      if (t < kn) {
+        x = g[t] - (T)a * d[t] * x;
+        buf[(size_t)(k1 - t) * plane + q] = x;
+      }
   }
 }
 
@@ -445,7 +463,7 @@ void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const P
   int gy = nz * ((nx + 2 * npy - 1) / (2 * npy));
   hipLaunchKernelGGL((k_dct_y<T, 0>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
   hipLaunchKernelGGL(k_tridiag<T>, dim3((unsigned)(((size_t)nx * ny + 255) / 256)), dim3(256), 0, st, L, s.dz,
-                     pw.piv, r);
+                     pw.piv, r, pw.part);
   hipLaunchKernelGGL((k_dct_y<T, 1>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
   hipLaunchKernelGGL((k_dct_x<T, 1>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, r, (T *)nullptr,
                      pw.delta, s, ph, f.p, f.fx, f.w, f.wm, f.fz, aux.pgradx);

@@ -45,13 +45,12 @@ __device__ inline T azv_at(const Layout &L, const T *ssv, const double *azzmin,
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_forces_x(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+__device__ inline void forces_x_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
                                                   const T *__restrict__ v, const T *__restrict__ w,
                                                   const T *__restrict__ um, const T *__restrict__ vm,
                                                   const T *__restrict__ wm, const T *__restrict__ ss,
                                                   const T *__restrict__ ssv, const double *__restrict__ azzmin,
                                                   const double *__restrict__ alphau, T *__restrict__ fx) {
-  HC_BOX_IJK(b);
   const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
   const T axmin = (T)ph.axmin, aymin = (T)ph.aymin;
@@ -83,13 +82,12 @@ __global__ void __launch_bounds__(256) k_forces_x(Layout L, Box b, StepArgs s, P
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_forces_y(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+__device__ inline void forces_y_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
                                                   const T *__restrict__ v, const T *__restrict__ w,
                                                   const T *__restrict__ um, const T *__restrict__ vm,
                                                   const T *__restrict__ wm, const T *__restrict__ ss,
                                                   const T *__restrict__ ssv, const double *__restrict__ azzmin,
                                                   const double *__restrict__ alphav, T *__restrict__ fy) {
-  HC_BOX_IJK(b);
   const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
   const T axmin = (T)ph.axmin, aymin = (T)ph.aymin;
@@ -120,14 +118,13 @@ __global__ void __launch_bounds__(256) k_forces_y(Layout L, Box b, StepArgs s, P
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_forces_z(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+__device__ inline void forces_z_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
                                                   const T *__restrict__ v, const T *__restrict__ w,
                                                   const T *__restrict__ um, const T *__restrict__ vm,
                                                   const T *__restrict__ wm, const T *__restrict__ ss,
                                                   const T *__restrict__ ssv, const T *__restrict__ rhop,
                                                   const double *__restrict__ azzmin, const double *__restrict__ alphae,
                                                   const double *__restrict__ alphaw, T *__restrict__ fz) {
-  HC_BOX_IJK(b);
   const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
   const T axmin = (T)ph.axmin;
@@ -161,6 +158,27 @@ __global__ void __launch_bounds__(256) k_forces_z(Layout L, Box b, StepArgs s, P
         coe_9 * (F3(u, i, j, k + 1) + F3(u, i - 1, j, k + 1) + F3(u, i, j, k) + F3(u, i - 1, j, k)) +
         coe_10 * (T)0.0;
   F3(fz, i, j, k) = r;
+}
+
+// one pass over the union of the three output boxes: the inputs shared by
+// fx, fy and fz are read from HBM once per block
+template <typename T>
+__global__ void __launch_bounds__(256) k_forces(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
+                                                const T *__restrict__ v, const T *__restrict__ w,
+                                                const T *__restrict__ um, const T *__restrict__ vm,
+                                                const T *__restrict__ wm, const T *__restrict__ ss,
+                                                const T *__restrict__ ssv, const T *__restrict__ rhop,
+                                                const double *__restrict__ azzmin, const double *__restrict__ alphau,
+                                                const double *__restrict__ alphav, const double *__restrict__ alphae0,
+                                                const double *__restrict__ alphaw, T *__restrict__ fx,
+                                                T *__restrict__ fy, T *__restrict__ fz) {
+  HC_BOX_IJK(b);  // i = 0..nx, j = 0..ny, k = 0..nz
+  if (j >= 1 && k >= 1)
+    forces_x_at<T>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphau, fx);
+  if (i >= 1 && k >= 1)
+    forces_y_at<T>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphav, fy);
+  if (i >= 1 && j >= 1)
+    forces_z_at<T>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, rhop, azzmin, alphae0, alphaw, fz);
 }
 
 // ---------------------------------------------------------------------------
@@ -403,15 +421,10 @@ __global__ void __launch_bounds__(256) k_refresh(size_t n, int leap, double asse
 template <typename T>
 void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                    const Aux &aux) {
-  Box bx{0, L.nx, 1, L.ny, 1, L.nz};
-  hipLaunchKernelGGL(k_forces_x<T>, box_grid(bx), box_block(), 0, st, L, bx, s, ph, f.u, f.v, f.w, f.um, f.vm,
-                     f.wm, f.ss, f.ssv, aux.azzmin, aux.alphau, f.fx);
-  Box by{1, L.nx, 0, L.ny, 1, L.nz};
-  hipLaunchKernelGGL(k_forces_y<T>, box_grid(by), box_block(), 0, st, L, by, s, ph, f.u, f.v, f.w, f.um, f.vm,
-                     f.wm, f.ss, f.ssv, aux.azzmin, aux.alphav, f.fy);
-  Box bz{1, L.nx, 1, L.ny, 0, L.nz};
-  hipLaunchKernelGGL(k_forces_z<T>, box_grid(bz), box_block(), 0, st, L, bz, s, ph, f.u, f.v, f.w, f.um, f.vm,
-                     f.wm, f.ss, f.ssv, f.rhop, aux.azzmin, aux.alphae0, aux.alphaw, f.fz);
+  Box b{0, L.nx, 0, L.ny, 0, L.nz};
+  hipLaunchKernelGGL(k_forces<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm,
+                     f.ss, f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy,
+                     f.fz);
 }
 
 template <typename T>

@@ -2,17 +2,18 @@
 //
 // mpdata-hotcross80.f:16-494.  One tracer call is
 //   pass 1: explicit diffusion + donor-cell upwind step      (k_pass1)
-//   pass 2/3: antidiffusive velocities (k_antidiff), limiter flags
-//            (k_flags), limited donor-cell step (k_limited)
+//   pass 2/3: antidiffusive velocities (k_antidiff), then the limiter flags
+//            and the limited donor-cell step (k_limited)
 //   sponge + source on the last pass, then the boundary images (k_bc).
 // The diffusivities kxt/kyt/kzt of MIXING (mixing80-revised2014.f:22-59) are
 // formed inside k_pass1 from ss/ss_v with the reference's expressions.
 //
 // The limiter (mpdata-hotcross80.f:660-690) zeroes face entries from both of
 // a face's cells; the reference does this by scattering zeros from every cell
-// (a benign write-write race under OpenMP).  Here each cell stores a flag
-// (1: c+dc <= cmin, 2: c+dc >= cmax) and every face reads the flags of its two
-// cells, which is race-free and gives the same zero set.
+// (a benign write-write race under OpenMP).  Here each cell has a flag
+// (1: c+dc <= cmin, 2: c+dc >= cmax), recomputed by each cell for itself and
+// its six neighbours, and every face reads the flags of its two cells, which
+// is race-free and gives the same zero set.
 //
 // Ghost cells: BOUNDARY_CONDITIONS leaves every ghost equal to an interior
 // cell (z: 0<-1, nz+1<-nz; x/y: clamped for bcond=1, periodic for bcond=0),
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(256) k_courant(Layout L, Box b, double tau3, d
 // split velocities of :527-565), optionally with limited face entries
 template <typename T>
 __device__ inline T donor(const Layout &L, const T *ca, const T *U, const T *V, const T *W, int i, int j, int k,
-                          const uint8_t *flag) {
+                          const int *fl7) {
   T u0 = F3(U, i - 1, j, k), u1 = F3(U, i, j, k);
   T v0 = F3(V, i, j - 1, k), v1 = F3(V, i, j, k);
   T w0 = F3(W, i, j, k - 1), w1 = F3(W, i, j, k);
@@ -54,16 +55,9 @@ __device__ inline T donor(const Layout &L, const T *ca, const T *U, const T *V, 
   T vpj1 = (T)0.5 * (v1 + fabs(v1)), vmj1 = (T)0.5 * (v1 - fabs(v1));
   T wpk0 = (T)0.5 * (w0 + fabs(w0)), wmk0 = (T)0.5 * (w0 - fabs(w0));
   T wpk1 = (T)0.5 * (w1 + fabs(w1)), wmk1 = (T)0.5 * (w1 - fabs(w1));
-  if (flag) {
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    auto fl = [&](int a, int bb, int c) -> int {
-      if (a < 1 || a > nx || bb < 1 || bb > ny || c < 1 || c > nz) return 0;
-      return flag[IX(L, a, bb, c)];
-    };
-    int fc = fl(i, j, k);
-    int fxm = fl(i - 1, j, k), fxp = fl(i + 1, j, k);
-    int fym = fl(i, j - 1, k), fyp = fl(i, j + 1, k);
-    int fzm = fl(i, j, k - 1), fzp = fl(i, j, k + 1);
+  if (fl7) {
+    // flags of the cell and its six neighbours: centre, x-, x+, y-, y+, z-, z+
+    int fc = fl7[0], fxm = fl7[1], fxp = fl7[2], fym = fl7[3], fyp = fl7[4], fzm = fl7[5], fzp = fl7[6];
     // x face i-1 (cells i-1, i) and face i (cells i, i+1)
     if (fxm == 1 || fc == 2) upi0 = (T)0.0;
     if (fc == 1 || fxm == 2) umi0 = (T)0.0;
@@ -172,33 +166,42 @@ __global__ void __launch_bounds__(256) k_antidiff(Layout L, Box b, const T *__re
   (void)nx; (void)ny; (void)nz;
 }
 
-// limiter flags from the unlimited corrective step, mpdata-hotcross80.f:660-690
+// limiter flag of one cell from its unlimited corrective step,
+// mpdata-hotcross80.f:660-690 (1: c+dc <= cmin, 2: c+dc >= cmax); cells
+// outside the interior carry no flag
 template <typename T>
-__global__ void __launch_bounds__(256) k_flags(Layout L, Box b, double cmin, double cmax, const T *__restrict__ cc,
-                                               const T *__restrict__ ud, const T *__restrict__ vd,
-                                               const T *__restrict__ wd, uint8_t *__restrict__ flag) {
-  HC_BOX_IJK(b);  // interior
-  T ce = donor<T>(L, cc, ud, vd, wd, i, j, k, nullptr);
-  T x = F3(cc, i, j, k) + ce;
-  uint8_t f = 0;
-  if (x <= (T)cmin) f = 1;
-  else if (x >= (T)cmax) f = 2;
-  flag[IX(L, i, j, k)] = f;
+__device__ inline int cell_flag(const Layout &L, const T *cc, const T *ud, const T *vd, const T *wd, int i, int j,
+                                int k, double cmin, double cmax) {
+  if (i < 1 || i > L.nx || j < 1 || j > L.ny || k < 1 || k > L.nz) return 0;
+  T x = F3(cc, i, j, k) + donor<T>(L, cc, ud, vd, wd, i, j, k, nullptr);
+  if (x <= (T)cmin) return 1;
+  if (x >= (T)cmax) return 2;
+  return 0;
 }
 
-// limited corrective step (mpdata-hotcross80.f:705-720, then c = c + ce);
-// on the last pass also the sponge and source of :356-381, with the pre-sponge
-// value kept in cpre for the boundary images
+// limited corrective step (mpdata-hotcross80.f:705-720, then c = c + ce).
+// The flags of the cell and of its six neighbours are recomputed here from
+// the antidiffusive velocities (read through the caches) instead of being
+// stored.  On the last pass also the sponge and source of :356-381, with the
+// pre-sponge value kept in cpre for the boundary images.
 template <typename T>
 __global__ void __launch_bounds__(256) k_limited(Layout L, Box b, StepArgs s, Phys ph, VentArgs va, int last,
-                                                 const T *__restrict__ cc, const T *__restrict__ ud,
-                                                 const T *__restrict__ vd, const T *__restrict__ wd,
-                                                 const uint8_t *__restrict__ flag, T *__restrict__ cout,
+                                                 double cmin, double cmax, const T *__restrict__ cc,
+                                                 const T *__restrict__ ud, const T *__restrict__ vd,
+                                                 const T *__restrict__ wd, T *__restrict__ cout,
                                                  T *__restrict__ cpre, const double *__restrict__ cbkg,
                                                  const double *__restrict__ alphae0, const double *__restrict__ alphaw,
                                                  const double *__restrict__ qpatch) {
   HC_BOX_IJK(b);  // interior
-  T ce = donor<T>(L, cc, ud, vd, wd, i, j, k, flag);
+  int f7[7];
+  f7[0] = cell_flag<T>(L, cc, ud, vd, wd, i, j, k, cmin, cmax);
+  f7[1] = cell_flag<T>(L, cc, ud, vd, wd, i - 1, j, k, cmin, cmax);
+  f7[2] = cell_flag<T>(L, cc, ud, vd, wd, i + 1, j, k, cmin, cmax);
+  f7[3] = cell_flag<T>(L, cc, ud, vd, wd, i, j - 1, k, cmin, cmax);
+  f7[4] = cell_flag<T>(L, cc, ud, vd, wd, i, j + 1, k, cmin, cmax);
+  f7[5] = cell_flag<T>(L, cc, ud, vd, wd, i, j, k - 1, cmin, cmax);
+  f7[6] = cell_flag<T>(L, cc, ud, vd, wd, i, j, k + 1, cmin, cmax);
+  T ce = donor<T>(L, cc, ud, vd, wd, i, j, k, f7);
   T cn = F3(cc, i, j, k) + ce;
   if (!last) {
     F3(cout, i, j, k) = cn;
@@ -331,18 +334,14 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   // pass 2
   hipLaunchKernelGGL(k_antidiff<T>, box_grid(ad), box_block(), 0, st, L, ad, c1, (T *)tw.uc, (T *)tw.vc,
                      (T *)tw.wc, (T *)tw.ud2, (T *)tw.vd2, (T *)tw.wd2);
-  hipLaunchKernelGGL(k_flags<T>, box_grid(in), box_block(), 0, st, L, in, ta.cmin, ta.cmax, c1, (T *)tw.ud2,
-                     (T *)tw.vd2, (T *)tw.wd2, tw.flag);
-  hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 0, c1, (T *)tw.ud2,
-                     (T *)tw.vd2, (T *)tw.wd2, tw.flag, c2, (T *)nullptr, c_bkg, aux.alphae0, aux.alphaw, qpatch);
+  hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 0, ta.cmin, ta.cmax, c1,
+                     (T *)tw.ud2, (T *)tw.vd2, (T *)tw.wd2, c2, (T *)nullptr, c_bkg, aux.alphae0, aux.alphaw, qpatch);
   hipLaunchKernelGGL(k_bc<T>, box_grid(all), box_block(), 0, st, L, all, ph, va, c2, c2, 0, 0.0);
   // pass 3 (the antidiffusive velocities of pass 2 play the Courant numbers)
   hipLaunchKernelGGL(k_antidiff<T>, box_grid(ad), box_block(), 0, st, L, ad, c2, (T *)tw.ud2, (T *)tw.vd2,
                      (T *)tw.wd2, (T *)tw.ud, (T *)tw.vd, (T *)tw.wd);
-  hipLaunchKernelGGL(k_flags<T>, box_grid(in), box_block(), 0, st, L, in, ta.cmin, ta.cmax, c2, (T *)tw.ud,
-                     (T *)tw.vd, (T *)tw.wd, tw.flag);
-  hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 1, c2, (T *)tw.ud,
-                     (T *)tw.vd, (T *)tw.wd, tw.flag, c, c1, c_bkg, aux.alphae0, aux.alphaw, qpatch);
+  hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 1, ta.cmin, ta.cmax, c2,
+                     (T *)tw.ud, (T *)tw.vd, (T *)tw.wd, c, c1, c_bkg, aux.alphae0, aux.alphaw, qpatch);
   hipLaunchKernelGGL(k_bc<T>, box_grid(all), box_block(), 0, st, L, all, ph, va, c, c1, ta.vent, ta.vent_value);
   if (ta.budgets) {
     int nb = L.ny * L.nz;

@@ -1,0 +1,63 @@
+"""Properties that hold at the headline size (256^3, FP64), where the oracle
+would take hours: the projected velocity is discretely divergence-free off the
+i = nx plane (where the pressure solve's compatibility correction acts), the
+pressure has zero mean, the state stays finite and the native loop
+(hc_step) reproduces the per-subroutine path bit for bit."""
+import numpy as np
+import pytest
+
+import gpu_common as G
+
+pytestmark = pytest.mark.gpu
+H = G.H
+
+
+def divergence(u, v, w, p):
+    nx, ny, nz = p.nx, p.ny, p.nz
+    # arrays are over the u/v/w bounds; interior cells i=1..nx-1, j=1..ny, k=1..nz
+    U = u[1:, 1:ny + 1, 1:nz + 1]           # i = 0..nx+1 -> use i=0..nx
+    du = (U[1:nx, :, :] - U[0:nx - 1, :, :]) / p.dx
+    V = v[1:nx, :, 1:nz + 1]                 # i = 1..nx-1, j = -1..ny+1
+    dv = (V[:, 2:ny + 2, :] - V[:, 1:ny + 1, :]) / p.dy
+    W = w[1:nx, 1:ny + 1, :]                 # k = -1..nz+1
+    dw = (W[:, :, 2:nz + 2] - W[:, :, 1:nz + 1]) / p.dz
+    return du + dv + dw
+
+
+def test_headline_size_projection_and_finiteness():
+    p = H.workload_params("les256")
+    print("les256 native loop, 4 steps", flush=True)
+    ctx = H.Context(p, H.FP64)
+    z, t, s = H.linear_stratification(p)
+    ctx.init_case(z, t, s, t_noise=1.0e-4, seed=3)
+    ctx.step(4)
+    u, v, w = ctx.download("u"), ctx.download("v"), ctx.download("w")
+    for a in (u, v, w):
+        core = np.abs(a) < 1e30
+        assert np.isfinite(a[core]).all()
+    d = divergence(u, v, w, p)
+    umax = max(np.abs(u[np.abs(u) < 1e30]).max(), 1e-30)
+    assert np.abs(d).max() <= 1e-9 * umax / p.dx, (np.abs(d).max(), umax)
+    pr = ctx.download("p")[1:p.nx + 1, 1:p.ny + 1, 1:p.nz + 1]
+    assert abs(pr.mean()) <= 1e-12 * max(np.abs(pr).max(), 1e-300)
+
+
+def test_native_loop_equals_subroutine_path():
+    p = H.workload_params("les64")
+    z, t, s = H.linear_stratification(p)
+    a = H.Context(p, H.FP64)
+    a.init_case(z, t, s, t_noise=1.0e-4, seed=5)
+    b = H.Context(p, H.FP64)
+    b.init_case(z, t, s, t_noise=1.0e-4, seed=5)
+    b._t_bkg = None
+    # the subroutine path needs the host profiles: rebuild them as hc_init_case does
+    import hc_testlib as T
+    o = T.Oracle(T.Params.from_buffer_copy(bytes(p)))
+    o.init_profiles(z, t, s)
+    b._t_bkg, b._s_bkg = o.get("t_bkg"), o.get("s_bkg")
+    n = 12
+    a.step(n)
+    for m in range(n):
+        b.driver_step(m, a.source_noise(m))
+    for f in ("u", "v", "w", "p", "t", "s", "c", "ss_v", "rhop"):
+        assert np.array_equal(a.download(f), b.download(f)), f
