@@ -85,9 +85,17 @@ def cpu_leg(nx, ny, nz, steps, threads):
     env.setdefault("OMP_PROC_BIND", "close")
     env.setdefault("OMP_STACKSIZE", "512M")
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", lib, str(nx), str(ny), str(nz), str(steps)]
+
+    def big_stack():
+        # the reference keeps grid-sized automatic arrays on the main
+        # thread's stack (OMP_STACKSIZE covers only the worker threads)
+        import resource
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
     with tempfile.TemporaryDirectory() as d:
         r = subprocess.run(cmd, cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
-                           timeout=900)
+                           timeout=900, preexec_fn=big_stack)
     if r.returncode != 0:
         sys.stderr.write(r.stderr[-2000:])
         return None

@@ -9,9 +9,11 @@ writes the one line that selects the grid, as the reference's own commented
 alternatives at array_size.inc:4-6 do), and compiles with amdflang:
 
   amdflang -c -O2 -ffixed-form -ffixed-line-length=132 -fdefault-real-8
-           -fopenmp -fd-lines-as-comments -fPIC
+           -fopenmp -fd-lines-as-comments -fPIC -mcmodel=medium
 
-(-fdefault-real-8 is flang's spelling of PGI's -r8, Hotcross-SAPELO2.sh:15).
+(-fdefault-real-8 is flang's spelling of PGI's -r8, Hotcross-SAPELO2.sh:15;
+-mcmodel=medium because the module and main-program arrays are static and
+pass 2 GiB at 256^3).
 
 hotcross80-revised2014.f holds FORCES, PRESSURE, MOMENTUM_U/V/W, REFRESH and
 INITIALIZE together with `program hotcross`, whose body includes the run's case
@@ -48,7 +50,7 @@ SOURCES = [
 ]
 INCLUDES = ["case.inc", "site_forcing.inc", "source80.inc"]
 FFLAGS = ["-O2", "-ffixed-form", "-ffixed-line-length=132", "-fdefault-real-8",
-          "-fopenmp", "-fd-lines-as-comments", "-fPIC"]
+          "-fopenmp", "-fd-lines-as-comments", "-fPIC", "-mcmodel=medium"]
 
 
 def libpath(nx, ny, nz):
@@ -78,7 +80,7 @@ def build(nx, ny, nz, quiet=True):
         run(["amdflang", "-c"] + FFLAGS + [f])
         objs.append(f.replace(".f", ".o"))
     run(["amdflang", "-c", "-O2", "-fdefault-real-8", "-fopenmp", "-fPIC",
-         "-I.", os.path.join(HERE, "ref_harness.f90"), "-o", "ref_harness.o"])
+         "-mcmodel=medium", "-I.", os.path.join(HERE, "ref_harness.f90"), "-o", "ref_harness.o"])
     run(["amdflang", "-shared", "-fopenmp", "-o", "libhcref.so",
          "ref_harness.o"] + objs)
     return libpath(nx, ny, nz)

@@ -53,6 +53,10 @@ struct hc_ctx {
   double acc[8] = {};
   int nxs = 0, nxe = 0, nys = 0, nye = 0, npatch = 0;
   bool map_valid = false;
+  // after REFRESH the reference's up/vp/wp equal u/v/w (u = up copies,
+  // hotcross80-revised2014.f:1421-1488); here the buffers are swapped instead and
+  // up_alias marks that the logical up is the u buffer until MOMENTUM writes it
+  bool up_alias[3] = {false, false, false};
   double sumup = 0, sumdown = 0;
   uint64_t seed = 12345;
   int next_step = 0;
@@ -264,7 +268,7 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
   ctx->esz = precision == HC_FP64 ? 8 : 4;
   {
     const char *e = getenv("HC_XCD_REMAP");
-    box_remap_flag() = (e && e[0] == '0') ? 0 : 1;
+    box_remap_flag() = (e && e[0] == '1') ? 1 : 0;
   }
   ctx->L = make_layout(P.nx, P.ny, P.nz);
   *out = ctx;
@@ -390,6 +394,15 @@ extern "C" int hc_synchronize(hc_ctx *ctx) {
   return HC_OK;
 }
 
+// copy the u buffer into the up buffer where the logical up still aliases u
+static int materialise_alias(hc_ctx *ctx, int comp) {
+  if (!ctx->up_alias[comp]) return HC_OK;
+  int uf = HC_U + comp, upf = HC_UP + comp;
+  HIPCHK(hipMemcpyAsync(ctx->f[upf], ctx->f[uf], ctx->L.n * ctx->esz, hipMemcpyDeviceToDevice, ctx->st));
+  ctx->up_alias[comp] = false;
+  return HC_OK;
+}
+
 static int check_box(hc_ctx *ctx, int field, const int32_t lo[3], const int32_t hi[3], int n[3]) {
   if (field < 0 || field >= HC_NFIELDS) return fail(ctx, HC_EINVAL, "bad field id");
   int fl[3], fh[3];
@@ -405,6 +418,10 @@ extern "C" int hc_upload(hc_ctx *ctx, int field, const double *host, const int32
   int n[3];
   int r = check_box(ctx, field, lo, hi, n);
   if (r) return r;
+  if (field >= HC_U && field <= HC_WP) {
+    r = materialise_alias(ctx, field % 3);
+    if (r) return r;
+  }
   size_t cnt = (size_t)n[0] * n[1] * n[2];
   HIPCHK(hipMemcpyAsync(ctx->d_stage, host, cnt * 8, hipMemcpyHostToDevice, ctx->st));
   dim3 g((unsigned)((cnt + 255) / 256));
@@ -422,14 +439,15 @@ extern "C" int hc_download(hc_ctx *ctx, int field, double *host, const int32_t l
   int n[3];
   int r = check_box(ctx, field, lo, hi, n);
   if (r) return r;
+  int src = (field >= HC_UP && field <= HC_WP && ctx->up_alias[field - HC_UP]) ? field - 3 : field;
   size_t cnt = (size_t)n[0] * n[1] * n[2];
   dim3 g((unsigned)((cnt + 255) / 256));
   if (ctx->prec == HC_FP64)
     hipLaunchKernelGGL(k_gather<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
-                       (const double *)ctx->f[field], ctx->d_stage);
+                       (const double *)ctx->f[src], ctx->d_stage);
   else
     hipLaunchKernelGGL(k_gather<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
-                       (const float *)ctx->f[field], ctx->d_stage);
+                       (const float *)ctx->f[src], ctx->d_stage);
   HIPCHK(hipMemcpyAsync(host, ctx->d_stage, cnt * 8, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return HC_OK;
@@ -651,6 +669,7 @@ extern "C" int hc_pressure(hc_ctx *ctx) {
 
 static int momentum(hc_ctx *ctx, int comp) {
   PhaseScope ps(ctx, "momentum");
+  ctx->up_alias[comp] = false;  // MOMENTUM writes the up buffer
   StepArgs s = step_args(ctx);
   VentArgs va = vent_args(ctx);
   if (ctx->prec == HC_FP64) launch_momentum<double>(ctx->st, comp, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux);
@@ -762,6 +781,7 @@ extern "C" int hc_refresh(hc_ctx *ctx) {
     ctx->f[t[0]] = ctx->f[t[1]];
     ctx->f[t[1]] = tmp;
   }
+  for (int q = 0; q < 3; ++q) ctx->up_alias[q] = true;
   return HC_OK;
 }
 

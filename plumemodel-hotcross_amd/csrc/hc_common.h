@@ -46,10 +46,11 @@ struct StepArgs {
   double acc[8];  // persistent azu/azv boundary accumulators
 };
 
-// XCD-aware block order on/off for the box launches (on unless the
-// environment sets HC_XCD_REMAP=0 when a context is created)
+// XCD-aware block order on/off for the box launches (off unless the
+// environment sets HC_XCD_REMAP=1 when a context is created; on MI355X at
+// 256^3 it measured 1.7% slower than the dispatcher's own order)
 inline int &box_remap_flag() {
-  static int flag = 1;
+  static int flag = 0;
   return flag;
 }
 

@@ -377,6 +377,23 @@ __global__ void __launch_bounds__(256) k_viscosity(Layout L, Box b, StepArgs s, 
 
 // ---------------------------------------------------------------------------
 // DENSITY, potential_density80.f:3-71: interior and the six faces.
+//
+// s**1.5 rounded correctly (but for exact values within ~2^-104 of a
+// rounding midpoint): r = sqrt(s) is correctly rounded, so s - r*r is exact
+// as one fma; sqrt(s) = r + (s - r*r)/(2r) to ~2^-106 relative, and the
+// product s*sqrt(s) is carried as p + (pe + s*c) with pe the exact error of
+// s*r.  A host libm pow is accurate to about half an ulp, so both round to
+// the same double except for rare near-midpoint values.
+__device__ inline double pow15(double s) {
+  if (!(s > 0.0) || isinf(s)) return pow(s, 1.5);
+  double r = sqrt(s);
+  double e = fma(-r, r, s);
+  double c = e / (2.0 * r);
+  double p = s * r;
+  double pe = fma(s, r, -p);
+  return p + fma(s, c, pe);
+}
+
 template <typename T>
 __device__ inline double rho_rho(const Phys &ph, double s, double t) {
   if (ph.eos == 1) return ph.eos_rho_ref * (1.0 - ph.eos_alpha * (t - ph.eos_t0) + ph.eos_beta * (s - ph.eos_s0));
@@ -384,7 +401,7 @@ __device__ inline double rho_rho(const Phys &ph, double s, double t) {
   double r0 = 999.842594 + 6.793952e-2 * t - 9.095290e-3 * t * t + 1.001685e-4 * t3 - 1.120083e-6 * t4 +
               6.536332e-9 * t5;
   return r0 + s * (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t * t - 8.2467e-7 * t3 + 5.3875e-9 * t4) +
-         pow(s, 1.5) * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t * t) + 4.8314e-4 * s * s;
+         pow15(s) * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t * t) + 4.8314e-4 * s * s;
 }
 
 template <typename T>

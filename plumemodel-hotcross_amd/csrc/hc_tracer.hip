@@ -220,14 +220,28 @@ __global__ void __launch_bounds__(256) k_limited(Layout L, Box b, StepArgs s, Ph
   F3(cout, i, j, k) = r;
 }
 
-// ghost cells of a scalar field from their interior images
+// ghost cells of a scalar field from their interior images.  One thread per
+// ghost cell: the x faces (with edges), then the y faces, then the z faces.
 template <typename T>
-__global__ void __launch_bounds__(256) k_bc(Layout L, Box b, Phys ph, VentArgs va, T *__restrict__ dst,
+__global__ void __launch_bounds__(256) k_bc(Layout L, Phys ph, VentArgs va, T *__restrict__ dst,
                                             const T *__restrict__ src, int vent, double vent_value) {
-  HC_BOX_IJK(b);  // 0..nx+1, 0..ny+1, 0..nz+1
   const int nx = L.nx, ny = L.ny, nz = L.nz;
-  bool gi = (i == 0 || i == nx + 1), gj = (j == 0 || j == ny + 1), gk = (k == 0 || k == nz + 1);
-  if (!gi && !gj && !gk) return;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long na = 2L * (ny + 2) * (nz + 2), nb = 2L * nx * (nz + 2), nc = 2L * nx * ny;
+  int i, j, k;
+  if (t < na) {
+    j = (int)(t % (ny + 2)); long r = t / (ny + 2);
+    k = (int)(r % (nz + 2)); i = (r / (nz + 2)) ? nx + 1 : 0;
+  } else if ((t -= na) < nb) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    k = (int)(r % (nz + 2)); j = (r / (nz + 2)) ? ny + 1 : 0;
+  } else if ((t -= nb) < nc) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    j = 1 + (int)(r % ny); k = (r / ny) ? nz + 1 : 0;
+  } else {
+    return;
+  }
+  bool gi = (i == 0 || i == nx + 1), gj = (j == 0 || j == ny + 1);
   int si = i, sj = j, sk = k;
   if (k == 0) sk = 1;
   if (k == nz + 1) sk = nz;
@@ -319,12 +333,13 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   StepArgs s2 = s;
   s2.tau3 = ta.tau3;
   Box in{1, L.nx, 1, L.ny, 1, L.nz};
-  Box all{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+  const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
+  const unsigned nbc = (unsigned)((nghost + 255) / 256);
   Box ad{0, L.nx, 0, L.ny, 0, L.nz};
   T *c1 = (T *)tw.c1, *c2 = (T *)tw.c2;
   hipLaunchKernelGGL(k_pass1<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, c, (T *)tw.uc, (T *)tw.vc,
                      (T *)tw.wc, f.ss, f.ssv, aux.azzmin, c_bkg, c1);
-  hipLaunchKernelGGL(k_bc<T>, box_grid(all), box_block(), 0, st, L, all, ph, va, c1, c1, 0, 0.0);
+  hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c1, c1, 0, 0.0);
   if (ph.ismol != 1) {
     // no corrective passes: sponge/source straight after pass 1 through the
     // limited kernel with zero antidiffusive velocities would change nothing;
@@ -336,13 +351,13 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
                      (T *)tw.wc, (T *)tw.ud2, (T *)tw.vd2, (T *)tw.wd2);
   hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 0, ta.cmin, ta.cmax, c1,
                      (T *)tw.ud2, (T *)tw.vd2, (T *)tw.wd2, c2, (T *)nullptr, c_bkg, aux.alphae0, aux.alphaw, qpatch);
-  hipLaunchKernelGGL(k_bc<T>, box_grid(all), box_block(), 0, st, L, all, ph, va, c2, c2, 0, 0.0);
+  hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c2, c2, 0, 0.0);
   // pass 3 (the antidiffusive velocities of pass 2 play the Courant numbers)
   hipLaunchKernelGGL(k_antidiff<T>, box_grid(ad), box_block(), 0, st, L, ad, c2, (T *)tw.ud2, (T *)tw.vd2,
                      (T *)tw.wd2, (T *)tw.ud, (T *)tw.vd, (T *)tw.wd);
   hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 1, ta.cmin, ta.cmax, c2,
                      (T *)tw.ud, (T *)tw.vd, (T *)tw.wd, c, c1, c_bkg, aux.alphae0, aux.alphaw, qpatch);
-  hipLaunchKernelGGL(k_bc<T>, box_grid(all), box_block(), 0, st, L, all, ph, va, c, c1, ta.vent, ta.vent_value);
+  hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c, c1, ta.vent, ta.vent_value);
   if (ta.budgets) {
     int nb = L.ny * L.nz;
     hipLaunchKernelGGL(k_budget<T>, dim3(nb), dim3(256), 0, st, L, ta.tau3, va, c, c_bkg, aux.alphae, qpatch,

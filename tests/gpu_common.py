@@ -34,3 +34,12 @@ def compare(a, b):
     if not np.array_equal(a[fin & ~msk], b[fin & ~msk]):
         return np.inf
     return T.rel_err(a, b, msk)
+
+
+def compare_abs(a, b):
+    """max |a-b| over the same cells as compare()."""
+    fin = np.isfinite(b)
+    msk = fin & (np.abs(b) < 1e30)
+    if not np.array_equal(a[fin & ~msk], b[fin & ~msk]):
+        return np.inf
+    return float(np.abs(a[msk] - b[msk]).max()) if msk.any() else 0.0
