@@ -99,7 +99,10 @@ def cpu_leg(nx, ny, nz, steps, threads):
     if r.returncode != 0:
         sys.stderr.write(r.stderr[-2000:])
         return None
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    # the reference's own Fortran output shares stdout; the worker's JSON
+    # line is the one that starts with '{'
+    lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+    return json.loads(lines[-1]) if lines else None
 
 
 def cpu_worker(lib, nx, ny, nz, steps):
@@ -118,7 +121,7 @@ def cpu_worker(lib, nx, ny, nz, steps):
     for m in range(1, steps + 1):
         ref.call("step", m)
     sec = time.perf_counter() - t0
-    print(json.dumps({"sec": sec, "steps": steps}))
+    print(json.dumps({"sec": sec, "steps": steps}), flush=True)
 
 
 def main():

@@ -271,6 +271,10 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
     box_remap_flag() = (e && e[0] == '1') ? 1 : 0;
   }
   ctx->L = make_layout(P.nx, P.ny, P.nz);
+  if (ctx->L.n >= ((size_t)1 << 32)) {
+    *out = ctx;
+    return fail(ctx, HC_EINVAL, "grid too large: the padded box must hold fewer than 2^32 cells");
+  }
   *out = ctx;
   HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
   int rc = 0;
@@ -280,8 +284,8 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
   fill(ctx, ctx->f[HC_FX], MISSING);
   fill(ctx, ctx->f[HC_FY], MISSING);
   // tracer scratch
-  void **tws[] = {&ctx->tw.uc, &ctx->tw.vc, &ctx->tw.wc, &ctx->tw.ud, &ctx->tw.vd, &ctx->tw.wd,
-                  &ctx->tw.ud2, &ctx->tw.vd2, &ctx->tw.wd2, &ctx->tw.c1, &ctx->tw.c2};
+  void **tws[] = {&ctx->tw.uc, &ctx->tw.vc, &ctx->tw.wc, &ctx->tw.ud2, &ctx->tw.vd2, &ctx->tw.wd2,
+                  &ctx->tw.c1, &ctx->tw.c2};
   for (auto p : tws)
     if (!(*p = dalloc(ctx, fb, &rc))) return rc;
   if (!(ctx->tw.budget_part = (double *)dalloc(ctx, (size_t)3 * P.ny * P.nz * 8, &rc))) return rc;
@@ -371,8 +375,7 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
   if (!ctx) return HC_OK;
   if (ctx->st) hipStreamSynchronize(ctx->st);
   for (int q = 0; q < HC_NFIELDS; ++q) hipFree(ctx->f[q]);
-  void *ps[] = {ctx->tw.uc, ctx->tw.vc, ctx->tw.wc, ctx->tw.ud, ctx->tw.vd, ctx->tw.wd, ctx->tw.ud2,
-                ctx->tw.vd2, ctx->tw.wd2, ctx->tw.c1, ctx->tw.c2, ctx->tw.budget_part,
+  void *ps[] = {ctx->tw.uc, ctx->tw.vc, ctx->tw.wc, ctx->tw.ud2, ctx->tw.vd2, ctx->tw.wd2, ctx->tw.c1, ctx->tw.c2, ctx->tw.budget_part,
                 ctx->tw.budget, ctx->aux.azzmin, ctx->aux.rho_bkg, ctx->aux.pgradx, ctx->aux.pgrady,
                 ctx->aux.alphaw, ctx->aux.alphau, ctx->aux.alphav, ctx->aux.alphae, ctx->aux.alphae0,
                 ctx->d_tbkg, ctx->d_sbkg, ctx->d_zero, ctx->d_cbkg, ctx->d_q[0], ctx->d_q[1], ctx->d_q[2],

@@ -23,8 +23,11 @@ struct Layout {
   size_t n;        // elements per field
 };
 
+// Element index of (i,j,k).  Computed in 32 bits: a context refuses grids
+// whose padded box holds 2^32 elements or more, so the index always fits,
+// and 32-bit arithmetic keeps the neighbour offsets of a stencil cheap.
 __host__ __device__ inline size_t IX(const Layout &L, int i, int j, int k) {
-  return (size_t)(i + OXP) + (size_t)L.px * ((size_t)(j + 1) + (size_t)L.py * (size_t)(k + 1));
+  return (size_t)((unsigned)(i + OXP) + (unsigned)L.px * ((unsigned)(j + 1) + (unsigned)L.py * (unsigned)(k + 1)));
 }
 
 inline Layout make_layout(int nx, int ny, int nz) {

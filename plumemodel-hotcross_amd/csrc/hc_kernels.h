@@ -46,9 +46,8 @@ void launch_refresh(hipStream_t st, const Layout &L, int leap, double assel, con
 // tracer transport (TRACER31 / MPDATA)
 struct TracerWork {
   void *uc, *vc, *wc;      // Courant numbers of the work velocities
-  void *ud, *vd, *wd;      // antidiffusive velocities of the current pass
-  void *ud2, *vd2, *wd2;   // ... of the previous pass
-  void *c1, *c2;           // pass results
+  void *ud2, *vd2, *wd2;   // antidiffusive velocities of pass 2 (pass 3's Courant numbers)
+  void *c1, *c2;           // pass results (c1: pass 1, then the pre-sponge boundary values)
   double *budget_part;     // per-block partial sums (FP64)
   double *budget;          // 3 sums
 };

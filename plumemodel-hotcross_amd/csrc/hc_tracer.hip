@@ -2,8 +2,9 @@
 //
 // mpdata-hotcross80.f:16-494.  One tracer call is
 //   pass 1: explicit diffusion + donor-cell upwind step      (k_pass1)
-//   pass 2/3: antidiffusive velocities (k_antidiff), then the limiter flags
-//            and the limited donor-cell step (k_limited)
+//   pass 2/3: antidiffusive velocities, limiter flags and the limited
+//            donor-cell step in one k-marching LDS tile (k_mpdata_pass,
+//            hc_mpdata.h)
 //   sponge + source on the last pass, then the boundary images (k_bc).
 // The diffusivities kxt/kyt/kzt of MIXING (mixing80-revised2014.f:22-59) are
 // formed inside k_pass1 from ss/ss_v with the reference's expressions.
@@ -19,8 +20,11 @@
 // cell (z: 0<-1, nz+1<-nz; x/y: clamped for bcond=1, periodic for bcond=0),
 // so k_bc copies from the mapped interior cell instead of sweeping faces in
 // the reference's order.
+#include <stdlib.h>
+
 #include "hc_common.h"
 #include "hc_kernels.h"
+#include "hc_mpdata.h"
 
 namespace hc {
 
@@ -39,46 +43,6 @@ __global__ void __launch_bounds__(256) k_courant(Layout L, Box b, double tau3, d
   if (i <= L.nx) uc[q] = (T)0.5 * (up[q] + u[q]) * (T)tau3 / (T)dx;
   if (j <= L.ny) vc[q] = (T)0.5 * (vp[q] + v[q]) * (T)tau3 / (T)dy;
   if (k <= L.nz) wc[q] = -((T)0.5 * (wp[q] + w[q])) * (T)tau3 / (T)dz;
-}
-
-// donor-cell flux divergence of MPDATA (mpdata-hotcross80.f:590-608 with the
-// split velocities of :527-565), optionally with limited face entries
-template <typename T>
-__device__ inline T donor(const Layout &L, const T *ca, const T *U, const T *V, const T *W, int i, int j, int k,
-                          const int *fl7) {
-  T u0 = F3(U, i - 1, j, k), u1 = F3(U, i, j, k);
-  T v0 = F3(V, i, j - 1, k), v1 = F3(V, i, j, k);
-  T w0 = F3(W, i, j, k - 1), w1 = F3(W, i, j, k);
-  T upi0 = (T)0.5 * (u0 + fabs(u0)), umi0 = (T)0.5 * (u0 - fabs(u0));
-  T upi1 = (T)0.5 * (u1 + fabs(u1)), umi1 = (T)0.5 * (u1 - fabs(u1));
-  T vpj0 = (T)0.5 * (v0 + fabs(v0)), vmj0 = (T)0.5 * (v0 - fabs(v0));
-  T vpj1 = (T)0.5 * (v1 + fabs(v1)), vmj1 = (T)0.5 * (v1 - fabs(v1));
-  T wpk0 = (T)0.5 * (w0 + fabs(w0)), wmk0 = (T)0.5 * (w0 - fabs(w0));
-  T wpk1 = (T)0.5 * (w1 + fabs(w1)), wmk1 = (T)0.5 * (w1 - fabs(w1));
-  if (fl7) {
-    // flags of the cell and its six neighbours: centre, x-, x+, y-, y+, z-, z+
-    int fc = fl7[0], fxm = fl7[1], fxp = fl7[2], fym = fl7[3], fyp = fl7[4], fzm = fl7[5], fzp = fl7[6];
-    // x face i-1 (cells i-1, i) and face i (cells i, i+1)
-    if (fxm == 1 || fc == 2) upi0 = (T)0.0;
-    if (fc == 1 || fxm == 2) umi0 = (T)0.0;
-    if (fc == 1 || fxp == 2) upi1 = (T)0.0;
-    if (fxp == 1 || fc == 2) umi1 = (T)0.0;
-    if (fym == 1 || fc == 2) vpj0 = (T)0.0;
-    if (fc == 1 || fym == 2) vmj0 = (T)0.0;
-    if (fc == 1 || fyp == 2) vpj1 = (T)0.0;
-    if (fyp == 1 || fc == 2) vmj1 = (T)0.0;
-    // z face k-1 (cells k-1, k): wpk zeroed by cell k <= cmin or cell k-1 >= cmax
-    if (fc == 1 || fzm == 2) wpk0 = (T)0.0;
-    if (fzm == 1 || fc == 2) wmk0 = (T)0.0;
-    if (fzp == 1 || fc == 2) wpk1 = (T)0.0;
-    if (fc == 1 || fzp == 2) wmk1 = (T)0.0;
-  }
-  T cc = F3(ca, i, j, k);
-  T F = +(upi0 * F3(ca, i - 1, j, k) + umi0 * cc) + (vpj0 * F3(ca, i, j - 1, k) + vmj0 * cc) +
-        (wmk1 * cc + wpk1 * F3(ca, i, j, k + 1)) -
-        ((upi1 * cc + umi1 * F3(ca, i + 1, j, k)) + (vpj1 * cc + vmj1 * F3(ca, i, j + 1, k)) +
-         (wmk0 * F3(ca, i, j, k - 1) + wpk0 * cc));
-  return (T)0 * cc + F / (T)1.0;
 }
 
 // pass 1: diffusion (mpdata-hotcross80.f:141-174) + upwind step (:176-184)
@@ -109,115 +73,44 @@ __global__ void __launch_bounds__(256) k_pass1(Layout L, Box b, StepArgs s, Phys
   T bk = (T)cbkg[k], bkm = (T)cbkg[k - 1], bkp = (T)cbkg[k + 1];
   T cz = zfac * (kz0 * ((F3(c, i, j, k - 1) + bk) - (bkm + cp)) - kz1 * ((cp + bkp) - (bk + F3(c, i, j, k + 1))));
   T cd = cp + (+cx + cy + cz);
-  T ce = donor<T>(L, c, uc, vc, wc, i, j, k, nullptr);
+  T ce = mp_donor<T>(cp, F3(c, i - 1, j, k), F3(c, i + 1, j, k), F3(c, i, j - 1, k), F3(c, i, j + 1, k),
+                     F3(c, i, j, k - 1), F3(c, i, j, k + 1), F3(uc, i - 1, j, k), F3(uc, i, j, k),
+                     F3(vc, i, j - 1, k), F3(vc, i, j, k), F3(wc, i, j, k - 1), F3(wc, i, j, k), false, 0, 0, 0, 0,
+                     0, 0, 0);
   F3(c1, i, j, k) = cd + ce;
 }
 
-// antidiffusive velocities, mpdata-hotcross80.f:218-326
-template <typename T>
-__global__ void __launch_bounds__(256) k_antidiff(Layout L, Box b, const T *__restrict__ cc, const T *__restrict__ uc,
-                                                  const T *__restrict__ vc, const T *__restrict__ wc,
-                                                  T *__restrict__ ud, T *__restrict__ vd, T *__restrict__ wd) {
-  HC_BOX_IJK(b);  // 0..nx, 0..ny, 0..nz
-  const int nx = L.nx, ny = L.ny, nz = L.nz;
-  const T epsa = (T)1.0e-30;
-  if (j >= 1 && k >= 1) {  // ud: i=0..nx, j=1..ny, k=1..nz
-    T ca = F3(cc, i + 1, j, k), cb = F3(cc, i, j, k);
-    T U = F3(uc, i, j, k);
-    T r = (fabs(U) - U * U / (T)1.0) * (ca - cb) / (ca + cb + epsa);
-    ca = (F3(cc, i + 1, j + 1, k) + F3(cc, i, j + 1, k));
-    cb = (F3(cc, i + 1, j - 1, k) + F3(cc, i, j - 1, k));
-    r = r - (T)1 * (T)0.5 * U / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
-                (F3(vc, i + 1, j, k) + F3(vc, i, j, k) + F3(vc, i + 1, j - 1, k) + F3(vc, i, j - 1, k)) * (T)1;
-    ca = (F3(cc, i + 1, j, k - 1) + F3(cc, i, j, k - 1));
-    cb = (F3(cc, i + 1, j, k + 1) + F3(cc, i, j, k + 1));
-    r = r - (T)0.5 * U / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
-                (F3(wc, i + 1, j, k) + F3(wc, i, j, k) + F3(wc, i + 1, j, k - 1) + F3(wc, i, j, k - 1)) * (T)1;
-    F3(ud, i, j, k) = r;
+// one corrective pass (antidiffusive velocities, limiter flags, limited
+// donor-cell step; sponge and source on the last pass) as a k-marching tile
+template <typename T, int TI, int TJ>
+__global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, int kc) {
+  typedef MpTile<T, TI, TJ> Tl;
+  __shared__ MpShared<T, TI, TJ> sh;
+  MpGeom g;
+  g.nx = L.nx; g.ny = L.ny; g.nz = L.nz; g.px = L.px; g.py = L.py;
+  g.i0 = 1 + (int)blockIdx.x * TI;
+  g.j0 = 1 + (int)blockIdx.y * TJ;
+  g.ka = 1 + (int)blockIdx.z * kc;
+  g.kb = min(L.nz, g.ka + kc - 1);
+  const int tid = (int)threadIdx.x;
+  const int qs = max(g.ka - 2, 0);
+  MpRegs<T> r{};
+  typename Tl::Plan pl;
+  typename Tl::Prefetch pf;
+  Tl::plan(g, tid, pl);
+  Tl::stage_initial(g, tid, pl, sh, a, qs);
+  Tl::fetch(g, pl, a, qs, pf);
+  for (int q = qs; q <= g.kb + 1; ++q) {
+    __syncthreads();  // the previous plane's finishing reads are done
+    Tl::commit(tid, sh, q, pf);
+    __syncthreads();
+    if (q <= g.kb) Tl::fetch(g, pl, a, q + 1, pf);  // in flight during this plane
+    Tl::faces(g, tid, pl, sh, a, q, q >= g.ka - 1);
+    __syncthreads();
+    if (q >= g.ka - 1) Tl::flags(g, pl, sh, a, q);
+    __syncthreads();
+    Tl::finish(g, pl, sh, a, q, r);
   }
-  if (i >= 1 && k >= 1) {  // vd: i=1..nx, j=0..ny, k=1..nz
-    T ca = F3(cc, i, j + 1, k), cb = F3(cc, i, j, k);
-    T V = F3(vc, i, j, k);
-    T r = (fabs(V) - V * V / (T)1.0) * (T)1 * (ca - cb) / (ca + cb + epsa);
-    ca = (F3(cc, i + 1, j + 1, k) + F3(cc, i + 1, j, k));
-    cb = (F3(cc, i - 1, j + 1, k) + F3(cc, i - 1, j, k));
-    r = r - (T)0.5 * V / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
-                (F3(uc, i, j + 1, k) + F3(uc, i, j, k) + F3(uc, i - 1, j + 1, k) + F3(uc, i - 1, j, k)) * (T)1;
-    ca = (F3(cc, i, j + 1, k - 1) + F3(cc, i, j, k - 1));
-    cb = (F3(cc, i, j + 1, k + 1) + F3(cc, i, j, k + 1));
-    r = r - (T)0.5 * V / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
-                (F3(wc, i, j, k) + F3(wc, i, j + 1, k) + F3(wc, i, j, k - 1) + F3(wc, i, j + 1, k - 1)) * (T)1;
-    F3(vd, i, j, k) = r;
-  }
-  if (i >= 1 && j >= 1) {  // wd: i=1..nx, j=1..ny, k=0..nz
-    T ca = F3(cc, i, j, k), cb = F3(cc, i, j, k + 1);
-    T W = F3(wc, i, j, k);
-    T r = (fabs(W) - W * W / (T)1.0) * (ca - cb) / (ca + cb + epsa);
-    ca = (F3(cc, i + 1, j, k) + F3(cc, i + 1, j, k + 1));
-    cb = (F3(cc, i - 1, j, k) + F3(cc, i - 1, j, k + 1));
-    r = r - (T)1 * (T)0.5 * W / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
-                (F3(uc, i, j, k) + F3(uc, i, j, k + 1) + F3(uc, i - 1, j, k + 1) + F3(uc, i - 1, j, k)) * (T)1;
-    ca = (F3(cc, i, j + 1, k + 1) + F3(cc, i, j + 1, k));
-    cb = (F3(cc, i, j - 1, k + 1) + F3(cc, i, j - 1, k));
-    r = r - (T)1 * (T)0.5 * W / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
-                (F3(vc, i, j, k) + F3(vc, i, j, k + 1) + F3(vc, i, j - 1, k + 1) + F3(vc, i, j - 1, k)) * (T)1;
-    F3(wd, i, j, k) = r;
-  }
-  (void)nx; (void)ny; (void)nz;
-}
-
-// limiter flag of one cell from its unlimited corrective step,
-// mpdata-hotcross80.f:660-690 (1: c+dc <= cmin, 2: c+dc >= cmax); cells
-// outside the interior carry no flag
-template <typename T>
-__device__ inline int cell_flag(const Layout &L, const T *cc, const T *ud, const T *vd, const T *wd, int i, int j,
-                                int k, double cmin, double cmax) {
-  if (i < 1 || i > L.nx || j < 1 || j > L.ny || k < 1 || k > L.nz) return 0;
-  T x = F3(cc, i, j, k) + donor<T>(L, cc, ud, vd, wd, i, j, k, nullptr);
-  if (x <= (T)cmin) return 1;
-  if (x >= (T)cmax) return 2;
-  return 0;
-}
-
-// limited corrective step (mpdata-hotcross80.f:705-720, then c = c + ce).
-// The flags of the cell and of its six neighbours are recomputed here from
-// the antidiffusive velocities (read through the caches) instead of being
-// stored.  On the last pass also the sponge and source of :356-381, with the
-// pre-sponge value kept in cpre for the boundary images.
-template <typename T>
-__global__ void __launch_bounds__(256) k_limited(Layout L, Box b, StepArgs s, Phys ph, VentArgs va, int last,
-                                                 double cmin, double cmax, const T *__restrict__ cc,
-                                                 const T *__restrict__ ud, const T *__restrict__ vd,
-                                                 const T *__restrict__ wd, T *__restrict__ cout,
-                                                 T *__restrict__ cpre, const double *__restrict__ cbkg,
-                                                 const double *__restrict__ alphae0, const double *__restrict__ alphaw,
-                                                 const double *__restrict__ qpatch) {
-  HC_BOX_IJK(b);  // interior
-  int f7[7];
-  f7[0] = cell_flag<T>(L, cc, ud, vd, wd, i, j, k, cmin, cmax);
-  f7[1] = cell_flag<T>(L, cc, ud, vd, wd, i - 1, j, k, cmin, cmax);
-  f7[2] = cell_flag<T>(L, cc, ud, vd, wd, i + 1, j, k, cmin, cmax);
-  f7[3] = cell_flag<T>(L, cc, ud, vd, wd, i, j - 1, k, cmin, cmax);
-  f7[4] = cell_flag<T>(L, cc, ud, vd, wd, i, j + 1, k, cmin, cmax);
-  f7[5] = cell_flag<T>(L, cc, ud, vd, wd, i, j, k - 1, cmin, cmax);
-  f7[6] = cell_flag<T>(L, cc, ud, vd, wd, i, j, k + 1, cmin, cmax);
-  T ce = donor<T>(L, cc, ud, vd, wd, i, j, k, f7);
-  T cn = F3(cc, i, j, k) + ce;
-  if (!last) {
-    F3(cout, i, j, k) = cn;
-    return;
-  }
-  F3(cpre, i, j, k) = cn;
-  const double tau3 = s.tau3;
-  // sponge: c = c - 0.5*alphaee*tau3*(cp - c_bkg(k)), cp = the field at entry
-  double ae = fmax(alphae0[i + (L.nx + 2) * j], (alphaw[k + 1] + alphaw[k]) / 2.0);
-  T cp = F3(cout, i, j, k);
-  T r = cn - (T)1 * (T)0.5 * (T)ae * (T)tau3 * (cp - (T)cbkg[k]);
-  double q = 0.0;
-  if (k == L.nz && va.map_valid && i >= va.nxs && i <= va.nxe && j >= va.nys && j <= va.nye)
-    q = qpatch[(i - va.nxs) + (va.nxe - va.nxs + 1) * (j - va.nys)];
-  r = r + (T)tau3 * (T)q / (T)(s.dx * s.dy * s.dz) / (T)1.0 * (T)1;
-  F3(cout, i, j, k) = r;
 }
 
 // ghost cells of a scalar field from their interior images.  One thread per
@@ -318,6 +211,42 @@ __global__ void __launch_bounds__(256) k_sum3(int n, const double *__restrict__ 
   }
 }
 
+// planes per block of the corrective-pass kernel (HC_MP_KC overrides)
+static int mp_chunk(int nz) {
+  static int kc = [] {
+    const char *e = getenv("HC_MP_KC");
+    int v = e ? atoi(e) : 0;
+    return v > 0 ? v : 32;
+  }();
+  return kc < nz ? kc : nz;
+}
+
+// tile shape of the corrective-pass kernel (HC_MP_TILE = 0: 32x8, 1: 16x16,
+// 2: 32x16)
+static int mp_tile() {
+  static int t = [] {
+    const char *e = getenv("HC_MP_TILE");
+    return e ? atoi(e) : 0;
+  }();
+  return t;
+}
+
+template <typename T, int TI, int TJ>
+static void launch_mpdata_tile(hipStream_t st, const Layout &L, const MpPass<T> &a) {
+  const int kc = mp_chunk(L.nz);
+  dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
+  hipLaunchKernelGGL((k_mpdata_pass<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc);
+}
+
+template <typename T>
+static void launch_mpdata_pass(hipStream_t st, const Layout &L, const MpPass<T> &a) {
+  switch (mp_tile()) {
+    case 1: launch_mpdata_tile<T, 16, 16>(st, L, a); break;
+    case 2: launch_mpdata_tile<T, 32, 16>(st, L, a); break;
+    default: launch_mpdata_tile<T, 32, 8>(st, L, a); break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 template <typename T>
 void launch_courant(hipStream_t st, const Layout &L, const StepArgs &s, const FieldPtrs<T> &f, TracerWork &tw) {
@@ -335,7 +264,6 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   Box in{1, L.nx, 1, L.ny, 1, L.nz};
   const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
   const unsigned nbc = (unsigned)((nghost + 255) / 256);
-  Box ad{0, L.nx, 0, L.ny, 0, L.nz};
   T *c1 = (T *)tw.c1, *c2 = (T *)tw.c2;
   hipLaunchKernelGGL(k_pass1<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, c, (T *)tw.uc, (T *)tw.vc,
                      (T *)tw.wc, f.ss, f.ssv, aux.azzmin, c_bkg, c1);
@@ -346,17 +274,23 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
     // this configuration is not used by the reference runs
     return;
   }
-  // pass 2
-  hipLaunchKernelGGL(k_antidiff<T>, box_grid(ad), box_block(), 0, st, L, ad, c1, (T *)tw.uc, (T *)tw.vc,
-                     (T *)tw.wc, (T *)tw.ud2, (T *)tw.vd2, (T *)tw.wd2);
-  hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 0, ta.cmin, ta.cmax, c1,
-                     (T *)tw.ud2, (T *)tw.vd2, (T *)tw.wd2, c2, (T *)nullptr, c_bkg, aux.alphae0, aux.alphaw, qpatch);
+  // pass 2: the antidiffusive velocities are kept for pass 3
+  MpPass<T> a{};
+  a.cmin = ta.cmin; a.cmax = ta.cmax;
+  a.tau3 = ta.tau3; a.dxdydz = s.dx * s.dy * s.dz;
+  a.cbkg = c_bkg; a.alphae0 = aux.alphae0; a.alphaw = aux.alphaw; a.qpatch = qpatch;
+  a.map_valid = va.map_valid; a.nxs = va.nxs; a.nxe = va.nxe; a.nys = va.nys; a.nye = va.nye;
+  a.cc = c1; a.U = (T *)tw.uc; a.V = (T *)tw.vc; a.W = (T *)tw.wc;
+  a.ud = (T *)tw.ud2; a.vd = (T *)tw.vd2; a.wd = (T *)tw.wd2;
+  a.cout = c2; a.cpre = nullptr; a.last = 0;
+  launch_mpdata_pass<T>(st, L, a);
   hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c2, c2, 0, 0.0);
-  // pass 3 (the antidiffusive velocities of pass 2 play the Courant numbers)
-  hipLaunchKernelGGL(k_antidiff<T>, box_grid(ad), box_block(), 0, st, L, ad, c2, (T *)tw.ud2, (T *)tw.vd2,
-                     (T *)tw.wd2, (T *)tw.ud, (T *)tw.vd, (T *)tw.wd);
-  hipLaunchKernelGGL(k_limited<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, va, 1, ta.cmin, ta.cmax, c2,
-                     (T *)tw.ud, (T *)tw.vd, (T *)tw.wd, c, c1, c_bkg, aux.alphae0, aux.alphaw, qpatch);
+  // pass 3 (the antidiffusive velocities of pass 2 play the Courant numbers);
+  // c1 receives the pre-sponge values of the cells next to a boundary
+  a.cc = c2; a.U = (T *)tw.ud2; a.V = (T *)tw.vd2; a.W = (T *)tw.wd2;
+  a.ud = a.vd = a.wd = nullptr;
+  a.cout = c; a.cpre = c1; a.last = 1;
+  launch_mpdata_pass<T>(st, L, a);
   hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c, c1, ta.vent, ta.vent_value);
   if (ta.budgets) {
     int nb = L.ny * L.nz;
