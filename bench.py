@@ -31,10 +31,24 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 # algorithmic HBM bytes per cell-step: 130.3 field moves (compulsory reads and
 # writes of every reference subroutine, counted once each) x 8 B in FP64
 FIELD_MOVES = 130.3
-# MPDATA corrective pass (k_antidiff + k_flags + k_limited of one tracer pass):
-# compulsory reads of c and the three face velocities, writes of the three
-# antidiffusive velocities and the result: 8 field moves per cell
-TRACER_PASS_MOVES = 8.0
+# MPDATA corrective pass, one launch of k_mpdata_pass: pass 2 reads c and the
+# three Courant numbers and writes the three antidiffusive velocities and the
+# result (8 field moves per cell); pass 3 reads c, the three antidiffusive
+# velocities and the field's entry value and writes the result (6 moves).
+# The mean over a tracer's two launches is 7 moves per cell.
+MPDATA_PASS_MOVES = 7.0
+
+
+def profiled_traffic(workload, precision):
+    """HBM bytes per launch of k_mpdata_pass from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (None if there is none)."""
+    path = os.path.join(ROOT, "profiles", "k_mpdata_pass_%s_%s_hbm.md" % (workload, precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch ="):
+            return float(line.split("=", 1)[1])
+    return None
 
 
 def init_context(H, name, precision):
@@ -147,19 +161,20 @@ def main():
     value = cells * args.steps / sec / 1e6
     ms_step = sec * 1e3 / args.steps
     esz = 8 if args.precision == "fp64" else 4
-    # roofline of the dominant kernel group: the MPDATA tracer transport
-    tr_ms, tr_n = phases.get("tracer31", (0.0, 0))
+    # roofline of the dominant kernel: the fused MPDATA corrective pass
+    # (k_mpdata_pass), timed per launch with HIP events on the context stream
     roof = None
-    if tr_n:
-        per_call = tr_ms / tr_n * 1e-3
-        # one TRACER31 call: pass 1 (7 moves) + two corrective passes (8 + 6)
-        # = 21 field moves per cell (reads of c, ss, ss_v, the Courant numbers,
-        # antidiffusive velocities; writes of the pass results)
-        alg = 21.0 * esz * cells
-        achieved = alg / per_call / 1e9
-        roof = {"bound": "hbm", "kernel": "tracer31 (MPDATA, one tracer)", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
-                "traffic": None, "avg_ms": round(per_call * 1e3, 4)}
+    k_ms, k_n = phases.get("mpdata_pass", (0.0, 0))
+    if k_n:
+        per_launch = k_ms / k_n * 1e-3
+        alg = MPDATA_PASS_MOVES * esz * cells
+        achieved = alg / per_launch / 1e9
+        roof = {"bound": "hbm", "kernel": "k_mpdata_pass (MPDATA corrective pass, mean of passes 2 and 3)",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved / (HBM_PEAK / 1e9), 4),
+                "traffic": profiled_traffic(args.workload, args.precision),
+                "avg_ms": round(per_launch * 1e3, 4), "launches": k_n,
+                "alg_bytes_per_launch": alg}
     step_bytes = FIELD_MOVES * esz * cells
     cpu = None
     if not args.no_cpu:

@@ -64,6 +64,10 @@ struct hc_ctx {
   // timing
   bool timing = false;
   std::vector<Phase> phases;
+  int mark_q = -1;
+  // multiply by the exact reciprocal of power-of-two spacings (HC_NO_P2=1
+  // at creation forces true divisions, for comparing the two forms)
+  bool allow_p2 = true;
 };
 
 #define HIPCHK(x)                                                                    \
@@ -129,6 +133,11 @@ static FieldPtrs<T> ptrs(hc_ctx *ctx) {
 static StepArgs step_args(hc_ctx *ctx) {
   StepArgs s{};
   s.dx = ctx->P.dx; s.dy = ctx->P.dy; s.dz = ctx->P.dz; s.dt = ctx->P.dt; s.tau3 = ctx->P.dt;
+  s.idx = 1.0 / s.dx; s.idy = 1.0 / s.dy; s.idz = 1.0 / s.dz;
+  s.p2 = ctx->prec == HC_FP64
+             ? (pow2_spacing<double>(s.dx) && pow2_spacing<double>(s.dy) && pow2_spacing<double>(s.dz))
+             : (pow2_spacing<float>(s.dx) && pow2_spacing<float>(s.dy) && pow2_spacing<float>(s.dz));
+  if (!ctx->allow_p2) s.p2 = 0;
   s.leap = ctx->leap; s.delta = ctx->delta; s.mtime = ctx->mtime;
   s.ubkg_m = ctx->ubkg_m; s.ubkg = ctx->ubkg; s.vbkg_m = ctx->vbkg_m; s.vbkg = ctx->vbkg;
   s.fbx = ctx->fbx; s.fby = ctx->fby; s.fc = ctx->fc; s.fc_y = ctx->fc_y;
@@ -269,11 +278,13 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
   {
     const char *e = getenv("HC_XCD_REMAP");
     box_remap_flag() = (e && e[0] == '1') ? 1 : 0;
+    const char *np2 = getenv("HC_NO_P2");
+    ctx->allow_p2 = !(np2 && np2[0] == '1');
   }
   ctx->L = make_layout(P.nx, P.ny, P.nz);
-  if (ctx->L.n >= ((size_t)1 << 32)) {
+  if (ctx->L.n >= ((size_t)1 << 31)) {
     *out = ctx;
-    return fail(ctx, HC_EINVAL, "grid too large: the padded box must hold fewer than 2^32 cells");
+    return fail(ctx, HC_EINVAL, "grid too large: the padded box must hold fewer than 2^31 cells");
   }
   *out = ctx;
   HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
@@ -720,6 +731,13 @@ extern "C" int hc_work_velocity(hc_ctx *ctx, double tau3) {
   return HC_OK;
 }
 
+// kernel timing of the corrective passes, as the phase "mpdata_pass"
+static void mark_pass(void *arg, int end) {
+  hc_ctx *ctx = (hc_ctx *)arg;
+  if (!end) ctx->mark_q = phase_begin(ctx, "mpdata_pass");
+  else phase_end(ctx, ctx->mark_q);
+}
+
 static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3, int vent,
                   double vent_value, int want_budget) {
   PhaseScope ps(ctx, "tracer31");
@@ -727,12 +745,13 @@ static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, dou
   StepArgs s = step_args(ctx);
   VentArgs va = vent_args(ctx);
   TracerArgs ta{cmin, cmax, tau3, vent, vent_value, want_budget};
+  MarkFn mk = ctx->timing ? mark_pass : nullptr;
   if (ctx->prec == HC_FP64)
     launch_tracer<double>(ctx->st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, ctx->tw,
-                          (double *)ctx->f[field], d_cbkg, ctx->d_q[which], ta);
+                          (double *)ctx->f[field], d_cbkg, ctx->d_q[which], ta, mk, ctx);
   else
     launch_tracer<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, ctx->tw,
-                         (float *)ctx->f[field], d_cbkg, ctx->d_q[which], ta);
+                         (float *)ctx->f[field], d_cbkg, ctx->d_q[which], ta, mk, ctx);
   return HC_OK;
 }
 

@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <math.h>
 #include <stdint.h>
 
 namespace hc {
@@ -23,11 +24,12 @@ struct Layout {
   size_t n;        // elements per field
 };
 
-// Element index of (i,j,k).  Computed in 32 bits: a context refuses grids
-// whose padded box holds 2^32 elements or more, so the index always fits,
-// and 32-bit arithmetic keeps the neighbour offsets of a stencil cheap.
-__host__ __device__ inline size_t IX(const Layout &L, int i, int j, int k) {
-  return (size_t)((unsigned)(i + OXP) + (unsigned)L.px * ((unsigned)(j + 1) + (unsigned)L.py * (unsigned)(k + 1)));
+// Element index of (i,j,k).  Computed in signed 32-bit arithmetic: a
+// context refuses grids whose padded box holds 2^31 elements or more, so the
+// index never overflows, and the compiler may fold a stencil's constant
+// neighbour offsets into the loads.
+__host__ __device__ inline int IX(const Layout &L, int i, int j, int k) {
+  return (i + OXP) + L.px * ((j + 1) + L.py * (k + 1));
 }
 
 inline Layout make_layout(int nx, int ny, int nz) {
@@ -43,6 +45,8 @@ inline Layout make_layout(int nx, int ny, int nz) {
 // Scalars of one timestep that kernels take by value.
 struct StepArgs {
   double dx, dy, dz, dt, tau3;
+  double idx, idy, idz;  // 1/dx, 1/dy, 1/dz
+  int p2;                // dx, dy, dz are powers of two (exact reciprocals)
   int leap, delta, mtime;
   double ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby;
   double fc, fc_y;
@@ -95,5 +99,30 @@ inline dim3 box_block() { return dim3(64, 4, 1); }
   if (i > (b).i1 || j > (b).j1) return;
 
 constexpr double MISSING = -1.0e+34;
+
+// A grid spacing as a divisor.  x / d for a power of two d equals x * (1/d)
+// exactly (both are one correctly rounded operation on the same real
+// number), so kernels instantiated with P2 = true multiply instead of
+// dividing; the results are bit-identical to the division.
+template <typename T, bool P2>
+struct Sp {
+  T d, inv;
+  __host__ __device__ Sp(T d_, T inv_) : d(d_), inv(inv_) {}
+};
+template <typename T, bool P2>
+__host__ __device__ inline T operator/(T x, const Sp<T, P2> &s) {
+  return P2 ? x * s.inv : x / s.d;
+}
+
+// d > 0 is a power of two whose reciprocal is a normal number of type T
+template <typename T>
+inline bool pow2_spacing(double d) {
+  T t = (T)d;
+  if (!(t > (T)0)) return false;
+  int e;
+  double m = frexp((double)t, &e);
+  T inv = (T)1 / t;
+  return m == 0.5 && inv * t == (T)1 && inv > (T)0 && (double)inv < 1e30;
+}
 
 }  // namespace hc

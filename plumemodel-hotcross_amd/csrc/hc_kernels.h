@@ -59,10 +59,13 @@ struct TracerArgs {
 };
 template <typename T>
 void launch_courant(hipStream_t st, const Layout &L, const StepArgs &s, const FieldPtrs<T> &f, TracerWork &tw);
+// optional hook around each launch of the corrective-pass kernel (for kernel
+// timing with events on the stream): called with 0 before and 1 after
+typedef void (*MarkFn)(void *arg, int end);
 template <typename T>
 void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const VentArgs &va,
                    const FieldPtrs<T> &f, const Aux &aux, TracerWork &tw, T *c, const double *c_bkg,
-                   const double *qpatch, const TracerArgs &ta);
+                   const double *qpatch, const TracerArgs &ta, MarkFn mark = nullptr, void *mark_arg = nullptr);
 template <typename T>
 void launch_vent_ghost(hipStream_t st, const Layout &L, const VentArgs &va, T *c, double value);
 template <typename T>

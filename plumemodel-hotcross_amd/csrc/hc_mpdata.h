@@ -185,8 +185,8 @@ HC_HD inline T mp_donor(T cc, T cxm, T cxp, T cym, T cyp, T czm, T czp, T u0, T 
 // Work of a block per plane, as flat lists over the block's threads:
 //   faces: ud on i in [i0-2, i0+TI] x j in [j0-1, j0+TJ], vd on
 //          [i0-1, i0+TI] x [j0-2, j0+TJ], wd on [i0-1, i0+TI] x [j0-1, j0+TJ],
-//          each list padded to a multiple of 64 so every wavefront works on
-//          one kind of face;
+//          each list laid over whole LDS rows and padded to a multiple of 64
+//          so every wavefront works on one kind of face;
 //   flags: cells [i0-1, i0+TI] x [j0-1, j0+TJ];
 //   finish: one tile cell per thread.
 // A thread's items are the same for every plane, so their LDS indexes and
@@ -197,15 +197,18 @@ struct MpTile {
   typedef MpShared<T, TI, TJ> Sh;
   static constexpr int SI = Sh::SI, SJ = Sh::SJ, S = Sh::S, NT = TI * TJ;
   static constexpr int pad64(int n) { return (n + 63) / 64 * 64; }
-  static constexpr int NIU = TI + 3, NJU = TJ + 2, NIV = TI + 2, NJV = TJ + 3, NIW = TI + 2, NJW = TJ + 2;
-  static constexpr int LU = pad64(NIU * NJU), LV = pad64(NIV * NJV), LW = pad64(NIW * NJW);
+  // every list runs over whole LDS rows (SI wide), so the LDS index of an
+  // item is the list's base plus its position: the lanes of a wavefront read
+  // consecutive LDS words.  The columns a list does not need are masked.
+  static constexpr int NJU = TJ + 2, NJV = TJ + 3, NJW = TJ + 2;  // rows: ud j0-1.., vd j0-2.., wd j0-1..
+  static constexpr int LU = pad64(SI * NJU), LV = pad64(SI * NJV), LW = pad64(SI * NJW);
   static constexpr int NF = LU + LV + LW, RF = (NF + NT - 1) / NT;
-  static constexpr int NIF = TI + 2, NJF = TJ + 2, NFL = NIF * NJF, RFL = (NFL + NT - 1) / NT;
+  static constexpr int NJF = TJ + 2, NFL = SI * NJF, RFL = (NFL + NT - 1) / NT;
   static constexpr int RS = (S + NT - 1) / NT;
   static constexpr int INTERIOR = 1 << 30;
 
   struct Plan {
-    int fp[RF];   // face item: LDS index, or -1
+    int fp[RF];   // face item: LDS index | kind << 28 (0 ud, 1 vd, 2 wd), or -1
     int fo[RF];   // face item: global (i,j) offset of an owned face, or -1
     int lp[RFL];  // flag cell: LDS index | INTERIOR, or -1
     int so[RS];   // staged element: global (i,j) offset, or -1 outside the box
@@ -217,35 +220,40 @@ struct MpTile {
   HC_HD static int ij(const MpGeom &g, int i, int j) { return i + 16 + g.px * (j + 1); }
   HC_HD static long plane(const MpGeom &g, int k) { return (long)g.px * g.py * (long)(k + 1); }
 
-  HC_HD static void plan(const MpGeom &g, int tid, Plan &pl) {
+  // rot rotates the assignment of face and flag items to wavefronts, so
+  // that the waves that carry an extra round differ from block to block
+  HC_HD static void plan(const MpGeom &g, int tid, int rot, Plan &pl) {
+    const int vt = (tid + NT - 64 * (rot % (NT / 64))) % NT;
     for (int r = 0; r < RF; ++r) {
-      int e = tid + r * NT, p = -1, o = -1;
-      if (e < LU) {  // ud
-        const int i = g.i0 - 2 + e % NIU, j = g.j0 - 1 + e / NIU;
-        if (e < NIU * NJU && i >= 0 && i <= g.nx && j >= 1 && j <= g.ny) {
+      int e = vt + r * NT, p = -1, o = -1;
+      if (e < LU) {  // ud: faces i in [i0-2, i0+TI], rows [j0-1, j0+TJ]
+        const int i = g.i0 - 2 + e % SI, j = g.j0 - 1 + e / SI;
+        if (e < SI * NJU && i <= g.i0 + TI && i >= 0 && i <= g.nx && j >= 1 && j <= g.ny) {
           p = at(g, i, j);
           if (j < g.j0 + TJ && ((i >= g.i0 && i < g.i0 + TI) || (i == 0 && g.i0 == 1))) o = ij(g, i, j);
         }
-      } else if ((e -= LU) < LV) {  // vd
-        const int i = g.i0 - 1 + e % NIV, j = g.j0 - 2 + e / NIV;
-        if (e < NIV * NJV && i >= 1 && i <= g.nx && j >= 0 && j <= g.ny) {
+      } else if ((e -= LU) < LV) {  // vd: columns [i0-1, i0+TI], faces j in [j0-2, j0+TJ]
+        const int i = g.i0 - 2 + e % SI, j = g.j0 - 2 + e / SI;
+        if (e < SI * NJV && i >= g.i0 - 1 && i <= g.i0 + TI && i >= 1 && i <= g.nx && j >= 0 && j <= g.ny) {
           p = at(g, i, j);
           if (i < g.i0 + TI && i >= g.i0 && ((j >= g.j0 && j < g.j0 + TJ) || (j == 0 && g.j0 == 1))) o = ij(g, i, j);
         }
-      } else if ((e -= LV) < LW) {  // wd
-        const int i = g.i0 - 1 + e % NIW, j = g.j0 - 1 + e / NIW;
-        if (e < NIW * NJW && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) {
+      } else if ((e -= LV) < LW) {  // wd: columns [i0-1, i0+TI], rows [j0-1, j0+TJ]
+        const int i = g.i0 - 2 + e % SI, j = g.j0 - 1 + e / SI;
+        if (e < SI * NJW && i >= g.i0 - 1 && i <= g.i0 + TI && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) {
           p = at(g, i, j);
           if (i >= g.i0 && i < g.i0 + TI && j >= g.j0 && j < g.j0 + TJ) o = ij(g, i, j);
         }
       }
-      pl.fp[r] = p;
+      const int kind = (vt + r * NT) < LU ? 0 : (vt + r * NT) < LU + LV ? 1 : 2;
+      pl.fp[r] = p < 0 ? -1 : (p | (kind << 28));
       pl.fo[r] = o;
     }
-    for (int r = 0; r < RFL; ++r) {
-      const int e = tid + r * NT;
-      const int i = g.i0 - 1 + e % NIF, j = g.j0 - 1 + e / NIF;
-      pl.lp[r] = e < NFL ? (at(g, i, j) | ((i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) ? INTERIOR : 0)) : -1;
+    for (int r = 0; r < RFL; ++r) {  // flags: cells [i0-1, i0+TI] x [j0-1, j0+TJ]
+      const int e = vt + r * NT;
+      const int i = g.i0 - 2 + e % SI, j = g.j0 - 1 + e / SI;
+      const bool need = e < NFL && i >= g.i0 - 1 && i <= g.i0 + TI;
+      pl.lp[r] = need ? (at(g, i, j) | ((i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) ? INTERIOR : 0)) : -1;
     }
     for (int r = 0; r < RS; ++r) {
       const int e = tid + r * NT;
@@ -319,8 +327,7 @@ struct MpTile {
   }
 
   // phase A: ud, vd, wd of plane q on the face ranges the flags need
-  HC_HD static void faces(const MpGeom &g, int tid, const Plan &pl, Sh &sh, const MpPass<T> &a, int q,
-                          bool full) {
+  HC_HD static void faces(const MpGeom &g, const Plan &pl, Sh &sh, const MpPass<T> &a, int q, bool full) {
     const bool quv = full && q >= 1 && q <= g.nz;
     const bool qw = q >= 0 && q <= g.nz;
     const bool wuv = q >= g.ka && q <= g.kb;
@@ -332,9 +339,9 @@ struct MpTile {
     const long pq = plane(g, q);
 #pragma unroll
     for (int r = 0; r < RF; ++r) {
-      const int e = tid + r * NT;
-      const int p = pl.fp[r];
-      if (p < 0) continue;
+      const int fp = pl.fp[r];
+      if (fp < 0) continue;
+      const int p = fp & ((1 << 28) - 1), kind = fp >> 28;
       auto C = [&](int di, int dj, int dk) {
         const T *b = dk < 0 ? cm : (dk > 0 ? cq : c0);
         return b[HC_MP_CHECK(p + di + SI * dj)];
@@ -342,13 +349,13 @@ struct MpTile {
       auto Uf = [&](int di, int dj, int dk) { return (dk > 0 ? u1 : u0)[HC_MP_CHECK(p + di + SI * dj)]; };
       auto Vf = [&](int di, int dj, int dk) { return (dk > 0 ? v1 : v0)[HC_MP_CHECK(p + di + SI * dj)]; };
       auto Wf = [&](int di, int dj, int dk) { return (dk < 0 ? wm : w0)[HC_MP_CHECK(p + di + SI * dj)]; };
-      if (e < LU) {
+      if (kind == 0) {
         if (quv) {
           T v = mp_anti_u<T>(C, Uf, Vf, Wf);
           sh.ud[p] = v;
           if (a.ud && wuv && pl.fo[r] >= 0) (a.ud + pq)[pl.fo[r]] = v;
         }
-      } else if (e < LU + LV) {
+      } else if (kind == 1) {
         if (quv) {
           T v = mp_anti_v<T>(C, Uf, Vf, Wf);
           sh.vd[p] = v;

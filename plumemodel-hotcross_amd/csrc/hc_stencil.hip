@@ -44,14 +44,14 @@ __device__ inline T azv_at(const Layout &L, const T *ssv, const double *azzmin,
   return v;
 }
 
-template <typename T>
+template <typename T, bool P2>
 __device__ inline void forces_x_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
                                                   const T *__restrict__ v, const T *__restrict__ w,
                                                   const T *__restrict__ um, const T *__restrict__ vm,
                                                   const T *__restrict__ wm, const T *__restrict__ ss,
                                                   const T *__restrict__ ssv, const double *__restrict__ azzmin,
                                                   const double *__restrict__ alphau, T *__restrict__ fx) {
-  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
   const T axmin = (T)ph.axmin, aymin = (T)ph.aymin;
   const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
@@ -81,14 +81,14 @@ __device__ inline void forces_x_at(const Layout &L, int i, int j, int k, const S
   F3(fx, i, j, k) = r;
 }
 
-template <typename T>
+template <typename T, bool P2>
 __device__ inline void forces_y_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
                                                   const T *__restrict__ v, const T *__restrict__ w,
                                                   const T *__restrict__ um, const T *__restrict__ vm,
                                                   const T *__restrict__ wm, const T *__restrict__ ss,
                                                   const T *__restrict__ ssv, const double *__restrict__ azzmin,
                                                   const double *__restrict__ alphav, T *__restrict__ fy) {
-  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
   const T axmin = (T)ph.axmin, aymin = (T)ph.aymin;
   const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
@@ -117,7 +117,7 @@ __device__ inline void forces_y_at(const Layout &L, int i, int j, int k, const S
   F3(fy, i, j, k) = r;
 }
 
-template <typename T>
+template <typename T, bool P2>
 __device__ inline void forces_z_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
                                                   const T *__restrict__ v, const T *__restrict__ w,
                                                   const T *__restrict__ um, const T *__restrict__ vm,
@@ -125,7 +125,7 @@ __device__ inline void forces_z_at(const Layout &L, int i, int j, int k, const S
                                                   const T *__restrict__ ssv, const T *__restrict__ rhop,
                                                   const double *__restrict__ azzmin, const double *__restrict__ alphae,
                                                   const double *__restrict__ alphaw, T *__restrict__ fz) {
-  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
   const T axmin = (T)ph.axmin;
   const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
@@ -162,7 +162,7 @@ __device__ inline void forces_z_at(const Layout &L, int i, int j, int k, const S
 
 // one pass over the union of the three output boxes: the inputs shared by
 // fx, fy and fz are read from HBM once per block
-template <typename T>
+template <typename T, bool P2>
 __global__ void __launch_bounds__(256) k_forces(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
                                                 const T *__restrict__ v, const T *__restrict__ w,
                                                 const T *__restrict__ um, const T *__restrict__ vm,
@@ -174,11 +174,11 @@ __global__ void __launch_bounds__(256) k_forces(Layout L, Box b, StepArgs s, Phy
                                                 T *__restrict__ fy, T *__restrict__ fz) {
   HC_BOX_IJK(b);  // i = 0..nx, j = 0..ny, k = 0..nz
   if (j >= 1 && k >= 1)
-    forces_x_at<T>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphau, fx);
+    forces_x_at<T, P2>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphau, fx);
   if (i >= 1 && k >= 1)
-    forces_y_at<T>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphav, fy);
+    forces_y_at<T, P2>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphav, fy);
   if (i >= 1 && j >= 1)
-    forces_z_at<T>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, rhop, azzmin, alphae0, alphaw, fz);
+    forces_z_at<T, P2>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, rhop, azzmin, alphae0, alphaw, fz);
 }
 
 // ---------------------------------------------------------------------------
@@ -186,14 +186,14 @@ __global__ void __launch_bounds__(256) k_forces(Layout L, Box b, StepArgs s, Phy
 // value of one cell of the written set, re-evaluating the interior formula at
 // the source cell of a boundary copy (a pure function of the inputs), so one
 // pass writes interior and ghosts with the reference's values.
-template <typename T>
+template <typename T, bool P2>
 __device__ inline T mom_u_val(const Layout &L, const StepArgs &s, const T *u, const T *um, const T *p,
                               const T *fx, const double *pgradx, int i, int j, int k) {
   return (T)(1 - s.delta) * F3(u, i, j, k) + (T)s.delta * F3(um, i, j, k) +
-         (T)s.dt * (T)s.leap * (-(F3(p, i + 1, j, k) - F3(p, i, j, k)) / (T)s.dx - (T)pgradx[k] + F3(fx, i, j, k));
+         (T)s.dt * (T)s.leap * (-(F3(p, i + 1, j, k) - F3(p, i, j, k)) / Sp<T, P2>((T)s.dx, (T)s.idx) - (T)pgradx[k] + F3(fx, i, j, k));
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ void __launch_bounds__(256) k_momentum_u(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
                                                     const T *__restrict__ um, const T *__restrict__ p,
                                                     const T *__restrict__ fx, const double *__restrict__ pgradx,
@@ -204,26 +204,26 @@ __global__ void __launch_bounds__(256) k_momentum_u(Layout L, Box b, StepArgs s,
   if ((int)gi + (int)gj + (int)gk > 1) return;  // edges keep -1e34
   T val;
   if (!gi && !gj && !gk) {
-    val = mom_u_val(L, s, u, um, p, fx, pgradx, i, j, k);
+    val = mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i, j, k);
   } else if (gk) {
-    if (k == 0) val = mom_u_val(L, s, u, um, p, fx, pgradx, i, j, 1);
-    else val = (T)ph.bottom_slip * mom_u_val(L, s, u, um, p, fx, pgradx, i, j, nz);
+    if (k == 0) val = mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i, j, 1);
+    else val = (T)ph.bottom_slip * mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i, j, nz);
   } else if (gj) {
-    val = mom_u_val(L, s, u, um, p, fx, pgradx, i, j == 0 ? 1 : ny, k);
+    val = mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i, j == 0 ? 1 : ny, k);
   } else {
-    val = mom_u_val(L, s, u, um, p, fx, pgradx, i == -1 ? 0 : nx, j, k);
+    val = mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i == -1 ? 0 : nx, j, k);
   }
   F3(up, i, j, k) = val;
 }
 
-template <typename T>
+template <typename T, bool P2>
 __device__ inline T mom_v_val(const Layout &L, const StepArgs &s, const T *v, const T *vm, const T *p,
                               const T *fy, const double *pgrady, int i, int j, int k) {
   return (T)(1 - s.delta) * F3(v, i, j, k) + (T)s.delta * F3(vm, i, j, k) +
-         (T)s.dt * (T)s.leap * (-(F3(p, i, j + 1, k) - F3(p, i, j, k)) / (T)s.dy - (T)pgrady[k] + F3(fy, i, j, k));
+         (T)s.dt * (T)s.leap * (-(F3(p, i, j + 1, k) - F3(p, i, j, k)) / Sp<T, P2>((T)s.dy, (T)s.idy) - (T)pgrady[k] + F3(fy, i, j, k));
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ void __launch_bounds__(256) k_momentum_v(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ v,
                                                     const T *__restrict__ vm, const T *__restrict__ p,
                                                     const T *__restrict__ fy, const double *__restrict__ pgrady,
@@ -234,23 +234,23 @@ __global__ void __launch_bounds__(256) k_momentum_v(Layout L, Box b, StepArgs s,
   if ((int)gi + (int)gj + (int)gk > 1) return;
   T val;
   if (!gi && !gj && !gk) {
-    val = mom_v_val(L, s, v, vm, p, fy, pgrady, i, j, k);
+    val = mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i, j, k);
   } else if (gk) {
-    if (k == 0) val = mom_v_val(L, s, v, vm, p, fy, pgrady, i, j, 1);
-    else val = (T)ph.bottom_slip * mom_v_val(L, s, v, vm, p, fy, pgrady, i, j, nz);
+    if (k == 0) val = mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i, j, 1);
+    else val = (T)ph.bottom_slip * mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i, j, nz);
   } else if (gi) {
-    val = mom_v_val(L, s, v, vm, p, fy, pgrady, i == 0 ? 1 : nx, j, k);
+    val = mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i == 0 ? 1 : nx, j, k);
   } else {
-    val = mom_v_val(L, s, v, vm, p, fy, pgrady, i, j == -1 ? 0 : ny, k);
+    val = mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i, j == -1 ? 0 : ny, k);
   }
   F3(vp, i, j, k) = val;
 }
 
-template <typename T>
+template <typename T, bool P2>
 __device__ inline T mom_w_val(const Layout &L, const StepArgs &s, const T *w, const T *wm, const T *p,
                               const T *fz, int i, int j, int k) {
   return (T)(1 - s.delta) * F3(w, i, j, k) + (T)s.delta * F3(wm, i, j, k) +
-         (T)s.leap * (T)s.dt * (-(F3(p, i, j, k + 1) - F3(p, i, j, k)) / (T)s.dz + F3(fz, i, j, k));
+         (T)s.leap * (T)s.dt * (-(F3(p, i, j, k + 1) - F3(p, i, j, k)) / Sp<T, P2>((T)s.dz, (T)s.idz) + F3(fz, i, j, k));
 }
 
 // vent value of wp at the bottom (k = nz, nz+1), :821-857
@@ -264,7 +264,7 @@ __device__ inline T w_vent(const VentArgs &va, const double *alphae, const Layou
   return (T)0.0;
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s, Phys ph, VentArgs va,
                                                     const T *__restrict__ w, const T *__restrict__ wm,
                                                     const T *__restrict__ p, const T *__restrict__ fz,
@@ -274,13 +274,13 @@ __global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s,
   bool ii = (i >= 1 && i <= nx), jj = (j >= 1 && j <= ny);
   auto inner = [&](int i2, int j2, int k2) -> T {
     // wp(i2,j2,k2) for i2,j2 interior after the interior update and bottom/top rows
-    if (k2 >= 1 && k2 <= nz - 1) return mom_w_val(L, s, w, wm, p, fz, i2, j2, k2);
+    if (k2 >= 1 && k2 <= nz - 1) return mom_w_val<T, P2>(L, s, w, wm, p, fz, i2, j2, k2);
     if (k2 == nz) return w_vent<T>(va, alphae, L, i2, j2);
     return (T)0.0;  // k2 = 0
   };
   T val;
   if (ii && jj) {
-    if (k >= 1 && k <= nz - 1) val = mom_w_val(L, s, w, wm, p, fz, i, j, k);
+    if (k >= 1 && k <= nz - 1) val = mom_w_val<T, P2>(L, s, w, wm, p, fz, i, j, k);
     else if (k == nz || k == nz + 1) val = w_vent<T>(va, alphae, L, i, j);
     else val = (T)0.0;  // k = -1, 0
   } else if (ii && (j == 0 || j == ny + 1) && k >= 0 && k <= nz) {
@@ -302,10 +302,10 @@ __global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s,
 // over its whole box (the reference zeroes it first, :24); ss_v over its
 // interior and z ghosts.  The complex root at :112 keeps its real part,
 // sqrt(max(0,x)).
-template <typename T>
+template <typename T, bool P2>
 __device__ inline T ss_val(const Layout &L, const Phys &ph, const StepArgs &s, const T *um, const T *vm,
                            const T *wm, int i, int j, int k) {
-  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   double ls = sqrt(s.dx * s.dy);
   T coef = (T)(ph.ismag_h * ((ph.cs_h * ls) * (ph.cs_h * ls)));
   T s11 = (F3(um, i, j, k) - F3(um, i - 1, j, k)) / dx;
@@ -319,10 +319,10 @@ __device__ inline T ss_val(const Layout &L, const Phys &ph, const StepArgs &s, c
   return coef * sqrt(q + (T)1.0e-30);
 }
 
-template <typename T>
+template <typename T, bool P2>
 __device__ inline T ssv_val(const Layout &L, const Phys &ph, const StepArgs &s, const T *um, const T *vm,
                             const T *wm, const T *rhop, const double *rho_bkg, int i, int j, int k) {
-  const T dx = (T)s.dx, dy = (T)s.dy, dz = (T)s.dz;
+  const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   T coef = (T)(ph.ismag_v * ((ph.cs_v * s.dz) * (ph.cs_v * s.dz)));
   T s13 = (T)0.25 *
           ((F3(um, i, j, k - 1) + F3(um, i - 1, j, k - 1) - F3(um, i, j, k + 1) - F3(um, i - 1, j, k + 1)) / dz +
@@ -337,12 +337,12 @@ __device__ inline T ssv_val(const Layout &L, const Phys &ph, const StepArgs &s, 
   // ~1e-4 of that
   double rkm = (double)F3(rhop, i, j, k - 1) + rho_bkg[k - 1];
   double rkp = (double)F3(rhop, i, j, k + 1) + rho_bkg[k + 1];
-  double ri = -(ph.g * (rkm - rkp) / 2.0 / s.dz / ph.rho0 / ((double)sv + 1.0e-30));
+  double ri = -(ph.g * (rkm - rkp) / 2.0 / Sp<double, P2>(s.dz, s.idz) / ph.rho0 / ((double)sv + 1.0e-30));
   double arg = 1.0 + ph.eps - ri / 3.0;
   return coef * sqrt(sv + (T)1.0e-30) * (T)sqrt(fmax(0.0, arg));
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ void __launch_bounds__(256) k_viscosity(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ um,
                                                    const T *__restrict__ vm, const T *__restrict__ wm,
                                                    const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
@@ -354,24 +354,24 @@ __global__ void __launch_bounds__(256) k_viscosity(Layout L, Box b, StepArgs s, 
   // ss
   T sval = (T)0;
   if (ng == 0) {
-    sval = ss_val(L, ph, s, um, vm, wm, i, j, k);
+    sval = ss_val<T, P2>(L, ph, s, um, vm, wm, i, j, k);
   } else if (ng == 1 && gi) {
     const T bx = (T)ph.bcondx;
-    T a = ss_val(L, ph, s, um, vm, wm, i == 0 ? 1 : nx, j, k);
-    T c = ss_val(L, ph, s, um, vm, wm, i == 0 ? nx : 1, j, k);
+    T a = ss_val<T, P2>(L, ph, s, um, vm, wm, i == 0 ? 1 : nx, j, k);
+    T c = ss_val<T, P2>(L, ph, s, um, vm, wm, i == 0 ? nx : 1, j, k);
     sval = a * bx + (T)(1 - ph.bcondx) * c;
   } else if (ng == 1 && gj) {
     const T by = (T)ph.bcondy;
-    T a = ss_val(L, ph, s, um, vm, wm, i, j == 0 ? 1 : ny, k);
-    T c = ss_val(L, ph, s, um, vm, wm, i, j == 0 ? ny : 1, k);
+    T a = ss_val<T, P2>(L, ph, s, um, vm, wm, i, j == 0 ? 1 : ny, k);
+    T c = ss_val<T, P2>(L, ph, s, um, vm, wm, i, j == 0 ? ny : 1, k);
     sval = a * by + (T)(1 - ph.bcondy) * c;
   }
   F3(ss, i, j, k) = sval;
   // ss_v
   if (ng == 0) {
-    F3(ssv, i, j, k) = ssv_val(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k);
+    F3(ssv, i, j, k) = ssv_val<T, P2>(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k);
   } else if (ng == 1 && gk) {
-    F3(ssv, i, j, k) = ssv_val(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k == 0 ? 1 : nz);
+    F3(ssv, i, j, k) = ssv_val<T, P2>(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k == 0 ? 1 : nz);
   }
 }
 
@@ -435,13 +435,19 @@ __global__ void __launch_bounds__(256) k_refresh(size_t n, int leap, double asse
 
 // ---------------------------------------------------------------------------
 // host launchers
+// launch the power-of-two-spacing instantiation when s.p2 is set
+#define HC_P2_LAUNCH(K, ...)                                   \
+  do {                                                         \
+    if (s.p2) hipLaunchKernelGGL((K<T, true>), __VA_ARGS__);   \
+    else hipLaunchKernelGGL((K<T, false>), __VA_ARGS__);       \
+  } while (0)
+
 template <typename T>
 void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                    const Aux &aux) {
   Box b{0, L.nx, 0, L.ny, 0, L.nz};
-  hipLaunchKernelGGL(k_forces<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm,
-                     f.ss, f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy,
-                     f.fz);
+  HC_P2_LAUNCH(k_forces, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm, f.ss,
+               f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy, f.fz);
 }
 
 template <typename T>
@@ -449,15 +455,15 @@ void launch_momentum(hipStream_t st, int comp, const Layout &L, const StepArgs &
                      const VentArgs &va, const FieldPtrs<T> &f, const Aux &aux) {
   if (comp == 0) {
     Box b{-1, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
-    hipLaunchKernelGGL(k_momentum_u<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.um, f.p, f.fx,
+    HC_P2_LAUNCH(k_momentum_u, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.um, f.p, f.fx,
                        aux.pgradx, f.up);
   } else if (comp == 1) {
     Box b{0, L.nx + 1, -1, L.ny + 1, 0, L.nz + 1};
-    hipLaunchKernelGGL(k_momentum_v<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.v, f.vm, f.p, f.fy,
+    HC_P2_LAUNCH(k_momentum_v, box_grid(b), box_block(), 0, st, L, b, s, ph, f.v, f.vm, f.p, f.fy,
                        aux.pgrady, f.vp);
   } else {
     Box b{0, L.nx + 1, 0, L.ny + 1, -1, L.nz + 1};
-    hipLaunchKernelGGL(k_momentum_w<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, va, f.w, f.wm, f.p, f.fz,
+    HC_P2_LAUNCH(k_momentum_w, box_grid(b), box_block(), 0, st, L, b, s, ph, va, f.w, f.wm, f.p, f.fz,
                        aux.alphae, f.wp);
   }
 }
@@ -466,7 +472,7 @@ template <typename T>
 void launch_viscosity(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                       const Aux &aux) {
   Box b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
-  hipLaunchKernelGGL(k_viscosity<T>, box_grid(b), box_block(), 0, st, L, b, s, ph, f.um, f.vm, f.wm, f.rhop,
+  HC_P2_LAUNCH(k_viscosity, box_grid(b), box_block(), 0, st, L, b, s, ph, f.um, f.vm, f.wm, f.rhop,
                      aux.rho_bkg, f.ss, f.ssv);
 }
 

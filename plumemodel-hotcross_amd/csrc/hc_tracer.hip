@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, i
   MpRegs<T> r{};
   typename Tl::Plan pl;
   typename Tl::Prefetch pf;
-  Tl::plan(g, tid, pl);
+  Tl::plan(g, tid, (int)(blockIdx.x + blockIdx.y + blockIdx.z), pl);
   Tl::stage_initial(g, tid, pl, sh, a, qs);
   Tl::fetch(g, pl, a, qs, pf);
   for (int q = qs; q <= g.kb + 1; ++q) {
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, i
     Tl::commit(tid, sh, q, pf);
     __syncthreads();
     if (q <= g.kb) Tl::fetch(g, pl, a, q + 1, pf);  // in flight during this plane
-    Tl::faces(g, tid, pl, sh, a, q, q >= g.ka - 1);
+    Tl::faces(g, pl, sh, a, q, q >= g.ka - 1);
     __syncthreads();
     if (q >= g.ka - 1) Tl::flags(g, pl, sh, a, q);
     __syncthreads();
@@ -258,7 +258,7 @@ void launch_courant(hipStream_t st, const Layout &L, const StepArgs &s, const Fi
 template <typename T>
 void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const VentArgs &va,
                    const FieldPtrs<T> &f, const Aux &aux, TracerWork &tw, T *c, const double *c_bkg,
-                   const double *qpatch, const TracerArgs &ta) {
+                   const double *qpatch, const TracerArgs &ta, MarkFn mark, void *mark_arg) {
   StepArgs s2 = s;
   s2.tau3 = ta.tau3;
   Box in{1, L.nx, 1, L.ny, 1, L.nz};
@@ -283,14 +283,18 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   a.cc = c1; a.U = (T *)tw.uc; a.V = (T *)tw.vc; a.W = (T *)tw.wc;
   a.ud = (T *)tw.ud2; a.vd = (T *)tw.vd2; a.wd = (T *)tw.wd2;
   a.cout = c2; a.cpre = nullptr; a.last = 0;
+  if (mark) mark(mark_arg, 0);
   launch_mpdata_pass<T>(st, L, a);
+  if (mark) mark(mark_arg, 1);
   hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c2, c2, 0, 0.0);
   // pass 3 (the antidiffusive velocities of pass 2 play the Courant numbers);
   // c1 receives the pre-sponge values of the cells next to a boundary
   a.cc = c2; a.U = (T *)tw.ud2; a.V = (T *)tw.vd2; a.W = (T *)tw.wd2;
   a.ud = a.vd = a.wd = nullptr;
   a.cout = c; a.cpre = c1; a.last = 1;
+  if (mark) mark(mark_arg, 0);
   launch_mpdata_pass<T>(st, L, a);
+  if (mark) mark(mark_arg, 1);
   hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c, c1, ta.vent, ta.vent_value);
   if (ta.budgets) {
     int nb = L.ny * L.nz;
@@ -365,7 +369,7 @@ void launch_cfl(hipStream_t st, const Layout &L, const FieldPtrs<T> &f, double *
                                   TracerWork &);                                                              \
   template void launch_tracer<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,                 \
                                  const VentArgs &, const FieldPtrs<T> &, const Aux &, TracerWork &, T *,      \
-                                 const double *, const double *, const TracerArgs &);                         \
+                                 const double *, const double *, const TracerArgs &, MarkFn, void *);         \
   template void launch_vent_ghost<T>(hipStream_t, const Layout &, const VentArgs &, T *, double);             \
   template void launch_cfl<T>(hipStream_t, const Layout &, const FieldPtrs<T> &, double *, double *);
 HC_INST_T(double)
