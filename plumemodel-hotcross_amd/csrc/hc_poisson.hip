@@ -23,6 +23,8 @@
 // The transforms are orthonormal DCT-II/III computed as one complex FFT per
 // pair of real sequences (Makhoul's reordering), mixed radix 4/2/3 Stockham
 // in LDS with an FP64 twiddle table.
+#include <stdlib.h>
+
 #include "hc_common.h"
 #include "hc_kernels.h"
 
@@ -435,11 +437,332 @@ __global__ void __launch_bounds__(256) k_pyghost(Layout L, StepArgs s, Phys ph, 
 }
 
 // ---------------------------------------------------------------------------
+// Transforms of compile-time length N (the sizes listed in FAST_SIZES).
+// Same mathematics as the kernels above, written for the hardware: the radix
+// plan, every index split and every divisor are compile-time constants; the
+// twiddle and DCT rotation tables sit in LDS; each Stockham stage runs in
+// place (all of a stage's reads into registers, a barrier, then its writes),
+// so a block holds one sequence buffer; global reads and writes run in
+// memory order with the Makhoul permutation applied on the LDS side.
+constexpr int fft_nst(int n) {
+  int s = 0;
+  while (n % 4 == 0) { n /= 4; ++s; }
+  while (n % 2 == 0) { n /= 2; ++s; }
+  while (n % 3 == 0) { n /= 3; ++s; }
+  return n == 1 ? s : -1;
+}
+constexpr int fft_radix(int n, int st) {
+  int s = 0;
+  while (n % 4 == 0) { if (s == st) return 4; n /= 4; ++s; }
+  while (n % 2 == 0) { if (s == st) return 2; n /= 2; ++s; }
+  while (n % 3 == 0) { if (s == st) return 3; n /= 3; ++s; }
+  return 1;
+}
+
+template <typename T, int R>
+__device__ inline void butterfly(cpx<T> *v, cpx<T> *y, int sign) {
+  if constexpr (R == 2) {
+    y[0] = cadd(v[0], v[1]);
+    y[1] = csub(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    cpx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+    cpx<T> t3 = sign < 0 ? cpx<T>{d.y, -d.x} : cpx<T>{-d.y, d.x};  // d * (sign * i)
+    y[0] = cadd(t0, t2);
+    y[2] = csub(t0, t2);
+    y[1] = cadd(t1, t3);
+    y[3] = csub(t1, t3);
+  } else {  // R == 3
+    const T c = (T)-0.5, sn = (T)(sign * 0.86602540378443864676);
+    cpx<T> s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
+    cpx<T> t1{v[0].x + c * s12.x, v[0].y + c * s12.y};
+    cpx<T> rot{-sn * d12.y, sn * d12.x};  // i * sn * d12
+    y[0] = cadd(v[0], s12);
+    y[1] = cadd(t1, rot);
+    y[2] = csub(t1, rot);
+  }
+}
+
+// stages ST.. of an in-place Stockham FFT of SEQ sequences of length N in
+// buf, NT threads; NS = product of the radices already applied
+template <typename T, int N, int SEQ, int NT, int ST, int NS>
+__device__ inline void fft_stages(cpx<T> *buf, const cpx<T> *tw, int sign) {
+  if constexpr (ST < fft_nst(N)) {
+    constexpr int R = fft_radix(N, ST);
+    constexpr int NB = N / R, STEP = N / (NS * R);
+    constexpr int TOT = SEQ * NB, NBT = (TOT + NT - 1) / NT;
+    cpx<T> v[NBT][R];
+#pragma unroll
+    for (int u = 0; u < NBT; ++u) {
+      const int b = (int)threadIdx.x + u * NT;
+      if (TOT % NT == 0 || b < TOT) {
+        const cpx<T> *in = buf + (b / NB) * N + b % NB;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u][r] = in[r * NB];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NBT; ++u) {
+      const int b = (int)threadIdx.x + u * NT;
+      if (TOT % NT == 0 || b < TOT) {
+        const int sq = b / NB, j = b % NB, jm = j % NS;
+        if constexpr (NS > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            cpx<T> w = tw[r * jm * STEP];
+            if (sign > 0) w.y = -w.y;
+            v[u][r] = cmul(v[u][r], w);
+          }
+        }
+        cpx<T> y[R];
+        butterfly<T, R>(v[u], y, sign);
+        cpx<T> *out = buf + sq * N + (j / NS) * NS * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r * NS] = y[r];
+      }
+    }
+    __syncthreads();
+    fft_stages<T, N, SEQ, NT, ST + 1, NS * R>(buf, tw, sign);
+  }
+}
+
+// LDS copies of the twiddles exp(-2 pi i m/N) and rotations (cos, sin)(pi k/2N)
+template <typename T, int N, int NT>
+__device__ inline void load_tables(cpx<T> *tw, cpx<T> *ct, const double *g_tw, const double *g_ct) {
+  for (int m = (int)threadIdx.x; m < N; m += NT) {
+    tw[m] = {(T)g_tw[2 * m], (T)g_tw[2 * m + 1]};
+    ct[m] = {(T)g_ct[2 * m], (T)g_ct[2 * m + 1]};
+  }
+}
+
+// position of x in the Makhoul order: v[n] = x[2n] (n < N/2), v[N-1-n] = x[2n+1]
+template <int N>
+__device__ inline int minv(int x) { return (x % 2 == 0) ? x / 2 : N - 1 - (x - 1) / 2; }
+
+// orthonormal DCT-II of the pair packed in Z at index k
+template <typename T, int N>
+__device__ inline void dct_post_t(const cpx<T> *Z, const cpx<T> *ct, int k, T &xa, T &xb) {
+  cpx<T> zk = Z[k], zc = Z[(N - k) % N];
+  zc.y = -zc.y;
+  cpx<T> A{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
+  cpx<T> D{zk.x - zc.x, zk.y - zc.y};
+  cpx<T> B{D.y * (T)0.5, -D.x * (T)0.5};  // D * (-i/2)
+  const T c = ct[k].x, sn = ct[k].y;
+  const T sk = k == 0 ? (T)sqrt(1.0 / N) : (T)sqrt(2.0 / N);
+  xa = sk * (A.x * c + A.y * sn);
+  xb = sk * (B.x * c + B.y * sn);
+}
+
+// orthonormal coefficients -> V[k] of one sequence (inverse pre-processing)
+template <typename T, int N>
+__device__ inline cpx<T> dct_pre_t(T xk, T xnk, int k, const cpx<T> *ct) {
+  const T sk = k == 0 ? (T)sqrt((double)N) : (T)sqrt(N / 2.0);
+  const T snk = (T)sqrt(N / 2.0);
+  T pp = xk * sk, qq = (k == 0) ? (T)0 : xnk * snk;
+  const T c = ct[k].x, sn = ct[k].y;
+  return {pp * c + qq * sn, pp * sn - qq * c};
+}
+
+template <int N>
+struct DctCfg {
+  static constexpr int SEQ = (2048 / N) < 1 ? 1 : 2048 / N;  // x: row pairs per block
+  static constexpr int SEQY = (2048 / N) < 1 ? 1 : ((2048 / N) > 8 ? 8 : 2048 / N);  // y: column pairs
+};
+
+// DCT along x, INV=0: r -> r (delta correction at x = N-1 first);
+// INV=1: r -> p with the x and z ghost faces
+template <typename T, int N, int INV>
+__global__ void __launch_bounds__(256) k_dctx_fast(Layout L, const double *__restrict__ g_tw,
+                                                   const double *__restrict__ g_ct, const T *__restrict__ in,
+                                                   T *__restrict__ out, const double *__restrict__ delta,
+                                                   StepArgs s, Phys ph, T *__restrict__ p, const T *__restrict__ fx,
+                                                   const T *__restrict__ w, const T *__restrict__ wm,
+                                                   const T *__restrict__ fz, const double *__restrict__ pgradx) {
+  constexpr int SEQ = DctCfg<N>::SEQ, NT = 256;
+  __shared__ cpx<T> buf[SEQ * N];
+  __shared__ cpx<T> tw[N], ct[N];
+  const int ny = L.ny, nz = L.nz;
+  const int nrows = ny * nz;
+  const int row0 = (int)blockIdx.x * 2 * SEQ;
+  load_tables<T, N, NT>(tw, ct, g_tw, g_ct);
+  if (!INV) {
+    const T corr = (T)(delta[0] / ((double)ny * nz));
+    for (int t = (int)threadIdx.x; t < 2 * SEQ * N; t += NT) {
+      const int rr = t / N, x = t % N;  // memory order
+      const int row = row0 + rr;
+      T v = row < nrows ? in[(size_t)row * N + x] : (T)0;
+      if (x == N - 1) v = v - corr;
+      T *dst = (T *)&buf[(rr >> 1) * N + minv<N>(x)];
+      dst[rr & 1] = v;
+    }
+  } else {
+    __syncthreads();  // ct
+    for (int t = (int)threadIdx.x; t < SEQ * N; t += NT) {
+      const int pq = t / N, k = t % N, nk = (N - k) % N;
+      const int ra = row0 + 2 * pq, rb = ra + 1;
+      T xa = ra < nrows ? in[(size_t)ra * N + k] : (T)0, xb = rb < nrows ? in[(size_t)rb * N + k] : (T)0;
+      T xna = ra < nrows ? in[(size_t)ra * N + nk] : (T)0, xnb = rb < nrows ? in[(size_t)rb * N + nk] : (T)0;
+      cpx<T> va = dct_pre_t<T, N>(xa, xna, k, ct), vb = dct_pre_t<T, N>(xb, xnb, k, ct);
+      buf[pq * N + k] = {va.x - vb.y, va.y + vb.x};  // va + i vb
+    }
+  }
+  __syncthreads();
+  fft_stages<T, N, SEQ, NT, 0, 1>(buf, tw, INV ? +1 : -1);
+  if (!INV) {
+    for (int t = (int)threadIdx.x; t < 2 * SEQ * N; t += NT) {
+      const int rr = t / N, k = t % N;
+      const int row = row0 + rr;
+      T xa, xb;
+      dct_post_t<T, N>(buf + (rr >> 1) * N, ct, k, xa, xb);
+      if (row < nrows) out[(size_t)row * N + k] = (rr & 1) ? xb : xa;
+    }
+  } else {
+    const T invN = (T)(1.0 / N);
+    for (int t = (int)threadIdx.x; t < 2 * SEQ * N; t += NT) {
+      const int rr = t / N, x = t % N;
+      const int row = row0 + rr;
+      if (row >= nrows) continue;
+      const cpx<T> z = buf[(rr >> 1) * N + minv<N>(x)];
+      const int j = 1 + row % ny, k = 1 + row / ny, i = x + 1;
+      const T val = ((rr & 1) ? z.y : z.x) * invN;
+      F3(p, i, j, k) = val;
+      // z ghost faces, hotcross80-revised2014.f:1348-1350
+      if (k == 1) {
+        T bd = -(F3(w, i, j, 0) - F3(wm, i, j, 0)) / (T)s.dt + F3(fz, i, j, 0);
+        F3(p, i, j, 0) = val - bd * (T)s.dz;
+      }
+      if (k == nz) {
+        T bd = -(F3(w, i, j, nz) - F3(wm, i, j, nz)) / (T)s.dt + F3(fz, i, j, nz);
+        F3(p, i, j, nz + 1) = val + bd * (T)s.dz;
+      }
+    }
+    // x ghost faces, :1352-1372, from the values at i = 1 and i = N (in LDS)
+    for (int t = (int)threadIdx.x; t < 2 * SEQ; t += NT) {
+      const int row = row0 + t;
+      if (row >= nrows) continue;
+      const int j = 1 + row % ny, k = 1 + row / ny;
+      const cpx<T> z1 = buf[(t >> 1) * N + minv<N>(0)], zn = buf[(t >> 1) * N + minv<N>(N - 1)];
+      const T p1 = ((t & 1) ? z1.y : z1.x) * invN, pn = ((t & 1) ? zn.y : zn.x) * invN;
+      T bxs = (T)(-(s.ubkg - s.ubkg_m) / s.dt - pgradx[k]) + F3(fx, 0, j, k);
+      T bxf = (T)(-(s.ubkg - s.ubkg_m) / s.dt - pgradx[k]) + F3(fx, N, j, k);
+      if (ph.open_box == 1) {
+        F3(p, 0, j, k) = (p1 - bxs * (T)s.dx);
+        F3(p, N + 1, j, k) = (pn + bxf * (T)s.dx);
+      } else {
+        T bx = (T)ph.bcondx;
+        F3(p, 0, j, k) = (p1 - bxs * (T)s.dx) * bx + (T)(1 - ph.bcondx) * pn;
+        F3(p, N + 1, j, k) = (pn + bxf * (T)s.dx) * bx + (T)(1 - ph.bcondx) * p1;
+      }
+    }
+  }
+}
+
+// DCT along y on tiles of 2*SEQY x-columns of one z-plane, in place on buf
+template <typename T, int N, int INV>
+__global__ void __launch_bounds__(256) k_dcty_fast(Layout L, const double *__restrict__ g_tw,
+                                                   const double *__restrict__ g_ct, T *__restrict__ data) {
+  constexpr int SEQ = DctCfg<N>::SEQY, TX = 2 * SEQ, NT = 256;
+  __shared__ cpx<T> buf[SEQ * N];
+  __shared__ cpx<T> tw[N], ct[N];
+  const int nx = L.nx;
+  const int ntile = (nx + TX - 1) / TX;
+  const int k = (int)blockIdx.x / ntile, x0 = ((int)blockIdx.x % ntile) * TX;
+  T *plane = data + (size_t)k * N * nx;
+  load_tables<T, N, NT>(tw, ct, g_tw, g_ct);
+  if (!INV) {
+    for (int t = (int)threadIdx.x; t < TX * N; t += NT) {
+      const int c = t % TX, n = t / TX, x = x0 + c;  // coalesced over c
+      T v = x < nx ? plane[(size_t)n * nx + x] : (T)0;
+      T *dst = (T *)&buf[(c >> 1) * N + minv<N>(n)];
+      dst[c & 1] = v;
+    }
+    __syncthreads();
+  } else {
+    // coefficients straight from global memory into the pre-processing
+    __syncthreads();  // ct
+    for (int t = (int)threadIdx.x; t < SEQ * N; t += NT) {
+      const int pq = t % SEQ, k2 = t / SEQ, nk = (N - k2) % N;
+      const int xa = x0 + 2 * pq, xb = xa + 1;
+      T ra = xa < nx ? plane[(size_t)k2 * nx + xa] : (T)0, rb = xb < nx ? plane[(size_t)k2 * nx + xb] : (T)0;
+      T rna = xa < nx ? plane[(size_t)nk * nx + xa] : (T)0, rnb = xb < nx ? plane[(size_t)nk * nx + xb] : (T)0;
+      cpx<T> va = dct_pre_t<T, N>(ra, rna, k2, ct), vb = dct_pre_t<T, N>(rb, rnb, k2, ct);
+      buf[pq * N + k2] = {va.x - vb.y, va.y + vb.x};
+    }
+    __syncthreads();
+  }
+  fft_stages<T, N, SEQ, NT, 0, 1>(buf, tw, INV ? +1 : -1);
+  if (!INV) {
+    for (int t = (int)threadIdx.x; t < TX * N; t += NT) {
+      const int c = t % TX, k2 = t / TX, x = x0 + c;
+      T xa, xb;
+      dct_post_t<T, N>(buf + (c >> 1) * N, ct, k2, xa, xb);
+      if (x < nx) plane[(size_t)k2 * nx + x] = (c & 1) ? xb : xa;
+    }
+  } else {
+    const T invN = (T)(1.0 / N);
+    for (int t = (int)threadIdx.x; t < TX * N; t += NT) {
+      const int c = t % TX, y = t / TX, x = x0 + c;
+      const cpx<T> z = buf[(c >> 1) * N + minv<N>(y)];
+      if (x < nx) plane[(size_t)y * nx + x] = ((c & 1) ? z.y : z.x) * invN;
+    }
+  }
+}
+
+// lengths with compile-time transforms; others use the generic kernels
+#define FAST_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+template <typename T, int INV>
+static bool launch_dctx_fast(hipStream_t st, const Layout &L, const double *twx, const double *ctx, const T *in,
+                             T *out, const double *delta, const StepArgs &s, const Phys &ph, T *p, const T *fx,
+                             const T *w, const T *wm, const T *fz, const double *pgradx) {
+  const int nrows = L.ny * L.nz;
+  switch (L.nx) {
+#define HC_X(NN)                                                                                         \
+  case NN: {                                                                                             \
+    const int g = (nrows + 2 * DctCfg<NN>::SEQ - 1) / (2 * DctCfg<NN>::SEQ);                             \
+    hipLaunchKernelGGL((k_dctx_fast<T, NN, INV>), dim3(g), dim3(256), 0, st, L, twx, ctx, in, out, delta, s, \
+                       ph, p, fx, w, wm, fz, pgradx);                                                    \
+    return true;                                                                                         \
+  }
+    FAST_SIZES(HC_X)
+#undef HC_X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int INV>
+static bool launch_dcty_fast(hipStream_t st, const Layout &L, const double *twy, const double *cty, T *data) {
+  switch (L.ny) {
+#define HC_Y(NN)                                                                        \
+  case NN: {                                                                            \
+    const int g = L.nz * ((L.nx + 2 * DctCfg<NN>::SEQY - 1) / (2 * DctCfg<NN>::SEQY));  \
+    hipLaunchKernelGGL((k_dcty_fast<T, NN, INV>), dim3(g), dim3(256), 0, st, L, twy, cty, data); \
+    return true;                                                                        \
+  }
+    FAST_SIZES(HC_Y)
+#undef HC_Y
+    default:
+      return false;
+  }
+}
+
+// ---------------------------------------------------------------------------
 static int pick_npair(int N, size_t elem) {
   // LDS: 2 buffers x npair x N complex elements, at most 64 KiB
   int np = 8;
   while (np > 1 && (size_t)2 * np * N * 2 * elem > 65536) np >>= 1;
   return np;
+}
+
+// compile-time transforms on (HC_GENERIC_DCT=1 selects the generic kernels,
+// for comparing the two)
+static bool fast_transforms() {
+  static bool f = [] {
+    const char *e = getenv("HC_GENERIC_DCT");
+    return !(e && e[0] == '1');
+  }();
+  return f;
 }
 
 template <typename T>
@@ -457,16 +780,24 @@ void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const P
   int gx = (nrows + 2 * npx - 1) / (2 * npx);
   const double *twx = pw.tw_x, *ctx = pw.tw_x + 2 * nx;
   const double *twy = pw.tw_y, *cty = pw.tw_y + 2 * ny;
-  hipLaunchKernelGGL((k_dct_x<T, 0>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, r, r, pw.delta, s, ph,
-                     (T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr,
-                     (const double *)nullptr);
+  const bool fast = fast_transforms();
+  if (!(fast && launch_dctx_fast<T, 0>(st, L, twx, ctx, r, r, pw.delta, s, ph, (T *)nullptr, (const T *)nullptr,
+                                       (const T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                                       (const double *)nullptr)))
+    hipLaunchKernelGGL((k_dct_x<T, 0>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, r, r, pw.delta, s, ph,
+                       (T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                       (const T *)nullptr, (const double *)nullptr);
   int gy = nz * ((nx + 2 * npy - 1) / (2 * npy));
-  hipLaunchKernelGGL((k_dct_y<T, 0>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
+  if (!(fast && launch_dcty_fast<T, 0>(st, L, twy, cty, r)))
+    hipLaunchKernelGGL((k_dct_y<T, 0>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
   hipLaunchKernelGGL(k_tridiag<T>, dim3((unsigned)(((size_t)nx * ny + 255) / 256)), dim3(256), 0, st, L, s.dz,
                      pw.piv, r, pw.part);
-  hipLaunchKernelGGL((k_dct_y<T, 1>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
-  hipLaunchKernelGGL((k_dct_x<T, 1>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, r, (T *)nullptr,
-                     pw.delta, s, ph, f.p, f.fx, f.w, f.wm, f.fz, aux.pgradx);
+  if (!(fast && launch_dcty_fast<T, 1>(st, L, twy, cty, r)))
+    hipLaunchKernelGGL((k_dct_y<T, 1>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
+  if (!(fast && launch_dctx_fast<T, 1>(st, L, twx, ctx, r, (T *)nullptr, pw.delta, s, ph, f.p, f.fx, f.w, f.wm,
+                                       f.fz, aux.pgradx)))
+    hipLaunchKernelGGL((k_dct_x<T, 1>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, r, (T *)nullptr,
+                       pw.delta, s, ph, f.p, f.fx, f.w, f.wm, f.fz, aux.pgradx);
   hipLaunchKernelGGL(k_pyghost<T>, dim3((nx + 255) / 256, nz), dim3(256), 0, st, L, s, ph, f.fy, aux.pgrady, f.p);
 }
 
