@@ -7,6 +7,10 @@
 // fewest passes over HBM, not for arithmetic.  Expressions keep the
 // reference's evaluation order term by term (and the library is built with
 // -ffp-contract=off) so the FP64 form reproduces the reference's rounding.
+#include <stdlib.h>
+
+#include <algorithm>
+
 #include "hc_common.h"
 #include "hc_kernels.h"
 
@@ -14,42 +18,50 @@ namespace hc {
 
 #define F3(a, i, j, k) (a)[IX(L, (i), (j), (k))]
 
+// a field read through the padded-box index
+template <typename T>
+struct GView {
+  const T *p;
+  Layout L;
+  __device__ T operator()(int i, int j, int k) const { return p[IX(L, i, j, k)]; }
+};
+
 // ---------------------------------------------------------------------------
 // FORCES, hotcross80-revised2014.f:890-1139.  azu/azv are formed on the fly;
 // their never-recomputed boundary entries are the persistent accumulators
 // acc[i] (the entrance ramp of :953-956 adds (7-i)*0.01/7 to them per call).
-template <typename T>
-__device__ inline T azu_at(const Layout &L, const T *ssv, const double *azzmin,
+template <typename T, class V>
+__device__ inline T azu_at(const Layout &L, const V &ssv, const double *azzmin,
                            const StepArgs &s, int i, int j, int k) {
   // azu(i,j,k) for i=0..nx, j=1..ny, k=0..nz
   if (i == 0) return k >= 1 ? (T)s.acc[0] : (T)0;
   if (i == L.nx) return (T)0;
-  T v = (T).25 * (F3(ssv, i, j, k) + F3(ssv, i + 1, j, k) + F3(ssv, i, j, k + 1) + F3(ssv, i + 1, j, k + 1)) +
+  T v = (T).25 * (ssv(i, j, k) + ssv(i + 1, j, k) + ssv(i, j, k + 1) + ssv(i + 1, j, k + 1)) +
         (T)azzmin[k];
   if (i <= 7 && k >= 1) v = v + (T)((double)(7 - i) * .01 / 7.0);
   return v;
 }
 
-template <typename T>
-__device__ inline T azv_at(const Layout &L, const T *ssv, const double *azzmin,
+template <typename T, class V>
+__device__ inline T azv_at(const Layout &L, const V &ssv, const double *azzmin,
                            const StepArgs &s, int i, int j, int k) {
   // azv(i,j,k) for i=1..nx, j=0..ny, k=0..nz
   if (j == 0 || j == L.ny) {
     if (k == 0 || i > 7) return (T)0;
     return (T)s.acc[i];
   }
-  T v = (T).25 * (F3(ssv, i, j, k) + F3(ssv, i, j + 1, k) + F3(ssv, i, j, k + 1) + F3(ssv, i, j + 1, k + 1)) +
+  T v = (T).25 * (ssv(i, j, k) + ssv(i, j + 1, k) + ssv(i, j, k + 1) + ssv(i, j + 1, k + 1)) +
         (T)azzmin[k];
   if (i <= 7 && k >= 1) v = v + (T)((double)(7 - i) * .01 / 7.0);
   return v;
 }
 
-template <typename T, bool P2>
-__device__ inline void forces_x_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
-                                                  const T *__restrict__ v, const T *__restrict__ w,
-                                                  const T *__restrict__ um, const T *__restrict__ vm,
-                                                  const T *__restrict__ wm, const T *__restrict__ ss,
-                                                  const T *__restrict__ ssv, const double *__restrict__ azzmin,
+template <typename T, bool P2, class V>
+__device__ inline void forces_x_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const V &u,
+                                                  const V &v, const V &w,
+                                                  const V &um, const V &vm,
+                                                  const V &wm, const V &ss,
+                                                  const V &ssv, const double *__restrict__ azzmin,
                                                   const double *__restrict__ alphau, T *__restrict__ fx) {
   const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
@@ -57,36 +69,36 @@ __device__ inline void forces_x_at(const Layout &L, int i, int j, int k, const S
   const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
           coe_4z = (T)((double)ph.bit4 / 4.0 / s.dz);
   const T coe_7 = (T)(ph.bit7 * s.fc / 4.0), coe_9 = (T)(ph.bit9 * s.fc_y / 4.0), coe_10 = (T)ph.bit10;
-  T axu_p = F3(ss, i + 1, j, k) + axmin, axu_c = F3(ss, i, j, k) + axmin;
-  T ayu_c = (T).25 * (F3(ss, i, j, k) + F3(ss, i + 1, j, k) + F3(ss, i, j + 1, k) + F3(ss, i + 1, j + 1, k)) + aymin;
-  T ayu_m = (T).25 * (F3(ss, i, j - 1, k) + F3(ss, i + 1, j - 1, k) + F3(ss, i, j, k) + F3(ss, i + 1, j, k)) + aymin;
-  T azu_c = azu_at(L, ssv, azzmin, s, i, j, k), azu_m = azu_at(L, ssv, azzmin, s, i, j, k - 1);
-  T umc = F3(um, i, j, k);
-  T dis = (T)2.0 * bit5 * (axu_p * (F3(um, i + 1, j, k) - umc) - axu_c * (umc - F3(um, i - 1, j, k))) / dx / dx +
-          bit5 * (ayu_c * (F3(um, i, j + 1, k) - umc) - ayu_m * (umc - F3(um, i, j - 1, k))) / dy / dy +
-          bit5 * (ayu_c * (F3(vm, i + 1, j, k) - F3(vm, i, j, k)) - ayu_m * (F3(vm, i + 1, j - 1, k) - F3(vm, i, j - 1, k))) / dx / dy +
-          bit2 * (azu_c * (F3(um, i, j, k + 1) - umc) - azu_m * (umc - F3(um, i, j, k - 1))) / dz / dz +
-          bit2 * (azu_c * (F3(wm, i + 1, j, k) - F3(wm, i, j, k)) - azu_m * (F3(wm, i + 1, j, k - 1) - F3(wm, i, j, k - 1))) / dx / dz;
-  T uc_ = F3(u, i, j, k);
-  T a1 = F3(u, i + 1, j, k) + uc_, a2 = uc_ + F3(u, i - 1, j, k);
+  T axu_p = ss(i + 1, j, k) + axmin, axu_c = ss(i, j, k) + axmin;
+  T ayu_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + aymin;
+  T ayu_m = (T).25 * (ss(i, j - 1, k) + ss(i + 1, j - 1, k) + ss(i, j, k) + ss(i + 1, j, k)) + aymin;
+  T azu_c = azu_at<T>(L, ssv, azzmin, s, i, j, k), azu_m = azu_at<T>(L, ssv, azzmin, s, i, j, k - 1);
+  T umc = um(i, j, k);
+  T dis = (T)2.0 * bit5 * (axu_p * (um(i + 1, j, k) - umc) - axu_c * (umc - um(i - 1, j, k))) / dx / dx +
+          bit5 * (ayu_c * (um(i, j + 1, k) - umc) - ayu_m * (umc - um(i, j - 1, k))) / dy / dy +
+          bit5 * (ayu_c * (vm(i + 1, j, k) - vm(i, j, k)) - ayu_m * (vm(i + 1, j - 1, k) - vm(i, j - 1, k))) / dx / dy +
+          bit2 * (azu_c * (um(i, j, k + 1) - umc) - azu_m * (umc - um(i, j, k - 1))) / dz / dz +
+          bit2 * (azu_c * (wm(i + 1, j, k) - wm(i, j, k)) - azu_m * (wm(i + 1, j, k - 1) - wm(i, j, k - 1))) / dx / dz;
+  T uc_ = u(i, j, k);
+  T a1 = u(i + 1, j, k) + uc_, a2 = uc_ + u(i - 1, j, k);
   T r = dis + (T)ph.bit3 * (T)0.0 - (T)0.5 * (T)alphau[(i + 1) + (L.nx + 3) * j] * (umc - (T)s.ubkg_m) -
         coe_4x * (a1 * a1 - a2 * a2) -
-        coe_4z * ((F3(w, i, j, k) + F3(w, i + 1, j, k)) * (F3(u, i, j, k + 1) + uc_) -
-                  (F3(w, i, j, k - 1) + F3(w, i + 1, j, k - 1)) * (uc_ + F3(u, i, j, k - 1))) -
-        coe_4y * ((F3(v, i, j, k) + F3(v, i + 1, j, k)) * (F3(u, i, j + 1, k) + uc_) -
-                  (F3(v, i, j - 1, k) + F3(v, i + 1, j - 1, k)) * (uc_ + F3(u, i, j - 1, k))) +
-        coe_7 * (F3(v, i + 1, j, k) + F3(v, i, j, k) + F3(v, i + 1, j - 1, k) + F3(v, i, j - 1, k)) -
-        coe_9 * (F3(w, i + 1, j, k) + F3(w, i + 1, j, k - 1) + F3(w, i, j, k) + F3(w, i, j, k - 1)) +
+        coe_4z * ((w(i, j, k) + w(i + 1, j, k)) * (u(i, j, k + 1) + uc_) -
+                  (w(i, j, k - 1) + w(i + 1, j, k - 1)) * (uc_ + u(i, j, k - 1))) -
+        coe_4y * ((v(i, j, k) + v(i + 1, j, k)) * (u(i, j + 1, k) + uc_) -
+                  (v(i, j - 1, k) + v(i + 1, j - 1, k)) * (uc_ + u(i, j - 1, k))) +
+        coe_7 * (v(i + 1, j, k) + v(i, j, k) + v(i + 1, j - 1, k) + v(i, j - 1, k)) -
+        coe_9 * (w(i + 1, j, k) + w(i + 1, j, k - 1) + w(i, j, k) + w(i, j, k - 1)) +
         coe_10 * (T)s.fbx;
   F3(fx, i, j, k) = r;
 }
 
-template <typename T, bool P2>
-__device__ inline void forces_y_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
-                                                  const T *__restrict__ v, const T *__restrict__ w,
-                                                  const T *__restrict__ um, const T *__restrict__ vm,
-                                                  const T *__restrict__ wm, const T *__restrict__ ss,
-                                                  const T *__restrict__ ssv, const double *__restrict__ azzmin,
+template <typename T, bool P2, class V>
+__device__ inline void forces_y_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const V &u,
+                                                  const V &v, const V &w,
+                                                  const V &um, const V &vm,
+                                                  const V &wm, const V &ss,
+                                                  const V &ssv, const double *__restrict__ azzmin,
                                                   const double *__restrict__ alphav, T *__restrict__ fy) {
   const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   const T bit2 = (T)ph.bit2, bit5 = (T)ph.bit5;
@@ -94,35 +106,35 @@ __device__ inline void forces_y_at(const Layout &L, int i, int j, int k, const S
   const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
           coe_4z = (T)((double)ph.bit4 / 4.0 / s.dz);
   const T coe_7 = (T)(ph.bit7 * s.fc / 4.0), coe_10 = (T)ph.bit10;
-  T axv_c = (T).25 * (F3(ss, i, j, k) + F3(ss, i + 1, j, k) + F3(ss, i, j + 1, k) + F3(ss, i + 1, j + 1, k)) + axmin;
-  T axv_m = (T).25 * (F3(ss, i - 1, j, k) + F3(ss, i, j, k) + F3(ss, i - 1, j + 1, k) + F3(ss, i, j + 1, k)) + axmin;
-  T ayv_p = F3(ss, i, j + 1, k) + aymin, ayv_c = F3(ss, i, j, k) + aymin;
-  T azv_c = azv_at(L, ssv, azzmin, s, i, j, k), azv_m = azv_at(L, ssv, azzmin, s, i, j, k - 1);
-  T vmc = F3(vm, i, j, k);
-  T dis = +bit5 * (axv_c * (F3(vm, i + 1, j, k) - vmc) - axv_m * (vmc - F3(vm, i - 1, j, k))) / dx / dx +
-          bit5 * (axv_c * (F3(um, i, j + 1, k) - F3(um, i, j, k)) - axv_m * (F3(um, i - 1, j + 1, k) - F3(um, i - 1, j, k))) / dx / dy +
-          (T)2.0 * bit5 * (ayv_p * (F3(vm, i, j + 1, k) - vmc) - ayv_c * (vmc - F3(vm, i, j - 1, k))) / dy / dy +
-          bit2 * (azv_c * (F3(vm, i, j, k + 1) - vmc) - azv_m * (vmc - F3(vm, i, j, k - 1))) / dz / dz +
-          bit2 * (azv_c * (F3(wm, i, j + 1, k) - F3(wm, i, j, k)) - azv_m * (F3(wm, i, j + 1, k - 1) - F3(wm, i, j, k - 1))) / dy / dz;
-  T vc_ = F3(v, i, j, k);
-  T b1 = F3(v, i, j + 1, k) + vc_, b2 = vc_ + F3(v, i, j - 1, k);
+  T axv_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + axmin;
+  T axv_m = (T).25 * (ss(i - 1, j, k) + ss(i, j, k) + ss(i - 1, j + 1, k) + ss(i, j + 1, k)) + axmin;
+  T ayv_p = ss(i, j + 1, k) + aymin, ayv_c = ss(i, j, k) + aymin;
+  T azv_c = azv_at<T>(L, ssv, azzmin, s, i, j, k), azv_m = azv_at<T>(L, ssv, azzmin, s, i, j, k - 1);
+  T vmc = vm(i, j, k);
+  T dis = +bit5 * (axv_c * (vm(i + 1, j, k) - vmc) - axv_m * (vmc - vm(i - 1, j, k))) / dx / dx +
+          bit5 * (axv_c * (um(i, j + 1, k) - um(i, j, k)) - axv_m * (um(i - 1, j + 1, k) - um(i - 1, j, k))) / dx / dy +
+          (T)2.0 * bit5 * (ayv_p * (vm(i, j + 1, k) - vmc) - ayv_c * (vmc - vm(i, j - 1, k))) / dy / dy +
+          bit2 * (azv_c * (vm(i, j, k + 1) - vmc) - azv_m * (vmc - vm(i, j, k - 1))) / dz / dz +
+          bit2 * (azv_c * (wm(i, j + 1, k) - wm(i, j, k)) - azv_m * (wm(i, j + 1, k - 1) - wm(i, j, k - 1))) / dy / dz;
+  T vc_ = v(i, j, k);
+  T b1 = v(i, j + 1, k) + vc_, b2 = vc_ + v(i, j - 1, k);
   T r = dis + (T)ph.bit3 * (T)0.0 - (T)0.5 * (T)alphav[i + (L.nx + 2) * (j + 1)] * (vmc - (T)s.vbkg_m) -
         coe_4y * (b1 * b1 - b2 * b2) -
-        coe_4z * ((F3(v, i, j, k + 1) + vc_) * (F3(w, i, j + 1, k) + F3(w, i, j, k)) -
-                  (vc_ + F3(v, i, j, k - 1)) * (F3(w, i, j + 1, k - 1) + F3(w, i, j, k - 1))) -
-        coe_4x * ((F3(v, i + 1, j, k) + vc_) * (F3(u, i, j, k) + F3(u, i, j + 1, k)) -
-                  (vc_ + F3(v, i - 1, j, k)) * (F3(u, i - 1, j, k) + F3(u, i - 1, j + 1, k))) -
-        coe_7 * (F3(u, i, j, k) + F3(u, i - 1, j, k) + F3(u, i, j + 1, k) + F3(u, i - 1, j + 1, k)) +
+        coe_4z * ((v(i, j, k + 1) + vc_) * (w(i, j + 1, k) + w(i, j, k)) -
+                  (vc_ + v(i, j, k - 1)) * (w(i, j + 1, k - 1) + w(i, j, k - 1))) -
+        coe_4x * ((v(i + 1, j, k) + vc_) * (u(i, j, k) + u(i, j + 1, k)) -
+                  (vc_ + v(i - 1, j, k)) * (u(i - 1, j, k) + u(i - 1, j + 1, k))) -
+        coe_7 * (u(i, j, k) + u(i - 1, j, k) + u(i, j + 1, k) + u(i - 1, j + 1, k)) +
         coe_10 * (T)s.fby;
   F3(fy, i, j, k) = r;
 }
 
-template <typename T, bool P2>
-__device__ inline void forces_z_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const T *__restrict__ u,
-                                                  const T *__restrict__ v, const T *__restrict__ w,
-                                                  const T *__restrict__ um, const T *__restrict__ vm,
-                                                  const T *__restrict__ wm, const T *__restrict__ ss,
-                                                  const T *__restrict__ ssv, const T *__restrict__ rhop,
+template <typename T, bool P2, class V, class VR>
+__device__ inline void forces_z_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const V &u,
+                                                  const V &v, const V &w,
+                                                  const V &um, const V &vm,
+                                                  const V &wm, const V &ss,
+                                                  const V &ssv, const VR &rhop,
                                                   const double *__restrict__ azzmin, const double *__restrict__ alphae,
                                                   const double *__restrict__ alphaw, T *__restrict__ fz) {
   const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
@@ -131,31 +143,31 @@ __device__ inline void forces_z_at(const Layout &L, int i, int j, int k, const S
   const T coe_4x = (T)((double)ph.bit4 / 4.0 / s.dx), coe_4y = (T)((double)ph.bit4 / 4.0 / s.dy),
           coe_4z = (T)((double)ph.bit4 / 4.0 / s.dz);
   const T coe_9 = (T)(ph.bit9 * s.fc_y / 4.0), coe_10 = (T)ph.bit10;
-  T ahp = F3(ss, i, j, k + 1) + axmin, ahm = F3(ss, i, j, k) + axmin;
+  T ahp = ss(i, j, k + 1) + axmin, ahm = ss(i, j, k) + axmin;
   T zk = (T)azzmin[k];
-  T s0 = F3(ssv, i, j, k + 1), s1 = F3(ssv, i, j, k);
-  T avxp = (T)0.25 * (s0 + s1 + F3(ssv, i + 1, j, k + 1) + F3(ssv, i + 1, j, k)) + zk;
-  T avxm = (T)0.25 * (s0 + s1 + F3(ssv, i - 1, j, k + 1) + F3(ssv, i - 1, j, k)) + zk;
-  T avyp = (T)0.25 * (s0 + s1 + F3(ssv, i, j + 1, k + 1) + F3(ssv, i, j + 1, k)) + zk;
-  T avym = (T)0.25 * (s0 + s1 + F3(ssv, i, j - 1, k + 1) + F3(ssv, i, j - 1, k)) + zk;
-  T wmc = F3(wm, i, j, k);
-  T dis = (T)2.0 * bit5 * (ahp * (F3(wm, i, j, k + 1) - wmc) - ahm * (wmc - F3(wm, i, j, k - 1))) / dz / dz +
-          bit2 * (avxp * (F3(wm, i + 1, j, k) - wmc) - avxm * (wmc - F3(wm, i - 1, j, k))) / dx / dx +
-          bit2 * (avxp * (F3(um, i, j, k) - F3(um, i, j, k + 1)) - avxm * (F3(um, i - 1, j, k) - F3(um, i - 1, j, k + 1))) / dz / dx +
-          bit2 * (avyp * (F3(wm, i, j + 1, k) - wmc) - avym * (wmc - F3(wm, i, j - 1, k))) / dy / dy +
-          bit2 * (avyp * (F3(vm, i, j, k) - F3(vm, i, j, k + 1)) - avym * (F3(vm, i, j - 1, k) - F3(vm, i, j - 1, k + 1))) / dz / dy;
-  T wc_ = F3(w, i, j, k);
-  T c1 = F3(w, i, j, k + 1) + wc_, c2 = wc_ + F3(w, i, j, k - 1);
+  T s0 = ssv(i, j, k + 1), s1 = ssv(i, j, k);
+  T avxp = (T)0.25 * (s0 + s1 + ssv(i + 1, j, k + 1) + ssv(i + 1, j, k)) + zk;
+  T avxm = (T)0.25 * (s0 + s1 + ssv(i - 1, j, k + 1) + ssv(i - 1, j, k)) + zk;
+  T avyp = (T)0.25 * (s0 + s1 + ssv(i, j + 1, k + 1) + ssv(i, j + 1, k)) + zk;
+  T avym = (T)0.25 * (s0 + s1 + ssv(i, j - 1, k + 1) + ssv(i, j - 1, k)) + zk;
+  T wmc = wm(i, j, k);
+  T dis = (T)2.0 * bit5 * (ahp * (wm(i, j, k + 1) - wmc) - ahm * (wmc - wm(i, j, k - 1))) / dz / dz +
+          bit2 * (avxp * (wm(i + 1, j, k) - wmc) - avxm * (wmc - wm(i - 1, j, k))) / dx / dx +
+          bit2 * (avxp * (um(i, j, k) - um(i, j, k + 1)) - avxm * (um(i - 1, j, k) - um(i - 1, j, k + 1))) / dz / dx +
+          bit2 * (avyp * (wm(i, j + 1, k) - wmc) - avym * (wmc - wm(i, j - 1, k))) / dy / dy +
+          bit2 * (avyp * (vm(i, j, k) - vm(i, j, k + 1)) - avym * (vm(i, j - 1, k) - vm(i, j - 1, k + 1))) / dz / dy;
+  T wc_ = w(i, j, k);
+  T c1 = w(i, j, k + 1) + wc_, c2 = wc_ + w(i, j, k - 1);
   // alphaee(i,j,k) = max(alphae(i,j), (alphaw(k)+alphaw(k-1))/2), sponge80.f:115-119
   // (alphae here is the mask before the periodic copies of :128-138)
   double ae = fmax(alphae[i + (L.nx + 2) * j], (alphaw[k + 1] + alphaw[k]) / 2.0);
-  T r = dis + (T)ph.bit3 * (T)ph.g * (F3(rhop, i, j, k) + F3(rhop, i, j, k + 1)) / (T)ph.rho0 / (T)2.0 -
+  T r = dis + (T)ph.bit3 * (T)ph.g * (rhop(i, j, k) + rhop(i, j, k + 1)) / (T)ph.rho0 / (T)2.0 -
         (T)0.0 * (T)alphaw[k + 1] * wmc - (T)0.5 * (T)ae * wmc - coe_4z * (c1 * c1 - c2 * c2) -
-        coe_4x * ((F3(w, i + 1, j, k) + wc_) * (F3(u, i, j, k) + F3(u, i, j, k + 1)) -
-                  (wc_ + F3(w, i - 1, j, k)) * (F3(u, i - 1, j, k) + F3(u, i - 1, j, k + 1))) -
-        coe_4y * ((F3(w, i, j + 1, k) + wc_) * (F3(v, i, j, k) + F3(v, i, j, k + 1)) -
-                  (wc_ + F3(w, i, j - 1, k)) * (F3(v, i, j - 1, k) + F3(v, i, j - 1, k + 1))) +
-        coe_9 * (F3(u, i, j, k + 1) + F3(u, i - 1, j, k + 1) + F3(u, i, j, k) + F3(u, i - 1, j, k)) +
+        coe_4x * ((w(i + 1, j, k) + wc_) * (u(i, j, k) + u(i, j, k + 1)) -
+                  (wc_ + w(i - 1, j, k)) * (u(i - 1, j, k) + u(i - 1, j, k + 1))) -
+        coe_4y * ((w(i, j + 1, k) + wc_) * (v(i, j, k) + v(i, j, k + 1)) -
+                  (wc_ + w(i, j - 1, k)) * (v(i, j - 1, k) + v(i, j - 1, k + 1))) +
+        coe_9 * (u(i, j, k + 1) + u(i - 1, j, k + 1) + u(i, j, k) + u(i - 1, j, k)) +
         coe_10 * (T)0.0;
   F3(fz, i, j, k) = r;
 }
@@ -173,12 +185,102 @@ __global__ void __launch_bounds__(256) k_forces(Layout L, Box b, StepArgs s, Phy
                                                 const double *__restrict__ alphaw, T *__restrict__ fx,
                                                 T *__restrict__ fy, T *__restrict__ fz) {
   HC_BOX_IJK(b);  // i = 0..nx, j = 0..ny, k = 0..nz
+  const GView<T> U{u, L}, Vv{v, L}, W{w, L}, UM{um, L}, VM{vm, L}, WM{wm, L}, SS{ss, L}, SV{ssv, L}, RH{rhop, L};
   if (j >= 1 && k >= 1)
-    forces_x_at<T, P2>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphau, fx);
+    forces_x_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphau, fx);
   if (i >= 1 && k >= 1)
-    forces_y_at<T, P2>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, azzmin, alphav, fy);
+    forces_y_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphav, fy);
   if (i >= 1 && j >= 1)
-    forces_z_at<T, P2>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, ssv, rhop, azzmin, alphae0, alphaw, fz);
+    forces_z_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, fz);
+}
+
+// The interior box i = 1..nx, j = 1..ny, k = 1..nz as a k-marching LDS
+// tile: a block owns a TI x TJ column of cells and keeps the planes k-1, k,
+// k+1 of the eight stencil inputs in LDS rings with a one-cell halo in i and
+// j.  Each step reads one new plane per field from HBM (prefetched into
+// registers a plane ahead), and the three force components of the cell are
+// formed from LDS with the same expressions as k_forces.  The faces i = 0
+// (fx), j = 0 (fy) and k = 0 (fz) are left to k_forces.
+template <typename T, int SI>
+struct LView {
+  const T *pm, *p0, *pp;  // planes k-1, k, k+1 of one field
+  int k, ib, jb;          // current plane, LDS origin (i0-1, j0-1)
+  __device__ T operator()(int i, int j, int kk) const {
+    const T *b = kk < k ? pm : (kk > k ? pp : p0);
+    return b[(i - ib) + SI * (j - jb)];
+  }
+};
+
+template <typename T, int TI, int TJ>
+struct ForcesTile {
+  static constexpr int SI = TI + 2, SJ = TJ + 2, S = SI * SJ, NT = TI * TJ, RS = (S + NT - 1) / NT;
+  static constexpr int NF = 8;  // u v w um vm wm ss ss_v
+  T f[NF][3][S];
+};
+
+template <typename T, bool P2, int TI, int TJ>
+__global__ void __launch_bounds__(TI *TJ) k_forces_tile(Layout L, StepArgs s, Phys ph, FieldPtrs<T> fp, int kc,
+                                                        const double *__restrict__ azzmin,
+                                                        const double *__restrict__ alphau,
+                                                        const double *__restrict__ alphav,
+                                                        const double *__restrict__ alphae0,
+                                                        const double *__restrict__ alphaw) {
+  typedef ForcesTile<T, TI, TJ> FT;
+  constexpr int SI = FT::SI, S = FT::S, NT = FT::NT, RS = FT::RS, NF = FT::NF;
+  __shared__ FT sh;
+  const int tid = (int)threadIdx.x;
+  const int i0 = 1 + (int)blockIdx.x * TI, j0 = 1 + (int)blockIdx.y * TJ;
+  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
+  const T *src[NF] = {fp.u, fp.v, fp.w, fp.um, fp.vm, fp.wm, fp.ss, fp.ssv};
+  // staged elements of this thread: (i,j) offsets in the padded box
+  int so[RS];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int e = tid + r * NT;
+    const int i = i0 - 1 + e % SI, j = j0 - 1 + e / SI;
+    so[r] = (e < S && i <= L.nx + 1 && j <= L.ny + 1) ? IX(L, i, j, -1) : -1;
+  }
+  const int pstride = L.px * L.py;
+  T pf[NF][RS];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) pf[f][r] = so[r] >= 0 ? src[f][so[r] + (k + 1) * pstride] : (T)0;
+  };
+  auto commit = [&](int k) {
+    const int slot = k % 3;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) {
+        const int e = tid + r * NT;
+        if (e < S) sh.f[f][slot][e] = pf[f][r];
+      }
+  };
+  fetch(ka - 1);
+  commit(ka - 1);
+  fetch(ka);
+  commit(ka);
+  fetch(ka + 1);
+  const int i = i0 + tid % TI, j = j0 + tid / TI;
+  const bool own = i <= L.nx && j <= L.ny;
+  for (int k = ka; k <= kb; ++k) {
+    __syncthreads();  // the previous step's reads of slot (k+1)%3 are done
+    commit(k + 1);
+    __syncthreads();
+    if (k < kb) fetch(k + 2);  // in flight during this step
+    if (own) {
+      const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
+      auto view = [&](int f) { return LView<T, SI>{sh.f[f][sm], sh.f[f][s0], sh.f[f][sp], k, i0 - 1, j0 - 1}; };
+      const LView<T, SI> U = view(0), Vv = view(1), W = view(2), UM = view(3), VM = view(4), WM = view(5),
+                         SS = view(6), SV = view(7);
+      const GView<T> RH{fp.rhop, L};
+      forces_x_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphau, fp.fx);
+      forces_y_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphav, fp.fy);
+      forces_z_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, fp.fz);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -436,6 +538,16 @@ __global__ void __launch_bounds__(256) k_refresh(size_t n, int leap, double asse
 // ---------------------------------------------------------------------------
 // host launchers
 // launch the power-of-two-spacing instantiation when s.p2 is set
+// the LDS-tiled FORCES (HC_FORCES_CELL=1 selects the cell-parallel kernel
+// for the whole box, for comparing the two)
+static bool forces_tiled() {
+  static bool t = [] {
+    const char *e = getenv("HC_FORCES_CELL");
+    return !(e && e[0] == '1');
+  }();
+  return t;
+}
+
 #define HC_P2_LAUNCH(K, ...)                                   \
   do {                                                         \
     if (s.p2) hipLaunchKernelGGL((K<T, true>), __VA_ARGS__);   \
@@ -445,9 +557,26 @@ __global__ void __launch_bounds__(256) k_refresh(size_t n, int leap, double asse
 template <typename T>
 void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                    const Aux &aux) {
-  Box b{0, L.nx, 0, L.ny, 0, L.nz};
-  HC_P2_LAUNCH(k_forces, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm, f.ss,
-               f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy, f.fz);
+  if (!forces_tiled()) {
+    Box b{0, L.nx, 0, L.ny, 0, L.nz};
+    HC_P2_LAUNCH(k_forces, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm, f.ss,
+                 f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy, f.fz);
+    return;
+  }
+  // faces i = 0, j = 0 and k = 0 with the cell-parallel kernel
+  const Box faces[3] = {{0, 0, 0, L.ny, 0, L.nz}, {1, L.nx, 0, 0, 0, L.nz}, {1, L.nx, 1, L.ny, 0, 0}};
+  for (const Box &b : faces)
+    HC_P2_LAUNCH(k_forces, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm, f.ss,
+                 f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy, f.fz);
+  constexpr int TI = 32, TJ = 8;
+  const int kc = std::min(L.nz, 32);
+  dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
+  if (s.p2)
+    hipLaunchKernelGGL((k_forces_tile<T, true, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f, kc, aux.azzmin,
+                       aux.alphau, aux.alphav, aux.alphae0, aux.alphaw);
+  else
+    hipLaunchKernelGGL((k_forces_tile<T, false, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f, kc, aux.azzmin,
+                       aux.alphau, aux.alphav, aux.alphae0, aux.alphaw);
 }
 
 template <typename T>
