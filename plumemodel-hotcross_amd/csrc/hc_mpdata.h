@@ -32,22 +32,28 @@
 #define HC_HD __host__ __device__
 #endif
 
-// a host build may define HC_MP_CHECK(x) to range-check every LDS index
+// a host build may define HC_MP_CHECK(x) / HC_MP_CHECKC(x) to range-check
+// every LDS index of a staged / computed plane
 #ifndef HC_MP_CHECK
 #define HC_MP_CHECK(x) (x)
+#endif
+#ifndef HC_MP_CHECKC
+#define HC_MP_CHECKC(x) (x)
 #endif
 
 namespace hc {
 
 // Shared-memory tile of one block.
+// Staged planes cover rows j0-2 .. j0+TJ+1 (S entries); the computed ones
+// (ud, vd, wd, flags) rows j0-2 .. j0+TJ only (SC entries), same pitch.
 template <typename T, int TI, int TJ>
 struct MpShared {
-  static constexpr int SI = TI + 4, SJ = TJ + 4, S = SI * SJ;
+  static constexpr int SI = TI + 4, SJ = TJ + 4, S = SI * SJ, SC = SI * (TJ + 3);
   T c[4][S];
   T U[2][S], V[2][S], W[2][S];
-  T ud[S], vd[S];
-  T wd[2][S];
-  unsigned char fl[2][S];
+  T ud[SC], vd[SC];
+  T wd[2][SC];
+  unsigned char fl[2][SC];
 };
 
 // Per-thread state that rolls from plane to plane.
@@ -185,8 +191,7 @@ HC_HD inline T mp_donor(T cc, T cxm, T cxp, T cym, T cyp, T czm, T czp, T u0, T 
 // Work of a block per plane, as flat lists over the block's threads:
 //   faces: ud on i in [i0-2, i0+TI] x j in [j0-1, j0+TJ], vd on
 //          [i0-1, i0+TI] x [j0-2, j0+TJ], wd on [i0-1, i0+TI] x [j0-1, j0+TJ],
-//          each list laid over whole LDS rows and padded to a multiple of 64
-//          so every wavefront works on one kind of face;
+//          packed back to back;
 //   flags: cells [i0-1, i0+TI] x [j0-1, j0+TJ];
 //   finish: one tile cell per thread.
 // A thread's items are the same for every plane, so their LDS indexes and
@@ -195,13 +200,14 @@ HC_HD inline T mp_donor(T cc, T cxm, T cxp, T cym, T cyp, T czm, T czp, T u0, T 
 template <typename T, int TI, int TJ>
 struct MpTile {
   typedef MpShared<T, TI, TJ> Sh;
-  static constexpr int SI = Sh::SI, SJ = Sh::SJ, S = Sh::S, NT = TI * TJ;
+  static constexpr int SI = Sh::SI, SJ = Sh::SJ, S = Sh::S, SC = Sh::SC, NT = TI * TJ;
   static constexpr int pad64(int n) { return (n + 63) / 64 * 64; }
-  // every list runs over whole LDS rows (SI wide), so the LDS index of an
-  // item is the list's base plus its position: the lanes of a wavefront read
-  // consecutive LDS words.  The columns a list does not need are masked.
-  static constexpr int NJU = TJ + 2, NJV = TJ + 3, NJW = TJ + 2;  // rows: ud j0-1.., vd j0-2.., wd j0-1..
-  static constexpr int LU = pad64(SI * NJU), LV = pad64(SI * NJV), LW = pad64(SI * NJW);
+  // The face lists are packed back to back without padding: ud over
+  // (TI+3) x (TJ+2) faces, vd over (TI+2) x (TJ+3), wd over (TI+2) x (TJ+2);
+  // only the two wavefronts that straddle a list boundary compute two kinds.
+  // The flag list runs over whole LDS rows.
+  static constexpr int NIU = TI + 3, NJU = TJ + 2, NIV = TI + 2, NJV = TJ + 3, NIW = TI + 2, NJW = TJ + 2;
+  static constexpr int LU = NIU * NJU, LV = NIV * NJV, LW = NIW * NJW;
   static constexpr int NF = LU + LV + LW, RF = (NF + NT - 1) / NT;
   static constexpr int NJF = TJ + 2, NFL = SI * NJF, RFL = (NFL + NT - 1) / NT;
   static constexpr int RS = (S + NT - 1) / NT;
@@ -221,36 +227,41 @@ struct MpTile {
   HC_HD static long plane(const MpGeom &g, int k) { return (long)g.px * g.py * (long)(k + 1); }
 
   // rot rotates the assignment of face and flag items to wavefronts, so
-  // that the waves that carry an extra round differ from block to block
+  // that the waves that carry an extra round differ from block to block; the
+  // flag items are rotated by a further half block so that the waves with an
+  // extra face round are not the ones with an extra flag round
   HC_HD static void plan(const MpGeom &g, int tid, int rot, Plan &pl) {
-    const int vt = (tid + NT - 64 * (rot % (NT / 64))) % NT;
+    const int nw = NT / 64;
+    const int vt = (tid + NT - 64 * (rot % nw)) % NT;
+    const int vf = (vt + 64 * (nw / 2)) % NT;
     for (int r = 0; r < RF; ++r) {
-      int e = vt + r * NT, p = -1, o = -1;
+      int e = vt + r * NT, p = -1, o = -1, kind = 0;
       if (e < LU) {  // ud: faces i in [i0-2, i0+TI], rows [j0-1, j0+TJ]
-        const int i = g.i0 - 2 + e % SI, j = g.j0 - 1 + e / SI;
-        if (e < SI * NJU && i <= g.i0 + TI && i >= 0 && i <= g.nx && j >= 1 && j <= g.ny) {
+        const int i = g.i0 - 2 + e % NIU, j = g.j0 - 1 + e / NIU;
+        if (i >= 0 && i <= g.nx && j >= 1 && j <= g.ny) {
           p = at(g, i, j);
           if (j < g.j0 + TJ && ((i >= g.i0 && i < g.i0 + TI) || (i == 0 && g.i0 == 1))) o = ij(g, i, j);
         }
       } else if ((e -= LU) < LV) {  // vd: columns [i0-1, i0+TI], faces j in [j0-2, j0+TJ]
-        const int i = g.i0 - 2 + e % SI, j = g.j0 - 2 + e / SI;
-        if (e < SI * NJV && i >= g.i0 - 1 && i <= g.i0 + TI && i >= 1 && i <= g.nx && j >= 0 && j <= g.ny) {
+        kind = 1;
+        const int i = g.i0 - 1 + e % NIV, j = g.j0 - 2 + e / NIV;
+        if (i >= 1 && i <= g.nx && j >= 0 && j <= g.ny) {
           p = at(g, i, j);
           if (i < g.i0 + TI && i >= g.i0 && ((j >= g.j0 && j < g.j0 + TJ) || (j == 0 && g.j0 == 1))) o = ij(g, i, j);
         }
       } else if ((e -= LV) < LW) {  // wd: columns [i0-1, i0+TI], rows [j0-1, j0+TJ]
-        const int i = g.i0 - 2 + e % SI, j = g.j0 - 1 + e / SI;
-        if (e < SI * NJW && i >= g.i0 - 1 && i <= g.i0 + TI && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) {
+        kind = 2;
+        const int i = g.i0 - 1 + e % NIW, j = g.j0 - 1 + e / NIW;
+        if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) {
           p = at(g, i, j);
           if (i >= g.i0 && i < g.i0 + TI && j >= g.j0 && j < g.j0 + TJ) o = ij(g, i, j);
         }
       }
-      const int kind = (vt + r * NT) < LU ? 0 : (vt + r * NT) < LU + LV ? 1 : 2;
       pl.fp[r] = p < 0 ? -1 : (p | (kind << 28));
       pl.fo[r] = o;
     }
     for (int r = 0; r < RFL; ++r) {  // flags: cells [i0-1, i0+TI] x [j0-1, j0+TJ]
-      const int e = vt + r * NT;
+      const int e = vf + r * NT;
       const int i = g.i0 - 2 + e % SI, j = g.j0 - 1 + e / SI;
       const bool need = e < NFL && i >= g.i0 - 1 && i <= g.i0 + TI;
       pl.lp[r] = need ? (at(g, i, j) | ((i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) ? INTERIOR : 0)) : -1;
@@ -352,19 +363,19 @@ struct MpTile {
       if (kind == 0) {
         if (quv) {
           T v = mp_anti_u<T>(C, Uf, Vf, Wf);
-          sh.ud[p] = v;
+          sh.ud[HC_MP_CHECKC(p)] = v;
           if (a.ud && wuv && pl.fo[r] >= 0) (a.ud + pq)[pl.fo[r]] = v;
         }
       } else if (kind == 1) {
         if (quv) {
           T v = mp_anti_v<T>(C, Uf, Vf, Wf);
-          sh.vd[p] = v;
+          sh.vd[HC_MP_CHECKC(p)] = v;
           if (a.vd && wuv && pl.fo[r] >= 0) (a.vd + pq)[pl.fo[r]] = v;
         }
       } else {
         if (qw) {
           T v = mp_anti_w<T>(C, Uf, Vf, Wf);
-          sh.wd[q & 1][p] = v;
+          sh.wd[q & 1][HC_MP_CHECKC(p)] = v;
           if (a.wd && ww && pl.fo[r] >= 0) (a.wd + pq)[pl.fo[r]] = v;
         }
       }
@@ -391,7 +402,7 @@ struct MpTile {
         if (x <= (T)a.cmin) f = 1;
         else if (x >= (T)a.cmax) f = 2;
       }
-      fl[HC_MP_CHECK(p)] = (unsigned char)f;
+      fl[HC_MP_CHECKC(p)] = (unsigned char)f;
     }
   }
 
@@ -399,7 +410,7 @@ struct MpTile {
   // values of plane q into registers and finish plane kk = q-1
   HC_HD static void finish(const MpGeom &g, const Plan &pl, Sh &sh, const MpPass<T> &a, int q, MpRegs<T> &r) {
     const int p = pl.cp;
-    const T nu0 = sh.ud[HC_MP_CHECK(p - 1)], nu1 = sh.ud[p], nv0 = sh.vd[HC_MP_CHECK(p - SI)], nv1 = sh.vd[p];
+    const T nu0 = sh.ud[HC_MP_CHECKC(p - 1)], nu1 = sh.ud[HC_MP_CHECKC(p)], nv0 = sh.vd[HC_MP_CHECKC(p - SI)], nv1 = sh.vd[p];
     const T nw = sh.wd[q & 1][p];
     const int nf = sh.fl[q & 1][p];
     const int kk = q - 1;
@@ -409,8 +420,8 @@ struct MpTile {
       T cc = c0[p];
       T ce = mp_donor<T>(cc, c0[HC_MP_CHECK(p - 1)], c0[HC_MP_CHECK(p + 1)], c0[HC_MP_CHECK(p - SI)],
                          c0[HC_MP_CHECK(p + SI)], sh.c[(kk - 1) & 3][p], sh.c[(kk + 1) & 3][p], r.u0, r.u1, r.v0,
-                         r.v1, r.wA, r.wB, true, r.fB, f0[HC_MP_CHECK(p - 1)], f0[HC_MP_CHECK(p + 1)],
-                         f0[HC_MP_CHECK(p - SI)], f0[HC_MP_CHECK(p + SI)], r.fA, nf);
+                         r.v1, r.wA, r.wB, true, r.fB, f0[HC_MP_CHECKC(p - 1)], f0[HC_MP_CHECKC(p + 1)],
+                         f0[HC_MP_CHECKC(p - SI)], f0[HC_MP_CHECKC(p + SI)], r.fA, nf);
       T cn = cc + ce;
       const long pk = plane(g, kk);
       if (!a.last) {

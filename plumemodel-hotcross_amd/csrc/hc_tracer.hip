@@ -222,11 +222,11 @@ static int mp_chunk(int nz) {
 }
 
 // tile shape of the corrective-pass kernel (HC_MP_TILE = 0: 32x8, 1: 16x16,
-// 2: 32x16)
+// 2: 32x16, the default: two 512-thread blocks of 79 KiB LDS per CU)
 static int mp_tile() {
   static int t = [] {
     const char *e = getenv("HC_MP_TILE");
-    return e ? atoi(e) : 0;
+    return e ? atoi(e) : 2;
   }();
   return t;
 }
