@@ -7,6 +7,9 @@ namespace hc {
 
 typedef hc_params Phys;
 
+// XCD-aware tile order for the LDS-tiled kernels, on unless HC_TILE_XCD=0
+int tile_remap();
+
 // vent / source patch state used by MOMENTUM_W and the tracer kernels
 struct VentArgs {
   int map_valid;            // isource_map set by a SOURCE_terms call

@@ -220,6 +220,7 @@ struct ForcesTile {
 
 template <typename T, bool P2, int TI, int TJ>
 __global__ void __launch_bounds__(TI *TJ) k_forces_tile(Layout L, StepArgs s, Phys ph, FieldPtrs<T> fp, int kc,
+                                                        int remap,
                                                         const double *__restrict__ azzmin,
                                                         const double *__restrict__ alphau,
                                                         const double *__restrict__ alphav,
@@ -229,8 +230,10 @@ __global__ void __launch_bounds__(TI *TJ) k_forces_tile(Layout L, StepArgs s, Ph
   constexpr int SI = FT::SI, S = FT::S, NT = FT::NT, RS = FT::RS, NF = FT::NF;
   __shared__ FT sh;
   const int tid = (int)threadIdx.x;
-  const int i0 = 1 + (int)blockIdx.x * TI, j0 = 1 + (int)blockIdx.y * TJ;
-  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
+  unsigned bx, by, bz;
+  box_block_coords(remap, bx, by, bz);  // XCD x gets the x-th slice of tiles
+  const int i0 = 1 + (int)bx * TI, j0 = 1 + (int)by * TJ;
+  const int ka = 1 + (int)bz * kc, kb = min(L.nz, ka + kc - 1);
   const T *src[NF] = {fp.u, fp.v, fp.w, fp.um, fp.vm, fp.wm, fp.ss, fp.ssv};
   // staged elements of this thread: (i,j) offsets in the padded box
   int so[RS];
@@ -404,52 +407,51 @@ __global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s,
 // over its whole box (the reference zeroes it first, :24); ss_v over its
 // interior and z ghosts.  The complex root at :112 keeps its real part,
 // sqrt(max(0,x)).
-template <typename T, bool P2>
-__device__ inline T ss_val(const Layout &L, const Phys &ph, const StepArgs &s, const T *um, const T *vm,
-                           const T *wm, int i, int j, int k) {
+template <typename T, bool P2, class V>
+__device__ inline T ss_val(const Layout &L, const Phys &ph, const StepArgs &s, const V &um, const V &vm,
+                           const V &wm, int i, int j, int k) {
   const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   double ls = sqrt(s.dx * s.dy);
   T coef = (T)(ph.ismag_h * ((ph.cs_h * ls) * (ph.cs_h * ls)));
-  T s11 = (F3(um, i, j, k) - F3(um, i - 1, j, k)) / dx;
-  T s22 = (F3(vm, i, j, k) - F3(vm, i, j - 1, k)) / dy;
-  T s33 = (F3(wm, i, j, k) - F3(wm, i, j, k - 1)) / dz;
+  T s11 = (um(i, j, k) - um(i - 1, j, k)) / dx;
+  T s22 = (vm(i, j, k) - vm(i, j - 1, k)) / dy;
+  T s33 = (wm(i, j, k) - wm(i, j, k - 1)) / dz;
   T s12 = (T)0.25 *
-          ((F3(um, i, j + 1, k) + F3(um, i - 1, j + 1, k) - F3(um, i, j - 1, k) - F3(um, i - 1, j - 1, k)) / dy +
-           (F3(vm, i + 1, j, k) + F3(vm, i + 1, j - 1, k) - F3(vm, i - 1, j, k) - F3(vm, i - 1, j - 1, k)) / dx) /
+          ((um(i, j + 1, k) + um(i - 1, j + 1, k) - um(i, j - 1, k) - um(i - 1, j - 1, k)) / dy +
+           (vm(i + 1, j, k) + vm(i + 1, j - 1, k) - vm(i - 1, j, k) - vm(i - 1, j - 1, k)) / dx) /
           (T)2.0;
   T q = (T)2.0 * (s11 * s11) + (T)2.0 * (s22 * s22) + (T)4.0 * (s12 * s12) + (T)2.0 * (s33 * s33);
   return coef * sqrt(q + (T)1.0e-30);
 }
 
-template <typename T, bool P2>
-__device__ inline T ssv_val(const Layout &L, const Phys &ph, const StepArgs &s, const T *um, const T *vm,
-                            const T *wm, const T *rhop, const double *rho_bkg, int i, int j, int k) {
+template <typename T, bool P2, class V, class VR>
+__device__ inline T ssv_val(const Layout &L, const Phys &ph, const StepArgs &s, const V &um, const V &vm,
+                            const V &wm, const VR &rhop, const double *rho_bkg, int i, int j, int k) {
   const Sp<T, P2> dx((T)s.dx, (T)s.idx), dy((T)s.dy, (T)s.idy), dz((T)s.dz, (T)s.idz);
   T coef = (T)(ph.ismag_v * ((ph.cs_v * s.dz) * (ph.cs_v * s.dz)));
   T s13 = (T)0.25 *
-          ((F3(um, i, j, k - 1) + F3(um, i - 1, j, k - 1) - F3(um, i, j, k + 1) - F3(um, i - 1, j, k + 1)) / dz +
-           (F3(wm, i + 1, j, k) + F3(wm, i + 1, j, k - 1) - F3(wm, i - 1, j, k) - F3(wm, i - 1, j, k - 1)) / dx) /
+          ((um(i, j, k - 1) + um(i - 1, j, k - 1) - um(i, j, k + 1) - um(i - 1, j, k + 1)) / dz +
+           (wm(i + 1, j, k) + wm(i + 1, j, k - 1) - wm(i - 1, j, k) - wm(i - 1, j, k - 1)) / dx) /
           (T)2.0;
   T s23 = (T)0.25 *
-          ((F3(vm, i, j, k - 1) + F3(vm, i, j - 1, k - 1) - F3(vm, i, j, k + 1) - F3(vm, i, j - 1, k + 1)) / dz +
-           (F3(wm, i, j + 1, k) + F3(wm, i, j + 1, k - 1) - F3(wm, i, j - 1, k) - F3(wm, i, j - 1, k - 1)) / dy) /
+          ((vm(i, j, k - 1) + vm(i, j - 1, k - 1) - vm(i, j, k + 1) - vm(i, j - 1, k + 1)) / dz +
+           (wm(i, j + 1, k) + wm(i, j + 1, k - 1) - wm(i, j - 1, k) - wm(i, j - 1, k - 1)) / dy) /
           (T)2.0;
   T sv = (T)4.0 * (s13 * s13) + (T)4.0 * (s23 * s23);
   // Richardson number in FP64: rho' + rho_bkg is ~1027 and its 1-m change is
   // ~1e-4 of that
-  double rkm = (double)F3(rhop, i, j, k - 1) + rho_bkg[k - 1];
-  double rkp = (double)F3(rhop, i, j, k + 1) + rho_bkg[k + 1];
+  double rkm = (double)rhop(i, j, k - 1) + rho_bkg[k - 1];
+  double rkp = (double)rhop(i, j, k + 1) + rho_bkg[k + 1];
   double ri = -(ph.g * (rkm - rkp) / 2.0 / Sp<double, P2>(s.dz, s.idz) / ph.rho0 / ((double)sv + 1.0e-30));
   double arg = 1.0 + ph.eps - ri / 3.0;
   return coef * sqrt(sv + (T)1.0e-30) * (T)sqrt(fmax(0.0, arg));
 }
 
-template <typename T, bool P2>
-__global__ void __launch_bounds__(256) k_viscosity(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ um,
-                                                   const T *__restrict__ vm, const T *__restrict__ wm,
-                                                   const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
-                                                   T *__restrict__ ss, T *__restrict__ ssv) {
-  HC_BOX_IJK(b);  // box: i=0..nx+1, j=0..ny+1, k=0..nz+1
+// one cell of the viscosity box (interior formula, ghost-cell rules)
+template <typename T, bool P2, class V, class VR>
+__device__ inline void visc_at(const Layout &L, const StepArgs &s, const Phys &ph, const V &um, const V &vm,
+                               const V &wm, const VR &rhop, const double *rho_bkg, T *ss, T *ssv, int i, int j,
+                               int k) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   bool gi = (i == 0 || i == nx + 1), gj = (j == 0 || j == ny + 1), gk = (k == 0 || k == nz + 1);
   int ng = (int)gi + (int)gj + (int)gk;
@@ -474,6 +476,107 @@ __global__ void __launch_bounds__(256) k_viscosity(Layout L, Box b, StepArgs s, 
     F3(ssv, i, j, k) = ssv_val<T, P2>(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k);
   } else if (ng == 1 && gk) {
     F3(ssv, i, j, k) = ssv_val<T, P2>(L, ph, s, um, vm, wm, rhop, rho_bkg, i, j, k == 0 ? 1 : nz);
+  }
+}
+
+template <typename T, bool P2>
+__global__ void __launch_bounds__(256) k_viscosity(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ um,
+                                                   const T *__restrict__ vm, const T *__restrict__ wm,
+                                                   const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
+                                                   T *__restrict__ ss, T *__restrict__ ssv) {
+  HC_BOX_IJK(b);  // box: i=0..nx+1, j=0..ny+1, k=0..nz+1
+  const GView<T> UM{um, L}, VM{vm, L}, WM{wm, L}, RH{rhop, L};
+  visc_at<T, P2>(L, s, ph, UM, VM, WM, RH, rho_bkg, ss, ssv, i, j, k);
+}
+
+// the ghost shell of the viscosity box, one thread per shell cell: the x
+// faces (with edges), the y faces, then the z faces
+template <typename T, bool P2>
+__global__ void __launch_bounds__(256) k_visc_shell(Layout L, StepArgs s, Phys ph, const T *__restrict__ um,
+                                                    const T *__restrict__ vm, const T *__restrict__ wm,
+                                                    const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
+                                                    T *__restrict__ ss, T *__restrict__ ssv) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long na = 2L * (ny + 2) * (nz + 2), nb = 2L * nx * (nz + 2), nc = 2L * nx * ny;
+  int i, j, k;
+  if (t < na) {
+    j = (int)(t % (ny + 2)); long r = t / (ny + 2);
+    k = (int)(r % (nz + 2)); i = (r / (nz + 2)) ? nx + 1 : 0;
+  } else if ((t -= na) < nb) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    k = (int)(r % (nz + 2)); j = (r / (nz + 2)) ? ny + 1 : 0;
+  } else if ((t -= nb) < nc) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    j = 1 + (int)(r % ny); k = (r / ny) ? nz + 1 : 0;
+  } else {
+    return;
+  }
+  const GView<T> UM{um, L}, VM{vm, L}, WM{wm, L}, RH{rhop, L};
+  visc_at<T, P2>(L, s, ph, UM, VM, WM, RH, rho_bkg, ss, ssv, i, j, k);
+}
+
+// the interior i = 1..nx, j = 1..ny, k = 1..nz as a k-marching LDS tile with
+// planes k-1, k, k+1 of um, vm, wm (one-cell halo), as k_forces_tile
+template <typename T, bool P2, int TI, int TJ>
+__global__ void __launch_bounds__(TI *TJ) k_visc_tile(Layout L, StepArgs s, Phys ph, const T *__restrict__ um,
+                                                      const T *__restrict__ vm, const T *__restrict__ wm,
+                                                      const T *__restrict__ rhop,
+                                                      const double *__restrict__ rho_bkg, T *__restrict__ ss,
+                                                      T *__restrict__ ssv, int kc, int remap) {
+  constexpr int SI = TI + 2, S = SI * (TJ + 2), NT = TI * TJ, RS = (S + NT - 1) / NT, NF = 3;
+  __shared__ T sh[NF][3][S];
+  const int tid = (int)threadIdx.x;
+  unsigned bx, by, bz;
+  box_block_coords(remap, bx, by, bz);
+  const int i0 = 1 + (int)bx * TI, j0 = 1 + (int)by * TJ;
+  const int ka = 1 + (int)bz * kc, kb = min(L.nz, ka + kc - 1);
+  const T *src[NF] = {um, vm, wm};
+  int so[RS];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int e = tid + r * NT;
+    const int i = i0 - 1 + e % SI, j = j0 - 1 + e / SI;
+    so[r] = (e < S && i <= L.nx + 1 && j <= L.ny + 1) ? IX(L, i, j, -1) : -1;
+  }
+  const int pstride = L.px * L.py;
+  T pf[NF][RS];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) pf[f][r] = so[r] >= 0 ? src[f][so[r] + (k + 1) * pstride] : (T)0;
+  };
+  auto commit = [&](int k) {
+    const int slot = k % 3;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) {
+        const int e = tid + r * NT;
+        if (e < S) sh[f][slot][e] = pf[f][r];
+      }
+  };
+  fetch(ka - 1);
+  commit(ka - 1);
+  fetch(ka);
+  commit(ka);
+  fetch(ka + 1);
+  const int i = i0 + tid % TI, j = j0 + tid / TI;
+  const bool own = i <= L.nx && j <= L.ny;
+  for (int k = ka; k <= kb; ++k) {
+    __syncthreads();
+    commit(k + 1);
+    __syncthreads();
+    if (k < kb) fetch(k + 2);
+    if (own) {
+      const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
+      auto view = [&](int f) { return LView<T, SI>{sh[f][sm], sh[f][s0], sh[f][sp], k, i0 - 1, j0 - 1}; };
+      const LView<T, SI> UM = view(0), VM = view(1), WM = view(2);
+      const GView<T> RH{rhop, L};
+      F3(ss, i, j, k) = ss_val<T, P2>(L, ph, s, UM, VM, WM, i, j, k);
+      F3(ssv, i, j, k) = ssv_val<T, P2>(L, ph, s, UM, VM, WM, RH, rho_bkg, i, j, k);
+    }
   }
 }
 
@@ -538,8 +641,8 @@ __global__ void __launch_bounds__(256) k_refresh(size_t n, int leap, double asse
 // ---------------------------------------------------------------------------
 // host launchers
 // launch the power-of-two-spacing instantiation when s.p2 is set
-// the LDS-tiled FORCES (HC_FORCES_CELL=1 selects the cell-parallel kernel
-// for the whole box, for comparing the two)
+// the LDS-tiled FORCES and VISCOSITY3D (HC_FORCES_CELL=1 selects the
+// cell-parallel kernels for the whole boxes, for comparing the two)
 static bool forces_tiled() {
   static bool t = [] {
     const char *e = getenv("HC_FORCES_CELL");
@@ -572,10 +675,10 @@ void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   const int kc = std::min(L.nz, 32);
   dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
   if (s.p2)
-    hipLaunchKernelGGL((k_forces_tile<T, true, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f, kc, aux.azzmin,
+    hipLaunchKernelGGL((k_forces_tile<T, true, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f, kc, tile_remap(), aux.azzmin,
                        aux.alphau, aux.alphav, aux.alphae0, aux.alphaw);
   else
-    hipLaunchKernelGGL((k_forces_tile<T, false, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f, kc, aux.azzmin,
+    hipLaunchKernelGGL((k_forces_tile<T, false, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f, kc, tile_remap(), aux.azzmin,
                        aux.alphau, aux.alphav, aux.alphae0, aux.alphaw);
 }
 
@@ -600,9 +703,24 @@ void launch_momentum(hipStream_t st, int comp, const Layout &L, const StepArgs &
 template <typename T>
 void launch_viscosity(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                       const Aux &aux) {
-  Box b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
-  HC_P2_LAUNCH(k_viscosity, box_grid(b), box_block(), 0, st, L, b, s, ph, f.um, f.vm, f.wm, f.rhop,
-                     aux.rho_bkg, f.ss, f.ssv);
+  if (!forces_tiled()) {
+    Box b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+    HC_P2_LAUNCH(k_viscosity, box_grid(b), box_block(), 0, st, L, b, s, ph, f.um, f.vm, f.wm, f.rhop, aux.rho_bkg,
+                 f.ss, f.ssv);
+    return;
+  }
+  const long nshell = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
+  HC_P2_LAUNCH(k_visc_shell, dim3((unsigned)((nshell + 255) / 256)), dim3(256), 0, st, L, s, ph, f.um, f.vm, f.wm,
+               f.rhop, aux.rho_bkg, f.ss, f.ssv);
+  constexpr int TI = 32, TJ = 8;
+  const int kc = std::min(L.nz, 32);
+  dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
+  if (s.p2)
+    hipLaunchKernelGGL((k_visc_tile<T, true, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f.um, f.vm, f.wm,
+                       f.rhop, aux.rho_bkg, f.ss, f.ssv, kc, tile_remap());
+  else
+    hipLaunchKernelGGL((k_visc_tile<T, false, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f.um, f.vm, f.wm,
+                       f.rhop, aux.rho_bkg, f.ss, f.ssv, kc, tile_remap());
 }
 
 template <typename T>

@@ -83,21 +83,23 @@ __global__ void __launch_bounds__(256) k_pass1(Layout L, Box b, StepArgs s, Phys
 // one corrective pass (antidiffusive velocities, limiter flags, limited
 // donor-cell step; sponge and source on the last pass) as a k-marching tile
 template <typename T, int TI, int TJ>
-__global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, int kc) {
+__global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, int kc, int remap) {
   typedef MpTile<T, TI, TJ> Tl;
   __shared__ MpShared<T, TI, TJ> sh;
+  unsigned bx, by, bz;
+  box_block_coords(remap, bx, by, bz);  // XCD x gets the x-th slice of tiles
   MpGeom g;
   g.nx = L.nx; g.ny = L.ny; g.nz = L.nz; g.px = L.px; g.py = L.py;
-  g.i0 = 1 + (int)blockIdx.x * TI;
-  g.j0 = 1 + (int)blockIdx.y * TJ;
-  g.ka = 1 + (int)blockIdx.z * kc;
+  g.i0 = 1 + (int)bx * TI;
+  g.j0 = 1 + (int)by * TJ;
+  g.ka = 1 + (int)bz * kc;
   g.kb = min(L.nz, g.ka + kc - 1);
   const int tid = (int)threadIdx.x;
   const int qs = max(g.ka - 2, 0);
   MpRegs<T> r{};
   typename Tl::Plan pl;
   typename Tl::Prefetch pf;
-  Tl::plan(g, tid, (int)(blockIdx.x + blockIdx.y + blockIdx.z), pl);
+  Tl::plan(g, tid, (int)(bx + by + bz), pl);
   Tl::stage_initial(g, tid, pl, sh, a, qs);
   Tl::fetch(g, pl, a, qs, pf);
   for (int q = qs; q <= g.kb + 1; ++q) {
@@ -221,6 +223,15 @@ static int mp_chunk(int nz) {
   return kc < nz ? kc : nz;
 }
 
+// XCD-aware tile order for the LDS-tiled kernels (HC_TILE_XCD=0 turns it off)
+int tile_remap() {
+  static int r = [] {
+    const char *e = getenv("HC_TILE_XCD");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return r;
+}
+
 // tile shape of the corrective-pass kernel (HC_MP_TILE = 0: 32x8, 1: 16x16,
 // 2: 32x16, the default: two 512-thread blocks of 79 KiB LDS per CU)
 static int mp_tile() {
@@ -235,7 +246,7 @@ template <typename T, int TI, int TJ>
 static void launch_mpdata_tile(hipStream_t st, const Layout &L, const MpPass<T> &a) {
   const int kc = mp_chunk(L.nz);
   dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
-  hipLaunchKernelGGL((k_mpdata_pass<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc);
+  hipLaunchKernelGGL((k_mpdata_pass<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc, tile_remap());
 }
 
 template <typename T>
