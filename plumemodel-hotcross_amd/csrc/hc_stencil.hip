@@ -13,18 +13,12 @@
 
 #include "hc_common.h"
 #include "hc_kernels.h"
+#include "hc_tile.h"
 
 namespace hc {
 
 #define F3(a, i, j, k) (a)[IX(L, (i), (j), (k))]
 
-// a field read through the padded-box index
-template <typename T>
-struct GView {
-  const T *p;
-  Layout L;
-  __device__ T operator()(int i, int j, int k) const { return p[IX(L, i, j, k)]; }
-};
 
 // ---------------------------------------------------------------------------
 // FORCES, hotcross80-revised2014.f:890-1139.  azu/azv are formed on the fly;
@@ -201,89 +195,27 @@ __global__ void __launch_bounds__(256) k_forces(Layout L, Box b, StepArgs s, Phy
 // registers a plane ahead), and the three force components of the cell are
 // formed from LDS with the same expressions as k_forces.  The faces i = 0
 // (fx), j = 0 (fy) and k = 0 (fz) are left to k_forces.
-template <typename T, int SI>
-struct LView {
-  const T *pm, *p0, *pp;  // planes k-1, k, k+1 of one field
-  int k, ib, jb;          // current plane, LDS origin (i0-1, j0-1)
-  __device__ T operator()(int i, int j, int kk) const {
-    const T *b = kk < k ? pm : (kk > k ? pp : p0);
-    return b[(i - ib) + SI * (j - jb)];
-  }
-};
-
-template <typename T, int TI, int TJ>
-struct ForcesTile {
-  static constexpr int SI = TI + 2, SJ = TJ + 2, S = SI * SJ, NT = TI * TJ, RS = (S + NT - 1) / NT;
-  static constexpr int NF = 8;  // u v w um vm wm ss ss_v
-  T f[NF][3][S];
-};
-
 template <typename T, bool P2, int TI, int TJ>
 __global__ void __launch_bounds__(TI *TJ) k_forces_tile(Layout L, StepArgs s, Phys ph, FieldPtrs<T> fp, int kc,
-                                                        int remap,
-                                                        const double *__restrict__ azzmin,
+                                                        int remap, const double *__restrict__ azzmin,
                                                         const double *__restrict__ alphau,
                                                         const double *__restrict__ alphav,
                                                         const double *__restrict__ alphae0,
                                                         const double *__restrict__ alphaw) {
-  typedef ForcesTile<T, TI, TJ> FT;
-  constexpr int SI = FT::SI, S = FT::S, NT = FT::NT, RS = FT::RS, NF = FT::NF;
-  __shared__ FT sh;
-  const int tid = (int)threadIdx.x;
+  __shared__ Ring3<T, TI, TJ, 8> sh;
   unsigned bx, by, bz;
   box_block_coords(remap, bx, by, bz);  // XCD x gets the x-th slice of tiles
   const int i0 = 1 + (int)bx * TI, j0 = 1 + (int)by * TJ;
   const int ka = 1 + (int)bz * kc, kb = min(L.nz, ka + kc - 1);
-  const T *src[NF] = {fp.u, fp.v, fp.w, fp.um, fp.vm, fp.wm, fp.ss, fp.ssv};
-  // staged elements of this thread: (i,j) offsets in the padded box
-  int so[RS];
-#pragma unroll
-  for (int r = 0; r < RS; ++r) {
-    const int e = tid + r * NT;
-    const int i = i0 - 1 + e % SI, j = j0 - 1 + e / SI;
-    so[r] = (e < S && i <= L.nx + 1 && j <= L.ny + 1) ? IX(L, i, j, -1) : -1;
-  }
-  const int pstride = L.px * L.py;
-  T pf[NF][RS];
-  auto fetch = [&](int k) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int r = 0; r < RS; ++r) pf[f][r] = so[r] >= 0 ? src[f][so[r] + (k + 1) * pstride] : (T)0;
-  };
-  auto commit = [&](int k) {
-    const int slot = k % 3;
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int r = 0; r < RS; ++r) {
-        const int e = tid + r * NT;
-        if (e < S) sh.f[f][slot][e] = pf[f][r];
-      }
-  };
-  fetch(ka - 1);
-  commit(ka - 1);
-  fetch(ka);
-  commit(ka);
-  fetch(ka + 1);
-  const int i = i0 + tid % TI, j = j0 + tid / TI;
-  const bool own = i <= L.nx && j <= L.ny;
-  for (int k = ka; k <= kb; ++k) {
-    __syncthreads();  // the previous step's reads of slot (k+1)%3 are done
-    commit(k + 1);
-    __syncthreads();
-    if (k < kb) fetch(k + 2);  // in flight during this step
-    if (own) {
-      const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
-      auto view = [&](int f) { return LView<T, SI>{sh.f[f][sm], sh.f[f][s0], sh.f[f][sp], k, i0 - 1, j0 - 1}; };
-      const LView<T, SI> U = view(0), Vv = view(1), W = view(2), UM = view(3), VM = view(4), WM = view(5),
-                         SS = view(6), SV = view(7);
-      const GView<T> RH{fp.rhop, L};
-      forces_x_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphau, fp.fx);
-      forces_y_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphav, fp.fy);
-      forces_z_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, fp.fz);
-    }
-  }
+  const T *src[8] = {fp.u, fp.v, fp.w, fp.um, fp.vm, fp.wm, fp.ss, fp.ssv};
+  ring3_march(sh, L, src, i0, j0, ka, kb, [&](int i, int j, int k, auto &view) {
+    const auto U = view(0), Vv = view(1), W = view(2), UM = view(3), VM = view(4), WM = view(5), SS = view(6),
+               SV = view(7);
+    const GView<T> RH{fp.rhop, L};
+    forces_x_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphau, fp.fx);
+    forces_y_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphav, fp.fy);
+    forces_z_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, fp.fz);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -524,60 +456,18 @@ __global__ void __launch_bounds__(TI *TJ) k_visc_tile(Layout L, StepArgs s, Phys
                                                       const T *__restrict__ rhop,
                                                       const double *__restrict__ rho_bkg, T *__restrict__ ss,
                                                       T *__restrict__ ssv, int kc, int remap) {
-  constexpr int SI = TI + 2, S = SI * (TJ + 2), NT = TI * TJ, RS = (S + NT - 1) / NT, NF = 3;
-  __shared__ T sh[NF][3][S];
-  const int tid = (int)threadIdx.x;
+  __shared__ Ring3<T, TI, TJ, 3> sh;
   unsigned bx, by, bz;
   box_block_coords(remap, bx, by, bz);
   const int i0 = 1 + (int)bx * TI, j0 = 1 + (int)by * TJ;
   const int ka = 1 + (int)bz * kc, kb = min(L.nz, ka + kc - 1);
-  const T *src[NF] = {um, vm, wm};
-  int so[RS];
-#pragma unroll
-  for (int r = 0; r < RS; ++r) {
-    const int e = tid + r * NT;
-    const int i = i0 - 1 + e % SI, j = j0 - 1 + e / SI;
-    so[r] = (e < S && i <= L.nx + 1 && j <= L.ny + 1) ? IX(L, i, j, -1) : -1;
-  }
-  const int pstride = L.px * L.py;
-  T pf[NF][RS];
-  auto fetch = [&](int k) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int r = 0; r < RS; ++r) pf[f][r] = so[r] >= 0 ? src[f][so[r] + (k + 1) * pstride] : (T)0;
-  };
-  auto commit = [&](int k) {
-    const int slot = k % 3;
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int r = 0; r < RS; ++r) {
-        const int e = tid + r * NT;
-        if (e < S) sh[f][slot][e] = pf[f][r];
-      }
-  };
-  fetch(ka - 1);
-  commit(ka - 1);
-  fetch(ka);
-  commit(ka);
-  fetch(ka + 1);
-  const int i = i0 + tid % TI, j = j0 + tid / TI;
-  const bool own = i <= L.nx && j <= L.ny;
-  for (int k = ka; k <= kb; ++k) {
-    __syncthreads();
-    commit(k + 1);
-    __syncthreads();
-    if (k < kb) fetch(k + 2);
-    if (own) {
-      const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
-      auto view = [&](int f) { return LView<T, SI>{sh[f][sm], sh[f][s0], sh[f][sp], k, i0 - 1, j0 - 1}; };
-      const LView<T, SI> UM = view(0), VM = view(1), WM = view(2);
-      const GView<T> RH{rhop, L};
-      F3(ss, i, j, k) = ss_val<T, P2>(L, ph, s, UM, VM, WM, i, j, k);
-      F3(ssv, i, j, k) = ssv_val<T, P2>(L, ph, s, UM, VM, WM, RH, rho_bkg, i, j, k);
-    }
-  }
+  const T *src[3] = {um, vm, wm};
+  ring3_march(sh, L, src, i0, j0, ka, kb, [&](int i, int j, int k, auto &view) {
+    const auto UM = view(0), VM = view(1), WM = view(2);
+    const GView<T> RH{rhop, L};
+    F3(ss, i, j, k) = ss_val<T, P2>(L, ph, s, UM, VM, WM, i, j, k);
+    F3(ssv, i, j, k) = ssv_val<T, P2>(L, ph, s, UM, VM, WM, RH, rho_bkg, i, j, k);
+  });
 }
 
 // ---------------------------------------------------------------------------

@@ -1,13 +1,13 @@
 // hc_tracer.hip - TRACER31 / MPDATA / BOUNDARY_CONDITIONS on gfx950.
 //
 // mpdata-hotcross80.f:16-494.  One tracer call is
-//   pass 1: explicit diffusion + donor-cell upwind step      (k_pass1)
+//   pass 1: explicit diffusion + donor-cell upwind step      (k_pass1_tile)
 //   pass 2/3: antidiffusive velocities, limiter flags and the limited
 //            donor-cell step in one k-marching LDS tile (k_mpdata_pass,
 //            hc_mpdata.h)
 //   sponge + source on the last pass, then the boundary images (k_bc).
 // The diffusivities kxt/kyt/kzt of MIXING (mixing80-revised2014.f:22-59) are
-// formed inside k_pass1 from ss/ss_v with the reference's expressions.
+// formed inside pass 1 from ss/ss_v with the reference's expressions.
 //
 // The limiter (mpdata-hotcross80.f:660-690) zeroes face entries from both of
 // a face's cells; the reference does this by scattering zeros from every cell
@@ -25,6 +25,7 @@
 #include "hc_common.h"
 #include "hc_kernels.h"
 #include "hc_mpdata.h"
+#include "hc_tile.h"
 
 namespace hc {
 
@@ -46,38 +47,56 @@ __global__ void __launch_bounds__(256) k_courant(Layout L, Box b, double tau3, d
 }
 
 // pass 1: diffusion (mpdata-hotcross80.f:141-174) + upwind step (:176-184)
-template <typename T>
-__global__ void __launch_bounds__(256) k_pass1(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ c,
-                                               const T *__restrict__ uc, const T *__restrict__ vc,
-                                               const T *__restrict__ wc, const T *__restrict__ ss,
-                                               const T *__restrict__ ssv, const double *__restrict__ azzmin,
-                                               const double *__restrict__ cbkg, T *__restrict__ c1) {
-  HC_BOX_IJK(b);  // interior
+template <typename T, class V>
+__device__ inline T pass1_at(const Layout &L, const StepArgs &s, const Phys &ph, const V &c, const V &uc,
+                             const V &vc, const V &wc, const V &ss, const V &ssv, const double *azzmin,
+                             const double *cbkg, int i, int j, int k) {
   const double pr = ph.prandtl;
   const T xfac = (T)((double)(1 * ph.bit5k) / s.dx / s.dx * s.tau3 * 1);
   const T yfac = (T)((double)(1 * ph.bit5k) / s.dy / s.dy * s.tau3 * 1);
   const T zfac = (T)((double)ph.bit2k / s.dz / s.dz * s.tau3 * 1);
   const T b5 = (T)ph.bit5k, b2 = (T)ph.bit2k;
   const T axp = (T)(ph.axmin / pr), ayp = (T)(ph.aymin / pr), tpr = (T)pr;
-  T cp = F3(c, i, j, k);
-  T kx1 = b5 * (((T).5 * (F3(ss, i, j, k) + F3(ss, i + 1, j, k))) / tpr + axp);
-  T kx0 = b5 * (((T).5 * (F3(ss, i - 1, j, k) + F3(ss, i, j, k))) / tpr + axp);
-  T ky1 = b5 * (((T).5 * (F3(ss, i, j, k) + F3(ss, i, j + 1, k))) / tpr + ayp);
-  T ky0 = b5 * (((T).5 * (F3(ss, i, j - 1, k) + F3(ss, i, j, k))) / tpr + ayp);
+  T cp = c(i, j, k);
+  T kx1 = b5 * (((T).5 * (ss(i, j, k) + ss(i + 1, j, k))) / tpr + axp);
+  T kx0 = b5 * (((T).5 * (ss(i - 1, j, k) + ss(i, j, k))) / tpr + axp);
+  T ky1 = b5 * (((T).5 * (ss(i, j, k) + ss(i, j + 1, k))) / tpr + ayp);
+  T ky0 = b5 * (((T).5 * (ss(i, j - 1, k) + ss(i, j, k))) / tpr + ayp);
   // kzt(k) = 0 at k = 0 and k = nz (mixing80-revised2014.f:49-59)
-  T kz1 = (k == L.nz) ? (T)0 : b2 * (((T)0.5 * (F3(ssv, i, j, k) + F3(ssv, i, j, k + 1))) / tpr + (T)(azzmin[k] / pr));
+  T kz1 = (k == L.nz) ? (T)0 : b2 * (((T)0.5 * (ssv(i, j, k) + ssv(i, j, k + 1))) / tpr + (T)(azzmin[k] / pr));
   T kz0 = (k == 1) ? (T)0
-                   : b2 * (((T)0.5 * (F3(ssv, i, j, k - 1) + F3(ssv, i, j, k))) / tpr + (T)(azzmin[k - 1] / pr));
-  T cx = xfac * (kx1 * (F3(c, i + 1, j, k) - cp) - kx0 * (cp - F3(c, i - 1, j, k)));
-  T cy = yfac * (ky1 * (F3(c, i, j + 1, k) - cp) - ky0 * (cp - F3(c, i, j - 1, k)));
+                   : b2 * (((T)0.5 * (ssv(i, j, k - 1) + ssv(i, j, k))) / tpr + (T)(azzmin[k - 1] / pr));
+  T cx = xfac * (kx1 * (c(i + 1, j, k) - cp) - kx0 * (cp - c(i - 1, j, k)));
+  T cy = yfac * (ky1 * (c(i, j + 1, k) - cp) - ky0 * (cp - c(i, j - 1, k)));
   T bk = (T)cbkg[k], bkm = (T)cbkg[k - 1], bkp = (T)cbkg[k + 1];
-  T cz = zfac * (kz0 * ((F3(c, i, j, k - 1) + bk) - (bkm + cp)) - kz1 * ((cp + bkp) - (bk + F3(c, i, j, k + 1))));
+  T cz = zfac * (kz0 * ((c(i, j, k - 1) + bk) - (bkm + cp)) - kz1 * ((cp + bkp) - (bk + c(i, j, k + 1))));
   T cd = cp + (+cx + cy + cz);
-  T ce = mp_donor<T>(cp, F3(c, i - 1, j, k), F3(c, i + 1, j, k), F3(c, i, j - 1, k), F3(c, i, j + 1, k),
-                     F3(c, i, j, k - 1), F3(c, i, j, k + 1), F3(uc, i - 1, j, k), F3(uc, i, j, k),
-                     F3(vc, i, j - 1, k), F3(vc, i, j, k), F3(wc, i, j, k - 1), F3(wc, i, j, k), false, 0, 0, 0, 0,
+  T ce = mp_donor<T>(cp, c(i - 1, j, k), c(i + 1, j, k), c(i, j - 1, k), c(i, j + 1, k),
+                     c(i, j, k - 1), c(i, j, k + 1), uc(i - 1, j, k), uc(i, j, k),
+                     vc(i, j - 1, k), vc(i, j, k), wc(i, j, k - 1), wc(i, j, k), false, 0, 0, 0, 0,
                      0, 0, 0);
-  F3(c1, i, j, k) = cd + ce;
+  return cd + ce;
+}
+
+
+// pass 1 as a k-marching LDS tile (planes k-1, k, k+1 of the six inputs)
+template <typename T, int TI, int TJ>
+__global__ void __launch_bounds__(TI *TJ) k_pass1_tile(Layout L, StepArgs s, Phys ph, const T *__restrict__ c,
+                                                       const T *__restrict__ uc, const T *__restrict__ vc,
+                                                       const T *__restrict__ wc, const T *__restrict__ ss,
+                                                       const T *__restrict__ ssv, const double *__restrict__ azzmin,
+                                                       const double *__restrict__ cbkg, T *__restrict__ c1, int kc,
+                                                       int remap) {
+  __shared__ Ring3<T, TI, TJ, 6> sh;
+  unsigned bx, by, bz;
+  box_block_coords(remap, bx, by, bz);
+  const int i0 = 1 + (int)bx * TI, j0 = 1 + (int)by * TJ;
+  const int ka = 1 + (int)bz * kc, kb = min(L.nz, ka + kc - 1);
+  const T *src[6] = {c, uc, vc, wc, ss, ssv};
+  ring3_march(sh, L, src, i0, j0, ka, kb, [&](int i, int j, int k, auto &view) {
+    F3(c1, i, j, k) = pass1_at<T>(L, s, ph, view(0), view(1), view(2), view(3), view(4), view(5), azzmin, cbkg, i,
+                                  j, k);
+  });
 }
 
 // one corrective pass (antidiffusive velocities, limiter flags, limited
@@ -272,12 +291,16 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
                    const double *qpatch, const TracerArgs &ta, MarkFn mark, void *mark_arg) {
   StepArgs s2 = s;
   s2.tau3 = ta.tau3;
-  Box in{1, L.nx, 1, L.ny, 1, L.nz};
   const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
   const unsigned nbc = (unsigned)((nghost + 255) / 256);
   T *c1 = (T *)tw.c1, *c2 = (T *)tw.c2;
-  hipLaunchKernelGGL(k_pass1<T>, box_grid(in), box_block(), 0, st, L, in, s2, ph, c, (T *)tw.uc, (T *)tw.vc,
-                     (T *)tw.wc, f.ss, f.ssv, aux.azzmin, c_bkg, c1);
+  {
+    constexpr int TI = 32, TJ = 8;
+    const int kc = L.nz < 32 ? L.nz : 32;
+    dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
+    hipLaunchKernelGGL((k_pass1_tile<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s2, ph, c, (T *)tw.uc,
+                       (T *)tw.vc, (T *)tw.wc, f.ss, f.ssv, aux.azzmin, c_bkg, c1, kc, tile_remap());
+  }
   hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c1, c1, 0, 0.0);
   if (ph.ismol != 1) {
     // no corrective passes: sponge/source straight after pass 1 through the
