@@ -68,6 +68,10 @@ struct hc_ctx {
   // multiply by the exact reciprocal of power-of-two spacings (HC_NO_P2=1
   // at creation forces true divisions, for comparing the two forms)
   bool allow_p2 = true;
+  // native loop: MOMENTUM also forms the Courant numbers and REFRESH's
+  // filtered fields (into spare[c]); HC_NO_FUSE=1 at creation turns it off
+  bool fuse = true;
+  void *spare[3] = {nullptr, nullptr, nullptr};
 };
 
 #define HIPCHK(x)                                                                    \
@@ -280,6 +284,8 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
     box_remap_flag() = (e && e[0] == '1') ? 1 : 0;
     const char *np2 = getenv("HC_NO_P2");
     ctx->allow_p2 = !(np2 && np2[0] == '1');
+    const char *nf = getenv("HC_NO_FUSE");
+    ctx->fuse = !(nf && nf[0] == '1');
   }
   ctx->L = make_layout(P.nx, P.ny, P.nz);
   if (ctx->L.n >= ((size_t)1 << 31)) {
@@ -393,6 +399,8 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
                 ctx->d_noise, ctx->d_stage, ctx->d_cflpart, ctx->d_cflout, ctx->pw.r, ctx->pw.piv, ctx->pw.part,
                 ctx->pw.delta, ctx->d_tw};
   for (void *p : ps)
+    if (p) hipFree(p);
+  for (void *p : ctx->spare)
     if (p) hipFree(p);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   for (auto &p : ctx->phases) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
@@ -820,6 +828,43 @@ extern "C" int hc_cfl_values(hc_ctx *ctx, double out[6]) {
 }
 
 // the reference's time loop body, hotcross80-revised2014.f:264-350
+// MOMENTUM of the native loop with the work velocity (tau3) and REFRESH's
+// filter fused in; the filtered field waits in spare[comp] until the loop's
+// REFRESH point, where the buffers rotate
+static int momentum_fused(hc_ctx *ctx, int comp, bool cour, double tau3) {
+  PhaseScope ps(ctx, "momentum");
+  ctx->up_alias[comp] = false;
+  if (!ctx->spare[comp]) {
+    int rc = 0;
+    if (!(ctx->spare[comp] = dalloc(ctx, ctx->L.n * ctx->esz, &rc))) return rc;
+  }
+  StepArgs s = step_args(ctx);
+  VentArgs va = vent_args(ctx);
+  void *c = comp == 0 ? ctx->tw.uc : comp == 1 ? ctx->tw.vc : ctx->tw.wc;
+  if (ctx->prec == HC_FP64) {
+    MomFuse<double> mf{cour ? (double *)c : nullptr, (double *)ctx->spare[comp], tau3, ctx->P.assel, ctx->leap};
+    launch_momentum<double>(ctx->st, comp, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, &mf);
+  } else {
+    MomFuse<float> mf{cour ? (float *)c : nullptr, (float *)ctx->spare[comp], tau3, ctx->P.assel, ctx->leap};
+    launch_momentum<float>(ctx->st, comp, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, &mf);
+  }
+  return HC_OK;
+}
+
+// REFRESH of the native loop after momentum_fused: u <- up (buffer swap),
+// um <- the filtered field, the old um buffer becomes the spare
+static void refresh_rotate(hc_ctx *ctx) {
+  int tri[3][3] = {{HC_U, HC_UP, HC_UM}, {HC_V, HC_VP, HC_VM}, {HC_W, HC_WP, HC_WM}};
+  for (int c = 0; c < 3; ++c) {
+    void *u = ctx->f[tri[c][0]], *up = ctx->f[tri[c][1]], *um = ctx->f[tri[c][2]];
+    ctx->f[tri[c][0]] = up;
+    ctx->f[tri[c][1]] = u;
+    ctx->f[tri[c][2]] = ctx->spare[c];
+    ctx->spare[c] = um;
+    ctx->up_alias[c] = true;
+  }
+}
+
 extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
   const hc_params &P = ctx->P;
   int nz = P.nz;
@@ -832,9 +877,11 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
     hc_body_force(ctx);
     hc_forces(ctx);
     hc_pressure(ctx);
-    momentum(ctx, 0);
-    momentum(ctx, 1);
-    momentum(ctx, 2);
+    const bool fuse = ctx->fuse;
+    for (int c = 0; c < 3; ++c) {
+      int r = fuse ? momentum_fused(ctx, c, P.toggle_t == 1, P.dt) : momentum(ctx, c);
+      if (r) return r;
+    }
     hc_viscosity3d(ctx);
     {
       PhaseScope ps(ctx, "source");
@@ -842,13 +889,18 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
     }
     int budgets = (m % 100 == 0);
     if (P.toggle_t == 1) {
-      hc_work_velocity(ctx, P.dt);
+      if (!fuse) hc_work_velocity(ctx, P.dt);
       tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], budgets);
     }
     if (P.toggle_s == 1) tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], budgets);
     if (P.bit3 == 1) hc_density(ctx);
     if (P.toggle_c == 1) tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, budgets);
-    hc_refresh(ctx);
+    if (fuse) {
+      PhaseScope ps(ctx, "refresh");
+      refresh_rotate(ctx);
+    } else {
+      hc_refresh(ctx);
+    }
     if (m % 10 == 0) {
       double cfl[6];
       int r = hc_cfl_values(ctx, cfl);

@@ -35,9 +35,18 @@ struct Aux {
 template <typename T>
 void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                    const Aux &aux);
+// Fused epilogue of MOMENTUM in the native loop: the work velocity's Courant
+// number and REFRESH's filtered value, formed from the new up of each cell
+template <typename T>
+struct MomFuse {
+  T *cour;      // uc / vc / wc, or null when the work velocities are not formed
+  T *umn;       // filtered field, the next um
+  double tau3, assel;
+  int leap;
+};
 template <typename T>
 void launch_momentum(hipStream_t st, int comp, const Layout &L, const StepArgs &s, const Phys &ph,
-                     const VentArgs &va, const FieldPtrs<T> &f, const Aux &aux);
+                     const VentArgs &va, const FieldPtrs<T> &f, const Aux &aux, const MomFuse<T> *mf = nullptr);
 template <typename T>
 void launch_viscosity(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                       const Aux &aux);

@@ -223,6 +223,25 @@ __global__ void __launch_bounds__(TI *TJ) k_forces_tile(Layout L, StepArgs s, Ph
 // value of one cell of the written set, re-evaluating the interior formula at
 // the source cell of a boundary copy (a pure function of the inputs), so one
 // pass writes interior and ghosts with the reference's values.
+// Fused epilogue of the native loop: with the new up of a cell at hand, also
+// the work velocity's Courant number (hotcross80-revised2014.f:303-305,
+// mpdata-hotcross80.f:96-102, the expressions of k_courant) and REFRESH's
+// filtered value (:1421-1488, the expression of k_refresh) into a spare
+// buffer that becomes um when the loop reaches REFRESH.  Every cell of the
+// component's box is visited, written by MOMENTUM or not.
+template <typename T, bool P2>
+__device__ inline void mom_fuse(const Layout &L, const StepArgs &s, const MomFuse<T> &mf, bool wr, T val,
+                                const T *u, const T *um, const T *up, int i, int j, int k, int comp, bool cour_box) {
+  const T upv = wr ? val : F3(up, i, j, k);
+  const T a = F3(u, i, j, k);
+  if (mf.cour && cour_box) {
+    if (comp == 0) F3(mf.cour, i, j, k) = (T)0.5 * (upv + a) * (T)mf.tau3 / Sp<T, P2>((T)s.dx, (T)s.idx);
+    else if (comp == 1) F3(mf.cour, i, j, k) = (T)0.5 * (upv + a) * (T)mf.tau3 / Sp<T, P2>((T)s.dy, (T)s.idy);
+    else F3(mf.cour, i, j, k) = -((T)0.5 * (upv + a)) * (T)mf.tau3 / Sp<T, P2>((T)s.dz, (T)s.idz);
+  }
+  F3(mf.umn, i, j, k) = mf.leap == 1 ? upv : a + (T)mf.assel * (upv - (T)2 * a + F3(um, i, j, k));
+}
+
 template <typename T, bool P2>
 __device__ inline T mom_u_val(const Layout &L, const StepArgs &s, const T *u, const T *um, const T *p,
                               const T *fx, const double *pgradx, int i, int j, int k) {
@@ -230,17 +249,19 @@ __device__ inline T mom_u_val(const Layout &L, const StepArgs &s, const T *u, co
          (T)s.dt * (T)s.leap * (-(F3(p, i + 1, j, k) - F3(p, i, j, k)) / Sp<T, P2>((T)s.dx, (T)s.idx) - (T)pgradx[k] + F3(fx, i, j, k));
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool FUSE>
 __global__ void __launch_bounds__(256) k_momentum_u(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ u,
                                                     const T *__restrict__ um, const T *__restrict__ p,
                                                     const T *__restrict__ fx, const double *__restrict__ pgradx,
-                                                    T *__restrict__ up) {
+                                                    T *__restrict__ up, MomFuse<T> mf) {
   HC_BOX_IJK(b);  // box: i=-1..nx+1, j=0..ny+1, k=0..nz+1
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   bool gi = (i == -1 || i == nx + 1), gj = (j == 0 || j == ny + 1), gk = (k == 0 || k == nz + 1);
-  if ((int)gi + (int)gj + (int)gk > 1) return;  // edges keep -1e34
-  T val;
-  if (!gi && !gj && !gk) {
+  const bool wr = (int)gi + (int)gj + (int)gk <= 1;  // edges keep -1e34
+  if (!FUSE && !wr) return;
+  T val = (T)0;
+  if (!wr) {
+  } else if (!gi && !gj && !gk) {
     val = mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i, j, k);
   } else if (gk) {
     if (k == 0) val = mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i, j, 1);
@@ -250,7 +271,8 @@ __global__ void __launch_bounds__(256) k_momentum_u(Layout L, Box b, StepArgs s,
   } else {
     val = mom_u_val<T, P2>(L, s, u, um, p, fx, pgradx, i == -1 ? 0 : nx, j, k);
   }
-  F3(up, i, j, k) = val;
+  if (wr) F3(up, i, j, k) = val;
+  if (FUSE) mom_fuse<T, P2>(L, s, mf, wr, val, u, um, up, i, j, k, 0, i >= 0 && i <= nx);
 }
 
 template <typename T, bool P2>
@@ -260,17 +282,19 @@ __device__ inline T mom_v_val(const Layout &L, const StepArgs &s, const T *v, co
          (T)s.dt * (T)s.leap * (-(F3(p, i, j + 1, k) - F3(p, i, j, k)) / Sp<T, P2>((T)s.dy, (T)s.idy) - (T)pgrady[k] + F3(fy, i, j, k));
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool FUSE>
 __global__ void __launch_bounds__(256) k_momentum_v(Layout L, Box b, StepArgs s, Phys ph, const T *__restrict__ v,
                                                     const T *__restrict__ vm, const T *__restrict__ p,
                                                     const T *__restrict__ fy, const double *__restrict__ pgrady,
-                                                    T *__restrict__ vp) {
+                                                    T *__restrict__ vp, MomFuse<T> mf) {
   HC_BOX_IJK(b);  // box: i=0..nx+1, j=-1..ny+1, k=0..nz+1
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   bool gi = (i == 0 || i == nx + 1), gj = (j == -1 || j == ny + 1), gk = (k == 0 || k == nz + 1);
-  if ((int)gi + (int)gj + (int)gk > 1) return;
-  T val;
-  if (!gi && !gj && !gk) {
+  const bool wr = (int)gi + (int)gj + (int)gk <= 1;
+  if (!FUSE && !wr) return;
+  T val = (T)0;
+  if (!wr) {
+  } else if (!gi && !gj && !gk) {
     val = mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i, j, k);
   } else if (gk) {
     if (k == 0) val = mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i, j, 1);
@@ -280,7 +304,8 @@ __global__ void __launch_bounds__(256) k_momentum_v(Layout L, Box b, StepArgs s,
   } else {
     val = mom_v_val<T, P2>(L, s, v, vm, p, fy, pgrady, i, j == -1 ? 0 : ny, k);
   }
-  F3(vp, i, j, k) = val;
+  if (wr) F3(vp, i, j, k) = val;
+  if (FUSE) mom_fuse<T, P2>(L, s, mf, wr, val, v, vm, vp, i, j, k, 1, j >= 0 && j <= ny);
 }
 
 template <typename T, bool P2>
@@ -301,11 +326,12 @@ __device__ inline T w_vent(const VentArgs &va, const double *alphae, const Layou
   return (T)0.0;
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool FUSE>
 __global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s, Phys ph, VentArgs va,
                                                     const T *__restrict__ w, const T *__restrict__ wm,
                                                     const T *__restrict__ p, const T *__restrict__ fz,
-                                                    const double *__restrict__ alphae, T *__restrict__ wp) {
+                                                    const double *__restrict__ alphae, T *__restrict__ wp,
+                                                    MomFuse<T> mf) {
   HC_BOX_IJK(b);  // box: i=0..nx+1, j=0..ny+1, k=-1..nz+1
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   bool ii = (i >= 1 && i <= nx), jj = (j >= 1 && j <= ny);
@@ -315,7 +341,8 @@ __global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s,
     if (k2 == nz) return w_vent<T>(va, alphae, L, i2, j2);
     return (T)0.0;  // k2 = 0
   };
-  T val;
+  T val = (T)0;
+  bool wr = true;
   if (ii && jj) {
     if (k >= 1 && k <= nz - 1) val = mom_w_val<T, P2>(L, s, w, wm, p, fz, i, j, k);
     else if (k == nz || k == nz + 1) val = w_vent<T>(va, alphae, L, i, j);
@@ -329,9 +356,11 @@ __global__ void __launch_bounds__(256) k_momentum_w(Layout L, Box b, StepArgs s,
     T src = (j >= 1) ? inner(i2, j, k) : (T)0 * inner(i2, 1, k);
     val = (T)0 * src;
   } else {
-    return;  // not written by MOMENTUM_W
+    wr = false;  // not written by MOMENTUM_W
+    if (!FUSE) return;
   }
-  F3(wp, i, j, k) = val;
+  if (wr) F3(wp, i, j, k) = val;
+  if (FUSE) mom_fuse<T, P2>(L, s, mf, wr, val, w, wm, wp, i, j, k, 2, k >= 0 && k <= nz);
 }
 
 // ---------------------------------------------------------------------------
@@ -572,21 +601,33 @@ void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
                        aux.alphau, aux.alphav, aux.alphae0, aux.alphaw);
 }
 
+template <typename T, class... A>
+static void mom_launch(void (*k00)(A...), void (*k01)(A...), void (*k10)(A...), void (*k11)(A...), bool p2, bool fuse,
+                       dim3 g, hipStream_t st, A... args) {
+  auto k = p2 ? (fuse ? k11 : k10) : (fuse ? k01 : k00);
+  hipLaunchKernelGGL(k, g, box_block(), 0, st, args...);
+}
+
 template <typename T>
 void launch_momentum(hipStream_t st, int comp, const Layout &L, const StepArgs &s, const Phys &ph,
-                     const VentArgs &va, const FieldPtrs<T> &f, const Aux &aux) {
+                     const VentArgs &va, const FieldPtrs<T> &f, const Aux &aux, const MomFuse<T> *mf) {
+  const MomFuse<T> m = mf ? *mf : MomFuse<T>{};
+  const bool fu = mf != nullptr;
   if (comp == 0) {
     Box b{-1, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
-    HC_P2_LAUNCH(k_momentum_u, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.um, f.p, f.fx,
-                       aux.pgradx, f.up);
+    mom_launch<T>(k_momentum_u<T, false, false>, k_momentum_u<T, false, true>, k_momentum_u<T, true, false>,
+                  k_momentum_u<T, true, true>, s.p2 != 0, fu, box_grid(b), st, L, b, s, ph, (const T *)f.u,
+                  (const T *)f.um, (const T *)f.p, (const T *)f.fx, (const double *)aux.pgradx, f.up, m);
   } else if (comp == 1) {
     Box b{0, L.nx + 1, -1, L.ny + 1, 0, L.nz + 1};
-    HC_P2_LAUNCH(k_momentum_v, box_grid(b), box_block(), 0, st, L, b, s, ph, f.v, f.vm, f.p, f.fy,
-                       aux.pgrady, f.vp);
+    mom_launch<T>(k_momentum_v<T, false, false>, k_momentum_v<T, false, true>, k_momentum_v<T, true, false>,
+                  k_momentum_v<T, true, true>, s.p2 != 0, fu, box_grid(b), st, L, b, s, ph, (const T *)f.v,
+                  (const T *)f.vm, (const T *)f.p, (const T *)f.fy, (const double *)aux.pgrady, f.vp, m);
   } else {
     Box b{0, L.nx + 1, 0, L.ny + 1, -1, L.nz + 1};
-    HC_P2_LAUNCH(k_momentum_w, box_grid(b), box_block(), 0, st, L, b, s, ph, va, f.w, f.wm, f.p, f.fz,
-                       aux.alphae, f.wp);
+    mom_launch<T>(k_momentum_w<T, false, false>, k_momentum_w<T, false, true>, k_momentum_w<T, true, false>,
+                  k_momentum_w<T, true, true>, s.p2 != 0, fu, box_grid(b), st, L, b, s, ph, va, (const T *)f.w,
+                  (const T *)f.wm, (const T *)f.p, (const T *)f.fz, (const double *)aux.alphae, f.wp, m);
   }
 }
 
@@ -629,7 +670,7 @@ void launch_refresh(hipStream_t st, const Layout &L, int leap, double assel, con
   template void launch_forces<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,                 \
                                  const FieldPtrs<T> &, const Aux &);                                          \
   template void launch_momentum<T>(hipStream_t, int, const Layout &, const StepArgs &, const Phys &,          \
-                                   const VentArgs &, const FieldPtrs<T> &, const Aux &);                      \
+                                   const VentArgs &, const FieldPtrs<T> &, const Aux &, const MomFuse<T> *);  \
   template void launch_viscosity<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,              \
                                     const FieldPtrs<T> &, const Aux &);                                       \
   template void launch_density<T>(hipStream_t, const Layout &, const Phys &, const FieldPtrs<T> &,            \
