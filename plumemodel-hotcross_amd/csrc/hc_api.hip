@@ -11,154 +11,11 @@
 
 #include "../../include/hotcross.h"
 #include "hc_common.h"
+#include "hc_ctx.h"
 #include "hc_kernels.h"
 #include "hc_rng.h"
 
 using namespace hc;
-
-struct Phase {
-  const char *name;
-  double ms;
-  int count;
-  hipEvent_t a, b;
-  bool pending;
-};
-
-struct hc_ctx {
-  hc_params P;
-  int prec;
-  size_t esz;
-  Layout L;
-  hipStream_t st = nullptr;
-  std::string err;
-  void *f[HC_NFIELDS] = {};
-  // case arrays (device + host copies)
-  Aux aux{};
-  double *d_tbkg = nullptr, *d_sbkg = nullptr, *d_zero = nullptr, *d_cbkg = nullptr;
-  std::vector<double> h_tbkg, h_sbkg, h_rho_bkg, h_azzmin, h_alphae, h_alphae0, h_alphau, h_alphav, h_alphaw;
-  TracerWork tw{};
-  PoissonWork pw{};
-  double *d_tw = nullptr;
-  double *d_q[3] = {};       // q_t, q_s, q_c on the patch
-  double *d_noise = nullptr; // 3 draws per patch cell
-  double *d_stage = nullptr; // host<->device staging
-  size_t stage_n = 0;
-  double *d_cflpart = nullptr, *d_cflout = nullptr;
-  double *h_pin = nullptr;   // pinned host scratch for small results
-  // step state
-  int mtime = -1, leap = 1, delta = 0;
-  double time = 0, tau = 0;
-  double ubkg_m = 0, ubkg = 0, ubkg_p = 0, vbkg_m = 0, vbkg = 0, vbkg_p = 0, fbx = 0, fby = 0;
-  double fc = 0, fc_y = 0;
-  double acc[8] = {};
-  int nxs = 0, nxe = 0, nys = 0, nye = 0, npatch = 0;
-  bool map_valid = false;
-  // after REFRESH the reference's up/vp/wp equal u/v/w (u = up copies,
-  // hotcross80-revised2014.f:1421-1488); here the buffers are swapped instead and
-  // up_alias marks that the logical up is the u buffer until MOMENTUM writes it
-  bool up_alias[3] = {false, false, false};
-  double sumup = 0, sumdown = 0;
-  uint64_t seed = 12345;
-  int next_step = 0;
-  double last_budget[3] = {};
-  // timing
-  bool timing = false;
-  std::vector<Phase> phases;
-  int mark_q = -1;
-  // multiply by the exact reciprocal of power-of-two spacings (HC_NO_P2=1
-  // at creation forces true divisions, for comparing the two forms)
-  bool allow_p2 = true;
-  // native loop: MOMENTUM also forms the Courant numbers and REFRESH's
-  // filtered fields (into spare[c]); HC_NO_FUSE=1 at creation turns it off
-  bool fuse = true;
-  void *spare[3] = {nullptr, nullptr, nullptr};
-};
-
-#define HIPCHK(x)                                                                    \
-  do {                                                                               \
-    hipError_t e_ = (x);                                                             \
-    if (e_ != hipSuccess) {                                                          \
-      ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);                     \
-      return HC_EDEVICE;                                                             \
-    }                                                                                \
-  } while (0)
-
-static int fail(hc_ctx *ctx, int code, const std::string &msg) {
-  if (ctx) ctx->err = msg;
-  return code;
-}
-
-// ---------------------------------------------------------------------------
-// phase timing with HIP events on the context stream
-static int phase_begin(hc_ctx *ctx, const char *name) {
-  if (!ctx->timing) return -1;
-  for (size_t q = 0; q < ctx->phases.size(); ++q)
-    if (!strcmp(ctx->phases[q].name, name)) {
-      Phase &p = ctx->phases[q];
-      if (p.pending) {
-        float ms = 0;
-        hipEventSynchronize(p.b);
-        hipEventElapsedTime(&ms, p.a, p.b);
-        p.ms += ms;
-        p.count++;
-        p.pending = false;
-      }
-      hipEventRecord(p.a, ctx->st);
-      return (int)q;
-    }
-  Phase p{name, 0.0, 0, nullptr, nullptr, false};
-  hipEventCreate(&p.a);
-  hipEventCreate(&p.b);
-  hipEventRecord(p.a, ctx->st);
-  ctx->phases.push_back(p);
-  return (int)ctx->phases.size() - 1;
-}
-static void phase_end(hc_ctx *ctx, int q) {
-  if (q < 0) return;
-  hipEventRecord(ctx->phases[q].b, ctx->st);
-  ctx->phases[q].pending = true;
-}
-struct PhaseScope {
-  hc_ctx *c;
-  int q;
-  PhaseScope(hc_ctx *c_, const char *n) : c(c_), q(phase_begin(c_, n)) {}
-  ~PhaseScope() { phase_end(c, q); }
-};
-
-// ---------------------------------------------------------------------------
-template <typename T>
-static FieldPtrs<T> ptrs(hc_ctx *ctx) {
-  FieldPtrs<T> p;
-  T **a = (T **)&p;
-  for (int q = 0; q < HC_NFIELDS; ++q) a[q] = (T *)ctx->f[q];
-  return p;
-}
-
-static StepArgs step_args(hc_ctx *ctx) {
-  StepArgs s{};
-  s.dx = ctx->P.dx; s.dy = ctx->P.dy; s.dz = ctx->P.dz; s.dt = ctx->P.dt; s.tau3 = ctx->P.dt;
-  s.idx = 1.0 / s.dx; s.idy = 1.0 / s.dy; s.idz = 1.0 / s.dz;
-  s.p2 = ctx->prec == HC_FP64
-             ? (pow2_spacing<double>(s.dx) && pow2_spacing<double>(s.dy) && pow2_spacing<double>(s.dz))
-             : (pow2_spacing<float>(s.dx) && pow2_spacing<float>(s.dy) && pow2_spacing<float>(s.dz));
-  if (!ctx->allow_p2) s.p2 = 0;
-  s.leap = ctx->leap; s.delta = ctx->delta; s.mtime = ctx->mtime;
-  s.ubkg_m = ctx->ubkg_m; s.ubkg = ctx->ubkg; s.vbkg_m = ctx->vbkg_m; s.vbkg = ctx->vbkg;
-  s.fbx = ctx->fbx; s.fby = ctx->fby; s.fc = ctx->fc; s.fc_y = ctx->fc_y;
-  for (int q = 0; q < 8; ++q) s.acc[q] = ctx->acc[q];
-  return s;
-}
-
-static VentArgs vent_args(hc_ctx *ctx) {
-  VentArgs v{};
-  v.map_valid = ctx->map_valid;
-  v.nxs = ctx->nxs; v.nxe = ctx->nxe; v.nys = ctx->nys; v.nye = ctx->nye;
-  double wup = ctx->P.wsource * fmin(1.0, (ctx->mtime * ctx->P.dt) / (ctx->P.wramp_hrs * 3600.0));
-  v.wup = wup;
-  v.sumup = ctx->sumup;
-  v.sumdown = ctx->sumdown;
-  return v;
-}
 
 // sumup/sumdown of MOMENTUM_W (hotcross80-revised2014.f:823-833)
 static void update_sums(hc_ctx *ctx) {
@@ -194,16 +51,11 @@ __global__ void k_gather(Layout L, int l0, int l1, int l2, int n0, int n1, int n
   int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
   dst[q] = (double)src[IX(L, l0 + i, l1 + j, l2 + k)];
 }
-template <typename T>
-__global__ void k_fill(size_t n, T *__restrict__ p, double v) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) p[q] = (T)v;
-}
 
 // q patch of SOURCE_terms (source80.inc:40-49) from the noise draws
 __global__ void k_source(int npatch, double fq_t, double fq_s, double wee, const double *__restrict__ noise,
-                         int use_hash, uint64_t seed, int mtime, double *__restrict__ qt, double *__restrict__ qs,
-                         double *__restrict__ qc) {
+                         int use_hash, uint64_t seed, int mtime, const double *__restrict__ qmap,
+                         double *__restrict__ qt, double *__restrict__ qs, double *__restrict__ qc) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= npatch) return;
   double r1, r2;
@@ -215,43 +67,12 @@ __global__ void k_source(int npatch, double fq_t, double fq_s, double wee, const
     r2 = noise[3 * n + 1];
   }
   qt[n] = fq_t * (1.0 + wee * (2.0 * (r1 - .5)));
+  if (qmap) qt[n] = qt[n] * qmap[n];
   qs[n] = fq_s * (1.0 + wee * (2.0 * (r2 - .5)));
   qc[n] = 1.0;
 }
 
 // ---------------------------------------------------------------------------
-static void field_bounds(int field, const hc_params &P, int lo[3], int hi[3]) {
-  lo[0] = 0; lo[1] = 0; lo[2] = 0;
-  hi[0] = P.nx + 1; hi[1] = P.ny + 1; hi[2] = P.nz + 1;
-  switch (field) {
-    case HC_U: case HC_UP: case HC_UM: case HC_FX: lo[0] = -1; break;
-    case HC_V: case HC_VP: case HC_VM: case HC_FY: lo[1] = -1; break;
-    case HC_W: case HC_WP: case HC_WM: case HC_FZ: lo[2] = -1; break;
-    default: break;
-  }
-}
-
-template <typename T>
-static void launch_fill(hc_ctx *ctx, void *p, double v) {
-  size_t n = ctx->L.n;
-  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->st, n, (T *)p, v);
-}
-static void fill(hc_ctx *ctx, void *p, double v) {
-  if (ctx->prec == HC_FP64) launch_fill<double>(ctx, p, v);
-  else launch_fill<float>(ctx, p, v);
-}
-
-static void *dalloc(hc_ctx *ctx, size_t bytes, int *rc) {
-  void *p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) {
-    *rc = HC_ENOMEM;
-    ctx->err = "hipMalloc failed for " + std::to_string(bytes) + " bytes";
-    return nullptr;
-  }
-  hipMemsetAsync(p, 0, bytes, ctx->st);
-  return p;
-}
-
 extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
   if (!prm || !out) return HC_EINVAL;
   *out = nullptr;
@@ -403,6 +224,8 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
   for (void *p : ctx->spare)
     if (p) hipFree(p);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
+  if (ctx->d_qmap) hipFree(ctx->d_qmap);
+  adj_free(ctx);
   for (auto &p : ctx->phases) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
   if (ctx->st) hipStreamDestroy(ctx->st);
   delete ctx;
@@ -657,16 +480,29 @@ extern "C" int hc_body_force(hc_ctx *ctx) {
   const hc_params &P = ctx->P;
   ctx->tau = ctx->leap * P.dt;
   int l = (int)lround(ctx->time / P.dt);
-  double d;
-  flow_series(P, l - 1, &ctx->ubkg_m, &d); flow_series(P, l - 1, &d, &ctx->vbkg_m);
-  flow_series(P, l, &ctx->ubkg, &d); flow_series(P, l, &d, &ctx->vbkg);
-  flow_series(P, l + 1, &ctx->ubkg_p, &d); flow_series(P, l + 1, &d, &ctx->vbkg_p);
-  double v4 = ((ctx->vbkg + ctx->vbkg) + ctx->vbkg) + ctx->vbkg;
-  double u4 = ((ctx->ubkg + ctx->ubkg) + ctx->ubkg) + ctx->ubkg;
-  ctx->fbx = +(ctx->ubkg_p - (1 - ctx->delta) * ctx->ubkg - ctx->delta * ctx->ubkg_m) / ctx->tau -
-             P.bit7 * ctx->fc * v4 / 4.0;
-  ctx->fby = +(ctx->vbkg_p - (1 - ctx->delta) * ctx->vbkg - ctx->delta * ctx->vbkg_m) / ctx->tau +
-             P.bit7 * ctx->fc * u4 / 4.0;
+  // the series at unit amplitude, then scaled by the crossflow amplitude
+  // (1 unless a caller sets it: x * 1.0 is exact)
+  double d, um1, u0, up1, vm1, v0, vp1;
+  flow_series(P, l - 1, &um1, &d); flow_series(P, l - 1, &d, &vm1);
+  flow_series(P, l, &u0, &d); flow_series(P, l, &d, &v0);
+  flow_series(P, l + 1, &up1, &d); flow_series(P, l + 1, &d, &vp1);
+  const double A = ctx->flow_scale;
+  ctx->ubkg_m = A * um1; ctx->ubkg = A * u0; ctx->ubkg_p = A * up1;
+  ctx->vbkg_m = A * vm1; ctx->vbkg = A * v0; ctx->vbkg_p = A * vp1;
+  auto fbx_of = [&](double upp, double uc, double umm, double vc) {
+    double v4 = ((vc + vc) + vc) + vc;
+    return +(upp - (1 - ctx->delta) * uc - ctx->delta * umm) / ctx->tau - P.bit7 * ctx->fc * v4 / 4.0;
+  };
+  auto fby_of = [&](double vpp, double vc, double vmm, double uc) {
+    double u4 = ((uc + uc) + uc) + uc;
+    return +(vpp - (1 - ctx->delta) * vc - ctx->delta * vmm) / ctx->tau + P.bit7 * ctx->fc * u4 / 4.0;
+  };
+  ctx->fbx = fbx_of(ctx->ubkg_p, ctx->ubkg, ctx->ubkg_m, ctx->vbkg);
+  ctx->fby = fby_of(ctx->vbkg_p, ctx->vbkg, ctx->vbkg_m, ctx->ubkg);
+  // every one of these scalars is linear in the amplitude
+  ctx->dflow[0] = um1; ctx->dflow[1] = u0; ctx->dflow[2] = vm1; ctx->dflow[3] = v0;
+  ctx->dflow[4] = fbx_of(up1, u0, um1, v0);
+  ctx->dflow[5] = fby_of(vp1, v0, vm1, u0);
   return HC_OK;
 }
 
@@ -715,7 +551,7 @@ extern "C" int hc_source_terms(hc_ctx *ctx, const double *noise) {
   double f = fmin(1.0, ctx->mtime * P.dt / (P.source_ramp_hrs * 3600.0));
   if (noise) HIPCHK(hipMemcpyAsync(ctx->d_noise, noise, (size_t)ctx->npatch * 3 * 8, hipMemcpyHostToDevice, ctx->st));
   hipLaunchKernelGGL(k_source, dim3((ctx->npatch + 255) / 256), dim3(256), 0, ctx->st, ctx->npatch, P.dqtdt * f,
-                     P.dqsdt * f, P.noise_amp, ctx->d_noise, noise ? 0 : 1, ctx->seed, ctx->mtime, ctx->d_q[0],
+                     P.dqsdt * f, P.noise_amp, ctx->d_noise, noise ? 0 : 1, ctx->seed, ctx->mtime, ctx->d_qmap, ctx->d_q[0],
                      ctx->d_q[1], ctx->d_q[2]);
   if (noise) HIPCHK(hipStreamSynchronize(ctx->st));
   if (!ctx->map_valid) {

@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "hc_common.h"
+#include "hc_cell.h"
 #include "hc_kernels.h"
 #include "hc_tile.h"
 
@@ -21,35 +22,7 @@ namespace hc {
 
 
 // ---------------------------------------------------------------------------
-// FORCES, hotcross80-revised2014.f:890-1139.  azu/azv are formed on the fly;
-// their never-recomputed boundary entries are the persistent accumulators
-// acc[i] (the entrance ramp of :953-956 adds (7-i)*0.01/7 to them per call).
-template <typename T, class V>
-__device__ inline T azu_at(const Layout &L, const V &ssv, const double *azzmin,
-                           const StepArgs &s, int i, int j, int k) {
-  // azu(i,j,k) for i=0..nx, j=1..ny, k=0..nz
-  if (i == 0) return k >= 1 ? (T)s.acc[0] : (T)0;
-  if (i == L.nx) return (T)0;
-  T v = (T).25 * (ssv(i, j, k) + ssv(i + 1, j, k) + ssv(i, j, k + 1) + ssv(i + 1, j, k + 1)) +
-        (T)azzmin[k];
-  if (i <= 7 && k >= 1) v = v + (T)((double)(7 - i) * .01 / 7.0);
-  return v;
-}
-
-template <typename T, class V>
-__device__ inline T azv_at(const Layout &L, const V &ssv, const double *azzmin,
-                           const StepArgs &s, int i, int j, int k) {
-  // azv(i,j,k) for i=1..nx, j=0..ny, k=0..nz
-  if (j == 0 || j == L.ny) {
-    if (k == 0 || i > 7) return (T)0;
-    return (T)s.acc[i];
-  }
-  T v = (T).25 * (ssv(i, j, k) + ssv(i, j + 1, k) + ssv(i, j, k + 1) + ssv(i, j + 1, k + 1)) +
-        (T)azzmin[k];
-  if (i <= 7 && k >= 1) v = v + (T)((double)(7 - i) * .01 / 7.0);
-  return v;
-}
-
+// FORCES, hotcross80-revised2014.f:890-1139 (azu_at / azv_at in hc_cell.h).
 template <typename T, bool P2, class V>
 __device__ inline void forces_x_at(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph, const V &u,
                                                   const V &v, const V &w,

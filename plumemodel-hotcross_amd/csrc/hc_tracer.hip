@@ -24,6 +24,7 @@
 
 #include "hc_common.h"
 #include "hc_kernels.h"
+#include "hc_cell.h"
 #include "hc_mpdata.h"
 #include "hc_tile.h"
 
@@ -45,39 +46,6 @@ __global__ void __launch_bounds__(256) k_courant(Layout L, Box b, double tau3, d
   if (j <= L.ny) vc[q] = (T)0.5 * (vp[q] + v[q]) * (T)tau3 / (T)dy;
   if (k <= L.nz) wc[q] = -((T)0.5 * (wp[q] + w[q])) * (T)tau3 / (T)dz;
 }
-
-// pass 1: diffusion (mpdata-hotcross80.f:141-174) + upwind step (:176-184)
-template <typename T, class V>
-__device__ inline T pass1_at(const Layout &L, const StepArgs &s, const Phys &ph, const V &c, const V &uc,
-                             const V &vc, const V &wc, const V &ss, const V &ssv, const double *azzmin,
-                             const double *cbkg, int i, int j, int k) {
-  const double pr = ph.prandtl;
-  const T xfac = (T)((double)(1 * ph.bit5k) / s.dx / s.dx * s.tau3 * 1);
-  const T yfac = (T)((double)(1 * ph.bit5k) / s.dy / s.dy * s.tau3 * 1);
-  const T zfac = (T)((double)ph.bit2k / s.dz / s.dz * s.tau3 * 1);
-  const T b5 = (T)ph.bit5k, b2 = (T)ph.bit2k;
-  const T axp = (T)(ph.axmin / pr), ayp = (T)(ph.aymin / pr), tpr = (T)pr;
-  T cp = c(i, j, k);
-  T kx1 = b5 * (((T).5 * (ss(i, j, k) + ss(i + 1, j, k))) / tpr + axp);
-  T kx0 = b5 * (((T).5 * (ss(i - 1, j, k) + ss(i, j, k))) / tpr + axp);
-  T ky1 = b5 * (((T).5 * (ss(i, j, k) + ss(i, j + 1, k))) / tpr + ayp);
-  T ky0 = b5 * (((T).5 * (ss(i, j - 1, k) + ss(i, j, k))) / tpr + ayp);
-  // kzt(k) = 0 at k = 0 and k = nz (mixing80-revised2014.f:49-59)
-  T kz1 = (k == L.nz) ? (T)0 : b2 * (((T)0.5 * (ssv(i, j, k) + ssv(i, j, k + 1))) / tpr + (T)(azzmin[k] / pr));
-  T kz0 = (k == 1) ? (T)0
-                   : b2 * (((T)0.5 * (ssv(i, j, k - 1) + ssv(i, j, k))) / tpr + (T)(azzmin[k - 1] / pr));
-  T cx = xfac * (kx1 * (c(i + 1, j, k) - cp) - kx0 * (cp - c(i - 1, j, k)));
-  T cy = yfac * (ky1 * (c(i, j + 1, k) - cp) - ky0 * (cp - c(i, j - 1, k)));
-  T bk = (T)cbkg[k], bkm = (T)cbkg[k - 1], bkp = (T)cbkg[k + 1];
-  T cz = zfac * (kz0 * ((c(i, j, k - 1) + bk) - (bkm + cp)) - kz1 * ((cp + bkp) - (bk + c(i, j, k + 1))));
-  T cd = cp + (+cx + cy + cz);
-  T ce = mp_donor<T>(cp, c(i - 1, j, k), c(i + 1, j, k), c(i, j - 1, k), c(i, j + 1, k),
-                     c(i, j, k - 1), c(i, j, k + 1), uc(i - 1, j, k), uc(i, j, k),
-                     vc(i, j - 1, k), vc(i, j, k), wc(i, j, k - 1), wc(i, j, k), false, 0, 0, 0, 0,
-                     0, 0, 0);
-  return cd + ce;
-}
-
 
 // pass 1 as a k-marching LDS tile (planes k-1, k, k+1 of the six inputs)
 template <typename T, int TI, int TJ>
@@ -156,19 +124,8 @@ __global__ void __launch_bounds__(256) k_bc(Layout L, Phys ph, VentArgs va, T *_
     return;
   }
   bool gi = (i == 0 || i == nx + 1), gj = (j == 0 || j == ny + 1);
-  int si = i, sj = j, sk = k;
-  if (k == 0) sk = 1;
-  if (k == nz + 1) sk = nz;
-  if (i == 0) si = ph.bcondx ? 1 : nx;
-  if (i == nx + 1) si = ph.bcondx ? nx : 1;
-  if (j == 0) sj = ph.bcondy ? 1 : ny;
-  if (j == ny + 1) sj = ph.bcondy ? ny : 1;
-  // corner columns of mpdata-hotcross80.f:485-492
-  int mx = ph.modulo_x, my = ph.modulo_y;
-  if (gi && gj && k >= 1 && k <= nz && ((mx == 1 && my == 0) || (mx == 0 && my == 1))) {
-    if (mx == 1) si = (i == 0) ? nx : 1;
-    else sj = (j == 0) ? ny : 1;
-  }
+  int si, sj, sk;
+  bc_source(L, ph, i, j, k, si, sj, sk);
   T val = src[IX(L, si, sj, sk)];
   if (vent && k == nz + 1 && !gi && !gj && va.map_valid && i >= va.nxs && i <= va.nxe && j >= va.nys &&
       j <= va.nye)
