@@ -182,6 +182,67 @@ int hc_step(hc_ctx *ctx, int32_t nsteps);
  * patch cells.  For checkers that replay a native run. */
 int hc_source_noise(hc_ctx *ctx, int32_t mtime, double *out);
 
+/* ---- controls and discrete adjoint --------------------------------------
+ * The reference has no adjoint; these entries differentiate the timestep
+ * above.  Each hc_adj_<routine> is the transpose of the linearisation of the
+ * forward entry of the same name at the context's current state (the state
+ * the forward entry would read): it takes the adjoint of what the forward
+ * entry writes from the adjoint fields, sets that adjoint to zero (the
+ * forward overwrites it) and adds to the adjoints of what the forward reads.
+ * Calling them in the reverse order of a forward sequence, each at the state
+ * its forward call saw, gives the adjoint of the sequence.  Non-smooth
+ * operations differentiate along the branch the forward took (|x| -> sign x,
+ * the MPDATA limiter's zeroed fluxes held fixed, sqrt(max(0,x)) -> 0 below
+ * zero). */
+
+/* Controls the adjoint differentiates: qmap (npatch values, NULL = 1) scales
+ * the vent heat flux of each source cell (q_t of source80.inc:46), flow_scale
+ * multiplies the crossflow series ut, vt (site_forcing.inc:51-72), cs > 0
+ * sets cs_h = cs_v (the Smagorinsky constants; cs <= 0 leaves them). */
+int hc_set_controls(hc_ctx *ctx, const double *qmap, double flow_scale, double cs);
+
+/* Adjoint fields use the forward field ids, plus the adjoints of the work
+ * velocities' Courant numbers (mpdata-hotcross80.f:96-102), which have the
+ * bounds of u, v and w. */
+#define HC_ADJ_UC 32
+#define HC_ADJ_VC 33
+#define HC_ADJ_WC 34
+/* Allocate (first call) and zero every adjoint field and gradient. */
+int hc_adj_reset(hc_ctx *ctx);
+int hc_adj_upload(hc_ctx *ctx, int field, const double *host, const int32_t lo[3], const int32_t hi[3]);
+int hc_adj_download(hc_ctx *ctx, int field, double *host, const int32_t lo[3], const int32_t hi[3]);
+
+int hc_adj_forces(hc_ctx *ctx);
+int hc_adj_pressure(hc_ctx *ctx);
+int hc_adj_momentum_u(hc_ctx *ctx);
+int hc_adj_momentum_v(hc_ctx *ctx);
+int hc_adj_momentum_w(hc_ctx *ctx);
+int hc_adj_viscosity3d(hc_ctx *ctx);
+int hc_adj_work_velocity(hc_ctx *ctx, double tau3);
+/* TRACER31 of field (T, S or C) with the arguments of hc_tracer31; the
+ * field's adjoint becomes the adjoint of its value before the call. */
+int hc_adj_tracer31(hc_ctx *ctx, int field, const double *c_bkg, double cmin, double cmax, double tau3);
+int hc_adj_vent_ghost(hc_ctx *ctx, int field);
+int hc_adj_density(hc_ctx *ctx);
+int hc_adj_refresh(hc_ctx *ctx);
+/* Gradients accumulated since hc_adj_reset: gq (npatch values, may be NULL)
+ * = dL/dqmap; out = dL/dflow_scale, dL/dcs_h, dL/dcs_v. */
+int hc_adj_gradients(hc_ctx *ctx, double *gq, double out[3]);
+
+/* Probe-misfit window.  Probes at interior cells (i,j,k) sample w and T
+ * every `every` steps; the loss is
+ *   L = sum over samples and probes of 0.5*(wgt_w (w - w_obs)^2 + wgt_t (T - T_obs)^2).
+ * hc_adj_window runs nsteps steps of the native loop from the current state
+ * (without the passive tracer c), keeping a checkpoint every `seg` steps in
+ * HBM, then sweeps backwards, recomputing each segment into a tape; obs holds
+ * (w_obs, T_obs) per probe per sample (nsteps/every samples), samples
+ * (optional) receives the model's values in the same order.  On return the
+ * context holds the window's end state; gq and gs as hc_adj_gradients. */
+int hc_adj_set_probes(hc_ctx *ctx, int32_t nprobe, const int32_t *ijk, int32_t every, double wgt_w,
+                      double wgt_t);
+int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples, double *loss,
+                  double *gq, double gs[3]);
+
 /* Per-phase device time accumulated since the last reset (ms), measured with
  * HIP events on the context stream when enabled.  names/ms have room for
  * `cap` entries; returns the number of phases. */

@@ -24,11 +24,6 @@ struct RView {
   HC_HD T operator()(int i, int j, int k) const { return p[IX(L, i, j, k)]; }
 };
 
-// d(ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby)/d(flow_scale) at one step
-struct FlowSens {
-  double dubkg_m, dubkg, dvbkg_m, dvbkg, dfbx, dfby;
-};
-
 template <typename T>
 HC_HD inline T adj_sign(T x) {
   return x > (T)0 ? (T)1 : (x < (T)0 ? (T)-1 : (T)0);

@@ -1,0 +1,1104 @@
+// hc_adjoint.hip - discrete adjoint of the HotCross timestep on gfx950.
+//
+// The adjoint of each reference subroutine is a set of kernels over the
+// same boxes as the forward kernels.  Every adjoint entry "consumes" the
+// adjoint of what its forward counterpart overwrites (sets it to zero) and
+// adds to the adjoints of what it reads, so the adjoint of a sequence of
+// subroutine calls is the sequence of adjoint calls in reverse order.
+//
+// Stencils scatter: a thread takes the adjoint of one output cell (or face)
+// and adds lam * d(output)/d(input) to each input's adjoint with hardware
+// FP64 atomic adds (per-cell reverse formulas in hc_adj.h).  The linear
+// maps with regular structure (MOMENTUM, the pressure solve, REFRESH, the
+// work velocities, DENSITY) are gathers without atomics.
+//
+// MPDATA's non-smooth parts follow the forward branch: the limiter flags
+// of both corrective passes are recomputed from the forward values
+// (c1, c2 and the antidiffusive velocities, recomputed from the pass inputs
+// by the forward kernels) and held fixed; |U| differentiates to sign(U).
+//
+// The window driver (hc_adj_window) runs the forward with a checkpoint
+// every `seg` steps, evaluates a probe-misfit loss, then sweeps the segments
+// backwards: each segment is recomputed from its checkpoint into a tape of
+// its states in HBM, and its steps are reversed one by one (the step's
+// intermediates are recomputed from its taped state).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "hc_adj.h"
+#include "hc_ctx.h"
+
+namespace hc {
+
+#define F3(a, i, j, k) (a)[IX(L, (i), (j), (k))]
+
+// accumulator: hardware FP64 / FP32 atomic add into a padded field
+template <typename T>
+struct AAcc {
+  T *p;
+  Layout L;
+  __device__ void operator()(int i, int j, int k, T v) const { unsafeAtomicAdd(p + IX(L, i, j, k), v); }
+};
+// views and accumulators relative to a face, for the MPDATA face formulas
+template <typename T>
+struct RelR {
+  const T *p;
+  Layout L;
+  int i, j, k;
+  __device__ T operator()(int a, int b, int c) const { return p[IX(L, i + a, j + b, k + c)]; }
+};
+template <typename T>
+struct RelA {
+  T *p;
+  Layout L;
+  int i, j, k;
+  __device__ void operator()(int a, int b, int c, T v) const { unsafeAtomicAdd(p + IX(L, i + a, j + b, k + c), v); }
+};
+
+// sum of one double per thread into *out (one atomic per block); every
+// thread of the block must call it
+__device__ inline void block_add(double v, double *out) {
+  const int tid = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+  const int nt = (int)(blockDim.x * blockDim.y * blockDim.z);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __shared__ double sh[16];
+  if (tid % 64 == 0) sh[tid / 64] = v;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (nt + 63) / 64; ++w) s += sh[w];
+    if (s != 0.0) atomicAdd(out, s);
+  }
+}
+
+struct Box3 {
+  int i0, i1, j0, j1, k0, k1;
+};
+inline dim3 grid3(const Box3 &b) {
+  return dim3((unsigned)((b.i1 - b.i0 + 1 + 63) / 64), (unsigned)((b.j1 - b.j0 + 1 + 3) / 4), (unsigned)(b.k1 - b.k0 + 1));
+}
+#define ADJ_IJK(b)                                            \
+  const int i = (b).i0 + (int)(blockIdx.x * 64 + threadIdx.x); \
+  const int j = (b).j0 + (int)(blockIdx.y * 4 + threadIdx.y);  \
+  const int k = (b).k0 + (int)blockIdx.z;                      \
+  const bool in = i <= (b).i1 && j <= (b).j1;
+
+// ---------------------------------------------------------------------------
+// FORCES
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_forces(Layout L, Box3 b, StepArgs s, Phys ph, FlowSens fs,
+                                                    FieldPtrs<T> f, const T *__restrict__ ss0,
+                                                    const T *__restrict__ ssv0, FieldPtrs<T> lam,
+                                                    const double *__restrict__ azzmin,
+                                                    const double *__restrict__ alphau,
+                                                    const double *__restrict__ alphav,
+                                                    const double *__restrict__ alphae0,
+                                                    const double *__restrict__ alphaw, double *gflow) {
+  ADJ_IJK(b);
+  double gf = 0.0;
+  if (in) {
+    const RView<T> U{f.u, L}, V{f.v, L}, W{f.w, L}, UM{f.um, L}, VM{f.vm, L}, WM{f.wm, L}, SS{ss0, L}, SV{ssv0, L},
+        RH{f.rhop, L};
+    const AAcc<T> au{lam.u, L}, av{lam.v, L}, aw{lam.w, L}, aum{lam.um, L}, avm{lam.vm, L}, awm{lam.wm, L},
+        ass{lam.ss, L}, assv{lam.ssv, L}, arh{lam.rhop, L};
+    if (j >= 1 && k >= 1) {
+      const T l = F3(lam.fx, i, j, k);
+      if (l != (T)0)
+        adj_forces_x<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphau, l, au, av, aw, aum, avm,
+                        awm, ass, assv, gf);
+    }
+    if (i >= 1 && k >= 1) {
+      const T l = F3(lam.fy, i, j, k);
+      if (l != (T)0)
+        adj_forces_y<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphav, l, au, av, aw, aum, avm,
+                        awm, ass, assv, gf);
+    }
+    if (i >= 1 && j >= 1) {
+      const T l = F3(lam.fz, i, j, k);
+      if (l != (T)0)
+        adj_forces_z<T>(L, i, j, k, s, ph, U, V, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, l, au, av, aw,
+                        aum, avm, awm, ass, assv, arh);
+    }
+  }
+  block_add(gf, gflow);
+}
+
+// ---------------------------------------------------------------------------
+// MOMENTUM_U/V/W: Lambda = the adjoint of the interior formula at a cell,
+// collected from the cell and from the ghost cells that copy it; then the
+// formula's (linear) adjoint
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_mom_lambda(Layout L, Box3 b, int comp, StepArgs s, Phys ph,
+                                                        const T *__restrict__ lup, T *__restrict__ lamb,
+                                                        T *__restrict__ lu, T *__restrict__ lum, T *__restrict__ lf) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  T a = F3(lup, i, j, k);
+  if (comp == 0) {
+    if (k == 1) a += F3(lup, i, j, 0);
+    if (k == nz) a += (T)ph.bottom_slip * F3(lup, i, j, nz + 1);
+    if (j == 1) a += F3(lup, i, 0, k);
+    if (j == ny) a += F3(lup, i, ny + 1, k);
+    if (i == 0) a += F3(lup, -1, j, k);
+    if (i == nx) a += F3(lup, nx + 1, j, k);
+  } else if (comp == 1) {
+    if (k == 1) a += F3(lup, i, j, 0);
+    if (k == nz) a += (T)ph.bottom_slip * F3(lup, i, j, nz + 1);
+    if (i == 1) a += F3(lup, 0, j, k);
+    if (i == nx) a += F3(lup, nx + 1, j, k);
+    if (j == 0) a += F3(lup, i, -1, k);
+    if (j == ny) a += F3(lup, i, ny + 1, k);
+  }
+  F3(lamb, i, j, k) = a;
+  F3(lu, i, j, k) += (T)(1 - s.delta) * a;
+  F3(lum, i, j, k) += (T)s.delta * a;
+  F3(lf, i, j, k) += (T)(s.dt * s.leap) * a;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_mom_p(Layout L, Box3 b, int comp, T c, const T *__restrict__ lamb,
+                                                   T *__restrict__ lp) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const T prev = comp == 0 ? F3(lamb, i - 1, j, k) : (comp == 1 ? F3(lamb, i, j - 1, k) : F3(lamb, i, j, k - 1));
+  F3(lp, i, j, k) += c * (F3(lamb, i, j, k) - prev);
+}
+
+// ---------------------------------------------------------------------------
+// VISCOSITY3D
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_visc(Layout L, Box3 b, StepArgs s, Phys ph, const T *__restrict__ um,
+                                                  const T *__restrict__ vm, const T *__restrict__ wm,
+                                                  const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
+                                                  const T *__restrict__ ss1, const T *__restrict__ ssv1,
+                                                  const T *__restrict__ lss, const T *__restrict__ lssv,
+                                                  T *__restrict__ lum, T *__restrict__ lvm, T *__restrict__ lwm,
+                                                  T *__restrict__ lrhop, double *gcs) {
+  ADJ_IJK(b);
+  double sh = 0.0, sv = 0.0;
+  if (in) {
+    const T a = visc_ss_written(L, i, j, k) ? F3(lss, i, j, k) : (T)0;
+    const T c = visc_ssv_written(L, i, j, k) ? F3(lssv, i, j, k) : (T)0;
+    if (a != (T)0 || c != (T)0) {
+      const RView<T> UM{um, L}, VM{vm, L}, WM{wm, L}, RH{rhop, L};
+      const AAcc<T> aum{lum, L}, avm{lvm, L}, awm{lwm, L}, arh{lrhop, L};
+      adj_visc_at<T>(L, s, ph, UM, VM, WM, RH, rho_bkg, a, c, i, j, k, aum, avm, awm, arh);
+      sh = (double)a * (double)F3(ss1, i, j, k);
+      sv = (double)c * (double)F3(ssv1, i, j, k);
+    }
+  }
+  block_add(sh, gcs);
+  block_add(sv, gcs + 1);
+}
+
+// ---------------------------------------------------------------------------
+// DENSITY, REFRESH, work velocities (elementwise)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_density(Layout L, Box3 b, Phys ph, const T *__restrict__ t,
+                                                     const T *__restrict__ sal, const T *__restrict__ lrhop,
+                                                     T *__restrict__ lt, T *__restrict__ ls) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const int ng = (int)(i == 0 || i == L.nx + 1) + (int)(j == 0 || j == L.ny + 1) + (int)(k == 0 || k == L.nz + 1);
+  if (ng > 1) return;
+  const T lr = F3(lrhop, i, j, k);
+  if (lr == (T)0) return;
+  double dt_, ds_;
+  drho(ph, (double)F3(sal, i, j, k), (double)F3(t, i, j, k), dt_, ds_);
+  F3(lt, i, j, k) += (T)((double)lr * dt_);
+  F3(ls, i, j, k) += (T)((double)lr * ds_);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_refresh(size_t n, int leap, double assel, T *__restrict__ lu,
+                                                     T *__restrict__ lup, T *__restrict__ lum) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const T a = lu[q], m = lum[q];
+  if (leap == 1) {
+    lup[q] += a + m;
+    lu[q] = (T)0;
+    lum[q] = (T)0;
+  } else {
+    lup[q] += a + (T)assel * m;
+    lu[q] = ((T)1 - (T)2 * (T)assel) * m;
+    lum[q] = (T)assel * m;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_courant(Layout L, Box3 b, double tau3, double dx, double dy, double dz,
+                                                     T *__restrict__ luc, T *__restrict__ lvc, T *__restrict__ lwc,
+                                                     T *__restrict__ lu, T *__restrict__ lup, T *__restrict__ lv,
+                                                     T *__restrict__ lvp, T *__restrict__ lw, T *__restrict__ lwp) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const size_t q = IX(L, i, j, k);
+  if (i <= L.nx) {
+    const T c = (T)(0.5 * tau3 / dx) * luc[q];
+    lup[q] += c; lu[q] += c; luc[q] = (T)0;
+  }
+  if (j <= L.ny) {
+    const T c = (T)(0.5 * tau3 / dy) * lvc[q];
+    lvp[q] += c; lv[q] += c; lvc[q] = (T)0;
+  }
+  if (k <= L.nz) {
+    const T c = -(T)(0.5 * tau3 / dz) * lwc[q];
+    lwp[q] += c; lw[q] += c; lwc[q] = (T)0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// TRACER31 / MPDATA
+
+// BOUNDARY_CONDITIONS transposed: a ghost's adjoint moves to its interior
+// image.  mode 0: in place (the ghost's adjoint is consumed); mode 1: from
+// the tracer's output into the pre-sponge accumulator, skipping the vent
+// ghost cells whose value the driver sets to a constant.
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_bc(Layout L, Phys ph, VentArgs va, int mode, int vent,
+                                                T *__restrict__ lsrc, T *__restrict__ ldst) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long na = 2L * (ny + 2) * (nz + 2), nb = 2L * nx * (nz + 2), nc = 2L * nx * ny;
+  int i, j, k;
+  if (t < na) {
+    j = (int)(t % (ny + 2)); long r = t / (ny + 2);
+    k = (int)(r % (nz + 2)); i = (r / (nz + 2)) ? nx + 1 : 0;
+  } else if ((t -= na) < nb) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    k = (int)(r % (nz + 2)); j = (r / (nz + 2)) ? ny + 1 : 0;
+  } else if ((t -= nb) < nc) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    j = 1 + (int)(r % ny); k = (r / ny) ? nz + 1 : 0;
+  } else {
+    return;
+  }
+  const bool gi = (i == 0 || i == nx + 1), gj = (j == 0 || j == ny + 1);
+  if (mode == 1 && vent && k == nz + 1 && !gi && !gj && va.map_valid && i >= va.nxs && i <= va.nxe &&
+      j >= va.nys && j <= va.nye)
+    return;
+  int si, sj, sk;
+  bc_source(L, ph, i, j, k, si, sj, sk);
+  const T v = F3(lsrc, i, j, k);
+  if (mode == 0) F3(lsrc, i, j, k) = (T)0;
+  if (v != (T)0) unsafeAtomicAdd(&F3(ldst, si, sj, sk), v);
+}
+
+// antidiffusive velocities of a pass on the faces the donor step uses:
+// ud on i = 0..nx, vd on j = 0..ny, wd on k = 0..nz (interior otherwise)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_anti_fwd(Layout L, Box3 b, const T *__restrict__ cc,
+                                                      const T *__restrict__ U, const T *__restrict__ V,
+                                                      const T *__restrict__ W, T *__restrict__ ud,
+                                                      T *__restrict__ vd, T *__restrict__ wd) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const RelR<T> C{cc, L, i, j, k}, Uf{U, L, i, j, k}, Vf{V, L, i, j, k}, Wf{W, L, i, j, k};
+  if (j >= 1 && k >= 1) F3(ud, i, j, k) = mp_anti_u<T>(C, Uf, Vf, Wf);
+  if (i >= 1 && k >= 1) F3(vd, i, j, k) = mp_anti_v<T>(C, Uf, Vf, Wf);
+  if (i >= 1 && j >= 1) F3(wd, i, j, k) = mp_anti_w<T>(C, Uf, Vf, Wf);
+}
+
+// limiter flags of the interior cells (mpdata-hotcross80.f:660-690)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_flags(Layout L, Box3 b, const T *__restrict__ cc,
+                                                   const T *__restrict__ ud, const T *__restrict__ vd,
+                                                   const T *__restrict__ wd, double cmin, double cmax,
+                                                   uint8_t *__restrict__ fl) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const T c = F3(cc, i, j, k);
+  const T x = c + mp_donor<T>(c, F3(cc, i - 1, j, k), F3(cc, i + 1, j, k), F3(cc, i, j - 1, k), F3(cc, i, j + 1, k),
+                              F3(cc, i, j, k - 1), F3(cc, i, j, k + 1), F3(ud, i - 1, j, k), F3(ud, i, j, k),
+                              F3(vd, i, j - 1, k), F3(vd, i, j, k), F3(wd, i, j, k - 1), F3(wd, i, j, k), false, 0, 0,
+                              0, 0, 0, 0, 0);
+  uint8_t f = 0;
+  if (x <= (T)cmin) f = 1;
+  else if (x >= (T)cmax) f = 2;
+  F3(fl, i, j, k) = f;
+}
+
+struct DonorLast {
+  int last;
+  double tau3, dxdydz;
+  const double *alphae0, *alphaw, *qbase;
+  double *gq;
+  int map_valid, nxs, nxe, nys, nye;
+};
+
+// the limited donor-cell step of a pass, reversed (interior cells); on the
+// last pass also the sponge and source
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_donor(Layout L, Box3 b, DonorLast dl, const T *__restrict__ lout,
+                                                   const T *__restrict__ lpre, const T *__restrict__ cc,
+                                                   const T *__restrict__ ud, const T *__restrict__ vd,
+                                                   const T *__restrict__ wd, const uint8_t *__restrict__ fl,
+                                                   T *__restrict__ lcc, T *__restrict__ lud, T *__restrict__ lvd,
+                                                   T *__restrict__ lwd, T *__restrict__ lentry) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const T lo = F3(lout, i, j, k);
+  const T ln = lo + (lpre ? F3(lpre, i, j, k) : (T)0);
+  if (dl.last && lo != (T)0) {
+    const double ae = fmax(dl.alphae0[i + (L.nx + 2) * j], (dl.alphaw[k + 1] + dl.alphaw[k]) / 2.0);
+    F3(lentry, i, j, k) += -(T)(0.5 * ae * dl.tau3) * lo;
+    if (k == L.nz && dl.map_valid && i >= dl.nxs && i <= dl.nxe && j >= dl.nys && j <= dl.nye) {
+      const int n = (i - dl.nxs) + (dl.nxe - dl.nxs + 1) * (j - dl.nys);
+      dl.gq[n] += (double)lo * dl.tau3 / dl.dxdydz * dl.qbase[n];
+    }
+  }
+  if (ln == (T)0) return;
+  T lc[7], lv[6];
+  adj_mp_donor<T>(F3(cc, i, j, k), F3(cc, i - 1, j, k), F3(cc, i + 1, j, k), F3(cc, i, j - 1, k), F3(cc, i, j + 1, k),
+                  F3(cc, i, j, k - 1), F3(cc, i, j, k + 1), F3(ud, i - 1, j, k), F3(ud, i, j, k),
+                  F3(vd, i, j - 1, k), F3(vd, i, j, k), F3(wd, i, j, k - 1), F3(wd, i, j, k), true,
+                  F3(fl, i, j, k), F3(fl, i - 1, j, k), F3(fl, i + 1, j, k), F3(fl, i, j - 1, k),
+                  F3(fl, i, j + 1, k), F3(fl, i, j, k - 1), F3(fl, i, j, k + 1), ln, lc, lv);
+  unsafeAtomicAdd(&F3(lcc, i, j, k), ln + lc[0]);
+  unsafeAtomicAdd(&F3(lcc, i - 1, j, k), lc[1]);
+  unsafeAtomicAdd(&F3(lcc, i + 1, j, k), lc[2]);
+  unsafeAtomicAdd(&F3(lcc, i, j - 1, k), lc[3]);
+  unsafeAtomicAdd(&F3(lcc, i, j + 1, k), lc[4]);
+  unsafeAtomicAdd(&F3(lcc, i, j, k - 1), lc[5]);
+  unsafeAtomicAdd(&F3(lcc, i, j, k + 1), lc[6]);
+  unsafeAtomicAdd(&F3(lud, i - 1, j, k), lv[0]);
+  unsafeAtomicAdd(&F3(lud, i, j, k), lv[1]);
+  unsafeAtomicAdd(&F3(lvd, i, j - 1, k), lv[2]);
+  unsafeAtomicAdd(&F3(lvd, i, j, k), lv[3]);
+  unsafeAtomicAdd(&F3(lwd, i, j, k - 1), lv[4]);
+  unsafeAtomicAdd(&F3(lwd, i, j, k), lv[5]);
+}
+
+// the antidiffusive velocities' formulas reversed (one thread per face
+// position, the three face kinds of k_adj_anti_fwd)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_anti(Layout L, Box3 b, const T *__restrict__ cc, const T *__restrict__ U,
+                                                  const T *__restrict__ V, const T *__restrict__ W,
+                                                  const T *__restrict__ lud, const T *__restrict__ lvd,
+                                                  const T *__restrict__ lwd, T *__restrict__ lcc, T *__restrict__ lU,
+                                                  T *__restrict__ lV, T *__restrict__ lW) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const RelR<T> C{cc, L, i, j, k}, Uf{U, L, i, j, k}, Vf{V, L, i, j, k}, Wf{W, L, i, j, k};
+  const RelA<T> ac{lcc, L, i, j, k}, aU{lU, L, i, j, k}, aV{lV, L, i, j, k}, aW{lW, L, i, j, k};
+  if (j >= 1 && k >= 1) adj_anti_u<T>(C, Uf, Vf, Wf, F3(lud, i, j, k), ac, aU, aV, aW);
+  if (i >= 1 && k >= 1) adj_anti_v<T>(C, Uf, Vf, Wf, F3(lvd, i, j, k), ac, aU, aV, aW);
+  if (i >= 1 && j >= 1) adj_anti_w<T>(C, Uf, Vf, Wf, F3(lwd, i, j, k), ac, aU, aV, aW);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_pass1(Layout L, Box3 b, StepArgs s, Phys ph, FieldPtrs<T> f,
+                                                   const T *__restrict__ c, const T *__restrict__ uc,
+                                                   const T *__restrict__ vc, const T *__restrict__ wc,
+                                                   const double *__restrict__ azzmin, const double *__restrict__ cbkg,
+                                                   const T *__restrict__ lc1, T *__restrict__ lentry,
+                                                   T *__restrict__ luc, T *__restrict__ lvc, T *__restrict__ lwc,
+                                                   T *__restrict__ lss, T *__restrict__ lssv) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const T l = F3(lc1, i, j, k);
+  if (l == (T)0) return;
+  const RView<T> C{c, L}, UC{uc, L}, VC{vc, L}, WC{wc, L}, SS{f.ss, L}, SV{f.ssv, L};
+  const AAcc<T> ac{lentry, L}, auc{luc, L}, avc{lvc, L}, awc{lwc, L}, ass{lss, L}, assv{lssv, L};
+  adj_pass1<T>(L, s, ph, C, UC, VC, WC, SS, SV, azzmin, cbkg, i, j, k, l, ac, auc, avc, awc, ass, assv);
+}
+
+// ---------------------------------------------------------------------------
+// probes
+__global__ void k_probe_sample(int n, const int *__restrict__ iw, const int *__restrict__ it, const double *w,
+                               const double *t, double *out) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  out[2 * q] = w[iw[q]];
+  out[2 * q + 1] = t[it[q]];
+}
+__global__ void k_probe_sample32(int n, const int *__restrict__ iw, const int *__restrict__ it, const float *w,
+                                 const float *t, double *out) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  out[2 * q] = w[iw[q]];
+  out[2 * q + 1] = t[it[q]];
+}
+template <typename T>
+__global__ void k_probe_seed(int n, const int *__restrict__ iw, const int *__restrict__ it,
+                             const double *__restrict__ samp, const double *__restrict__ obs, double ww, double wt,
+                             T *lw, T *lt) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  unsafeAtomicAdd(lw + iw[q], (T)(ww * (samp[2 * q] - obs[2 * q])));
+  unsafeAtomicAdd(lt + it[q], (T)(wt * (samp[2 * q + 1] - obs[2 * q + 1])));
+}
+
+}  // namespace hc
+
+// ===========================================================================
+// adjoint state of a context
+static const int NTAPE = 11;
+static const int TAPE_FIELDS[NTAPE] = {HC_U, HC_V, HC_W, HC_UM, HC_VM, HC_WM, HC_SS, HC_SSV, HC_RHOP, HC_T, HC_S};
+
+struct HostState {
+  double time;
+  double acc[8];
+  int next_step;
+  bool map_valid;
+};
+
+struct AdjCtx {
+  void *lam[HC_NFIELDS] = {};
+  void *luc[3] = {};
+  void *lamb = nullptr;  // MOMENTUM's Lambda
+  void *lcnew = nullptr;
+  // tracer recompute and its adjoints
+  void *cw = nullptr, *c1 = nullptr, *c2 = nullptr, *cpre = nullptr, *ud2[3] = {}, *ud3[3] = {};
+  void *lc1 = nullptr, *lc2 = nullptr, *lpre = nullptr, *lud2[3] = {}, *lud3[3] = {};
+  uint8_t *fl2 = nullptr, *fl3 = nullptr;
+  void *g = nullptr;          // compact pressure work
+  double *d_sc = nullptr;     // [0] d/d flow_scale, [1] sum lam_ss*ss, [2] sum lam_ssv*ssv, [3] 0.0
+  double *d_part = nullptr;   // nz+2 partial sums
+  double *d_gq = nullptr;     // npatch
+  // window
+  void *tape = nullptr;       // (nck + seg + 1) states of NTAPE fields
+  size_t tape_slots = 0;
+  int nprobe = 0, every = 1;
+  double wgt_w = 1.0, wgt_t = 1.0;
+  int *d_iw = nullptr, *d_it = nullptr;
+  double *d_samp = nullptr, *d_obs = nullptr;
+  size_t nsamp_cap = 0;
+};
+
+void adj_free(hc_ctx *ctx) {
+  AdjCtx *A = ctx->adj;
+  if (!A) return;
+  hipStreamSynchronize(ctx->st);
+  std::vector<void *> ps;
+  for (void *p : A->lam) ps.push_back(p);
+  for (void *p : A->luc) ps.push_back(p);
+  for (int q = 0; q < 3; ++q) { ps.push_back(A->ud2[q]); ps.push_back(A->ud3[q]); ps.push_back(A->lud2[q]); ps.push_back(A->lud3[q]); }
+  void *rest[] = {A->lamb, A->lcnew, A->cw, A->c1, A->c2, A->cpre, A->lc1, A->lc2, A->lpre, A->fl2, A->fl3, A->g,
+                  A->d_sc, A->d_part, A->d_gq, A->tape, A->d_iw, A->d_it, A->d_samp, A->d_obs};
+  for (void *p : rest) ps.push_back(p);
+  for (void *p : ps)
+    if (p) hipFree(p);
+  delete A;
+  ctx->adj = nullptr;
+}
+
+static int adj_alloc(hc_ctx *ctx) {
+  if (ctx->adj) return HC_OK;
+  AdjCtx *A = new AdjCtx();
+  ctx->adj = A;
+  const size_t fb = ctx->L.n * ctx->esz;
+  int rc = 0;
+  for (int q = 0; q < HC_NFIELDS; ++q)
+    if (!(A->lam[q] = dalloc(ctx, fb, &rc))) return rc;
+  void **fields[] = {&A->luc[0], &A->luc[1], &A->luc[2], &A->lamb, &A->lcnew, &A->cw, &A->c1, &A->c2, &A->cpre,
+                     &A->ud2[0], &A->ud2[1], &A->ud2[2], &A->ud3[0], &A->ud3[1], &A->ud3[2], &A->lc1, &A->lc2,
+                     &A->lpre, &A->lud2[0], &A->lud2[1], &A->lud2[2], &A->lud3[0], &A->lud3[1], &A->lud3[2]};
+  for (auto p : fields)
+    if (!(*p = dalloc(ctx, fb, &rc))) return rc;
+  if (!(A->fl2 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
+  if (!(A->fl3 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
+  if (!(A->g = dalloc(ctx, (size_t)ctx->P.nx * ctx->P.ny * ctx->P.nz * ctx->esz, &rc))) return rc;
+  if (!(A->d_sc = (double *)dalloc(ctx, 64, &rc))) return rc;
+  if (!(A->d_part = (double *)dalloc(ctx, (size_t)(ctx->P.nz + 2) * 8, &rc))) return rc;
+  if (!(A->d_gq = (double *)dalloc(ctx, (size_t)ctx->npatch * 8, &rc))) return rc;
+  return HC_OK;
+}
+
+static int adj_need(hc_ctx *ctx) {
+  int r = adj_alloc(ctx);
+  if (r) return r;
+  return HC_OK;
+}
+
+template <typename T>
+static FieldPtrs<T> lptrs(hc_ctx *ctx) {
+  FieldPtrs<T> p;
+  T **a = (T **)&p;
+  for (int q = 0; q < HC_NFIELDS; ++q) a[q] = (T *)ctx->adj->lam[q];
+  return p;
+}
+
+static FlowSens flow_sens(hc_ctx *ctx) {
+  FlowSens f;
+  f.dubkg_m = ctx->dflow[0]; f.dubkg = ctx->dflow[1]; f.dvbkg_m = ctx->dflow[2]; f.dvbkg = ctx->dflow[3];
+  f.dfbx = ctx->dflow[4]; f.dfby = ctx->dflow[5];
+  return f;
+}
+
+static void zero(hc_ctx *ctx, void *p) { hipMemsetAsync(p, 0, ctx->L.n * ctx->esz, ctx->st); }
+
+static Box3 interior(const Layout &L) { return Box3{1, L.nx, 1, L.ny, 1, L.nz}; }
+
+// ---------------------------------------------------------------------------
+// per-routine adjoints (templated on the precision)
+template <typename T>
+static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
+  AdjCtx *A = ctx->adj;
+  const Layout &L = ctx->L;
+  Box3 b{0, L.nx, 0, L.ny, 0, L.nz};
+  FieldPtrs<T> lam = lptrs<T>(ctx);
+  hipLaunchKernelGGL(k_adj_forces<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, step_args(ctx), ctx->P, flow_sens(ctx),
+                     ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
+                     ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc);
+  zero(ctx, lam.fx); zero(ctx, lam.fy); zero(ctx, lam.fz);
+}
+
+template <typename T>
+static void adj_momentum_t(hc_ctx *ctx, int comp) {
+  AdjCtx *A = ctx->adj;
+  const Layout &L = ctx->L;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  FieldPtrs<T> lam = lptrs<T>(ctx);
+  StepArgs s = step_args(ctx);
+  T *lup = comp == 0 ? lam.up : comp == 1 ? lam.vp : lam.wp;
+  T *lu = comp == 0 ? lam.u : comp == 1 ? lam.v : lam.w;
+  T *lum = comp == 0 ? lam.um : comp == 1 ? lam.vm : lam.wm;
+  T *lf = comp == 0 ? lam.fx : comp == 1 ? lam.fy : lam.fz;
+  Box3 fb = comp == 0 ? Box3{0, nx, 1, ny, 1, nz} : comp == 1 ? Box3{1, nx, 0, ny, 1, nz} : Box3{1, nx, 1, ny, 1, nz - 1};
+  Box3 pb = comp == 0 ? Box3{0, nx + 1, 1, ny, 1, nz} : comp == 1 ? Box3{1, nx, 0, ny + 1, 1, nz} : Box3{1, nx, 1, ny, 1, nz};
+  zero(ctx, A->lamb);
+  hipLaunchKernelGGL(k_adj_mom_lambda<T>, grid3(fb), dim3(64, 4), 0, ctx->st, L, fb, comp, s, ctx->P, (const T *)lup,
+                     (T *)A->lamb, lu, lum, lf);
+  const double d = comp == 0 ? s.dx : comp == 1 ? s.dy : s.dz;
+  hipLaunchKernelGGL(k_adj_mom_p<T>, grid3(pb), dim3(64, 4), 0, ctx->st, L, pb, comp, (T)(s.dt * s.leap / d),
+                     (const T *)A->lamb, lam.p);
+  zero(ctx, lup);
+}
+
+template <typename T>
+static void adj_pressure_t(hc_ctx *ctx) {
+  AdjCtx *A = ctx->adj;
+  launch_pressure_adj<T>(ctx->st, ctx->L, step_args(ctx), ctx->P, ptrs<T>(ctx), lptrs<T>(ctx), ctx->aux, ctx->pw,
+                         flow_sens(ctx), A->d_sc, (T *)A->g, A->d_sc + 3, A->d_part);
+}
+
+template <typename T>
+static void adj_viscosity_t(hc_ctx *ctx, const T *ss1, const T *ssv1) {
+  AdjCtx *A = ctx->adj;
+  const Layout &L = ctx->L;
+  Box3 b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+  FieldPtrs<T> f = ptrs<T>(ctx), lam = lptrs<T>(ctx);
+  hipLaunchKernelGGL(k_adj_visc<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, step_args(ctx), ctx->P, (const T *)f.um,
+                     (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop, (const double *)ctx->aux.rho_bkg, ss1, ssv1,
+                     (const T *)lam.ss, (const T *)lam.ssv, lam.um, lam.vm, lam.wm, lam.rhop, A->d_sc + 1);
+  zero(ctx, lam.ss); zero(ctx, lam.ssv);
+}
+
+template <typename T>
+static void adj_density_t(hc_ctx *ctx, const T *t1, const T *s1) {
+  const Layout &L = ctx->L;
+  Box3 b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+  FieldPtrs<T> lam = lptrs<T>(ctx);
+  hipLaunchKernelGGL(k_adj_density<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, ctx->P, t1, s1, (const T *)lam.rhop,
+                     lam.t, lam.s);
+  zero(ctx, lam.rhop);
+}
+
+template <typename T>
+static void adj_refresh_t(hc_ctx *ctx) {
+  FieldPtrs<T> lam = lptrs<T>(ctx);
+  size_t n = ctx->L.n;
+  T *tri[3][3] = {{lam.u, lam.up, lam.um}, {lam.v, lam.vp, lam.vm}, {lam.w, lam.wp, lam.wm}};
+  for (auto &t : tri)
+    hipLaunchKernelGGL(k_adj_refresh<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->st, n, ctx->leap,
+                       ctx->P.assel, t[0], t[1], t[2]);
+}
+
+template <typename T>
+static void adj_courant_t(hc_ctx *ctx, double tau3) {
+  AdjCtx *A = ctx->adj;
+  const Layout &L = ctx->L;
+  Box3 b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+  FieldPtrs<T> lam = lptrs<T>(ctx);
+  hipLaunchKernelGGL(k_adj_courant<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, tau3, ctx->P.dx, ctx->P.dy, ctx->P.dz,
+                     (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2], lam.u, lam.up, lam.v, lam.vp, lam.w, lam.wp);
+}
+
+// TRACER31 for field `field` at the current state: lam[field] holds the
+// adjoint of the call's output and receives the adjoint of its input
+template <typename T>
+static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3,
+                         int vent, const double *qpatch) {
+  AdjCtx *A = ctx->adj;
+  const Layout &L = ctx->L;
+  hipStream_t st = ctx->st;
+  const size_t fb = L.n * sizeof(T);
+  StepArgs s = step_args(ctx);
+  s.tau3 = tau3;
+  VentArgs va = vent_args(ctx);
+  FieldPtrs<T> f = ptrs<T>(ctx);
+  const T *c = (const T *)ctx->f[field];
+  // recompute the passes' intermediates on a copy of the entry value
+  hipMemcpyAsync(A->cw, c, fb, hipMemcpyDeviceToDevice, st);
+  TracerWork tw{};
+  tw.uc = ctx->tw.uc; tw.vc = ctx->tw.vc; tw.wc = ctx->tw.wc;
+  tw.ud2 = A->ud2[0]; tw.vd2 = A->ud2[1]; tw.wd2 = A->ud2[2];
+  tw.c1 = A->c1; tw.c2 = A->c2; tw.cpre = A->cpre;
+  tw.budget_part = ctx->tw.budget_part; tw.budget = ctx->tw.budget;
+  TracerArgs ta{cmin, cmax, tau3, 0, 0.0, 0};
+  launch_tracer<T>(st, L, s, ctx->P, va, f, ctx->aux, tw, (T *)A->cw, d_cbkg, qpatch, ta);
+  const Box3 fbx{0, L.nx, 0, L.ny, 0, L.nz}, ib = interior(L);
+  const T *ud2 = (const T *)A->ud2[0], *vd2 = (const T *)A->ud2[1], *wd2 = (const T *)A->ud2[2];
+  T *ud3 = (T *)A->ud3[0], *vd3 = (T *)A->ud3[1], *wd3 = (T *)A->ud3[2];
+  hipLaunchKernelGGL(k_adj_anti_fwd<T>, grid3(fbx), dim3(64, 4), 0, st, L, fbx, (const T *)A->c2, ud2, vd2, wd2, ud3,
+                     vd3, wd3);
+  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, (const T *)A->c1, ud2, vd2, wd2, cmin,
+                     cmax, A->fl2);
+  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, (const T *)A->c2, (const T *)ud3,
+                     (const T *)vd3, (const T *)wd3, cmin, cmax, A->fl3);
+  // adjoint buffers
+  void *zs[] = {A->lcnew, A->lpre, A->lc1, A->lc2, A->lud2[0], A->lud2[1], A->lud2[2], A->lud3[0], A->lud3[1], A->lud3[2]};
+  for (void *p : zs) zero(ctx, p);
+  T *lout = (T *)A->lam[field];
+  const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
+  const unsigned nbc = (unsigned)((nghost + 255) / 256);
+  // final BOUNDARY_CONDITIONS from the pre-sponge values
+  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 1, vent, lout, (T *)A->lpre);
+  // pass 3
+  DonorLast dl{};
+  dl.last = 1; dl.tau3 = tau3; dl.dxdydz = ctx->P.dx * ctx->P.dy * ctx->P.dz;
+  dl.alphae0 = ctx->aux.alphae0; dl.alphaw = ctx->aux.alphaw; dl.qbase = ctx->d_qbase;
+  dl.gq = (field == HC_T) ? A->d_gq : nullptr;
+  dl.map_valid = (field == HC_T && ctx->d_qbase) ? va.map_valid : 0;
+  dl.nxs = va.nxs; dl.nxe = va.nxe; dl.nys = va.nys; dl.nye = va.nye;
+  hipLaunchKernelGGL(k_adj_donor<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, dl, (const T *)lout, (const T *)A->lpre,
+                     (const T *)A->c2, (const T *)ud3, (const T *)vd3, (const T *)wd3, (const uint8_t *)A->fl3,
+                     (T *)A->lc2, (T *)A->lud3[0], (T *)A->lud3[1], (T *)A->lud3[2], (T *)A->lcnew);
+  hipLaunchKernelGGL(k_adj_anti<T>, grid3(fbx), dim3(64, 4), 0, st, L, fbx, (const T *)A->c2, ud2, vd2, wd2,
+                     (const T *)A->lud3[0], (const T *)A->lud3[1], (const T *)A->lud3[2], (T *)A->lc2,
+                     (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2]);
+  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc2, (T *)A->lc2);
+  // pass 2
+  DonorLast d2{};
+  hipLaunchKernelGGL(k_adj_donor<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, d2, (const T *)A->lc2, (const T *)nullptr,
+                     (const T *)A->c1, ud2, vd2, wd2, (const uint8_t *)A->fl2, (T *)A->lc1, (T *)A->lud2[0],
+                     (T *)A->lud2[1], (T *)A->lud2[2], (T *)nullptr);
+  hipLaunchKernelGGL(k_adj_anti<T>, grid3(fbx), dim3(64, 4), 0, st, L, fbx, (const T *)A->c1, (const T *)ctx->tw.uc,
+                     (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)A->lud2[0], (const T *)A->lud2[1],
+                     (const T *)A->lud2[2], (T *)A->lc1, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2]);
+  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc1, (T *)A->lc1);
+  // pass 1
+  FieldPtrs<T> lam = lptrs<T>(ctx);
+  hipLaunchKernelGGL(k_adj_pass1<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, s, ctx->P, f, c, (const T *)ctx->tw.uc,
+                     (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const double *)ctx->aux.azzmin, d_cbkg,
+                     (const T *)A->lc1, (T *)A->lcnew, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2], lam.ss,
+                     lam.ssv);
+  // the entry value's adjoint becomes the field's
+  void *tmp = A->lam[field];
+  A->lam[field] = A->lcnew;
+  A->lcnew = tmp;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+#define ADJ_DISPATCH(fn, ...)                        \
+  do {                                               \
+    if (ctx->prec == HC_FP64) fn<double>(__VA_ARGS__); \
+    else fn<float>(__VA_ARGS__);                     \
+  } while (0)
+
+static int adj_field_ptr(hc_ctx *ctx, int field, void **p, int *bounds_field) {
+  if (field >= 0 && field < HC_NFIELDS) {
+    *p = ctx->adj->lam[field];
+    *bounds_field = field;
+    return HC_OK;
+  }
+  if (field >= HC_ADJ_UC && field <= HC_ADJ_WC) {
+    *p = ctx->adj->luc[field - HC_ADJ_UC];
+    *bounds_field = field == HC_ADJ_UC ? HC_U : field == HC_ADJ_VC ? HC_V : HC_W;
+    return HC_OK;
+  }
+  return fail(ctx, HC_EINVAL, "bad adjoint field id");
+}
+
+extern "C" int hc_adj_reset(hc_ctx *ctx) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  AdjCtx *A = ctx->adj;
+  for (void *p : A->lam) zero(ctx, p);
+  for (void *p : A->luc) zero(ctx, p);
+  HIPCHK(hipMemsetAsync(A->d_sc, 0, 64, ctx->st));
+  HIPCHK(hipMemsetAsync(A->d_gq, 0, (size_t)ctx->npatch * 8, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
+}
+
+extern "C" int hc_adj_upload(hc_ctx *ctx, int field, const double *host, const int32_t lo[3], const int32_t hi[3]) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  void *p;
+  int bf;
+  if ((r = adj_field_ptr(ctx, field, &p, &bf))) return r;
+  return box_copy(ctx, p, bf, (double *)host, lo, hi, true);
+}
+
+extern "C" int hc_adj_download(hc_ctx *ctx, int field, double *host, const int32_t lo[3], const int32_t hi[3]) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  void *p;
+  int bf;
+  if ((r = adj_field_ptr(ctx, field, &p, &bf))) return r;
+  return box_copy(ctx, p, bf, host, lo, hi, false);
+}
+
+extern "C" int hc_adj_forces(hc_ctx *ctx) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_forces");
+  if (ctx->prec == HC_FP64) adj_forces_t<double>(ctx, (const double *)ctx->f[HC_SS], (const double *)ctx->f[HC_SSV]);
+  else adj_forces_t<float>(ctx, (const float *)ctx->f[HC_SS], (const float *)ctx->f[HC_SSV]);
+  return HC_OK;
+}
+
+extern "C" int hc_adj_pressure(hc_ctx *ctx) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_pressure");
+  ADJ_DISPATCH(adj_pressure_t, ctx);
+  return HC_OK;
+}
+
+static int adj_mom(hc_ctx *ctx, int comp) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_momentum");
+  ADJ_DISPATCH(adj_momentum_t, ctx, comp);
+  return HC_OK;
+}
+extern "C" int hc_adj_momentum_u(hc_ctx *ctx) { return adj_mom(ctx, 0); }
+extern "C" int hc_adj_momentum_v(hc_ctx *ctx) { return adj_mom(ctx, 1); }
+extern "C" int hc_adj_momentum_w(hc_ctx *ctx) { return adj_mom(ctx, 2); }
+
+extern "C" int hc_adj_viscosity3d(hc_ctx *ctx) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_viscosity3d");
+  if (ctx->prec == HC_FP64)
+    adj_viscosity_t<double>(ctx, (const double *)ctx->f[HC_SS], (const double *)ctx->f[HC_SSV]);
+  else
+    adj_viscosity_t<float>(ctx, (const float *)ctx->f[HC_SS], (const float *)ctx->f[HC_SSV]);
+  return HC_OK;
+}
+
+extern "C" int hc_adj_work_velocity(hc_ctx *ctx, double tau3) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_work_velocity");
+  ADJ_DISPATCH(adj_courant_t, ctx, tau3);
+  return HC_OK;
+}
+
+extern "C" int hc_adj_tracer31(hc_ctx *ctx, int field, const double *c_bkg, double cmin, double cmax, double tau3) {
+  if (field != HC_T && field != HC_S && field != HC_C) return fail(ctx, HC_EINVAL, "tracer field must be T, S or C");
+  if (ctx->P.ismol != 1) return fail(ctx, HC_EINVAL, "only ismol = 1 (two corrective passes) is supported");
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_tracer31");
+  HIPCHK(hipMemcpyAsync(ctx->d_cbkg, c_bkg, (size_t)(ctx->P.nz + 2) * 8, hipMemcpyHostToDevice, ctx->st));
+  const int which = field == HC_T ? 0 : field == HC_S ? 1 : 2;
+  ADJ_DISPATCH(adj_tracer_t, ctx, field, ctx->d_cbkg, cmin, cmax, tau3, 0, ctx->d_q[which]);
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
+}
+
+template <typename T>
+__global__ void k_adj_vent(Layout L, VentArgs va, T *lc) {
+  int i = va.nxs + (int)threadIdx.x, j = va.nys + (int)blockIdx.x;
+  if (!va.map_valid || i > va.nxe || j > va.nye) return;
+  lc[IX(L, i, j, L.nz + 1)] = (T)0;
+}
+
+extern "C" int hc_adj_vent_ghost(hc_ctx *ctx, int field) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  VentArgs va = vent_args(ctx);
+  if (!va.map_valid) return HC_OK;
+  dim3 g(va.nye - va.nys + 1), b(va.nxe - va.nxs + 1);
+  if (ctx->prec == HC_FP64) hipLaunchKernelGGL(k_adj_vent<double>, g, b, 0, ctx->st, ctx->L, va, (double *)ctx->adj->lam[field]);
+  else hipLaunchKernelGGL(k_adj_vent<float>, g, b, 0, ctx->st, ctx->L, va, (float *)ctx->adj->lam[field]);
+  return HC_OK;
+}
+
+extern "C" int hc_adj_density(hc_ctx *ctx) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_density");
+  if (ctx->prec == HC_FP64) adj_density_t<double>(ctx, (const double *)ctx->f[HC_T], (const double *)ctx->f[HC_S]);
+  else adj_density_t<float>(ctx, (const float *)ctx->f[HC_T], (const float *)ctx->f[HC_S]);
+  return HC_OK;
+}
+
+extern "C" int hc_adj_refresh(hc_ctx *ctx) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  PhaseScope ps(ctx, "adj_refresh");
+  ADJ_DISPATCH(adj_refresh_t, ctx);
+  return HC_OK;
+}
+
+extern "C" int hc_adj_gradients(hc_ctx *ctx, double *gq, double out[3]) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  AdjCtx *A = ctx->adj;
+  double sc[4];
+  HIPCHK(hipMemcpyAsync(sc, A->d_sc, 4 * 8, hipMemcpyDeviceToHost, ctx->st));
+  if (gq) HIPCHK(hipMemcpyAsync(gq, A->d_gq, (size_t)ctx->npatch * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  out[0] = sc[0];
+  out[1] = ctx->P.cs_h != 0.0 ? 2.0 * sc[1] / ctx->P.cs_h : 0.0;
+  out[2] = ctx->P.cs_v != 0.0 ? 2.0 * sc[2] / ctx->P.cs_v : 0.0;
+  return HC_OK;
+}
+
+extern "C" int hc_set_controls(hc_ctx *ctx, const double *qmap, double flow_scale, double cs) {
+  if (!(flow_scale == flow_scale)) return fail(ctx, HC_EINVAL, "flow_scale is NaN");
+  ctx->flow_scale = flow_scale;
+  if (cs > 0.0) {
+    ctx->P.cs_h = cs;
+    ctx->P.cs_v = cs;
+  }
+  if (qmap) {
+    if (!ctx->d_qmap) {
+      int rc = 0;
+      if (!(ctx->d_qmap = (double *)dalloc(ctx, (size_t)ctx->npatch * 8, &rc))) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_qmap, qmap, (size_t)ctx->npatch * 8, hipMemcpyHostToDevice, ctx->st));
+  } else if (ctx->d_qmap) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    hipFree(ctx->d_qmap);
+    ctx->d_qmap = nullptr;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// window: forward with checkpoints, probe-misfit loss, reverse sweep
+extern "C" int hc_adj_set_probes(hc_ctx *ctx, int32_t nprobe, const int32_t *ijk, int32_t every, double wgt_w,
+                                 double wgt_t) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  AdjCtx *A = ctx->adj;
+  if (nprobe < 1 || every < 1) return fail(ctx, HC_EINVAL, "need at least one probe and every >= 1");
+  const hc_params &P = ctx->P;
+  std::vector<int> iw(nprobe), it(nprobe);
+  for (int q = 0; q < nprobe; ++q) {
+    int i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+    if (i < 1 || i > P.nx || j < 1 || j > P.ny || k < 1 || k > P.nz - 1)
+      return fail(ctx, HC_EINVAL, "probe outside the interior (1..nx, 1..ny, 1..nz-1)");
+    iw[q] = IX(ctx->L, i, j, k);
+    it[q] = IX(ctx->L, i, j, k);
+  }
+  if (A->d_iw) { hipFree(A->d_iw); hipFree(A->d_it); }
+  int rc = 0;
+  if (!(A->d_iw = (int *)dalloc(ctx, (size_t)nprobe * 4, &rc))) return rc;
+  if (!(A->d_it = (int *)dalloc(ctx, (size_t)nprobe * 4, &rc))) return rc;
+  HIPCHK(hipMemcpy(A->d_iw, iw.data(), (size_t)nprobe * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(A->d_it, it.data(), (size_t)nprobe * 4, hipMemcpyHostToDevice));
+  A->nprobe = nprobe;
+  A->every = every;
+  A->wgt_w = wgt_w;
+  A->wgt_t = wgt_t;
+  return HC_OK;
+}
+
+static HostState host_state(hc_ctx *ctx) {
+  HostState h;
+  h.time = ctx->time;
+  for (int q = 0; q < 8; ++q) h.acc[q] = ctx->acc[q];
+  h.next_step = ctx->next_step;
+  h.map_valid = ctx->map_valid;
+  return h;
+}
+
+static void set_host_state(hc_ctx *ctx, const HostState &h) {
+  ctx->time = h.time;
+  for (int q = 0; q < 8; ++q) ctx->acc[q] = h.acc[q];
+  ctx->next_step = h.next_step;
+  if (ctx->map_valid != h.map_valid) {
+    ctx->map_valid = h.map_valid;
+    update_sums(ctx);
+  }
+}
+
+static void *slot_ptr(hc_ctx *ctx, size_t slot, int f) {
+  return (char *)ctx->adj->tape + (slot * NTAPE + f) * ctx->L.n * ctx->esz;
+}
+static void save_slot(hc_ctx *ctx, size_t slot) {
+  for (int f = 0; f < NTAPE; ++f)
+    hipMemcpyAsync(slot_ptr(ctx, slot, f), ctx->f[TAPE_FIELDS[f]], ctx->L.n * ctx->esz, hipMemcpyDeviceToDevice, ctx->st);
+}
+static void load_slot(hc_ctx *ctx, size_t slot) {
+  for (int f = 0; f < NTAPE; ++f)
+    hipMemcpyAsync(ctx->f[TAPE_FIELDS[f]], slot_ptr(ctx, slot, f), ctx->L.n * ctx->esz, hipMemcpyDeviceToDevice, ctx->st);
+}
+
+// the steps of the native loop up to the work velocities (no tracers)
+static void step_head(hc_ctx *ctx, int m) {
+  hc_begin_step(ctx, m, ctx->time + ctx->P.dt);
+  hc_body_force(ctx);
+  hc_forces(ctx);
+  hc_pressure(ctx);
+  hc_momentum_u(ctx);
+  hc_momentum_v(ctx);
+  hc_momentum_w(ctx);
+  hc_viscosity3d(ctx);
+  hc_source_terms(ctx, nullptr);
+  hc_work_velocity(ctx, ctx->P.dt);
+}
+
+// one forward step of the window (the native loop without the passive
+// tracer c, which no loss here depends on, and without the CFL check)
+static void fwd_step(hc_ctx *ctx, int m) {
+  step_head(ctx, m);
+  if (ctx->P.toggle_t == 1) native_tracer(ctx, HC_T);
+  if (ctx->P.toggle_s == 1) native_tracer(ctx, HC_S);
+  if (ctx->P.bit3 == 1) hc_density(ctx);
+  hc_refresh(ctx);
+  ctx->next_step = m + 1;
+}
+
+// the adjoint of step m, with the live state holding the step's
+// recomputed head (step_head from its taped state); t1/s1 are T and S after
+// the step, ss0/ssv0 the viscosities at its start
+template <typename T>
+static void adj_step(hc_ctx *ctx, int m, const T *t1, const T *s1, const T *ss0, const T *ssv0) {
+  (void)m;
+  const hc_params &P = ctx->P;
+  adj_refresh_t<T>(ctx);
+  if (P.bit3 == 1) adj_density_t<T>(ctx, t1, s1);
+  double tmin, smax;
+  native_bounds(ctx, &tmin, &smax);
+  const int nz = P.nz;
+  if (P.toggle_s == 1)
+    adj_tracer_t<T>(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, ctx->d_q[1]);
+  if (P.toggle_t == 1)
+    adj_tracer_t<T>(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, ctx->d_q[0]);
+  (void)nz;
+  adj_courant_t<T>(ctx, P.dt);
+  adj_viscosity_t<T>(ctx, (const T *)ctx->f[HC_SS], (const T *)ctx->f[HC_SSV]);
+  adj_momentum_t<T>(ctx, 2);
+  adj_momentum_t<T>(ctx, 1);
+  adj_momentum_t<T>(ctx, 0);
+  adj_pressure_t<T>(ctx);
+  adj_forces_t<T>(ctx, ss0, ssv0);
+}
+
+template <typename T>
+static void probe_sample(hc_ctx *ctx, size_t q) {
+  AdjCtx *A = ctx->adj;
+  dim3 g((A->nprobe + 255) / 256);
+  if (sizeof(T) == 8)
+    hipLaunchKernelGGL(k_probe_sample, g, dim3(256), 0, ctx->st, A->nprobe, A->d_iw, A->d_it,
+                       (const double *)ctx->f[HC_W], (const double *)ctx->f[HC_T], A->d_samp + 2 * A->nprobe * q);
+  else
+    hipLaunchKernelGGL(k_probe_sample32, g, dim3(256), 0, ctx->st, A->nprobe, A->d_iw, A->d_it,
+                       (const float *)ctx->f[HC_W], (const float *)ctx->f[HC_T], A->d_samp + 2 * A->nprobe * q);
+}
+
+template <typename T>
+static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double *samples, double *loss, double *gq,
+                    double gs[3]) {
+  AdjCtx *A = ctx->adj;
+  const int m0 = ctx->next_step;
+  const int nck = (nsteps + seg - 1) / seg;  // checkpoints at n = 0, seg, ...
+  const size_t slots = (size_t)nck + 1 + (size_t)seg + 1;
+  const size_t slot_bytes = (size_t)NTAPE * ctx->L.n * ctx->esz;
+  if (A->tape_slots < slots) {
+    if (A->tape) { hipStreamSynchronize(ctx->st); hipFree(A->tape); A->tape = nullptr; A->tape_slots = 0; }
+    if (hipMalloc(&A->tape, slots * slot_bytes) != hipSuccess) {
+      A->tape = nullptr;
+      return fail(ctx, HC_ENOMEM, "checkpoint tape: hipMalloc of " + std::to_string(slots * slot_bytes) +
+                                      " bytes failed (use a smaller segment length)");
+    }
+    A->tape_slots = slots;
+  }
+  const int nsamp = nsteps / A->every;
+  const size_t ns = (size_t)nsamp * 2 * A->nprobe;
+  if (A->nsamp_cap < ns) {
+    if (A->d_samp) { hipFree(A->d_samp); hipFree(A->d_obs); }
+    int rc = 0;
+    if (!(A->d_samp = (double *)dalloc(ctx, ns * 8, &rc))) return rc;
+    if (!(A->d_obs = (double *)dalloc(ctx, ns * 8, &rc))) return rc;
+    A->nsamp_cap = ns;
+  }
+  if (ns) HIPCHK(hipMemcpyAsync(A->d_obs, obs, ns * 8, hipMemcpyHostToDevice, ctx->st));
+  const size_t tape0 = (size_t)nck + 1;  // first slot of the segment tape
+  std::vector<HostState> ck(nck + 1), tp(seg + 1);
+  // forward with checkpoints
+  for (int n = 0; n < nsteps; ++n) {
+    if (n % seg == 0) {
+      save_slot(ctx, n / seg);
+      ck[n / seg] = host_state(ctx);
+    }
+    fwd_step(ctx, m0 + n);
+    if ((n + 1) % A->every == 0) probe_sample<T>(ctx, (size_t)((n + 1) / A->every - 1));
+  }
+  save_slot(ctx, nck);
+  ck[nck] = host_state(ctx);
+  std::vector<double> samp(ns);
+  if (ns) HIPCHK(hipMemcpyAsync(samp.data(), A->d_samp, ns * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  double L2 = 0.0;
+  for (size_t q = 0; q < ns; q += 2) {
+    const double dw = samp[q] - obs[q], dtt = samp[q + 1] - obs[q + 1];
+    L2 += 0.5 * (A->wgt_w * dw * dw + A->wgt_t * dtt * dtt);
+  }
+  *loss = L2;
+  if (samples && ns) memcpy(samples, samp.data(), ns * 8);
+  // reverse sweep
+  int r = hc_adj_reset(ctx);
+  if (r) return r;
+  for (int sg = nck - 1; sg >= 0; --sg) {
+    const int a = sg * seg, b = std::min(nsteps, a + seg);
+    load_slot(ctx, sg);
+    set_host_state(ctx, ck[sg]);
+    for (int n = a; n < b; ++n) {
+      save_slot(ctx, tape0 + (n - a));
+      tp[n - a] = host_state(ctx);
+      fwd_step(ctx, m0 + n);
+    }
+    save_slot(ctx, tape0 + (b - a));
+    tp[b - a] = host_state(ctx);
+    for (int n = b - 1; n >= a; --n) {
+      FieldPtrs<T> lam = lptrs<T>(ctx);
+      if ((n + 1) % A->every == 0) {
+        const size_t q = (size_t)((n + 1) / A->every - 1);
+        hipLaunchKernelGGL(k_probe_seed<T>, dim3((A->nprobe + 255) / 256), dim3(256), 0, ctx->st, A->nprobe, A->d_iw,
+                           A->d_it, A->d_samp + 2 * A->nprobe * q, A->d_obs + 2 * A->nprobe * q, A->wgt_w, A->wgt_t,
+                           lam.w, lam.t);
+      }
+      load_slot(ctx, tape0 + (n - a));
+      set_host_state(ctx, tp[n - a]);
+      step_head(ctx, m0 + n);
+      const size_t s1 = tape0 + (n - a) + 1, s0 = tape0 + (n - a);
+      adj_step<T>(ctx, m0 + n, (const T *)slot_ptr(ctx, s1, 9), (const T *)slot_ptr(ctx, s1, 10),
+                  (const T *)slot_ptr(ctx, s0, 6), (const T *)slot_ptr(ctx, s0, 7));
+    }
+  }
+  // continue from the window's end state
+  load_slot(ctx, nck);
+  set_host_state(ctx, ck[nck]);
+  for (int c = 0; c < 3; ++c) ctx->up_alias[c] = false;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, HC_EDEVICE, std::string("adjoint window: ") + hipGetErrorString(e));
+  return hc_adj_gradients(ctx, gq, gs);
+}
+
+extern "C" int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples,
+                             double *loss, double *gq, double gs[3]) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  if (ctx->h_tbkg.empty()) return fail(ctx, HC_EINVAL, "profiles not set (hc_init_case or hc_set_profiles)");
+  if (!ctx->adj->d_iw) return fail(ctx, HC_EINVAL, "no probes (hc_adj_set_probes)");
+  if (nsteps < 1 || seg < 1) return fail(ctx, HC_EINVAL, "nsteps and seg must be positive");
+  if (ctx->P.ismol != 1) return fail(ctx, HC_EINVAL, "only ismol = 1 (two corrective passes) is supported");
+  PhaseScope ps(ctx, "adj_window");
+  if (ctx->prec == HC_FP64) return window_t<double>(ctx, nsteps, seg, obs, samples, loss, gq, gs);
+  return window_t<float>(ctx, nsteps, seg, obs, samples, loss, gq, gs);
+}

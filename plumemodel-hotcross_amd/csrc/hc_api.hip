@@ -18,7 +18,7 @@
 using namespace hc;
 
 // sumup/sumdown of MOMENTUM_W (hotcross80-revised2014.f:823-833)
-static void update_sums(hc_ctx *ctx) {
+void update_sums(hc_ctx *ctx) {
   int nx = ctx->P.nx, ny = ctx->P.ny;
   double su = 0, sd = 0;
   for (int j = 1; j <= ny; ++j)
@@ -55,7 +55,8 @@ __global__ void k_gather(Layout L, int l0, int l1, int l2, int n0, int n1, int n
 // q patch of SOURCE_terms (source80.inc:40-49) from the noise draws
 __global__ void k_source(int npatch, double fq_t, double fq_s, double wee, const double *__restrict__ noise,
                          int use_hash, uint64_t seed, int mtime, const double *__restrict__ qmap,
-                         double *__restrict__ qt, double *__restrict__ qs, double *__restrict__ qc) {
+                         double *__restrict__ qt, double *__restrict__ qs, double *__restrict__ qc,
+                         double *__restrict__ qbase) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= npatch) return;
   double r1, r2;
@@ -67,6 +68,7 @@ __global__ void k_source(int npatch, double fq_t, double fq_s, double wee, const
     r2 = noise[3 * n + 1];
   }
   qt[n] = fq_t * (1.0 + wee * (2.0 * (r1 - .5)));
+  qbase[n] = qt[n];
   if (qmap) qt[n] = qt[n] * qmap[n];
   qs[n] = fq_s * (1.0 + wee * (2.0 * (r2 - .5)));
   qc[n] = 1.0;
@@ -155,6 +157,7 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
   for (int q = 0; q < 3; ++q)
     if (!(ctx->d_q[q] = (double *)dalloc(ctx, (size_t)ctx->npatch * 8, &rc))) return rc;
   if (!(ctx->d_noise = (double *)dalloc(ctx, (size_t)ctx->npatch * 3 * 8, &rc))) return rc;
+  if (!(ctx->d_qbase = (double *)dalloc(ctx, (size_t)ctx->npatch * 8, &rc))) return rc;
   ctx->stage_n = (size_t)(P.nx + 3) * (P.ny + 3) * (P.nz + 3);
   if (!(ctx->d_stage = (double *)dalloc(ctx, ctx->stage_n * 8, &rc))) return rc;
   int ncfl = (P.ny + 1) * (P.nz + 1);
@@ -217,7 +220,7 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
                 ctx->tw.budget, ctx->aux.azzmin, ctx->aux.rho_bkg, ctx->aux.pgradx, ctx->aux.pgrady,
                 ctx->aux.alphaw, ctx->aux.alphau, ctx->aux.alphav, ctx->aux.alphae, ctx->aux.alphae0,
                 ctx->d_tbkg, ctx->d_sbkg, ctx->d_zero, ctx->d_cbkg, ctx->d_q[0], ctx->d_q[1], ctx->d_q[2],
-                ctx->d_noise, ctx->d_stage, ctx->d_cflpart, ctx->d_cflout, ctx->pw.r, ctx->pw.piv, ctx->pw.part,
+                ctx->d_noise, ctx->d_qbase, ctx->d_stage, ctx->d_cflpart, ctx->d_cflout, ctx->pw.r, ctx->pw.piv, ctx->pw.part,
                 ctx->pw.delta, ctx->d_tw};
   for (void *p : ps)
     if (p) hipFree(p);
@@ -259,43 +262,49 @@ static int check_box(hc_ctx *ctx, int field, const int32_t lo[3], const int32_t 
   return HC_OK;
 }
 
-extern "C" int hc_upload(hc_ctx *ctx, int field, const double *host, const int32_t lo[3], const int32_t hi[3]) {
+// copy between a host Fortran box and a padded device field (dev) with the
+// bounds of field id bounds_field; up = host to device
+int box_copy(hc_ctx *ctx, void *dev, int bounds_field, double *host, const int32_t lo[3], const int32_t hi[3],
+             bool up) {
   int n[3];
-  int r = check_box(ctx, field, lo, hi, n);
+  int r = check_box(ctx, bounds_field, lo, hi, n);
   if (r) return r;
-  if (field >= HC_U && field <= HC_WP) {
-    r = materialise_alias(ctx, field % 3);
-    if (r) return r;
-  }
   size_t cnt = (size_t)n[0] * n[1] * n[2];
-  HIPCHK(hipMemcpyAsync(ctx->d_stage, host, cnt * 8, hipMemcpyHostToDevice, ctx->st));
   dim3 g((unsigned)((cnt + 255) / 256));
-  if (ctx->prec == HC_FP64)
-    hipLaunchKernelGGL(k_scatter<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
-                       ctx->d_stage, (double *)ctx->f[field]);
-  else
-    hipLaunchKernelGGL(k_scatter<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
-                       ctx->d_stage, (float *)ctx->f[field]);
+  if (up) {
+    HIPCHK(hipMemcpyAsync(ctx->d_stage, host, cnt * 8, hipMemcpyHostToDevice, ctx->st));
+    if (ctx->prec == HC_FP64)
+      hipLaunchKernelGGL(k_scatter<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                         ctx->d_stage, (double *)dev);
+    else
+      hipLaunchKernelGGL(k_scatter<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                         ctx->d_stage, (float *)dev);
+  } else {
+    if (ctx->prec == HC_FP64)
+      hipLaunchKernelGGL(k_gather<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                         (const double *)dev, ctx->d_stage);
+    else
+      hipLaunchKernelGGL(k_gather<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                         (const float *)dev, ctx->d_stage);
+    HIPCHK(hipMemcpyAsync(host, ctx->d_stage, cnt * 8, hipMemcpyDeviceToHost, ctx->st));
+  }
   HIPCHK(hipStreamSynchronize(ctx->st));
   return HC_OK;
 }
 
+extern "C" int hc_upload(hc_ctx *ctx, int field, const double *host, const int32_t lo[3], const int32_t hi[3]) {
+  if (field < 0 || field >= HC_NFIELDS) return fail(ctx, HC_EINVAL, "bad field id");
+  if (field >= HC_U && field <= HC_WP) {
+    int r = materialise_alias(ctx, field % 3);
+    if (r) return r;
+  }
+  return box_copy(ctx, ctx->f[field], field, (double *)host, lo, hi, true);
+}
+
 extern "C" int hc_download(hc_ctx *ctx, int field, double *host, const int32_t lo[3], const int32_t hi[3]) {
-  int n[3];
-  int r = check_box(ctx, field, lo, hi, n);
-  if (r) return r;
+  if (field < 0 || field >= HC_NFIELDS) return fail(ctx, HC_EINVAL, "bad field id");
   int src = (field >= HC_UP && field <= HC_WP && ctx->up_alias[field - HC_UP]) ? field - 3 : field;
-  size_t cnt = (size_t)n[0] * n[1] * n[2];
-  dim3 g((unsigned)((cnt + 255) / 256));
-  if (ctx->prec == HC_FP64)
-    hipLaunchKernelGGL(k_gather<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
-                       (const double *)ctx->f[src], ctx->d_stage);
-  else
-    hipLaunchKernelGGL(k_gather<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
-                       (const float *)ctx->f[src], ctx->d_stage);
-  HIPCHK(hipMemcpyAsync(host, ctx->d_stage, cnt * 8, hipMemcpyDeviceToHost, ctx->st));
-  HIPCHK(hipStreamSynchronize(ctx->st));
-  return HC_OK;
+  return box_copy(ctx, ctx->f[src], field, host, lo, hi, false);
 }
 
 extern "C" int hc_set_profiles(hc_ctx *ctx, const double *t_bkg, const double *s_bkg, const double *rho_bkg,
@@ -552,7 +561,7 @@ extern "C" int hc_source_terms(hc_ctx *ctx, const double *noise) {
   if (noise) HIPCHK(hipMemcpyAsync(ctx->d_noise, noise, (size_t)ctx->npatch * 3 * 8, hipMemcpyHostToDevice, ctx->st));
   hipLaunchKernelGGL(k_source, dim3((ctx->npatch + 255) / 256), dim3(256), 0, ctx->st, ctx->npatch, P.dqtdt * f,
                      P.dqsdt * f, P.noise_amp, ctx->d_noise, noise ? 0 : 1, ctx->seed, ctx->mtime, ctx->d_qmap, ctx->d_q[0],
-                     ctx->d_q[1], ctx->d_q[2]);
+                     ctx->d_q[1], ctx->d_q[2], ctx->d_qbase);
   if (noise) HIPCHK(hipStreamSynchronize(ctx->st));
   if (!ctx->map_valid) {
     ctx->map_valid = true;
@@ -597,6 +606,29 @@ static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, dou
     launch_tracer<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, ctx->tw,
                          (float *)ctx->f[field], d_cbkg, ctx->d_q[which], ta, mk, ctx);
   return HC_OK;
+}
+
+// cmin of T and cmax of S in the native loop (hotcross80-revised2014.f:307,
+// 321-324): the background profiles' extremes
+void native_bounds(hc_ctx *ctx, double *tmin, double *smax) {
+  *tmin = 1e300;
+  *smax = -1e300;
+  for (int k = 0; k <= ctx->P.nz + 1; ++k) {
+    *tmin = fmin(*tmin, ctx->h_tbkg[k]);
+    *smax = fmax(*smax, ctx->h_sbkg[k]);
+  }
+}
+
+// TRACER31 for T, S or C as the native loop calls it (bounds, background,
+// vent ghost cells of T and S), without budgets
+int native_tracer(hc_ctx *ctx, int field) {
+  const hc_params &P = ctx->P;
+  double tmin, smax;
+  native_bounds(ctx, &tmin, &smax);
+  const int nz = P.nz;
+  if (field == HC_T) return tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], 0);
+  if (field == HC_S) return tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], 0);
+  return tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, 0);
 }
 
 extern "C" int hc_tracer31(hc_ctx *ctx, int field, const double *c_bkg, double cmin, double cmax, double tau3,

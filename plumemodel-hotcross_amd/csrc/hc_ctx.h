@@ -41,6 +41,7 @@ struct hc_ctx {
   double *d_tw = nullptr;
   double *d_q[3] = {};       // q_t, q_s, q_c on the patch
   double *d_noise = nullptr; // 3 draws per patch cell
+  double *d_qbase = nullptr; // q_t of the patch before the heat-flux map factor
   double *d_stage = nullptr; // host<->device staging
   size_t stage_n = 0;
   double *d_cflpart = nullptr, *d_cflout = nullptr;
@@ -83,6 +84,12 @@ struct hc_ctx {
 };
 
 void adj_free(hc_ctx *ctx);
+// shared by the C-ABI translation units (hc_api.hip)
+int box_copy(hc_ctx *ctx, void *dev, int bounds_field, double *host, const int32_t lo[3], const int32_t hi[3],
+             bool up);
+void update_sums(hc_ctx *ctx);
+void native_bounds(hc_ctx *ctx, double *tmin, double *smax);
+int native_tracer(hc_ctx *ctx, int field);
 
 #define HIPCHK(x)                                                                    \
   do {                                                                               \

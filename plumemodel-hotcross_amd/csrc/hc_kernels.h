@@ -32,6 +32,11 @@ struct Aux {
   double *alphaw;   // (-1:nz+1)
 };
 
+// d(ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby)/d(flow_scale) at one step
+struct FlowSens {
+  double dubkg_m, dubkg, dvbkg_m, dvbkg, dfbx, dfby;
+};
+
 template <typename T>
 void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                    const Aux &aux);
@@ -60,6 +65,7 @@ struct TracerWork {
   void *uc, *vc, *wc;      // Courant numbers of the work velocities
   void *ud2, *vd2, *wd2;   // antidiffusive velocities of pass 2 (pass 3's Courant numbers)
   void *c1, *c2;           // pass results (c1: pass 1, then the pre-sponge boundary values)
+  void *cpre;              // if set, receives the pre-sponge boundary values instead of c1
   double *budget_part;     // per-block partial sums (FP64)
   double *budget;          // 3 sums
 };
@@ -98,5 +104,13 @@ template <typename T>
 void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
                      const Aux &aux, PoissonWork &pw);
 void poisson_setup(const Layout &L, const Phys &ph, PoissonWork &pw, hipStream_t st);
+// adjoint of PRESSURE: consumes lam.p (the adjoint of the pressure field,
+// ghosts included) and adds to lam.um, vm, wm, w, fx, fy, fz; gflow (one
+// FP64 device value) collects d/d(flow_scale) through the boundary data.
+// g is a compact (nx,ny,nz) work array; zero points to a device 0.0.
+template <typename T>
+void launch_pressure_adj(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                         const FieldPtrs<T> &lam, const Aux &aux, PoissonWork &pw, const FlowSens &fs,
+                         double *gflow, T *g, const double *zero, double *part);
 
 }  // namespace hc

@@ -801,9 +801,272 @@ void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const P
   hipLaunchKernelGGL(k_pyghost<T>, dim3((nx + 255) / 256, nz), dim3(256), 0, st, L, s, ph, f.fy, aux.pgrady, f.p);
 }
 
+
+// ---------------------------------------------------------------------------
+// Adjoint of PRESSURE.  The forward map is affine in its inputs:
+//   r  = RHS(um, vm, wm, w, fx, fy, fz) + boundary constants    (k_rhs)
+//   r' = r - e_nx * sum(r) / (ny nz)                               (delta rule)
+//   X  = Dy Dx r'        (orthonormal DCT-II, x then y)
+//   Y  = S X             (z tridiagonal per (kx,ky); the (0,0) column with
+//                         its last pivot 1, then its k-mean removed)
+//   p  = Dx^-1 Dy^-1 Y,  ghosts of p from p and the boundary data.
+// Its transpose runs the steps backwards: the ghost rules' adjoint, Dy Dx
+// (the inverse of an orthonormal transform is its transpose), the transposed
+// Thomas recurrences, Dx^-1 Dy^-1, the delta rule's transpose
+// (subtract the i = nx plane's mean from every cell), and the divergence's
+// transpose (a difference in the other direction).
+
+// interior adjoint of p in the compact layout, plus the boundary data's
+// share of the ghost adjoints (one thread per interior cell; the ghost cells
+// next to it are handled by it alone)
+template <typename T>
+__global__ void __launch_bounds__(256) k_padj_ghost(Layout L, StepArgs s, Phys ph, FlowSens fs,
+                                                    const T *__restrict__ lp, T *__restrict__ lfx,
+                                                    T *__restrict__ lfy, T *__restrict__ lfz, T *__restrict__ lw,
+                                                    T *__restrict__ lwm, T *__restrict__ g, double *gflow) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x), j = 1 + (int)blockIdx.y, k = 1 + (int)blockIdx.z;
+  double gf = 0.0;
+  if (i <= nx) {
+    T v = F3(lp, i, j, k);
+    // z ghosts (:1348-1350): p(i,j,0) = p(i,j,1) - bd0 dz, p(i,j,nz+1) = p(i,j,nz) + bdn dz,
+    // bd = -(w - wm)/dt + fz at k = 0 / nz
+    if (k == 1) {
+      const T a = F3(lp, i, j, 0);
+      v += a;
+      F3(lw, i, j, 0) += a * (T)(s.dz / s.dt);
+      F3(lwm, i, j, 0) -= a * (T)(s.dz / s.dt);
+      F3(lfz, i, j, 0) -= a * (T)s.dz;
+    }
+    if (k == nz) {
+      const T a = F3(lp, i, j, nz + 1);
+      v += a;
+      F3(lw, i, j, nz) -= a * (T)(s.dz / s.dt);
+      F3(lwm, i, j, nz) += a * (T)(s.dz / s.dt);
+      F3(lfz, i, j, nz) += a * (T)s.dz;
+    }
+    // x ghosts (:1352-1372), bxs/bxf = -(ubkg - ubkg_m)/dt - pgradx + fx(0|nx)
+    const T bx = ph.open_box == 1 ? (T)1 : (T)ph.bcondx, obx = ph.open_box == 1 ? (T)0 : (T)(1 - ph.bcondx);
+    const double dbx = -(fs.dubkg - fs.dubkg_m) / s.dt;
+    if (i == 1) {
+      const T a0 = F3(lp, 0, j, k), an = F3(lp, nx + 1, j, k);
+      v += bx * a0 + obx * an;
+      F3(lfx, 0, j, k) -= a0 * bx * (T)s.dx;
+      gf += -(double)(a0 * bx) * s.dx * dbx;
+    }
+    if (i == nx) {
+      const T a0 = F3(lp, 0, j, k), an = F3(lp, nx + 1, j, k);
+      v += bx * an + obx * a0;
+      F3(lfx, nx, j, k) += an * bx * (T)s.dx;
+      gf += (double)(an * bx) * s.dx * dbx;
+    }
+    // y ghosts (:1374-1381)
+    const T by = (T)ph.bcondy, oby = (T)(1 - ph.bcondy);
+    const double dby = -(fs.dvbkg - fs.dvbkg_m) / s.dt;
+    if (j == 1) {
+      const T a0 = F3(lp, i, 0, k), an = F3(lp, i, ny + 1, k);
+      v += by * a0 + oby * an;
+      F3(lfy, i, 0, k) -= a0 * by * (T)s.dy;
+      gf += -(double)(a0 * by) * s.dy * dby;
+    }
+    if (j == ny) {
+      const T a0 = F3(lp, i, 0, k), an = F3(lp, i, ny + 1, k);
+      v += by * an + oby * a0;
+      F3(lfy, i, ny, k) += an * by * (T)s.dy;
+      gf += (double)(an * by) * s.dy * dby;
+    }
+    g[((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1)] = v;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = gf;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sh[0] != 0.0) atomicAdd(gflow, sh[0]);
+}
+
+// transposed Thomas recurrences of k_tridiag, per (kx,ky) column
+template <typename T>
+__global__ void __launch_bounds__(256) k_tridiag_adj(Layout L, double dz, const double *__restrict__ piv,
+                                                     T *__restrict__ buf) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= plane) return;
+  const double a = 1.0 / (dz * dz);
+  if (q == 0) {
+    // the (0,0) column in FP64: mean removal (a symmetric projection) first
+    double mean = 0.0;
+    for (int k = 0; k < nz; ++k) mean += (double)buf[k * plane];
+    mean /= nz;
+    // back substitution x_k = y_k - a d_k x_{k+1}, transposed (k ascending)
+    double lx_next = (double)buf[0] - mean;
+    for (int k = 0; k < nz - 1; ++k) {
+      const double ly = lx_next;
+      buf[k * plane] = (T)ly;  // holds lambda_y
+      lx_next = ((double)buf[(k + 1) * plane] - mean) - a * piv[k * plane] * ly;
+    }
+    buf[(nz - 1) * plane] = (T)lx_next;
+    // forward elimination y_k = (g_k - a y_{k-1}) d_k, transposed (k descending)
+    double ly = (double)buf[(nz - 1) * plane];
+    for (int k = nz - 1; k >= 1; --k) {
+      const double lg = piv[k * plane] * ly;
+      buf[k * plane] = (T)lg;
+      ly = (double)buf[(k - 1) * plane] - a * lg;
+    }
+    buf[0] = (T)(piv[0] * ly);
+    return;
+  }
+  T lx = buf[q];
+  for (int k = 0; k < nz - 1; ++k) {
+    const T ly = lx;
+    buf[(size_t)k * plane + q] = ly;
+    lx = buf[(size_t)(k + 1) * plane + q] - (T)a * (T)piv[(size_t)k * plane + q] * ly;
+  }
+  buf[(size_t)(nz - 1) * plane + q] = lx;
+  T ly = lx;
+  for (int k = nz - 1; k >= 1; --k) {
+    const T lg = (T)piv[(size_t)k * plane + q] * ly;
+    buf[(size_t)k * plane + q] = lg;
+    ly = buf[(size_t)(k - 1) * plane + q] - (T)a * lg;
+  }
+  buf[q] = (T)piv[q] * ly;
+}
+
+// sum of the adjoint over the plane i = nx (the delta rule's transpose)
+template <typename T>
+__global__ void __launch_bounds__(256) k_padj_plane(Layout L, const T *__restrict__ pad, double *__restrict__ part) {
+  const int k = 1 + (int)blockIdx.x;
+  double a = 0;
+  for (int j = 1 + (int)threadIdx.x; j <= L.ny; j += blockDim.x) a += (double)F3(pad, L.nx, j, k);
+  __shared__ double sh[256];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// transpose of k_rhs: one thread per face (i,j,k), i,j,k = 0..n
+template <typename T>
+__global__ void __launch_bounds__(256) k_padj_rhs(Layout L, Box b, StepArgs s, FlowSens fs,
+                                                  const T *__restrict__ pad, const double *__restrict__ dsum,
+                                                  T *__restrict__ lum, T *__restrict__ lvm, T *__restrict__ lwm,
+                                                  T *__restrict__ lw, T *__restrict__ lfx, T *__restrict__ lfy,
+                                                  T *__restrict__ lfz, double *gflow) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)blockIdx.y, k = (int)blockIdx.z;
+  (void)b;
+  double gf = 0.0;
+  if (i <= nx) {
+    const T corr = (T)(dsum[0] / ((double)ny * nz));
+    auto LR = [&](int ii, int jj, int kk) -> T {
+      return (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny && kk >= 1 && kk <= nz) ? F3(pad, ii, jj, kk) - corr : (T)0;
+    };
+    const double dx = s.dx, dy = s.dy, dz = s.dz, dt = s.dt;
+    const double dlx = 1.0 / ((nx * dx - 0.0) / nx), dly = 1.0 / ((ny * dy - 0.0) / ny), dlz = 1.0 / ((nz * dz - 0.0) / nz);
+    const T cm = (T)(1.0 / dt / s.leap);
+    if (j >= 1 && k >= 1) {
+      const T d = LR(i, j, k) - LR(i + 1, j, k);
+      F3(lum, i, j, k) += d * (T)(1.0 / dx) * cm;
+      T f = d * (T)(1.0 / dx);
+      const double dbd = -(fs.dubkg - fs.dubkg_m) / dt;
+      if (i == 0) { f += LR(1, j, k) * (T)dlx; gf += (double)LR(1, j, k) * dlx * dbd; }
+      if (i == nx) { f -= LR(nx, j, k) * (T)dlx; gf -= (double)LR(nx, j, k) * dlx * dbd; }
+      F3(lfx, i, j, k) += f;
+    }
+    if (i >= 1 && k >= 1) {
+      const T d = LR(i, j, k) - LR(i, j + 1, k);
+      F3(lvm, i, j, k) += d * (T)(1.0 / dy) * cm;
+      T f = d * (T)(1.0 / dy);
+      const double dbd = -(fs.dvbkg - fs.dvbkg_m) / dt;
+      if (j == 0) { f += LR(i, 1, k) * (T)dly; gf += (double)LR(i, 1, k) * dly * dbd; }
+      if (j == ny) { f -= LR(i, ny, k) * (T)dly; gf -= (double)LR(i, ny, k) * dly * dbd; }
+      F3(lfy, i, j, k) += f;
+    }
+    if (i >= 1 && j >= 1) {
+      const T d = LR(i, j, k) - LR(i, j, k + 1);
+      T wmv = d * (T)(1.0 / dz) * cm;
+      T f = d * (T)(1.0 / dz);
+      if (k == 0) {
+        const T a = LR(i, j, 1) * (T)dlz;
+        f += a;
+        F3(lw, i, j, 0) -= a * (T)(1.0 / dt);
+        wmv += a * (T)(1.0 / dt);
+      }
+      if (k == nz) {
+        const T a = LR(i, j, nz) * (T)dlz;
+        f -= a;
+        F3(lw, i, j, nz) += a * (T)(1.0 / dt);
+        wmv -= a * (T)(1.0 / dt);
+      }
+      F3(lwm, i, j, k) += wmv;
+      F3(lfz, i, j, k) += f;
+    }
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = gf;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sh[0] != 0.0) atomicAdd(gflow, sh[0]);
+}
+
+template <typename T>
+void launch_pressure_adj(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                         const FieldPtrs<T> &lam, const Aux &aux, PoissonWork &pw, const FlowSens &fs,
+                         double *gflow, T *g, const double *zero, double *part) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  hipLaunchKernelGGL(k_padj_ghost<T>, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, st, L, s, ph, fs,
+                     (const T *)lam.p, lam.fx, lam.fy, lam.fz, lam.w, lam.wm, g, gflow);
+  FftPlan px = make_plan(nx), py = make_plan(ny);
+  int npx = pick_npair(nx, sizeof(T)), npy = pick_npair(ny, sizeof(T));
+  size_t shx = (size_t)2 * npx * nx * 2 * sizeof(T), shy = (size_t)2 * npy * ny * 2 * sizeof(T);
+  int gx = (ny * nz + 2 * npx - 1) / (2 * npx);
+  int gy = nz * ((nx + 2 * npy - 1) / (2 * npy));
+  const double *twx = pw.tw_x, *ctx = pw.tw_x + 2 * nx;
+  const double *twy = pw.tw_y, *cty = pw.tw_y + 2 * ny;
+  const bool fast = fast_transforms();
+  // Dy Dx (forward transforms, no delta rule)
+  if (!(fast && launch_dctx_fast<T, 0>(st, L, twx, ctx, g, g, zero, s, ph, (T *)nullptr, (const T *)nullptr,
+                                       (const T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                                       (const double *)nullptr)))
+    hipLaunchKernelGGL((k_dct_x<T, 0>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, g, g, zero, s, ph,
+                       (T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+                       (const T *)nullptr, (const double *)nullptr);
+  if (!(fast && launch_dcty_fast<T, 0>(st, L, twy, cty, g)))
+    hipLaunchKernelGGL((k_dct_y<T, 0>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, g);
+  hipLaunchKernelGGL(k_tridiag_adj<T>, dim3((unsigned)(((size_t)nx * ny + 255) / 256)), dim3(256), 0, st, L, s.dz,
+                     pw.piv, g);
+  // Dx^-1 Dy^-1 into the interior of lam.p (its ghosts get values that are
+  // not used and cleared with the rest of lam.p afterwards)
+  if (!(fast && launch_dcty_fast<T, 1>(st, L, twy, cty, g)))
+    hipLaunchKernelGGL((k_dct_y<T, 1>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, g);
+  if (!(fast && launch_dctx_fast<T, 1>(st, L, twx, ctx, g, (T *)nullptr, zero, s, ph, lam.p, f.fx, f.w, f.wm,
+                                       f.fz, aux.pgradx)))
+    hipLaunchKernelGGL((k_dct_x<T, 1>), dim3(gx), dim3(256), shx, st, L, px, npx, twx, ctx, g, (T *)nullptr,
+                       zero, s, ph, lam.p, f.fx, f.w, f.wm, f.fz, aux.pgradx);
+  hipLaunchKernelGGL(k_padj_plane<T>, dim3(nz), dim3(256), 0, st, L, (const T *)lam.p, part);
+  hipLaunchKernelGGL(k_sum1, dim3(1), dim3(256), 0, st, nz, part, part + nz);
+  Box b{0, nx, 0, ny, 0, nz};
+  hipLaunchKernelGGL(k_padj_rhs<T>, dim3((nx + 1 + 255) / 256, ny + 1, nz + 1), dim3(256), 0, st, L, b, s, fs,
+                     (const T *)lam.p, (const double *)(part + nz), lam.um, lam.vm, lam.wm, lam.w, lam.fx, lam.fy,
+                     lam.fz, gflow);
+  hipMemsetAsync(lam.p, 0, L.n * sizeof(T), st);
+}
+
 #define HC_INST_P(T)                                                                                          \
   template void launch_pressure<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,               \
-                                   const FieldPtrs<T> &, const Aux &, PoissonWork &);
+                                   const FieldPtrs<T> &, const Aux &, PoissonWork &);                        \
+  template void launch_pressure_adj<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,           \
+                                       const FieldPtrs<T> &, const FieldPtrs<T> &, const Aux &, PoissonWork &,  \
+                                       const FlowSens &, double *, T *, const double *, double *);
 HC_INST_P(double)
 HC_INST_P(float)
 

@@ -282,11 +282,12 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   // c1 receives the pre-sponge values of the cells next to a boundary
   a.cc = c2; a.U = (T *)tw.ud2; a.V = (T *)tw.vd2; a.W = (T *)tw.wd2;
   a.ud = a.vd = a.wd = nullptr;
-  a.cout = c; a.cpre = c1; a.last = 1;
+  T *cpre = tw.cpre ? (T *)tw.cpre : c1;
+  a.cout = c; a.cpre = cpre; a.last = 1;
   if (mark) mark(mark_arg, 0);
   launch_mpdata_pass<T>(st, L, a);
   if (mark) mark(mark_arg, 1);
-  hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c, c1, ta.vent, ta.vent_value);
+  hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c, cpre, ta.vent, ta.vent_value);
   if (ta.budgets) {
     int nb = L.ny * L.nz;
     hipLaunchKernelGGL(k_budget<T>, dim3(nb), dim3(256), 0, st, L, ta.tau3, va, c, c_bkg, aux.alphae, qpatch,

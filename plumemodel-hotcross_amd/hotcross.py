@@ -18,6 +18,9 @@ LIB_PATH = os.path.join(HERE, "libhotcross.so")
 FIELDS = ("u", "v", "w", "up", "vp", "wp", "um", "vm", "wm", "p", "t", "s", "c", "ss",
           "ss_v", "rhop", "fx", "fy", "fz")
 FIELD_ID = {n: q for q, n in enumerate(FIELDS)}
+# adjoint-only fields: the work velocities' Courant numbers (u-, v-, w-bounds)
+ADJ_ID = dict(FIELD_ID, uc=32, vc=33, wc=34)
+ADJ_BOUNDS = {"uc": "u", "vc": "v", "wc": "w"}
 FP64, FP32 = 0, 1
 
 
@@ -93,6 +96,19 @@ def lib():
             "hc_step": [vp, i32], "hc_source_noise": [vp, i32, dp],
             "hc_phase_timing": [vp, ctypes.c_int],
             "hc_phase_times": [vp, ctypes.POINTER(ctypes.c_char_p), dp, dp, i32],
+            "hc_set_controls": [vp, dp, dbl, dbl],
+            "hc_adj_reset": [vp],
+            "hc_adj_upload": [vp, ctypes.c_int, dp, dp, dp],
+            "hc_adj_download": [vp, ctypes.c_int, dp, dp, dp],
+            "hc_adj_forces": [vp], "hc_adj_pressure": [vp],
+            "hc_adj_momentum_u": [vp], "hc_adj_momentum_v": [vp], "hc_adj_momentum_w": [vp],
+            "hc_adj_viscosity3d": [vp], "hc_adj_work_velocity": [vp, dbl],
+            "hc_adj_tracer31": [vp, ctypes.c_int, dp, dbl, dbl, dbl],
+            "hc_adj_vent_ghost": [vp, ctypes.c_int],
+            "hc_adj_density": [vp], "hc_adj_refresh": [vp],
+            "hc_adj_gradients": [vp, dp, dp],
+            "hc_adj_set_probes": [vp, i32, dp, i32, dbl, dbl],
+            "hc_adj_window": [vp, i32, i32, dp, dp, dp, dp, dp],
         }
         for n, a in sig.items():
             f = getattr(L, n)
@@ -277,6 +293,96 @@ class Context:
         out = np.zeros(3 * 4096)
         n = self.L.hc_source_noise(self.h, m, _arr(out))
         return out[: 3 * n].copy()
+
+    def npatch(self):
+        return len(self.source_noise(0)) // 3
+
+    # ---- controls and discrete adjoint ---------------------------------------
+    def set_controls(self, qmap=None, flow_scale=1.0, cs=0.0):
+        """Heat-flux multiplier per source cell (None = 1), crossflow
+        amplitude factor, Smagorinsky constant (cs <= 0 keeps the case's)."""
+        q = None if qmap is None else np.ascontiguousarray(qmap, dtype=np.float64)
+        self._check(self.L.hc_set_controls(self.h, None if q is None else _arr(q), float(flow_scale), float(cs)))
+
+    def adj_reset(self):
+        self._check(self.L.hc_adj_reset(self.h))
+
+    def _adj_box(self, name):
+        lo, hi = self.bounds(ADJ_BOUNDS.get(name, name))
+        return lo, hi, tuple(h - l + 1 for l, h in zip(lo, hi))
+
+    def adj_upload(self, name, arr):
+        lo, hi, shape = self._adj_box(name)
+        a = np.asfortranarray(np.asarray(arr, dtype=np.float64).reshape(shape, order="F"))
+        self._check(self.L.hc_adj_upload(self.h, ADJ_ID[name], _arr(a), _arr(np.array(lo, dtype=np.int32)),
+                                         _arr(np.array(hi, dtype=np.int32))))
+
+    def adj_download(self, name):
+        lo, hi, shape = self._adj_box(name)
+        a = np.zeros(shape, dtype=np.float64, order="F")
+        self._check(self.L.hc_adj_download(self.h, ADJ_ID[name], _arr(a), _arr(np.array(lo, dtype=np.int32)),
+                                           _arr(np.array(hi, dtype=np.int32))))
+        return a
+
+    def adj_forces(self):
+        self._check(self.L.hc_adj_forces(self.h))
+
+    def adj_pressure(self):
+        self._check(self.L.hc_adj_pressure(self.h))
+
+    def adj_momentum_u(self):
+        self._check(self.L.hc_adj_momentum_u(self.h))
+
+    def adj_momentum_v(self):
+        self._check(self.L.hc_adj_momentum_v(self.h))
+
+    def adj_momentum_w(self):
+        self._check(self.L.hc_adj_momentum_w(self.h))
+
+    def adj_viscosity3d(self):
+        self._check(self.L.hc_adj_viscosity3d(self.h))
+
+    def adj_work_velocity(self, tau3):
+        self._check(self.L.hc_adj_work_velocity(self.h, tau3))
+
+    def adj_tracer31(self, name, c_bkg, cmin, cmax, tau3):
+        cb = np.ascontiguousarray(c_bkg, dtype=np.float64)
+        self._check(self.L.hc_adj_tracer31(self.h, FIELD_ID[name], _arr(cb), cmin, cmax, tau3))
+
+    def adj_vent_ghost(self, name):
+        self._check(self.L.hc_adj_vent_ghost(self.h, FIELD_ID[name]))
+
+    def adj_density(self):
+        self._check(self.L.hc_adj_density(self.h))
+
+    def adj_refresh(self):
+        self._check(self.L.hc_adj_refresh(self.h))
+
+    def adj_gradients(self):
+        """(dL/dqmap per source cell, [dL/dflow_scale, dL/dcs_h, dL/dcs_v])."""
+        gq = np.zeros(self.npatch())
+        out = np.zeros(3)
+        self._check(self.L.hc_adj_gradients(self.h, _arr(gq), _arr(out)))
+        return gq, out
+
+    def set_probes(self, ijk, every, wgt_w=1.0, wgt_t=1.0):
+        a = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))
+        self._probes = len(a)
+        self._every = int(every)
+        self._check(self.L.hc_adj_set_probes(self.h, len(a), _arr(a), int(every), float(wgt_w), float(wgt_t)))
+
+    def adj_window(self, nsteps, seg, obs):
+        """Forward nsteps with a checkpoint every seg steps, probe-misfit loss
+        against obs (shape (nsteps // every, nprobe, 2): w, T), reverse sweep.
+        Returns (loss, samples, dL/dqmap, [dL/dflow_scale, dL/dcs_h, dL/dcs_v])."""
+        ns = nsteps // self._every
+        o = np.ascontiguousarray(np.asarray(obs, dtype=np.float64).reshape(ns * self._probes * 2))
+        samp = np.zeros_like(o)
+        loss = np.zeros(1)
+        gq = np.zeros(self.npatch())
+        gs = np.zeros(3)
+        self._check(self.L.hc_adj_window(self.h, nsteps, seg, _arr(o), _arr(samp), _arr(loss), _arr(gq), _arr(gs)))
+        return float(loss[0]), samp.reshape(ns, self._probes, 2), gq, gs
 
     def phase_timing(self, enable=True):
         self._check(self.L.hc_phase_timing(self.h, 1 if enable else 0))
