@@ -59,10 +59,12 @@ def init_context(H, name, precision):
     return p, ctx
 
 
-def gpu_leg(args):
+def gpu_leg(args, precision=None, workload=None):
     import hotcross as H
-    prec = H.FP64 if args.precision == "fp64" else H.FP32
-    p, ctx = init_context(H, args.workload, prec)
+    precision = precision or args.precision
+    workload = workload or args.workload
+    prec = H.FP64 if precision == "fp64" else H.FP32
+    p, ctx = init_context(H, workload, prec)
     ctx.step(args.warmup)
     ctx.synchronize()
     try:
@@ -87,6 +89,39 @@ def gpu_leg(args):
     ctx.phase_timing(False)
     ctx.close()
     return p, sec, cells, phases, nprof
+
+
+def adjoint_leg(workload="les128", nsteps=400, seg=20):
+    """The adjoint window of the headline adjoint configuration: nsteps steps
+    of the FP64 timestep, 25 probes on the plane 20 m above the vent sampled
+    at 1 Hz, a checkpoint every seg steps, then the reverse sweep (segments
+    recomputed into an HBM tape, adjoint of every step).  Timed beside the
+    same number of plain forward steps (hc_step) from the same state."""
+    import hotcross as H
+    import adjoint as AD
+    p, ctx = init_context(H, workload, H.FP64)
+    ctx.step(2)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(nsteps)
+    ctx.synchronize()
+    fwd = time.perf_counter() - t0
+    probes = AD.plane_probes(p)
+    every = AD.samples_every(p)
+    ctx.set_probes(probes, every, 1.0e4, 1.0)
+    obs = np.zeros((nsteps // every, len(probes), 2))
+    t0 = time.perf_counter()
+    loss, _, gq, gs = ctx.adj_window(nsteps, seg, obs)
+    ctx.synchronize()
+    win = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": "%s, %d steps, checkpoint every %d, %d probes at 1 Hz" % (workload, nsteps, seg, len(probes)),
+            "forward_ms_per_step": round(fwd / nsteps * 1e3, 4),
+            "window_ms_per_step": round(win / nsteps * 1e3, 4),
+            "ratio": round((win - fwd) / fwd, 3),
+            "note": "ratio = (window - plain forward) / plain forward: the reverse sweep (segment recompute + "
+                    "adjoint of every step) per forward step",
+            "finite": bool(np.isfinite(loss) and np.all(np.isfinite(gq)) and np.all(np.isfinite(gs)))}
 
 
 def cpu_leg(nx, ny, nz, steps, threads):
@@ -146,6 +181,8 @@ def main():
     ap.add_argument("--workload", default="les256")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form's line")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window timing")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-worker", nargs=5)
     args = ap.parse_args()
@@ -185,6 +222,16 @@ def main():
                    "cores": threads, "kind": "reference",
                    "sample": "%d steps of the reference's time loop body at %dx%dx%d (amdflang -O2 OpenMP, "
                              "reference constants), after one untimed step" % (r["steps"], p.nx, p.ny, p.nz)}
+    fp32 = None
+    if args.precision == "fp64" and not args.no_fp32:
+        p32, sec32, _, ph32, n32 = gpu_leg(args, "fp32")
+        v32 = cells * args.steps / sec32 / 1e6
+        fp32 = {"value": round(v32, 3), "unit": "Mcell-steps/s", "ms_per_step": round(sec32 * 1e3 / args.steps, 4),
+                "speedup_vs_fp64": round(v32 / value, 3), "dtype": "f32 (FP64 islands)",
+                "err_vs_fp64": "tests/test_gpu_fp32.py: max error relative to max|field| after 50 steps"}
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg()
     out = {
         "metric": "Mcell-steps/s FP64 vs FP32 at stated err vs FP64; adjoint/forward time ratio",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -198,6 +245,8 @@ def main():
                           "frac": round(step_bytes / (ms_step * 1e-3) / HBM_PEAK, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "fp32": fp32,
+        "adjoint": adj,
         "phases_ms_per_step": {k: round(v[0] / nprof, 4) for k, v in phases.items()},
     }
     print(json.dumps(out))

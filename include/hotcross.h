@@ -237,11 +237,13 @@ int hc_adj_gradients(hc_ctx *ctx, double *gq, double out[3]);
  * HBM, then sweeps backwards, recomputing each segment into a tape; obs holds
  * (w_obs, T_obs) per probe per sample (nsteps/every samples), samples
  * (optional) receives the model's values in the same order.  On return the
- * context holds the window's end state; gq and gs as hc_adj_gradients. */
+ * context holds the window's end state (end_state = 1, to continue the run)
+ * or its initial state and step counter (end_state = 0, to evaluate the
+ * window again); gq and gs as hc_adj_gradients. */
 int hc_adj_set_probes(hc_ctx *ctx, int32_t nprobe, const int32_t *ijk, int32_t every, double wgt_w,
                       double wgt_t);
 int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples, double *loss,
-                  double *gq, double gs[3]);
+                  double *gq, double gs[3], int32_t end_state);
 
 /* Per-phase device time accumulated since the last reset (ms), measured with
  * HIP events on the context stream when enabled.  names/ms have room for

@@ -1003,7 +1003,7 @@ static void probe_sample(hc_ctx *ctx, size_t q) {
 
 template <typename T>
 static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double *samples, double *loss, double *gq,
-                    double gs[3]) {
+                    double gs[3], int end_state) {
   AdjCtx *A = ctx->adj;
   const int m0 = ctx->next_step;
   const int nck = (nsteps + seg - 1) / seg;  // checkpoints at n = 0, seg, ...
@@ -1081,9 +1081,11 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
                   (const T *)slot_ptr(ctx, s0, 6), (const T *)slot_ptr(ctx, s0, 7));
     }
   }
-  // continue from the window's end state
-  load_slot(ctx, nck);
-  set_host_state(ctx, ck[nck]);
+  // the window's end state (to continue the run) or its initial state (to
+  // evaluate the window again)
+  const int keep = end_state ? nck : 0;
+  load_slot(ctx, keep);
+  set_host_state(ctx, ck[keep]);
   for (int c = 0; c < 3; ++c) ctx->up_alias[c] = false;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ctx, HC_EDEVICE, std::string("adjoint window: ") + hipGetErrorString(e));
@@ -1091,7 +1093,7 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
 }
 
 extern "C" int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples,
-                             double *loss, double *gq, double gs[3]) {
+                             double *loss, double *gq, double gs[3], int32_t end_state) {
   int r = adj_need(ctx);
   if (r) return r;
   if (ctx->h_tbkg.empty()) return fail(ctx, HC_EINVAL, "profiles not set (hc_init_case or hc_set_profiles)");
@@ -1099,6 +1101,6 @@ extern "C" int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const dou
   if (nsteps < 1 || seg < 1) return fail(ctx, HC_EINVAL, "nsteps and seg must be positive");
   if (ctx->P.ismol != 1) return fail(ctx, HC_EINVAL, "only ismol = 1 (two corrective passes) is supported");
   PhaseScope ps(ctx, "adj_window");
-  if (ctx->prec == HC_FP64) return window_t<double>(ctx, nsteps, seg, obs, samples, loss, gq, gs);
-  return window_t<float>(ctx, nsteps, seg, obs, samples, loss, gq, gs);
+  if (ctx->prec == HC_FP64) return window_t<double>(ctx, nsteps, seg, obs, samples, loss, gq, gs, end_state);
+  return window_t<float>(ctx, nsteps, seg, obs, samples, loss, gq, gs, end_state);
 }

@@ -108,7 +108,7 @@ def lib():
             "hc_adj_density": [vp], "hc_adj_refresh": [vp],
             "hc_adj_gradients": [vp, dp, dp],
             "hc_adj_set_probes": [vp, i32, dp, i32, dbl, dbl],
-            "hc_adj_window": [vp, i32, i32, dp, dp, dp, dp, dp],
+            "hc_adj_window": [vp, i32, i32, dp, dp, dp, dp, dp, i32],
         }
         for n, a in sig.items():
             f = getattr(L, n)
@@ -371,17 +371,20 @@ class Context:
         self._every = int(every)
         self._check(self.L.hc_adj_set_probes(self.h, len(a), _arr(a), int(every), float(wgt_w), float(wgt_t)))
 
-    def adj_window(self, nsteps, seg, obs):
+    def adj_window(self, nsteps, seg, obs, end_state=True):
         """Forward nsteps with a checkpoint every seg steps, probe-misfit loss
         against obs (shape (nsteps // every, nprobe, 2): w, T), reverse sweep.
-        Returns (loss, samples, dL/dqmap, [dL/dflow_scale, dL/dcs_h, dL/dcs_v])."""
+        Returns (loss, samples, dL/dqmap, [dL/dflow_scale, dL/dcs_h, dL/dcs_v]);
+        the context then holds the window's end state, or (end_state=False)
+        its initial state and step counter."""
         ns = nsteps // self._every
         o = np.ascontiguousarray(np.asarray(obs, dtype=np.float64).reshape(ns * self._probes * 2))
         samp = np.zeros_like(o)
         loss = np.zeros(1)
         gq = np.zeros(self.npatch())
         gs = np.zeros(3)
-        self._check(self.L.hc_adj_window(self.h, nsteps, seg, _arr(o), _arr(samp), _arr(loss), _arr(gq), _arr(gs)))
+        self._check(self.L.hc_adj_window(self.h, nsteps, seg, _arr(o), _arr(samp), _arr(loss), _arr(gq), _arr(gs),
+                                         1 if end_state else 0))
         return float(loss[0]), samp.reshape(ns, self._probes, 2), gq, gs
 
     def phase_timing(self, enable=True):
