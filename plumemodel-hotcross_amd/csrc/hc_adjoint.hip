@@ -57,9 +57,11 @@ struct RelA {
   __device__ void operator()(int a, int b, int c, T v) const { unsafeAtomicAdd(p + IX(L, i + a, j + b, k + c), v); }
 };
 
-// sum of one double per thread into *out (one atomic per block); every
-// thread of the block must call it
-__device__ inline void block_add(double v, double *out) {
+// sum of one double per thread into one of the 256 slots at `slots` (one
+// atomic per block, spread over the slots so that blocks do not queue on one
+// address; the slots are summed on the host); every thread of the block must
+// call it
+__device__ inline void block_add(double v, double *slots) {
   const int tid = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
   const int nt = (int)(blockDim.x * blockDim.y * blockDim.z);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -69,7 +71,8 @@ __device__ inline void block_add(double v, double *out) {
   if (tid == 0) {
     double s = 0.0;
     for (int w = 0; w < (nt + 63) / 64; ++w) s += sh[w];
-    if (s != 0.0) atomicAdd(out, s);
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (s != 0.0) atomicAdd(slots + (b & (ADJ_SLOTS - 1)), s);
   }
 }
 
@@ -84,6 +87,76 @@ inline dim3 grid3(const Box3 &b) {
   const int j = (b).j0 + (int)(blockIdx.y * 4 + threadIdx.y);  \
   const int k = (b).k0 + (int)blockIdx.z;                      \
   const bool in = i <= (b).i1 && j <= (b).j1;
+
+// ---------------------------------------------------------------------------
+// k-marching LDS accumulation.  A block owns a TI x TJ column of output
+// cells and marches down the planes ka..kb; the contributions of the
+// outputs of plane k land in three LDS planes (k-1, k, k+1) of every adjoint
+// field the formula feeds, over the tile plus a one-cell halo.  After plane
+// k, plane k-1 receives nothing more from this block: it is added to HBM
+// (one atomic per touched cell and field, instead of one per contribution)
+// and its ring slot cleared for plane k+2.
+template <typename T, int TI, int TJ, int NF>
+struct AdjRing {
+  static constexpr int SI = TI + 2, SJ = TJ + 2, S = SI * SJ, NT = TI * TJ;
+  T a[NF][3][S];
+};
+
+template <typename T, int SI, int S>
+struct LAcc {
+  T *ring;  // [3][S] of one field
+  int ib, jb;
+  __device__ void operator()(int i, int j, int k, T v) const {
+    atomicAdd(&ring[(k % 3) * S + (i - ib) + SI * (j - jb)], v);
+  }
+};
+// the same, relative to a face (MPDATA face formulas)
+template <typename T, int SI, int S>
+struct LRel {
+  T *ring;
+  int ib, jb, i, j, k;
+  __device__ void operator()(int a, int b, int c, T v) const {
+    atomicAdd(&ring[((k + c) % 3) * S + (i + a - ib) + SI * (j + b - jb)], v);
+  }
+};
+
+template <typename T, int TI, int TJ, int NF, class Body>
+__device__ inline void adj_march(AdjRing<T, TI, TJ, NF> &sh, const Layout &L, T *const *dst, int i0, int j0, int ka,
+                                 int kb, Body &&body) {
+  typedef AdjRing<T, TI, TJ, NF> R;
+  constexpr int SI = R::SI, S = R::S, NT = R::NT;
+  const int tid = (int)threadIdx.x;
+  T *flat = &sh.a[0][0][0];
+  for (int e = tid; e < NF * 3 * S; e += NT) flat[e] = (T)0;
+  __syncthreads();
+  const int i = i0 + tid % TI, j = j0 + tid / TI;
+  const bool own = i <= L.nx && j <= L.ny;
+  auto flush = [&](int k) {
+    const int slot = k % 3;
+    for (int e = tid; e < NF * S; e += NT) {
+      const int f = e / S, r = e - f * S;
+      T *cell = &sh.a[f][slot][r];
+      const T v = *cell;
+      const int ii = i0 - 1 + r % SI, jj = j0 - 1 + r / SI;
+      if (v != (T)0 && ii <= L.nx + 1 && jj <= L.ny + 1) unsafeAtomicAdd(dst[f] + IX(L, ii, jj, k), v);
+      *cell = (T)0;
+    }
+  };
+  for (int k = ka; k <= kb; ++k) {
+    if (own) body(i, j, k);
+    __syncthreads();
+    flush(k - 1);
+    __syncthreads();
+  }
+  flush(kb);
+  flush(kb + 1);
+}
+
+template <int TI, int TJ>
+inline dim3 adj_tile_grid(const Layout &L, int kc) {
+  return dim3((unsigned)((L.nx + TI - 1) / TI), (unsigned)((L.ny + TJ - 1) / TJ), (unsigned)((L.nz + kc - 1) / kc));
+}
+constexpr int ATI = 32, ATJ = 8, AKC = 32;
 
 // ---------------------------------------------------------------------------
 // FORCES
@@ -122,6 +195,44 @@ __global__ void __launch_bounds__(256) k_adj_forces(Layout L, Box3 b, StepArgs s
                         aum, avm, awm, ass, assv, arh);
     }
   }
+  block_add(gf, gflow);
+}
+
+
+// FORCES, interior cells i, j, k >= 1 of all three components as a tile
+template <typename T>
+__global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs s, Phys ph, FlowSens fs,
+                                                              FieldPtrs<T> f, const T *__restrict__ ss0,
+                                                              const T *__restrict__ ssv0, FieldPtrs<T> lam,
+                                                              const double *__restrict__ azzmin,
+                                                              const double *__restrict__ alphau,
+                                                              const double *__restrict__ alphav,
+                                                              const double *__restrict__ alphae0,
+                                                              const double *__restrict__ alphaw, double *gflow, int kc) {
+  typedef AdjRing<T, ATI, ATJ, 9> R;
+  __shared__ R sh;
+  const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
+  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
+  T *const dst[9] = {lam.u, lam.v, lam.w, lam.um, lam.vm, lam.wm, lam.ss, lam.ssv, lam.rhop};
+  const RView<T> U{f.u, L}, V{f.v, L}, W{f.w, L}, UM{f.um, L}, VM{f.vm, L}, WM{f.wm, L}, SS{ss0, L}, SV{ssv0, L},
+      RH{f.rhop, L};
+  typedef LAcc<T, R::SI, R::S> A;
+  const A au{sh.a[0][0], i0 - 1, j0 - 1}, av{sh.a[1][0], i0 - 1, j0 - 1}, aw{sh.a[2][0], i0 - 1, j0 - 1},
+      aum{sh.a[3][0], i0 - 1, j0 - 1}, avm{sh.a[4][0], i0 - 1, j0 - 1}, awm{sh.a[5][0], i0 - 1, j0 - 1},
+      ass{sh.a[6][0], i0 - 1, j0 - 1}, assv{sh.a[7][0], i0 - 1, j0 - 1}, arh{sh.a[8][0], i0 - 1, j0 - 1};
+  double gf = 0.0;
+  adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
+    const T lx = F3(lam.fx, i, j, k), ly = F3(lam.fy, i, j, k), lz = F3(lam.fz, i, j, k);
+    if (lx != (T)0)
+      adj_forces_x<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphau, lx, au, av, aw, aum, avm, awm,
+                      ass, assv, gf);
+    if (ly != (T)0)
+      adj_forces_y<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphav, ly, au, av, aw, aum, avm, awm,
+                      ass, assv, gf);
+    if (lz != (T)0)
+      adj_forces_z<T>(L, i, j, k, s, ph, U, V, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, lz, au, av, aw, aum,
+                      avm, awm, ass, assv, arh);
+  });
   block_add(gf, gflow);
 }
 
@@ -191,7 +302,79 @@ __global__ void __launch_bounds__(256) k_adj_visc(Layout L, Box3 b, StepArgs s, 
     }
   }
   block_add(sh, gcs);
-  block_add(sv, gcs + 1);
+  block_add(sv, gcs + ADJ_SLOTS);
+}
+
+
+// VISCOSITY3D: interior cells as a tile, the ghost shell cell by cell
+template <typename T>
+__global__ void __launch_bounds__(ATI *ATJ) k_adj_visc_tile(Layout L, StepArgs s, Phys ph, const T *__restrict__ um,
+                                                            const T *__restrict__ vm, const T *__restrict__ wm,
+                                                            const T *__restrict__ rhop,
+                                                            const double *__restrict__ rho_bkg,
+                                                            const T *__restrict__ ss1, const T *__restrict__ ssv1,
+                                                            const T *__restrict__ lss, const T *__restrict__ lssv,
+                                                            T *lum, T *lvm, T *lwm, T *lrhop, double *gcs, int kc) {
+  typedef AdjRing<T, ATI, ATJ, 4> R;
+  __shared__ R sh;
+  const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
+  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
+  T *const dst[4] = {lum, lvm, lwm, lrhop};
+  const RView<T> UM{um, L}, VM{vm, L}, WM{wm, L}, RH{rhop, L};
+  typedef LAcc<T, R::SI, R::S> A;
+  const A aum{sh.a[0][0], i0 - 1, j0 - 1}, avm{sh.a[1][0], i0 - 1, j0 - 1}, awm{sh.a[2][0], i0 - 1, j0 - 1},
+      arh{sh.a[3][0], i0 - 1, j0 - 1};
+  double sh_ = 0.0, sv_ = 0.0;
+  adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
+    const T a = F3(lss, i, j, k), c = F3(lssv, i, j, k);
+    adj_ss_val<T>(L, ph, s, UM, VM, WM, i, j, k, a, aum, avm, awm);
+    adj_ssv_val<T>(L, ph, s, UM, VM, WM, RH, rho_bkg, i, j, k, c, aum, avm, awm, arh);
+    sh_ += (double)a * (double)F3(ss1, i, j, k);
+    sv_ += (double)c * (double)F3(ssv1, i, j, k);
+  });
+  block_add(sh_, gcs);
+  block_add(sv_, gcs + ADJ_SLOTS);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_visc_shell(Layout L, StepArgs s, Phys ph, const T *__restrict__ um,
+                                                        const T *__restrict__ vm, const T *__restrict__ wm,
+                                                        const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
+                                                        const T *__restrict__ ss1, const T *__restrict__ ssv1,
+                                                        const T *__restrict__ lss, const T *__restrict__ lssv,
+                                                        T *__restrict__ lum, T *__restrict__ lvm, T *__restrict__ lwm,
+                                                        T *__restrict__ lrhop, double *gcs) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long na = 2L * (ny + 2) * (nz + 2), nb = 2L * nx * (nz + 2), nc = 2L * nx * ny;
+  int i = 0, j = 0, k = 0;
+  bool in = true;
+  if (t < na) {
+    j = (int)(t % (ny + 2)); long r = t / (ny + 2);
+    k = (int)(r % (nz + 2)); i = (r / (nz + 2)) ? nx + 1 : 0;
+  } else if ((t -= na) < nb) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    k = (int)(r % (nz + 2)); j = (r / (nz + 2)) ? ny + 1 : 0;
+  } else if ((t -= nb) < nc) {
+    i = 1 + (int)(t % nx); long r = t / nx;
+    j = 1 + (int)(r % ny); k = (r / ny) ? nz + 1 : 0;
+  } else {
+    in = false;
+  }
+  double sh = 0.0, sv = 0.0;
+  if (in) {
+    const T a = visc_ss_written(L, i, j, k) ? F3(lss, i, j, k) : (T)0;
+    const T c = visc_ssv_written(L, i, j, k) ? F3(lssv, i, j, k) : (T)0;
+    if (a != (T)0 || c != (T)0) {
+      const RView<T> UM{um, L}, VM{vm, L}, WM{wm, L}, RH{rhop, L};
+      const AAcc<T> aum{lum, L}, avm{lvm, L}, awm{lwm, L}, arh{lrhop, L};
+      adj_visc_at<T>(L, s, ph, UM, VM, WM, RH, rho_bkg, a, c, i, j, k, aum, avm, awm, arh);
+      sh = (double)a * (double)F3(ss1, i, j, k);
+      sv = (double)c * (double)F3(ssv1, i, j, k);
+    }
+  }
+  block_add(sh, gcs);
+  block_add(sv, gcs + ADJ_SLOTS);
 }
 
 // ---------------------------------------------------------------------------
@@ -407,6 +590,200 @@ __global__ void __launch_bounds__(256) k_adj_pass1(Layout L, Box3 b, StepArgs s,
   adj_pass1<T>(L, s, ph, C, UC, VC, WC, SS, SV, azzmin, cbkg, i, j, k, l, ac, auc, avc, awc, ass, assv);
 }
 
+
+// the limited donor step reversed, as a tile (interior cells)
+template <typename T>
+__global__ void __launch_bounds__(ATI *ATJ) k_adj_donor_tile(Layout L, DonorLast dl, const T *__restrict__ lout,
+                                                             const T *__restrict__ lpre, const T *__restrict__ cc,
+                                                             const T *__restrict__ ud, const T *__restrict__ vd,
+                                                             const T *__restrict__ wd, const uint8_t *__restrict__ fl,
+                                                             T *lcc, T *lud, T *lvd, T *lwd, T *lentry, int kc) {
+  typedef AdjRing<T, ATI, ATJ, 4> R;
+  __shared__ R sh;
+  const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
+  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
+  T *const dst[4] = {lcc, lud, lvd, lwd};
+  typedef LAcc<T, R::SI, R::S> A;
+  const A ac{sh.a[0][0], i0 - 1, j0 - 1}, au{sh.a[1][0], i0 - 1, j0 - 1}, av{sh.a[2][0], i0 - 1, j0 - 1},
+      aw{sh.a[3][0], i0 - 1, j0 - 1};
+  adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
+    const T lo = F3(lout, i, j, k);
+    const T ln = lo + (lpre ? F3(lpre, i, j, k) : (T)0);
+    if (dl.last && lo != (T)0) {
+      const double ae = fmax(dl.alphae0[i + (L.nx + 2) * j], (dl.alphaw[k + 1] + dl.alphaw[k]) / 2.0);
+      F3(lentry, i, j, k) += -(T)(0.5 * ae * dl.tau3) * lo;
+      if (k == L.nz && dl.map_valid && i >= dl.nxs && i <= dl.nxe && j >= dl.nys && j <= dl.nye) {
+        const int n = (i - dl.nxs) + (dl.nxe - dl.nxs + 1) * (j - dl.nys);
+        dl.gq[n] += (double)lo * dl.tau3 / dl.dxdydz * dl.qbase[n];
+      }
+    }
+    if (ln == (T)0) return;
+    T lc[7], lv[6];
+    adj_mp_donor<T>(F3(cc, i, j, k), F3(cc, i - 1, j, k), F3(cc, i + 1, j, k), F3(cc, i, j - 1, k),
+                    F3(cc, i, j + 1, k), F3(cc, i, j, k - 1), F3(cc, i, j, k + 1), F3(ud, i - 1, j, k), F3(ud, i, j, k),
+                    F3(vd, i, j - 1, k), F3(vd, i, j, k), F3(wd, i, j, k - 1), F3(wd, i, j, k), true,
+                    F3(fl, i, j, k), F3(fl, i - 1, j, k), F3(fl, i + 1, j, k), F3(fl, i, j - 1, k),
+                    F3(fl, i, j + 1, k), F3(fl, i, j, k - 1), F3(fl, i, j, k + 1), ln, lc, lv);
+    ac(i, j, k, ln + lc[0]);
+    ac(i - 1, j, k, lc[1]); ac(i + 1, j, k, lc[2]); ac(i, j - 1, k, lc[3]); ac(i, j + 1, k, lc[4]);
+    ac(i, j, k - 1, lc[5]); ac(i, j, k + 1, lc[6]);
+    au(i - 1, j, k, lv[0]); au(i, j, k, lv[1]); av(i, j - 1, k, lv[2]); av(i, j, k, lv[3]);
+    aw(i, j, k - 1, lv[4]); aw(i, j, k, lv[5]);
+  });
+}
+
+
+// The limited donor-cell step of a pass reversed as a gather over the box
+// 0..n+1 (no atomics).  In flux form the step is cn = cc + sum of signed
+// face fluxes: an x (or y) face between L = i and R = i+1 carries
+// f = p cc(L) + m cc(R) with p = (u+|u|)/2, m = (u-|u|)/2 times the
+// limiter's masks, taken from L and given to R; a z face carries
+// g = wm cc(L) + wp cc(R), given to L and taken from R (mp_donor).  So each
+// cell collects lam_f * (d flux / d cc) from its six faces, with
+// lam_f = lam_n(R) - lam_n(L) (x, y) or lam_n(L) - lam_n(R) (z), and each
+// face's velocity adjoint is lam_f * d flux / d u, written by the face's
+// L cell.  lam_n is the adjoint of the pass's result, zero at ghost cells.
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_donor_gather(Layout L, Box3 b, DonorLast dl, const T *__restrict__ lout,
+                                                          const T *__restrict__ lpre, const T *__restrict__ cc,
+                                                          const T *__restrict__ ud, const T *__restrict__ vd,
+                                                          const T *__restrict__ wd, const uint8_t *__restrict__ fl,
+                                                          T *__restrict__ lcc, T *__restrict__ lud,
+                                                          T *__restrict__ lvd, T *__restrict__ lwd,
+                                                          T *__restrict__ lentry) {
+  ADJ_IJK(b);
+  if (!in) return;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  auto inner = [&](int a, int c, int d) { return a >= 1 && a <= nx && c >= 1 && c <= ny && d >= 1 && d <= nz; };
+  auto lnv = [&](int a, int c, int d) -> T {
+    if (!inner(a, c, d)) return (T)0;
+    return F3(lout, a, c, d) + (lpre ? F3(lpre, a, c, d) : (T)0);
+  };
+  // split velocity and its derivative with a mask
+  auto pos = [](T u, bool on, T &v, T &dv) {
+    v = on ? (T)0.5 * (u + mp_fabs(u)) : (T)0;
+    dv = on ? (T)0.5 * ((T)1 + adj_sign(u)) : (T)0;
+  };
+  auto neg = [](T u, bool on, T &v, T &dv) {
+    v = on ? (T)0.5 * (u - mp_fabs(u)) : (T)0;
+    dv = on ? (T)0.5 * ((T)1 - adj_sign(u)) : (T)0;
+  };
+  // masks: A = !(fL == 1 || fR == 2), B = !(fR == 1 || fL == 2)
+  auto mA = [](int fL, int fR) { return !(fL == 1 || fR == 2); };
+  auto mB = [](int fL, int fR) { return !(fR == 1 || fL == 2); };
+  const bool cin = inner(i, j, k);
+  const T lc = lnv(i, j, k);
+  T acc = lc;
+  if (dl.last && cin) {
+    const T lo = F3(lout, i, j, k);
+    const double ae = fmax(dl.alphae0[i + (nx + 2) * j], (dl.alphaw[k + 1] + dl.alphaw[k]) / 2.0);
+    F3(lentry, i, j, k) += -(T)(0.5 * ae * dl.tau3) * lo;
+    if (k == nz && dl.map_valid && i >= dl.nxs && i <= dl.nxe && j >= dl.nys && j <= dl.nye) {
+      const int n = (i - dl.nxs) + (dl.nxe - dl.nxs + 1) * (j - dl.nys);
+      dl.gq[n] += (double)lo * dl.tau3 / dl.dxdydz * dl.qbase[n];
+    }
+  }
+  const T c0 = F3(cc, i, j, k);
+  const int f0 = F3(fl, i, j, k);
+  T v, dv, v2, dv2;
+  if (j >= 1 && j <= ny && k >= 1 && k <= nz) {
+    if (i <= nx) {  // x face i: L = (i,j,k), R = (i+1,j,k)
+      const T u = F3(ud, i, j, k);
+      const int fR = F3(fl, i + 1, j, k);
+      pos(u, mA(f0, fR), v, dv);
+      neg(u, mB(f0, fR), v2, dv2);
+      const T lf = lnv(i + 1, j, k) - lc;
+      acc += lf * v;
+      F3(lud, i, j, k) += lf * (dv * c0 + dv2 * F3(cc, i + 1, j, k));
+    }
+    if (i >= 1) {  // x face i-1: L = (i-1,j,k), R = (i,j,k)
+      const T u = F3(ud, i - 1, j, k);
+      neg(u, mB(F3(fl, i - 1, j, k), f0), v2, dv2);
+      acc += (lc - lnv(i - 1, j, k)) * v2;
+    }
+  }
+  if (i >= 1 && i <= nx && k >= 1 && k <= nz) {
+    if (j <= ny) {  // y face j
+      const T u = F3(vd, i, j, k);
+      const int fR = F3(fl, i, j + 1, k);
+      pos(u, mA(f0, fR), v, dv);
+      neg(u, mB(f0, fR), v2, dv2);
+      const T lf = lnv(i, j + 1, k) - lc;
+      acc += lf * v;
+      F3(lvd, i, j, k) += lf * (dv * c0 + dv2 * F3(cc, i, j + 1, k));
+    }
+    if (j >= 1) {  // y face j-1
+      const T u = F3(vd, i, j - 1, k);
+      neg(u, mB(F3(fl, i, j - 1, k), f0), v2, dv2);
+      acc += (lc - lnv(i, j - 1, k)) * v2;
+    }
+  }
+  if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+    if (k <= nz) {  // z face k: g = wm cc(L) + wp cc(R), L = k, R = k+1
+      const T w = F3(wd, i, j, k);
+      const int fR = F3(fl, i, j, k + 1);
+      neg(w, mA(f0, fR), v2, dv2);  // wm
+      pos(w, mB(f0, fR), v, dv);    // wp
+      const T lg = lc - lnv(i, j, k + 1);
+      acc += lg * v2;
+      F3(lwd, i, j, k) += lg * (dv2 * c0 + dv * F3(cc, i, j, k + 1));
+    }
+    if (k >= 1) {  // z face k-1: L = k-1, R = k
+      const T w = F3(wd, i, j, k - 1);
+      pos(w, mB(F3(fl, i, j, k - 1), f0), v, dv);
+      acc += (lnv(i, j, k - 1) - lc) * v;
+    }
+  }
+  F3(lcc, i, j, k) += acc;
+}
+
+// the antidiffusive velocities reversed on the faces of the interior cells
+// (ud, vd, wd at (i, j, k) with i, j, k >= 1) as a tile
+template <typename T>
+__global__ void __launch_bounds__(ATI *ATJ) k_adj_anti_tile(Layout L, const T *__restrict__ cc,
+                                                            const T *__restrict__ U, const T *__restrict__ V,
+                                                            const T *__restrict__ W, const T *__restrict__ lud,
+                                                            const T *__restrict__ lvd, const T *__restrict__ lwd,
+                                                            T *lcc, T *lU, T *lV, T *lW, int kc) {
+  typedef AdjRing<T, ATI, ATJ, 4> R;
+  __shared__ R sh;
+  const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
+  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
+  T *const dst[4] = {lcc, lU, lV, lW};
+  typedef LRel<T, R::SI, R::S> A;
+  adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
+    const RelR<T> C{cc, L, i, j, k}, Uf{U, L, i, j, k}, Vf{V, L, i, j, k}, Wf{W, L, i, j, k};
+    const A ac{sh.a[0][0], i0 - 1, j0 - 1, i, j, k}, aU{sh.a[1][0], i0 - 1, j0 - 1, i, j, k},
+        aV{sh.a[2][0], i0 - 1, j0 - 1, i, j, k}, aW{sh.a[3][0], i0 - 1, j0 - 1, i, j, k};
+    adj_anti_u<T>(C, Uf, Vf, Wf, F3(lud, i, j, k), ac, aU, aV, aW);
+    adj_anti_v<T>(C, Uf, Vf, Wf, F3(lvd, i, j, k), ac, aU, aV, aW);
+    adj_anti_w<T>(C, Uf, Vf, Wf, F3(lwd, i, j, k), ac, aU, aV, aW);
+  });
+}
+
+// pass 1 reversed, as a tile
+template <typename T>
+__global__ void __launch_bounds__(ATI *ATJ) k_adj_pass1_tile(Layout L, StepArgs s, Phys ph, FieldPtrs<T> f,
+                                                             const T *__restrict__ c, const T *__restrict__ uc,
+                                                             const T *__restrict__ vc, const T *__restrict__ wc,
+                                                             const double *__restrict__ azzmin,
+                                                             const double *__restrict__ cbkg, const T *__restrict__ lc1,
+                                                             T *lentry, T *luc, T *lvc, T *lwc, T *lss, T *lssv, int kc) {
+  typedef AdjRing<T, ATI, ATJ, 6> R;
+  __shared__ R sh;
+  const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
+  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
+  T *const dst[6] = {lentry, luc, lvc, lwc, lss, lssv};
+  const RView<T> C{c, L}, UC{uc, L}, VC{vc, L}, WC{wc, L}, SS{f.ss, L}, SV{f.ssv, L};
+  typedef LAcc<T, R::SI, R::S> A;
+  const A ac{sh.a[0][0], i0 - 1, j0 - 1}, auc{sh.a[1][0], i0 - 1, j0 - 1}, avc{sh.a[2][0], i0 - 1, j0 - 1},
+      awc{sh.a[3][0], i0 - 1, j0 - 1}, ass{sh.a[4][0], i0 - 1, j0 - 1}, assv{sh.a[5][0], i0 - 1, j0 - 1};
+  adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
+    const T l = F3(lc1, i, j, k);
+    if (l != (T)0) adj_pass1<T>(L, s, ph, C, UC, VC, WC, SS, SV, azzmin, cbkg, i, j, k, l, ac, auc, avc, awc, ass, assv);
+  });
+}
+
 // ---------------------------------------------------------------------------
 // probes
 __global__ void k_probe_sample(int n, const int *__restrict__ iw, const int *__restrict__ it, const double *w,
@@ -457,7 +834,9 @@ struct AdjCtx {
   void *lc1 = nullptr, *lc2 = nullptr, *lpre = nullptr, *lud2[3] = {}, *lud3[3] = {};
   uint8_t *fl2 = nullptr, *fl3 = nullptr;
   void *g = nullptr;          // compact pressure work
-  double *d_sc = nullptr;     // [0] d/d flow_scale, [1] sum lam_ss*ss, [2] sum lam_ssv*ssv, [3] 0.0
+  // ADJ_SLOTS partial sums each of d/d flow_scale, sum lam_ss*ss and
+  // sum lam_ssv*ssv, then a device 0.0
+  double *d_sc = nullptr;
   double *d_part = nullptr;   // nz+2 partial sums
   double *d_gq = nullptr;     // npatch
   // window
@@ -503,7 +882,7 @@ static int adj_alloc(hc_ctx *ctx) {
   if (!(A->fl2 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
   if (!(A->fl3 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
   if (!(A->g = dalloc(ctx, (size_t)ctx->P.nx * ctx->P.ny * ctx->P.nz * ctx->esz, &rc))) return rc;
-  if (!(A->d_sc = (double *)dalloc(ctx, 64, &rc))) return rc;
+  if (!(A->d_sc = (double *)dalloc(ctx, (3 * ADJ_SLOTS + 1) * 8, &rc))) return rc;
   if (!(A->d_part = (double *)dalloc(ctx, (size_t)(ctx->P.nz + 2) * 8, &rc))) return rc;
   if (!(A->d_gq = (double *)dalloc(ctx, (size_t)ctx->npatch * 8, &rc))) return rc;
   return HC_OK;
@@ -540,11 +919,21 @@ template <typename T>
 static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
   AdjCtx *A = ctx->adj;
   const Layout &L = ctx->L;
-  Box3 b{0, L.nx, 0, L.ny, 0, L.nz};
   FieldPtrs<T> lam = lptrs<T>(ctx);
-  hipLaunchKernelGGL(k_adj_forces<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, step_args(ctx), ctx->P, flow_sens(ctx),
-                     ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
-                     ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc);
+  const StepArgs s = step_args(ctx);
+  const FlowSens fs = flow_sens(ctx);
+  // the faces i = 0 (fx), j = 0 (fy), k = 0 (fz) cell by cell, the interior
+  // as LDS-accumulating tiles
+  const Box3 faces[3] = {{0, 0, 0, L.ny, 0, L.nz}, {1, L.nx, 0, 0, 0, L.nz}, {1, L.nx, 1, L.ny, 0, 0}};
+  for (const Box3 &b : faces)
+    hipLaunchKernelGGL(k_adj_forces<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, s, ctx->P, fs, ptrs<T>(ctx), ss0,
+                       ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav, ctx->aux.alphae0,
+                       ctx->aux.alphaw, A->d_sc);
+  const int kc = std::min(L.nz, AKC);
+  const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
+  hipLaunchKernelGGL(k_adj_forces_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
+                     fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
+                     ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc, kc);
   zero(ctx, lam.fx); zero(ctx, lam.fy); zero(ctx, lam.fz);
 }
 
@@ -574,18 +963,26 @@ template <typename T>
 static void adj_pressure_t(hc_ctx *ctx) {
   AdjCtx *A = ctx->adj;
   launch_pressure_adj<T>(ctx->st, ctx->L, step_args(ctx), ctx->P, ptrs<T>(ctx), lptrs<T>(ctx), ctx->aux, ctx->pw,
-                         flow_sens(ctx), A->d_sc, (T *)A->g, A->d_sc + 3, A->d_part);
+                         flow_sens(ctx), A->d_sc, (T *)A->g, A->d_sc + 3 * ADJ_SLOTS, A->d_part);
 }
 
 template <typename T>
 static void adj_viscosity_t(hc_ctx *ctx, const T *ss1, const T *ssv1) {
   AdjCtx *A = ctx->adj;
   const Layout &L = ctx->L;
-  Box3 b{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
   FieldPtrs<T> f = ptrs<T>(ctx), lam = lptrs<T>(ctx);
-  hipLaunchKernelGGL(k_adj_visc<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, step_args(ctx), ctx->P, (const T *)f.um,
-                     (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop, (const double *)ctx->aux.rho_bkg, ss1, ssv1,
-                     (const T *)lam.ss, (const T *)lam.ssv, lam.um, lam.vm, lam.wm, lam.rhop, A->d_sc + 1);
+  const StepArgs s = step_args(ctx);
+  const long nshell = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
+  hipLaunchKernelGGL(k_adj_visc_shell<T>, dim3((unsigned)((nshell + 255) / 256)), dim3(256), 0, ctx->st, L, s, ctx->P,
+                     (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
+                     (const double *)ctx->aux.rho_bkg, ss1, ssv1, (const T *)lam.ss, (const T *)lam.ssv, lam.um,
+                     lam.vm, lam.wm, lam.rhop, A->d_sc + ADJ_SLOTS);
+  const int kc = std::min(L.nz, AKC);
+  const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
+  hipLaunchKernelGGL(k_adj_visc_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
+                     (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
+                     (const double *)ctx->aux.rho_bkg, ss1, ssv1, (const T *)lam.ss, (const T *)lam.ssv, lam.um,
+                     lam.vm, lam.wm, lam.rhop, A->d_sc + ADJ_SLOTS, kc);
   zero(ctx, lam.ss); zero(ctx, lam.ssv);
 }
 
@@ -639,14 +1036,13 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   tw.uc = ctx->tw.uc; tw.vc = ctx->tw.vc; tw.wc = ctx->tw.wc;
   tw.ud2 = A->ud2[0]; tw.vd2 = A->ud2[1]; tw.wd2 = A->ud2[2];
   tw.c1 = A->c1; tw.c2 = A->c2; tw.cpre = A->cpre;
+  tw.ud3 = A->ud3[0]; tw.vd3 = A->ud3[1]; tw.wd3 = A->ud3[2];
   tw.budget_part = ctx->tw.budget_part; tw.budget = ctx->tw.budget;
   TracerArgs ta{cmin, cmax, tau3, 0, 0.0, 0};
   launch_tracer<T>(st, L, s, ctx->P, va, f, ctx->aux, tw, (T *)A->cw, d_cbkg, qpatch, ta);
-  const Box3 fbx{0, L.nx, 0, L.ny, 0, L.nz}, ib = interior(L);
+  const Box3 ib = interior(L);
   const T *ud2 = (const T *)A->ud2[0], *vd2 = (const T *)A->ud2[1], *wd2 = (const T *)A->ud2[2];
   T *ud3 = (T *)A->ud3[0], *vd3 = (T *)A->ud3[1], *wd3 = (T *)A->ud3[2];
-  hipLaunchKernelGGL(k_adj_anti_fwd<T>, grid3(fbx), dim3(64, 4), 0, st, L, fbx, (const T *)A->c2, ud2, vd2, wd2, ud3,
-                     vd3, wd3);
   hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, (const T *)A->c1, ud2, vd2, wd2, cmin,
                      cmax, A->fl2);
   hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, (const T *)A->c2, (const T *)ud3,
@@ -666,28 +1062,39 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   dl.gq = (field == HC_T) ? A->d_gq : nullptr;
   dl.map_valid = (field == HC_T && ctx->d_qbase) ? va.map_valid : 0;
   dl.nxs = va.nxs; dl.nxe = va.nxe; dl.nys = va.nys; dl.nye = va.nye;
-  hipLaunchKernelGGL(k_adj_donor<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, dl, (const T *)lout, (const T *)A->lpre,
-                     (const T *)A->c2, (const T *)ud3, (const T *)vd3, (const T *)wd3, (const uint8_t *)A->fl3,
-                     (T *)A->lc2, (T *)A->lud3[0], (T *)A->lud3[1], (T *)A->lud3[2], (T *)A->lcnew);
-  hipLaunchKernelGGL(k_adj_anti<T>, grid3(fbx), dim3(64, 4), 0, st, L, fbx, (const T *)A->c2, ud2, vd2, wd2,
-                     (const T *)A->lud3[0], (const T *)A->lud3[1], (const T *)A->lud3[2], (T *)A->lc2,
-                     (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2]);
+  const int kc = std::min(L.nz, AKC);
+  const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc), tb(ATI * ATJ);
+  // the antidiffusive velocities' adjoint: boundary faces cell by cell,
+  // the interior faces as tiles
+  const Box3 afaces[3] = {{0, 0, 1, L.ny, 1, L.nz}, {1, L.nx, 0, 0, 1, L.nz}, {1, L.nx, 1, L.ny, 0, 0}};
+  auto anti = [&](const T *cc, const T *U, const T *V, const T *W, const T *lu, const T *lv, const T *lw, T *lcc,
+                  T *lU, T *lV, T *lW) {
+    for (const Box3 &b : afaces)
+      hipLaunchKernelGGL(k_adj_anti<T>, grid3(b), dim3(64, 4), 0, st, L, b, cc, U, V, W, lu, lv, lw, lcc, lU, lV, lW);
+    hipLaunchKernelGGL(k_adj_anti_tile<T>, tg, tb, 0, st, L, cc, U, V, W, lu, lv, lw, lcc, lU, lV, lW, kc);
+  };
+  const Box3 full{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
+  hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, dl, (const T *)lout,
+                     (const T *)A->lpre, (const T *)A->c2, (const T *)ud3, (const T *)vd3, (const T *)wd3,
+                     (const uint8_t *)A->fl3, (T *)A->lc2, (T *)A->lud3[0], (T *)A->lud3[1], (T *)A->lud3[2],
+                     (T *)A->lcnew);
+  anti((const T *)A->c2, ud2, vd2, wd2, (const T *)A->lud3[0], (const T *)A->lud3[1], (const T *)A->lud3[2],
+       (T *)A->lc2, (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2]);
   hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc2, (T *)A->lc2);
   // pass 2
   DonorLast d2{};
-  hipLaunchKernelGGL(k_adj_donor<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, d2, (const T *)A->lc2, (const T *)nullptr,
-                     (const T *)A->c1, ud2, vd2, wd2, (const uint8_t *)A->fl2, (T *)A->lc1, (T *)A->lud2[0],
-                     (T *)A->lud2[1], (T *)A->lud2[2], (T *)nullptr);
-  hipLaunchKernelGGL(k_adj_anti<T>, grid3(fbx), dim3(64, 4), 0, st, L, fbx, (const T *)A->c1, (const T *)ctx->tw.uc,
-                     (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)A->lud2[0], (const T *)A->lud2[1],
-                     (const T *)A->lud2[2], (T *)A->lc1, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2]);
+  hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, d2, (const T *)A->lc2,
+                     (const T *)nullptr, (const T *)A->c1, ud2, vd2, wd2, (const uint8_t *)A->fl2, (T *)A->lc1,
+                     (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2], (T *)nullptr);
+  anti((const T *)A->c1, (const T *)ctx->tw.uc, (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)A->lud2[0],
+       (const T *)A->lud2[1], (const T *)A->lud2[2], (T *)A->lc1, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2]);
   hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc1, (T *)A->lc1);
   // pass 1
   FieldPtrs<T> lam = lptrs<T>(ctx);
-  hipLaunchKernelGGL(k_adj_pass1<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, s, ctx->P, f, c, (const T *)ctx->tw.uc,
+  hipLaunchKernelGGL(k_adj_pass1_tile<T>, tg, tb, 0, st, L, s, ctx->P, f, c, (const T *)ctx->tw.uc,
                      (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const double *)ctx->aux.azzmin, d_cbkg,
                      (const T *)A->lc1, (T *)A->lcnew, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2], lam.ss,
-                     lam.ssv);
+                     lam.ssv, kc);
   // the entry value's adjoint becomes the field's
   void *tmp = A->lam[field];
   A->lam[field] = A->lcnew;
@@ -722,7 +1129,7 @@ extern "C" int hc_adj_reset(hc_ctx *ctx) {
   AdjCtx *A = ctx->adj;
   for (void *p : A->lam) zero(ctx, p);
   for (void *p : A->luc) zero(ctx, p);
-  HIPCHK(hipMemsetAsync(A->d_sc, 0, 64, ctx->st));
+  HIPCHK(hipMemsetAsync(A->d_sc, 0, (3 * ADJ_SLOTS + 1) * 8, ctx->st));
   HIPCHK(hipMemsetAsync(A->d_gq, 0, (size_t)ctx->npatch * 8, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return HC_OK;
@@ -845,13 +1252,16 @@ extern "C" int hc_adj_gradients(hc_ctx *ctx, double *gq, double out[3]) {
   int r = adj_need(ctx);
   if (r) return r;
   AdjCtx *A = ctx->adj;
-  double sc[4];
-  HIPCHK(hipMemcpyAsync(sc, A->d_sc, 4 * 8, hipMemcpyDeviceToHost, ctx->st));
+  std::vector<double> sc(3 * ADJ_SLOTS);
+  HIPCHK(hipMemcpyAsync(sc.data(), A->d_sc, 3 * ADJ_SLOTS * 8, hipMemcpyDeviceToHost, ctx->st));
   if (gq) HIPCHK(hipMemcpyAsync(gq, A->d_gq, (size_t)ctx->npatch * 8, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
-  out[0] = sc[0];
-  out[1] = ctx->P.cs_h != 0.0 ? 2.0 * sc[1] / ctx->P.cs_h : 0.0;
-  out[2] = ctx->P.cs_v != 0.0 ? 2.0 * sc[2] / ctx->P.cs_v : 0.0;
+  double sum[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < 3; ++q)
+    for (int e = 0; e < ADJ_SLOTS; ++e) sum[q] += sc[q * ADJ_SLOTS + e];
+  out[0] = sum[0];
+  out[1] = ctx->P.cs_h != 0.0 ? 2.0 * sum[1] / ctx->P.cs_h : 0.0;
+  out[2] = ctx->P.cs_v != 0.0 ? 2.0 * sum[2] / ctx->P.cs_v : 0.0;
   return HC_OK;
 }
 
@@ -1007,7 +1417,7 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   AdjCtx *A = ctx->adj;
   const int m0 = ctx->next_step;
   const int nck = (nsteps + seg - 1) / seg;  // checkpoints at n = 0, seg, ...
-  const size_t slots = (size_t)nck + 1 + (size_t)seg + 1;
+  const size_t slots = (size_t)nck + 1 + (size_t)seg;
   const size_t slot_bytes = (size_t)NTAPE * ctx->L.n * ctx->esz;
   if (A->tape_slots < slots) {
     if (A->tape) { hipStreamSynchronize(ctx->st); hipFree(A->tape); A->tape = nullptr; A->tape_slots = 0; }
@@ -1056,15 +1466,19 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   if (r) return r;
   for (int sg = nck - 1; sg >= 0; --sg) {
     const int a = sg * seg, b = std::min(nsteps, a + seg);
+    // the segment's states: X_a is checkpoint sg, X_b checkpoint sg + 1,
+    // X_a+1 .. X_b-1 recomputed into the tape
+    auto slot_of = [&](int n) -> size_t {
+      return n == a ? (size_t)sg : (n == b ? (size_t)sg + 1 : tape0 + (size_t)(n - a));
+    };
     load_slot(ctx, sg);
     set_host_state(ctx, ck[sg]);
-    for (int n = a; n < b; ++n) {
-      save_slot(ctx, tape0 + (n - a));
-      tp[n - a] = host_state(ctx);
+    tp[0] = ck[sg];
+    for (int n = a; n < b - 1; ++n) {
       fwd_step(ctx, m0 + n);
+      save_slot(ctx, tape0 + (n + 1 - a));
+      tp[n + 1 - a] = host_state(ctx);
     }
-    save_slot(ctx, tape0 + (b - a));
-    tp[b - a] = host_state(ctx);
     for (int n = b - 1; n >= a; --n) {
       FieldPtrs<T> lam = lptrs<T>(ctx);
       if ((n + 1) % A->every == 0) {
@@ -1073,10 +1487,10 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
                            A->d_it, A->d_samp + 2 * A->nprobe * q, A->d_obs + 2 * A->nprobe * q, A->wgt_w, A->wgt_t,
                            lam.w, lam.t);
       }
-      load_slot(ctx, tape0 + (n - a));
+      if (n != b - 1 || n == a) load_slot(ctx, slot_of(n));  // X_b-1 is live after the recompute
       set_host_state(ctx, tp[n - a]);
       step_head(ctx, m0 + n);
-      const size_t s1 = tape0 + (n - a) + 1, s0 = tape0 + (n - a);
+      const size_t s1 = slot_of(n + 1), s0 = slot_of(n);
       adj_step<T>(ctx, m0 + n, (const T *)slot_ptr(ctx, s1, 9), (const T *)slot_ptr(ctx, s1, 10),
                   (const T *)slot_ptr(ctx, s0, 6), (const T *)slot_ptr(ctx, s0, 7));
     }

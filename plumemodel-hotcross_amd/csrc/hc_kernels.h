@@ -32,6 +32,9 @@ struct Aux {
   double *alphaw;   // (-1:nz+1)
 };
 
+// partial-sum slots per scalar sensitivity of the adjoint (a power of two)
+constexpr int ADJ_SLOTS = 256;
+
 // d(ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby)/d(flow_scale) at one step
 struct FlowSens {
   double dubkg_m, dubkg, dvbkg_m, dvbkg, dfbx, dfby;
@@ -66,6 +69,7 @@ struct TracerWork {
   void *ud2, *vd2, *wd2;   // antidiffusive velocities of pass 2 (pass 3's Courant numbers)
   void *c1, *c2;           // pass results (c1: pass 1, then the pre-sponge boundary values)
   void *cpre;              // if set, receives the pre-sponge boundary values instead of c1
+  void *ud3, *vd3, *wd3;   // if set, receive pass 3's antidiffusive velocities
   double *budget_part;     // per-block partial sums (FP64)
   double *budget;          // 3 sums
 };
@@ -106,7 +110,8 @@ void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const P
 void poisson_setup(const Layout &L, const Phys &ph, PoissonWork &pw, hipStream_t st);
 // adjoint of PRESSURE: consumes lam.p (the adjoint of the pressure field,
 // ghosts included) and adds to lam.um, vm, wm, w, fx, fy, fz; gflow (one
-// FP64 device value) collects d/d(flow_scale) through the boundary data.
+// FP64 device slots, ADJ_SLOTS of them) collects d/d(flow_scale) through the
+// boundary data.
 // g is a compact (nx,ny,nz) work array; zero points to a device 0.0.
 template <typename T>
 void launch_pressure_adj(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,

@@ -878,13 +878,16 @@ __global__ void __launch_bounds__(256) k_padj_ghost(Layout L, StepArgs s, Phys p
     g[((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1)] = v;
   }
   __shared__ double sh[256];
-  sh[threadIdx.x] = gf;
+  const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+  sh[tid] = gf;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    if (tid < off) sh[tid] += sh[tid + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && sh[0] != 0.0) atomicAdd(gflow, sh[0]);
+  // one of ADJ_SLOTS partial sums (blocks spread over them)
+  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (tid == 0 && sh[0] != 0.0) atomicAdd(gflow + (bid & (ADJ_SLOTS - 1)), sh[0]);
 }
 
 // transposed Thomas recurrences of k_tridiag, per (kx,ky) column
@@ -959,10 +962,11 @@ __global__ void __launch_bounds__(256) k_padj_rhs(Layout L, Box b, StepArgs s, F
                                                   T *__restrict__ lw, T *__restrict__ lfx, T *__restrict__ lfy,
                                                   T *__restrict__ lfz, double *gflow) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)blockIdx.y, k = (int)blockIdx.z;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y),
+            k = (int)blockIdx.z;
   (void)b;
   double gf = 0.0;
-  if (i <= nx) {
+  if (i <= nx && j <= ny) {
     const T corr = (T)(dsum[0] / ((double)ny * nz));
     auto LR = [&](int ii, int jj, int kk) -> T {
       return (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny && kk >= 1 && kk <= nz) ? F3(pad, ii, jj, kk) - corr : (T)0;
@@ -1009,13 +1013,16 @@ __global__ void __launch_bounds__(256) k_padj_rhs(Layout L, Box b, StepArgs s, F
     }
   }
   __shared__ double sh[256];
-  sh[threadIdx.x] = gf;
+  const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+  sh[tid] = gf;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    if (tid < off) sh[tid] += sh[tid + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && sh[0] != 0.0) atomicAdd(gflow, sh[0]);
+  // one of ADJ_SLOTS partial sums (blocks spread over them)
+  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (tid == 0 && sh[0] != 0.0) atomicAdd(gflow + (bid & (ADJ_SLOTS - 1)), sh[0]);
 }
 
 template <typename T>
@@ -1055,7 +1062,7 @@ void launch_pressure_adj(hipStream_t st, const Layout &L, const StepArgs &s, con
   hipLaunchKernelGGL(k_padj_plane<T>, dim3(nz), dim3(256), 0, st, L, (const T *)lam.p, part);
   hipLaunchKernelGGL(k_sum1, dim3(1), dim3(256), 0, st, nz, part, part + nz);
   Box b{0, nx, 0, ny, 0, nz};
-  hipLaunchKernelGGL(k_padj_rhs<T>, dim3((nx + 1 + 255) / 256, ny + 1, nz + 1), dim3(256), 0, st, L, b, s, fs,
+  hipLaunchKernelGGL(k_padj_rhs<T>, dim3((nx + 1 + 63) / 64, (ny + 1 + 3) / 4, nz + 1), dim3(64, 4), 0, st, L, b, s, fs,
                      (const T *)lam.p, (const double *)(part + nz), lam.um, lam.vm, lam.wm, lam.w, lam.fx, lam.fy,
                      lam.fz, gflow);
   hipMemsetAsync(lam.p, 0, L.n * sizeof(T), st);

@@ -281,7 +281,7 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   // pass 3 (the antidiffusive velocities of pass 2 play the Courant numbers);
   // c1 receives the pre-sponge values of the cells next to a boundary
   a.cc = c2; a.U = (T *)tw.ud2; a.V = (T *)tw.vd2; a.W = (T *)tw.wd2;
-  a.ud = a.vd = a.wd = nullptr;
+  a.ud = (T *)tw.ud3; a.vd = (T *)tw.vd3; a.wd = (T *)tw.wd3;
   T *cpre = tw.cpre ? (T *)tw.cpre : c1;
   a.cout = c; a.cpre = cpre; a.last = 1;
   if (mark) mark(mark_arg, 0);
