@@ -4,13 +4,21 @@ MI355X (FP64 plain form, 256^3 workload), with the dominant kernel's HBM
 roofline and the reference's OpenMP CPU path timed beside it.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload les256]
-                [--precision fp64|fp32] [--no-cpu]
+                [--precision fp64|fp32] [--no-cpu] [--no-fp32] [--no-adjoint]
+                [--only-adjoint] [--adjoint-workload les128] [--adjoint-steps 400]
+                [--adjoint-seg 20]
 
 A step is one pass of the reference's time loop body
 (hotcross80-revised2014.f:264-350: body force, forces, pressure solve,
 momentum, Smagorinsky viscosity, source, three MPDATA tracers, density,
 Asselin/leapfrog refresh, CFL every 10 steps) over the whole grid, with the
 state resident in HBM.  Mcell-steps/s = nx*ny*nz*K / seconds.
+
+The same line carries the FP32 form's throughput on the same workload
+("fp32") and the adjoint's cost ("adjoint"): an adjoint window of the
+128^3 adjoint configuration (400 steps, a checkpoint every 20, 25 probes at
+1 Hz) timed beside the same number of plain forward steps; its ratio is
+(window - forward) / forward, the reverse sweep per forward step.
 """
 import argparse
 import ctypes
@@ -183,6 +191,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form's line")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint window timing")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="time only the adjoint window (prints its JSON object), e.g. for a profiler")
+    ap.add_argument("--adjoint-workload", default="les128")
+    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-seg", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-worker", nargs=5)
     args = ap.parse_args()
@@ -194,6 +207,9 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.only_adjoint:
+        print(json.dumps(adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg)))
+        return
     p, sec, cells, phases, nprof = gpu_leg(args)
     value = cells * args.steps / sec / 1e6
     ms_step = sec * 1e3 / args.steps
@@ -231,7 +247,7 @@ def main():
                 "err_vs_fp64": "tests/test_gpu_fp32.py: max error relative to max|field| after 50 steps"}
     adj = None
     if not args.no_adjoint:
-        adj = adjoint_leg()
+        adj = adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg)
     out = {
         "metric": "Mcell-steps/s FP64 vs FP32 at stated err vs FP64; adjoint/forward time ratio",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,

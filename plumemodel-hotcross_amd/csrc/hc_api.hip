@@ -93,6 +93,12 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
     while (n % 3 == 0) n /= 3;
     return n == 1;
   };
+  if (P.nz > 1024) {
+    // the pressure solve keeps a z column in LDS
+    int r = fail(ctx, HC_EINVAL, "grid must have nz <= 1024");
+    *out = ctx;
+    return r;
+  }
   if (P.nx < 8 || P.ny < 4 || P.nz < 3 || !fft_ok(P.nx) || !fft_ok(P.ny)) {
     // nx >= 8: the entrance ramp of FORCES touches i = 0..7
     // (hotcross80-revised2014.f:953-956); nx, ny = 2^a 3^b, even, for the

@@ -83,6 +83,19 @@ struct MpPass {
 HC_HD inline double mp_fabs(double x) { return fabs(x); }
 HC_HD inline float mp_fabs(float x) { return fabsf(x); }
 
+// the antidiffusive ratios (a - b) / (a + b + eps): an IEEE division in FP64
+// (the reference's rounding); in the FP32 form the device uses the quotient
+// through the hardware reciprocal (within 2 ulp), which the form's stated
+// error against FP64 absorbs
+HC_HD inline double mp_ratio(double a, double b) { return a / b; }
+HC_HD inline float mp_ratio(float a, float b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __fdividef(a, b);
+#else
+  return a / b;
+#endif
+}
+
 struct MpGeom {
   int nx, ny, nz;
   int px, py;   // padded pitches (Layout)
@@ -103,14 +116,14 @@ HC_HD inline T mp_anti_u(const FC &C, const FU &U, const FV &V, const FW &W) {
   const T epsa = (T)1.0e-30;
   T ca = C(1, 0, 0), cb = C(0, 0, 0);
   T Uc = U(0, 0, 0);
-  T r = (mp_fabs(Uc) - Uc * Uc / (T)1.0) * (ca - cb) / (ca + cb + epsa);
+  T r = (mp_fabs(Uc) - Uc * Uc / (T)1.0) * mp_ratio(ca - cb, ca + cb + epsa);
   ca = (C(1, 1, 0) + C(0, 1, 0));
   cb = (C(1, -1, 0) + C(0, -1, 0));
-  r = r - (T)1 * (T)0.5 * Uc / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
+  r = r - (T)1 * (T)0.5 * Uc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
               (V(1, 0, 0) + V(0, 0, 0) + V(1, -1, 0) + V(0, -1, 0)) * (T)1;
   ca = (C(1, 0, -1) + C(0, 0, -1));
   cb = (C(1, 0, 1) + C(0, 0, 1));
-  r = r - (T)0.5 * Uc / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
+  r = r - (T)0.5 * Uc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
               (W(1, 0, 0) + W(0, 0, 0) + W(1, 0, -1) + W(0, 0, -1)) * (T)1;
   return r;
 }
@@ -121,14 +134,14 @@ HC_HD inline T mp_anti_v(const FC &C, const FU &U, const FV &V, const FW &W) {
   const T epsa = (T)1.0e-30;
   T ca = C(0, 1, 0), cb = C(0, 0, 0);
   T Vc = V(0, 0, 0);
-  T r = (mp_fabs(Vc) - Vc * Vc / (T)1.0) * (T)1 * (ca - cb) / (ca + cb + epsa);
+  T r = (mp_fabs(Vc) - Vc * Vc / (T)1.0) * (T)1 * mp_ratio(ca - cb, ca + cb + epsa);
   ca = (C(1, 1, 0) + C(1, 0, 0));
   cb = (C(-1, 1, 0) + C(-1, 0, 0));
-  r = r - (T)0.5 * Vc / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
+  r = r - (T)0.5 * Vc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
               (U(0, 1, 0) + U(0, 0, 0) + U(-1, 1, 0) + U(-1, 0, 0)) * (T)1;
   ca = (C(0, 1, -1) + C(0, 0, -1));
   cb = (C(0, 1, 1) + C(0, 0, 1));
-  r = r - (T)0.5 * Vc / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
+  r = r - (T)0.5 * Vc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
               (W(0, 0, 0) + W(0, 1, 0) + W(0, 0, -1) + W(0, 1, -1)) * (T)1;
   return r;
 }
@@ -139,14 +152,14 @@ HC_HD inline T mp_anti_w(const FC &C, const FU &U, const FV &V, const FW &W) {
   const T epsa = (T)1.0e-30;
   T ca = C(0, 0, 0), cb = C(0, 0, 1);
   T Wc = W(0, 0, 0);
-  T r = (mp_fabs(Wc) - Wc * Wc / (T)1.0) * (ca - cb) / (ca + cb + epsa);
+  T r = (mp_fabs(Wc) - Wc * Wc / (T)1.0) * mp_ratio(ca - cb, ca + cb + epsa);
   ca = (C(1, 0, 0) + C(1, 0, 1));
   cb = (C(-1, 0, 0) + C(-1, 0, 1));
-  r = r - (T)1 * (T)0.5 * Wc / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
+  r = r - (T)1 * (T)0.5 * Wc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
               (U(0, 0, 0) + U(0, 0, 1) + U(-1, 0, 1) + U(-1, 0, 0)) * (T)1;
   ca = (C(0, 1, 1) + C(0, 1, 0));
   cb = (C(0, -1, 1) + C(0, -1, 0));
-  r = r - (T)1 * (T)0.5 * Wc / (T)1.0 * (ca - cb) / (ca + cb + epsa) * (T)0.25 *
+  r = r - (T)1 * (T)0.5 * Wc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
               (V(0, 0, 0) + V(0, 0, 1) + V(0, -1, 1) + V(0, -1, 0)) * (T)1;
   return r;
 }

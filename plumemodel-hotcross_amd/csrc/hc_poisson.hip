@@ -361,34 +361,48 @@ __global__ void __launch_bounds__(256) k_dct_y(Layout L, FftPlan pl, int npair, 
 
 // Thomas solve along z per (kx,ky), hs3crt_composite80.f:408-431 in the z
 // direction; the (0,0) column in FP64 with the k-mean removed
+// the column of the (0,0) mode lives in LDS while one thread runs its
+// recurrences (nz <= TRI_MAXZ); the block that holds it loads and stores it
+// with all its threads
+constexpr int TRI_MAXZ = 1024;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const double *__restrict__ piv,
                                                  T *__restrict__ buf, double *__restrict__ y00) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= plane) return;
   const double a = 1.0 / (dz * dz);
-  if (q == 0) {
-    // the singular (0,0) column: FP64 throughout (y00 holds the sweep), last
-    // pivot 1 (set with the table), then the k-mean removed
-    double y = (double)buf[0] * piv[0];
-    y00[0] = y;
-    for (int k = 1; k < nz; ++k) {
-      y = ((double)buf[k * plane] - a * y) * piv[k * plane];
-      y00[k] = y;
+  if (blockIdx.x == 0) {
+    // the singular (0,0) column: FP64 throughout, last pivot 1 (set with the
+    // table), then the k-mean removed
+    __shared__ double g[TRI_MAXZ], d[TRI_MAXZ];
+    for (int k = (int)threadIdx.x; k < nz; k += blockDim.x) {
+      g[k] = (double)buf[k * plane];
+      d[k] = piv[k * plane];
     }
-    double x = y, sum = x;
-    for (int k = nz - 2; k >= 0; --k) {
-      x = y00[k] - a * piv[k * plane] * x;
-      y00[k] = x;
-      sum += x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double y = g[0] * d[0];
+      g[0] = y;
+      for (int k = 1; k < nz; ++k) {
+        y = (g[k] - a * y) * d[k];
+        g[k] = y;
+      }
+      double x = y, sum = x;
+      for (int k = nz - 2; k >= 0; --k) {
+        x = g[k] - a * d[k] * x;
+        g[k] = x;
+        sum += x;
+      }
+      g[nz - 1] = y;
+      y00[0] = sum / nz;
     }
-    y00[nz - 1] = y;
-    double mean = sum / nz;
-    for (int k = 0; k < nz; ++k) buf[k * plane] = (T)(y00[k] - mean);
-    return;
+    __syncthreads();
+    const double mean = y00[0];
+    for (int k = (int)threadIdx.x; k < nz; k += blockDim.x) buf[k * plane] = (T)(g[k] - mean);
   }
+  if (q == 0 || q >= plane) return;
   // the sweeps are sequential in k; loads are issued in batches of KB levels
   // ahead of the recurrence so their latency overlaps
   constexpr int KB = 8;
@@ -897,31 +911,41 @@ __global__ void __launch_bounds__(256) k_tridiag_adj(Layout L, double dz, const 
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= plane) return;
   const double a = 1.0 / (dz * dz);
-  if (q == 0) {
-    // the (0,0) column in FP64: mean removal (a symmetric projection) first
-    double mean = 0.0;
-    for (int k = 0; k < nz; ++k) mean += (double)buf[k * plane];
-    mean /= nz;
-    // back substitution x_k = y_k - a d_k x_{k+1}, transposed (k ascending)
-    double lx_next = (double)buf[0] - mean;
-    for (int k = 0; k < nz - 1; ++k) {
-      const double ly = lx_next;
-      buf[k * plane] = (T)ly;  // holds lambda_y
-      lx_next = ((double)buf[(k + 1) * plane] - mean) - a * piv[k * plane] * ly;
+  if (blockIdx.x == 0) {
+    // the (0,0) column in FP64 and in LDS: mean removal (a symmetric
+    // projection) first, then the recurrences transposed
+    __shared__ double g[TRI_MAXZ], d[TRI_MAXZ];
+    for (int k = (int)threadIdx.x; k < nz; k += blockDim.x) {
+      g[k] = (double)buf[k * plane];
+      d[k] = piv[k * plane];
     }
-    buf[(nz - 1) * plane] = (T)lx_next;
-    // forward elimination y_k = (g_k - a y_{k-1}) d_k, transposed (k descending)
-    double ly = (double)buf[(nz - 1) * plane];
-    for (int k = nz - 1; k >= 1; --k) {
-      const double lg = piv[k * plane] * ly;
-      buf[k * plane] = (T)lg;
-      ly = (double)buf[(k - 1) * plane] - a * lg;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mean = 0.0;
+      for (int k = 0; k < nz; ++k) mean += g[k];
+      mean /= nz;
+      // back substitution x_k = y_k - a d_k x_{k+1}, transposed (k ascending)
+      double lx_next = g[0] - mean;
+      for (int k = 0; k < nz - 1; ++k) {
+        const double ly = lx_next;
+        lx_next = (g[k + 1] - mean) - a * d[k] * ly;
+        g[k] = ly;
+      }
+      g[nz - 1] = lx_next;
+      // forward elimination y_k = (g_k - a y_{k-1}) d_k, transposed (k descending)
+      double ly = g[nz - 1];
+      for (int k = nz - 1; k >= 1; --k) {
+        const double lg = d[k] * ly;
+        ly = g[k - 1] - a * lg;
+        g[k] = lg;
+      }
+      g[0] = d[0] * ly;
     }
-    buf[0] = (T)(piv[0] * ly);
-    return;
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < nz; k += blockDim.x) buf[k * plane] = (T)g[k];
   }
+  if (q == 0 || q >= plane) return;
   T lx = buf[q];
   for (int k = 0; k < nz - 1; ++k) {
     const T ly = lx;
