@@ -13,6 +13,8 @@ module hotcross_capi
   integer(c_int), parameter :: HC_U = 0, HC_V = 1, HC_W = 2, HC_UP = 3, HC_VP = 4, HC_WP = 5, HC_UM = 6, &
                                HC_VM = 7, HC_WM = 8, HC_P = 9, HC_T = 10, HC_S = 11, HC_C = 12, HC_SS = 13, &
                                HC_SSV = 14, HC_RHOP = 15, HC_FX = 16, HC_FY = 17, HC_FZ = 18
+  ! adjoint-only fields: the work velocities' Courant numbers
+  integer(c_int), parameter :: HC_ADJ_UC = 32, HC_ADJ_VC = 33, HC_ADJ_WC = 34
 
   type, bind(C) :: hc_params_t
     integer(c_int32_t) :: nx, ny, nz, nt
@@ -162,6 +164,97 @@ module hotcross_capi
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsteps
+    end function
+    ! ---- controls and discrete adjoint (no reference counterpart) ----------
+    integer(c_int) function hc_set_controls(ctx, qmap, flow_scale, cs) bind(C, name="hc_set_controls")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, qmap
+      real(c_double), value :: flow_scale, cs
+    end function
+    integer(c_int) function hc_adj_reset(ctx) bind(C, name="hc_adj_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_upload(ctx, field, host, lo, hi) bind(C, name="hc_adj_upload")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: field
+      integer(c_int32_t), intent(in) :: lo(3), hi(3)
+    end function
+    integer(c_int) function hc_adj_download(ctx, field, host, lo, hi) bind(C, name="hc_adj_download")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: field
+      integer(c_int32_t), intent(in) :: lo(3), hi(3)
+    end function
+    integer(c_int) function hc_adj_forces(ctx) bind(C, name="hc_adj_forces")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_pressure(ctx) bind(C, name="hc_adj_pressure")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_momentum_u(ctx) bind(C, name="hc_adj_momentum_u")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_momentum_v(ctx) bind(C, name="hc_adj_momentum_v")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_momentum_w(ctx) bind(C, name="hc_adj_momentum_w")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_viscosity3d(ctx) bind(C, name="hc_adj_viscosity3d")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_work_velocity(ctx, tau3) bind(C, name="hc_adj_work_velocity")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tau3
+    end function
+    integer(c_int) function hc_adj_tracer31(ctx, field, c_bkg, cmin, cmax, tau3) bind(C, name="hc_adj_tracer31")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      real(c_double), intent(in) :: c_bkg(*)
+      real(c_double), value :: cmin, cmax, tau3
+    end function
+    integer(c_int) function hc_adj_vent_ghost(ctx, field) bind(C, name="hc_adj_vent_ghost")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+    end function
+    integer(c_int) function hc_adj_density(ctx) bind(C, name="hc_adj_density")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_refresh(ctx) bind(C, name="hc_adj_refresh")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_adj_gradients(ctx, gq, out) bind(C, name="hc_adj_gradients")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, gq
+      real(c_double), intent(out) :: out(3)
+    end function
+    integer(c_int) function hc_adj_set_probes(ctx, nprobe, ijk, every, wgt_w, wgt_t) bind(C, name="hc_adj_set_probes")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nprobe, every
+      integer(c_int32_t), intent(in) :: ijk(3, *)
+      real(c_double), value :: wgt_w, wgt_t
+    end function
+    integer(c_int) function hc_adj_window(ctx, nsteps, seg, obs, samples, loss, gq, gs, end_state) &
+        bind(C, name="hc_adj_window")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx, samples, gq
+      integer(c_int32_t), value :: nsteps, seg, end_state
+      real(c_double), intent(in) :: obs(*)
+      real(c_double), intent(out) :: loss, gs(3)
     end function
   end interface
 
