@@ -814,14 +814,18 @@ __global__ void k_probe_seed(int n, const int *__restrict__ iw, const int *__res
 
 // ===========================================================================
 // adjoint state of a context
-static const int NTAPE = 11;
-static const int TAPE_FIELDS[NTAPE] = {HC_U, HC_V, HC_W, HC_UM, HC_VM, HC_WM, HC_SS, HC_SSV, HC_RHOP, HC_T, HC_S};
+// a state of the window: the 11 fields the step reads, and the up, vp, wp
+// buffers, whose cells MOMENTUM does not write become u, v, w at REFRESH
+static const int NTAPE = 14;
+static const int TAPE_FIELDS[NTAPE] = {HC_U,    HC_V,   HC_W, HC_UM, HC_VM, HC_WM, HC_SS,
+                                       HC_SSV,  HC_RHOP, HC_T, HC_S,  HC_UP, HC_VP, HC_WP};
 
 struct HostState {
   double time;
   double acc[8];
   int next_step;
   bool map_valid;
+  bool up_alias[3];
 };
 
 struct AdjCtx {
@@ -1323,6 +1327,7 @@ static HostState host_state(hc_ctx *ctx) {
   for (int q = 0; q < 8; ++q) h.acc[q] = ctx->acc[q];
   h.next_step = ctx->next_step;
   h.map_valid = ctx->map_valid;
+  for (int c = 0; c < 3; ++c) h.up_alias[c] = ctx->up_alias[c];
   return h;
 }
 
@@ -1330,6 +1335,7 @@ static void set_host_state(hc_ctx *ctx, const HostState &h) {
   ctx->time = h.time;
   for (int q = 0; q < 8; ++q) ctx->acc[q] = h.acc[q];
   ctx->next_step = h.next_step;
+  for (int c = 0; c < 3; ++c) ctx->up_alias[c] = h.up_alias[c];
   if (ctx->map_valid != h.map_valid) {
     ctx->map_valid = h.map_valid;
     update_sums(ctx);
@@ -1500,7 +1506,6 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   const int keep = end_state ? nck : 0;
   load_slot(ctx, keep);
   set_host_state(ctx, ck[keep]);
-  for (int c = 0; c < 3; ++c) ctx->up_alias[c] = false;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ctx, HC_EDEVICE, std::string("adjoint window: ") + hipGetErrorString(e));
   return hc_adj_gradients(ctx, gq, gs);
