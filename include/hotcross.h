@@ -182,6 +182,24 @@ int hc_step(hc_ctx *ctx, int32_t nsteps);
  * patch cells.  For checkers that replay a native run. */
 int hc_source_noise(hc_ctx *ctx, int32_t mtime, double *out);
 
+/* ---- plane statistics ---------------------------------------------------
+ * Running FP64 time sums over the planes k0..k1 (1 <= k0 <= k1 <= nz), for
+ * the time-mean w, T, TKE and heat flux of the plume runs (the reference
+ * writes 1 Hz planes, put_block1.f:242-330, and averages them afterwards).
+ * Per cell (i,j,k), x fastest then y then k, HC_NSTAT sums laid out
+ * stat-major (sums[s*ncell + cell]):
+ *   0 w at the w point (as stored: positive toward increasing k, i.e.
+ *     downward; the writers output -w, put_block1.f:265), 1 T,
+ *   2-4 the cell-centred u, v, w, 5-7 their squares, 8 T^2, 9 (cell w)*T.
+ * hc_stats_reset zeroes the sums (k0 = k1 = 0 frees them); hc_step adds one
+ * sample after every `every`-th step (m+1 divisible by every; 0 = only
+ * hc_stats_accumulate adds).  hc_stats_get returns the sums and the sample
+ * count (synchronises). */
+#define HC_NSTAT 10
+int hc_stats_reset(hc_ctx *ctx, int32_t k0, int32_t k1, int32_t every);
+int hc_stats_accumulate(hc_ctx *ctx);
+int hc_stats_get(hc_ctx *ctx, double *sums, int32_t *count);
+
 /* ---- controls and discrete adjoint --------------------------------------
  * The reference has no adjoint; these entries differentiate the timestep
  * above.  Each hc_adj_<routine> is the transpose of the linearisation of the

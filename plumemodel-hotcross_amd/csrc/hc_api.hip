@@ -235,6 +235,7 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->d_qmap) hipFree(ctx->d_qmap);
   adj_free(ctx);
+  stats_free(ctx);
   for (auto &p : ctx->phases) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
   if (ctx->st) hipStreamDestroy(ctx->st);
   delete ctx;
@@ -775,6 +776,7 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
     } else {
       hc_refresh(ctx);
     }
+    if (ctx->stats_every > 0 && (m + 1) % ctx->stats_every == 0) stats_accumulate(ctx);
     if (m % 10 == 0) {
       double cfl[6];
       int r = hc_cfl_values(ctx, cfl);

@@ -81,9 +81,15 @@ struct hc_ctx {
   // d/d(flow_scale) of ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby at this step
   double dflow[6] = {};
   AdjCtx *adj = nullptr;
+  // running time sums over the planes k0..k1 (hc_stats.hip); the native loop
+  // adds to them after every stats_every-th step (0: only on request)
+  double *d_stats = nullptr;
+  int stats_k0 = 0, stats_k1 = 0, stats_every = 0, stats_count = 0;
 };
 
 void adj_free(hc_ctx *ctx);
+void stats_free(hc_ctx *ctx);
+int stats_accumulate(hc_ctx *ctx);
 // shared by the C-ABI translation units (hc_api.hip)
 int box_copy(hc_ctx *ctx, void *dev, int bounds_field, double *host, const int32_t lo[3], const int32_t hi[3],
              bool up);

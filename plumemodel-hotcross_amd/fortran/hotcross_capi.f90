@@ -15,6 +15,8 @@ module hotcross_capi
                                HC_SSV = 14, HC_RHOP = 15, HC_FX = 16, HC_FY = 17, HC_FZ = 18
   ! adjoint-only fields: the work velocities' Courant numbers
   integer(c_int), parameter :: HC_ADJ_UC = 32, HC_ADJ_VC = 33, HC_ADJ_WC = 34
+  ! running sums per cell of a statistics slab (hc_stats_get)
+  integer(c_int), parameter :: HC_NSTAT = 10
 
   type, bind(C) :: hc_params_t
     integer(c_int32_t) :: nx, ny, nz, nt
@@ -164,6 +166,21 @@ module hotcross_capi
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsteps
+    end function
+    ! ---- plane statistics: time sums over planes k0..k1 ---------------------
+    integer(c_int) function hc_stats_reset(ctx, k0, k1, every) bind(C, name="hc_stats_reset")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: k0, k1, every
+    end function
+    integer(c_int) function hc_stats_accumulate(ctx) bind(C, name="hc_stats_accumulate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hc_stats_get(ctx, sums, count) bind(C, name="hc_stats_get")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx, sums
+      integer(c_int32_t), intent(out) :: count
     end function
     ! ---- controls and discrete adjoint (no reference counterpart) ----------
     integer(c_int) function hc_set_controls(ctx, qmap, flow_scale, cs) bind(C, name="hc_set_controls")

@@ -94,6 +94,8 @@ def lib():
             "hc_vent_ghost": [vp, ctypes.c_int, dbl],
             "hc_density": [vp], "hc_refresh": [vp], "hc_cfl_values": [vp, dp],
             "hc_step": [vp, i32], "hc_source_noise": [vp, i32, dp],
+            "hc_stats_reset": [vp, i32, i32, i32], "hc_stats_accumulate": [vp],
+            "hc_stats_get": [vp, dp, dp],
             "hc_phase_timing": [vp, ctypes.c_int],
             "hc_phase_times": [vp, ctypes.POINTER(ctypes.c_char_p), dp, dp, i32],
             "hc_set_controls": [vp, dp, dbl, dbl],
@@ -123,6 +125,31 @@ def lib():
 
 def _arr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# sums of hc_stats_get, in order (include/hotcross.h)
+NSTAT = 10
+STAT_NAMES = ("w", "t", "uc", "vc", "wc", "uc2", "vc2", "wc2", "t2", "wct")
+
+
+def plane_statistics(sums, count):
+    """Time means from the running sums of a statistics slab.  Returns a dict
+    of arrays shaped (nk, ny, nx): w_up (the time-mean vertical velocity at
+    the w points, positive upward as the writers output it,
+    put_block1.f:265), t (mean temperature), tke (0.5 times the summed
+    variances of the cell-centred velocity components), t_var (temperature
+    variance) and wt_flux (<w'T'> with w positive upward)."""
+    if count < 1:
+        raise ValueError("no samples accumulated")
+    m = {n: sums[q] / count for q, n in enumerate(STAT_NAMES)}
+    var = lambda sq, mean: np.maximum(sq - mean * mean, 0.0)
+    return {
+        "w_up": -m["w"],
+        "t": m["t"],
+        "tke": 0.5 * (var(m["uc2"], m["uc"]) + var(m["vc2"], m["vc"]) + var(m["wc2"], m["wc"])),
+        "t_var": var(m["t2"], m["t"]),
+        "wt_flux": -(m["wct"] - m["wc"] * m["t"]),
+    }
 
 
 class Context:
@@ -296,6 +323,30 @@ class Context:
 
     def npatch(self):
         return len(self.source_noise(0)) // 3
+
+    # ---- plane statistics ------------------------------------------------------
+    def stats_reset(self, k0, k1=None, every=0):
+        """Zero the running sums over planes k0..k1 (k1 defaults to k0); the
+        native loop adds a sample after every `every`-th step (0: only
+        stats_accumulate adds)."""
+        k1 = k0 if k1 is None else k1
+        self._check(self.L.hc_stats_reset(self.h, k0, k1, every))
+        self._stats_k = (k0, k1)
+
+    def stats_accumulate(self):
+        self._check(self.L.hc_stats_accumulate(self.h))
+
+    def stats_get(self):
+        """(sums with shape (NSTAT, k1-k0+1, ny, nx), sample count)."""
+        k0, k1 = self._stats_k
+        sums = np.zeros((NSTAT, k1 - k0 + 1, self.ny, self.nx))
+        cnt = np.zeros(1, dtype=np.int32)
+        self._check(self.L.hc_stats_get(self.h, _arr(sums), _arr(cnt)))
+        return sums, int(cnt[0])
+
+    def stats(self):
+        """Time means of the slab (plane_statistics of stats_get)."""
+        return plane_statistics(*self.stats_get())
 
     # ---- controls and discrete adjoint ---------------------------------------
     def set_controls(self, qmap=None, flow_scale=1.0, cs=0.0):
