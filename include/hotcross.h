@@ -34,6 +34,7 @@ extern "C" {
 #define HC_EDEVICE (-2)  /* HIP runtime error                             */
 #define HC_ENOMEM (-3)   /* device allocation failed                      */
 #define HC_ECFL (-4)     /* "stop on impending cfl violation"             */
+#define HC_ENAN (-5)     /* non-finite velocity found by the CFL check    */
 
 #define HC_FP64 0        /* plain form: every field and kernel in FP64    */
 #define HC_FP32 1        /* chosen form: FP32 fields, FP64 islands        */
@@ -105,6 +106,12 @@ int hc_upload(hc_ctx *ctx, int field, const double *host, const int32_t lo[3],
               const int32_t hi[3]);
 int hc_download(hc_ctx *ctx, int field, double *host, const int32_t lo[3],
                 const int32_t hi[3]);
+/* FP32 copy of scale*field over lo..hi, converted on the device: what the
+ * reference's netCDF writers store (real(field, kind=4),
+ * write_snapshota.f:269-374; w is written as -w, scale = -1,
+ * put_block1.f:265).  Half the PCIe bytes of hc_download. */
+int hc_download_f32(hc_ctx *ctx, int field, float *host, const int32_t lo[3],
+                    const int32_t hi[3], double scale);
 
 /* Background profiles, each of length nz+2 (index 0..nz+1): t_bkg, s_bkg,
  * rho_bkg (tracer_module), azzmin (viscosity_module).  Set by the
@@ -168,19 +175,28 @@ int hc_density(hc_ctx *ctx);
 int hc_refresh(hc_ctx *ctx);
 /* CFL_VALUES, hotcross80-revised2014.f:374-438: out = maxu, minu, maxv,
  * minv, maxw, minw (Courant numbers).  Returns HC_ECFL when the largest
- * magnitude exceeds 0.75. */
+ * magnitude exceeds 0.75, and HC_ENAN (out all NaN) when any velocity it
+ * scans is NaN or infinite (the NaN bailout the reference keeps commented
+ * out at :417-431; min/max alone pass over NaN).  The state is left as it
+ * is, for the caller's emergency snapshot (the reference's :407-411). */
 int hc_cfl_values(hc_ctx *ctx, double out[6]);
 
 /* The whole reference time loop body (hotcross80-revised2014.f:264-350) for
  * nsteps steps starting at the context's next step, with the source noise
  * from the counter-based generator.  CFL every 10 steps as in the
- * reference; returns HC_ECFL on the bailout condition. */
+ * reference; returns HC_ECFL or HC_ENAN on the bailout conditions, after
+ * the step whose check failed (the state stays on the device). */
 int hc_step(hc_ctx *ctx, int32_t nsteps);
 
 /* The three uniform draws per source-patch cell that hc_step's generator
  * uses at step mtime (patch order as hc_source_terms); returns the number of
  * patch cells.  For checkers that replay a native run. */
 int hc_source_noise(hc_ctx *ctx, int32_t mtime, double *out);
+
+/* Model time and step counters: time and mtime of the last step begun
+ * (hotcross80-revised2014.f:266-268), and the step hc_step runs next.  Any
+ * pointer may be NULL. */
+int hc_clock(hc_ctx *ctx, double *time, int32_t *mtime, int32_t *next_step);
 
 /* ---- plane statistics ---------------------------------------------------
  * Running FP64 time sums over the planes k0..k1 (1 <= k0 <= k1 <= nz), for

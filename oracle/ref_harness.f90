@@ -339,6 +339,16 @@ subroutine href_clear_cbkg()
   cmin = 0.0; cmax = 1.0e+10; c_bkg = 0.0
 end subroutine
 
+! the reference's CFL_VALUES (hotcross80-revised2014.f:374-438) on the current
+! state: it writes maxu, minu, maxv, minv, maxw, minw and mtime to unit 81
+! (format 6e15.7,i10); flushed here so the caller can read the line back.
+! Only for states under the 0.75 bailout, whose path calls the netCDF writer.
+subroutine href_cfl_values() bind(C, name="href_cfl_values")
+  implicit none
+  call cfl_values
+  flush(81)
+end subroutine
+
 subroutine href_scalars(out) bind(C, name="href_scalars")
   use iso_c_binding
   use grid_module

@@ -52,6 +52,19 @@ __global__ void k_gather(Layout L, int l0, int l1, int l2, int n0, int n1, int n
   dst[q] = (double)src[IX(L, l0 + i, l1 + j, l2 + k)];
 }
 
+// FP32 copy of a Fortran box for the output writers (the reference writes
+// real(..., kind=4) of each field, write_snapshota.f:269-374), times `scale`
+// (-1 for w, which the writers output as -w, put_block1.f:265)
+template <typename T>
+__global__ void k_gather32(Layout L, int l0, int l1, int l2, int n0, int n1, int n2, const T *__restrict__ src,
+                           double scale, float *__restrict__ dst) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)n0 * n1 * n2;
+  if (q >= n) return;
+  int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
+  dst[q] = (float)(scale * (double)src[IX(L, l0 + i, l1 + j, l2 + k)]);
+}
+
 // q patch of SOURCE_terms (source80.inc:40-49) from the noise draws
 __global__ void k_source(int npatch, double fq_t, double fq_s, double wee, const double *__restrict__ noise,
                          int use_hash, uint64_t seed, int mtime, const double *__restrict__ qmap,
@@ -166,8 +179,7 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
   if (!(ctx->d_qbase = (double *)dalloc(ctx, (size_t)ctx->npatch * 8, &rc))) return rc;
   ctx->stage_n = (size_t)(P.nx + 3) * (P.ny + 3) * (P.nz + 3);
   if (!(ctx->d_stage = (double *)dalloc(ctx, ctx->stage_n * 8, &rc))) return rc;
-  int ncfl = (P.ny + 1) * (P.nz + 1);
-  if (!(ctx->d_cflpart = (double *)dalloc(ctx, (size_t)6 * ncfl * 8, &rc))) return rc;
+  if (!(ctx->d_cflpart = (double *)dalloc(ctx, (size_t)CFL_NPART * CFL_BLOCKS * 8, &rc))) return rc;
   if (!(ctx->d_cflout = (double *)dalloc(ctx, 64, &rc))) return rc;
   HIPCHK(hipHostMalloc((void **)&ctx->h_pin, 4096, hipHostMallocDefault));
   // pressure solve
@@ -306,6 +318,27 @@ extern "C" int hc_upload(hc_ctx *ctx, int field, const double *host, const int32
     if (r) return r;
   }
   return box_copy(ctx, ctx->f[field], field, (double *)host, lo, hi, true);
+}
+
+extern "C" int hc_download_f32(hc_ctx *ctx, int field, float *host, const int32_t lo[3], const int32_t hi[3],
+                               double scale) {
+  if (field < 0 || field >= HC_NFIELDS) return fail(ctx, HC_EINVAL, "bad field id");
+  int n[3];
+  int r = check_box(ctx, field, lo, hi, n);
+  if (r) return r;
+  const int src = (field >= HC_UP && field <= HC_WP && ctx->up_alias[field - HC_UP]) ? field - 3 : field;
+  const size_t cnt = (size_t)n[0] * n[1] * n[2];
+  dim3 g((unsigned)((cnt + 255) / 256));
+  float *stage = (float *)ctx->d_stage;
+  if (ctx->prec == HC_FP64)
+    hipLaunchKernelGGL(k_gather32<double>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                       (const double *)ctx->f[src], scale, stage);
+  else
+    hipLaunchKernelGGL(k_gather32<float>, g, dim3(256), 0, ctx->st, ctx->L, lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                       (const float *)ctx->f[src], scale, stage);
+  HIPCHK(hipMemcpyAsync(host, stage, cnt * 4, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return HC_OK;
 }
 
 extern "C" int hc_download(hc_ctx *ctx, int field, double *host, const int32_t lo[3], const int32_t hi[3]) {
@@ -693,10 +726,16 @@ extern "C" int hc_refresh(hc_ctx *ctx) {
 extern "C" int hc_cfl_values(hc_ctx *ctx, double out[6]) {
   if (ctx->prec == HC_FP64) launch_cfl<double>(ctx->st, ctx->L, ptrs<double>(ctx), ctx->d_cflpart, ctx->d_cflout);
   else launch_cfl<float>(ctx->st, ctx->L, ptrs<float>(ctx), ctx->d_cflpart, ctx->d_cflout);
-  HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_cflout, 6 * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(ctx->h_pin, ctx->d_cflout, CFL_NPART * 8, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   const hc_params &P = ctx->P;
   for (int q = 0; q < 6; ++q) out[q] = ctx->h_pin[q] * P.dt / (q < 2 ? P.dx : q < 4 ? P.dy : P.dz);
+  ctx->nonfinite = (int64_t)ctx->h_pin[6];
+  if (ctx->nonfinite > 0) {
+    for (int q = 0; q < 6; ++q) out[q] = NAN;
+    return fail(ctx, HC_ENAN, "stop on NaN cfl violation (" + std::to_string(ctx->nonfinite) +
+                                  " non-finite velocities at step " + std::to_string(ctx->mtime) + ")");
+  }
   double m = fmax(fmax(fmax(out[0], out[2]), fmax(-out[1], -out[3])), fmax(out[4], -out[5]));
   if (m > 0.75) return fail(ctx, HC_ECFL, "stop on impending cfl violation");
   return HC_OK;
@@ -785,6 +824,14 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, HC_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   }
+  return HC_OK;
+}
+
+extern "C" int hc_clock(hc_ctx *ctx, double *time, int32_t *mtime, int32_t *next_step) {
+  if (!ctx) return HC_EINVAL;
+  if (time) *time = ctx->time;
+  if (mtime) *mtime = ctx->mtime;
+  if (next_step) *next_step = ctx->next_step;
   return HC_OK;
 }
 

@@ -85,6 +85,8 @@ struct hc_ctx {
   // adds to them after every stats_every-th step (0: only on request)
   double *d_stats = nullptr;
   int stats_k0 = 0, stats_k1 = 0, stats_every = 0, stats_count = 0;
+  // non-finite velocities found by the last CFL check
+  int64_t nonfinite = 0;
 };
 
 void adj_free(hc_ctx *ctx);

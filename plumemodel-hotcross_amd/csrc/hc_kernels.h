@@ -34,6 +34,10 @@ struct Aux {
 
 // partial-sum slots per scalar sensitivity of the adjoint (a power of two)
 constexpr int ADJ_SLOTS = 256;
+// CFL_VALUES: partial results per block (six extremes and a non-finite count)
+// and the number of blocks of its reduction
+constexpr int CFL_NPART = 7;
+constexpr int CFL_BLOCKS = 1024;
 
 // d(ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby)/d(flow_scale) at one step
 struct FlowSens {

@@ -303,55 +303,81 @@ void launch_vent_ghost(hipStream_t st, const Layout &L, const VentArgs &va, T *c
   hipLaunchKernelGGL(k_vent<T>, dim3(va.nye - va.nys + 1), dim3(va.nxe - va.nxs + 1), 0, st, L, va, c, value);
 }
 
-// CFL_VALUES reductions, hotcross80-revised2014.f:395-402
+// CFL_VALUES reductions, hotcross80-revised2014.f:395-402, plus a count of
+// non-finite velocities (the NaN bailout the reference keeps commented out at
+// :417-431).  One wave per (j,k) row at a time, rows strided over the grid's
+// waves; fmin/fmax ignore NaN, so the count is what catches a NaN state.
+// part holds CFL_NPART values per block: max u, min u, max v, min v, max w,
+// min w, non-finite count.
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Layout L, const T *__restrict__ u, const T *__restrict__ v,
                                              const T *__restrict__ w, double *__restrict__ part) {
-  int j = (int)blockIdx.x % (L.ny + 1), k = (int)blockIdx.x / (L.ny + 1);  // j,k = 0..ny, 0..nz
-  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-  for (int i = (int)threadIdx.x; i <= L.nx; i += blockDim.x) {
-    if (j >= 1 && k >= 1) { double a = F3(u, i, j, k); mn[0] = fmin(mn[0], a); mx[0] = fmax(mx[0], a); }
-    if (i >= 1 && k >= 1) { double a = F3(v, i, j, k); mn[1] = fmin(mn[1], a); mx[1] = fmax(mx[1], a); }
-    if (i >= 1 && j >= 1) { double a = F3(w, i, j, k); mn[2] = fmin(mn[2], a); mx[2] = fmax(mx[2], a); }
-  }
-  __shared__ double sh[6][256];
-  for (int q = 0; q < 3; ++q) { sh[2 * q][threadIdx.x] = mx[q]; sh[2 * q + 1][threadIdx.x] = mn[q]; }
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off)
-      for (int q = 0; q < 3; ++q) {
-        sh[2 * q][threadIdx.x] = fmax(sh[2 * q][threadIdx.x], sh[2 * q][threadIdx.x + off]);
-        sh[2 * q + 1][threadIdx.x] = fmin(sh[2 * q + 1][threadIdx.x], sh[2 * q + 1][threadIdx.x + off]);
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int nrows = (L.ny + 1) * (L.nz + 1), nwaves = (int)gridDim.x * 4;
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300}, bad = 0.0;
+  for (int row = (int)blockIdx.x * 4 + wv; row < nrows; row += nwaves) {
+    const int j = row % (L.ny + 1), k = row / (L.ny + 1);  // j,k = 0..ny, 0..nz
+    for (int i = lane; i <= L.nx; i += 64) {
+      if (j >= 1 && k >= 1) {
+        double a = F3(u, i, j, k);
+        mn[0] = fmin(mn[0], a); mx[0] = fmax(mx[0], a); bad += !(fabs(a) <= 1.7976931348623157e308);
       }
-    __syncthreads();
+      if (i >= 1 && k >= 1) {
+        double a = F3(v, i, j, k);
+        mn[1] = fmin(mn[1], a); mx[1] = fmax(mx[1], a); bad += !(fabs(a) <= 1.7976931348623157e308);
+      }
+      if (i >= 1 && j >= 1) {
+        double a = F3(w, i, j, k);
+        mn[2] = fmin(mn[2], a); mx[2] = fmax(mx[2], a); bad += !(fabs(a) <= 1.7976931348623157e308);
+      }
+    }
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 6; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = sh[q][0];
+  double r[CFL_NPART] = {mx[0], mn[0], mx[1], mn[1], mx[2], mn[2], bad};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < CFL_NPART; ++q) {
+      double o = __shfl_xor(r[q], off, 64);
+      r[q] = q == 6 ? r[q] + o : (q % 2 == 0) ? fmax(r[q], o) : fmin(r[q], o);
+    }
+  __shared__ double sh[4][CFL_NPART];
+  if (lane == 0)
+    for (int q = 0; q < CFL_NPART; ++q) sh[wv][q] = r[q];
+  __syncthreads();
+  if (threadIdx.x < CFL_NPART) {
+    const int q = (int)threadIdx.x;
+    double a = sh[0][q];
+    for (int x = 1; x < 4; ++x) a = q == 6 ? a + sh[x][q] : (q % 2 == 0) ? fmax(a, sh[x][q]) : fmin(a, sh[x][q]);
+    part[(size_t)q * gridDim.x + blockIdx.x] = a;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_cfl_final(int n, const double *__restrict__ part, double *__restrict__ out) {
-  __shared__ double sh[6][256];
-  for (int q = 0; q < 6; ++q) {
-    double a = (q % 2 == 0) ? -1e300 : 1e300;
-    for (int r = threadIdx.x; r < n; r += 256)
-      a = (q % 2 == 0) ? fmax(a, part[(size_t)q * n + r]) : fmin(a, part[(size_t)q * n + r]);
+  __shared__ double sh[CFL_NPART][256];
+  for (int q = 0; q < CFL_NPART; ++q) {
+    double a = q == 6 ? 0.0 : (q % 2 == 0) ? -1e300 : 1e300;
+    for (int r = threadIdx.x; r < n; r += 256) {
+      double b = part[(size_t)q * n + r];
+      a = q == 6 ? a + b : (q % 2 == 0) ? fmax(a, b) : fmin(a, b);
+    }
     sh[q][threadIdx.x] = a;
   }
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off)
-      for (int q = 0; q < 6; ++q)
-        sh[q][threadIdx.x] = (q % 2 == 0) ? fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + off])
-                                          : fmin(sh[q][threadIdx.x], sh[q][threadIdx.x + off]);
+      for (int q = 0; q < CFL_NPART; ++q) {
+        double a = sh[q][threadIdx.x], b = sh[q][threadIdx.x + off];
+        sh[q][threadIdx.x] = q == 6 ? a + b : (q % 2 == 0) ? fmax(a, b) : fmin(a, b);
+      }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 6; ++q) out[q] = sh[q][0];
+  if (threadIdx.x < CFL_NPART) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 template <typename T>
 void launch_cfl(hipStream_t st, const Layout &L, const FieldPtrs<T> &f, double *part, double *out) {
-  int nb = (L.ny + 1) * (L.nz + 1);
+  const int nrows = (L.ny + 1) * (L.nz + 1);
+  const int nb = std::min(CFL_BLOCKS, (nrows + 3) / 4);
   hipLaunchKernelGGL(k_cfl<T>, dim3(nb), dim3(256), 0, st, L, f.u, f.v, f.w, part);
   hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, st, nb, part, out);
 }

@@ -8,7 +8,8 @@ module hotcross_capi
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: HC_OK = 0, HC_EINVAL = -1, HC_EDEVICE = -2, HC_ENOMEM = -3, HC_ECFL = -4
+  integer(c_int), parameter :: HC_OK = 0, HC_EINVAL = -1, HC_EDEVICE = -2, HC_ENOMEM = -3, HC_ECFL = -4, &
+                               HC_ENAN = -5
   integer(c_int), parameter :: HC_FP64 = 0, HC_FP32 = 1
   integer(c_int), parameter :: HC_U = 0, HC_V = 1, HC_W = 2, HC_UP = 3, HC_VP = 4, HC_WP = 5, HC_UM = 6, &
                                HC_VM = 7, HC_WM = 8, HC_P = 9, HC_T = 10, HC_S = 11, HC_C = 12, HC_SS = 13, &

@@ -28,6 +28,7 @@ contains
     if (r /= HC_OK) then
       write (6, *) 'hotcross: ', where, ': ', trim(hc_message(hc))
       if (r == HC_ECFL) stop 'stop on impending cfl violation'
+      if (r == HC_ENAN) stop 'stop on NaN cfl violation'
       stop 'hotcross device error'
     endif
   end subroutine

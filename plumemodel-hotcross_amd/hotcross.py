@@ -22,6 +22,8 @@ FIELD_ID = {n: q for q, n in enumerate(FIELDS)}
 ADJ_ID = dict(FIELD_ID, uc=32, vc=33, wc=34)
 ADJ_BOUNDS = {"uc": "u", "vc": "v", "wc": "w"}
 FP64, FP32 = 0, 1
+# error codes of include/hotcross.h
+HC_EINVAL, HC_EDEVICE, HC_ENOMEM, HC_ECFL, HC_ENAN = -1, -2, -3, -4, -5
 
 
 class HcError(RuntimeError):
@@ -96,6 +98,8 @@ def lib():
             "hc_step": [vp, i32], "hc_source_noise": [vp, i32, dp],
             "hc_stats_reset": [vp, i32, i32, i32], "hc_stats_accumulate": [vp],
             "hc_stats_get": [vp, dp, dp],
+            "hc_clock": [vp, dp, dp, dp],
+            "hc_download_f32": [vp, ctypes.c_int, dp, dp, dp, dbl],
             "hc_phase_timing": [vp, ctypes.c_int],
             "hc_phase_times": [vp, ctypes.POINTER(ctypes.c_char_p), dp, dp, i32],
             "hc_set_controls": [vp, dp, dbl, dbl],
@@ -202,6 +206,26 @@ class Context:
         hi_a = np.array(hi, dtype=np.int32)
         self._check(self.L.hc_download(self.h, FIELD_ID[name], _arr(a), _arr(lo_a), _arr(hi_a)))
         return a
+
+    def download_f32(self, name, scale=1.0, lo=None, hi=None):
+        """FP32 copy of scale * field over lo..hi (default: the field's
+        bounds), converted on the device; Fortran-ordered like download."""
+        blo, bhi = self.bounds(name)
+        lo = blo if lo is None else lo
+        hi = bhi if hi is None else hi
+        shape = tuple(h - l + 1 for l, h in zip(lo, hi))
+        a = np.zeros(shape, dtype=np.float32, order="F")
+        lo_a = np.array(lo, dtype=np.int32)
+        hi_a = np.array(hi, dtype=np.int32)
+        self._check(self.L.hc_download_f32(self.h, FIELD_ID[name], _arr(a), _arr(lo_a), _arr(hi_a), scale))
+        return a
+
+    def clock(self):
+        """(time, mtime) of the last step begun and the next step of hc_step."""
+        t = np.zeros(1)
+        m = np.zeros(2, dtype=np.int32)
+        self._check(self.L.hc_clock(self.h, _arr(t), _arr(m[0:1]), _arr(m[1:2])))
+        return float(t[0]), int(m[0]), int(m[1])
 
     def set_profiles(self, t_bkg, s_bkg, rho_bkg, azzmin):
         arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (t_bkg, s_bkg, rho_bkg, azzmin)]

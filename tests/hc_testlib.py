@@ -324,7 +324,8 @@ def ref_lib_path(nx, ny, nz):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import build_ref
     path = build_ref.libpath(nx, ny, nz)
-    if not os.path.exists(path):
+    srcs = [os.path.join(ROOT, "oracle", f) for f in ("ref_harness.f90", "build_ref.py")]
+    if not os.path.exists(path) or os.path.getmtime(path) < max(os.path.getmtime(f) for f in srcs):
         build_ref.build(nx, ny, nz)
     return path
 

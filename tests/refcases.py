@@ -141,3 +141,51 @@ def oracle_sequence(o, steps, calls, noises):
                 o.tracer("c", 0.0, 1.0e10, prm.dt)
             else:
                 o.call(c)
+
+
+def plant_cfl_extremes(state, prm, seed):
+    """Velocities for CFL_VALUES checks: seeded values of magnitude below
+    0.2 dx/dt inside the scanned ranges (u: i=0..nx, j=1..ny, k=1..nz; v:
+    i=1..nx, j=0..ny, k=1..nz; w: i=1..nx, j=1..ny, k=0..nz,
+    hotcross80-revised2014.f:395-402), the range's extremes planted on its
+    first and last cells, and larger values (0.6 dx/dt, still under the
+    bailout) on the cells just outside every range face, which a scan over
+    the wrong range would report."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = prm.nx, prm.ny, prm.nz
+    out = dict(state)
+    for name, d, lo in (("u", prm.dx, (0, 1, 1)), ("v", prm.dy, (1, 0, 1)), ("w", prm.dz, (1, 1, 0))):
+        b = T.bounds(name, nx, ny, nz)
+        a = np.array(state[name], dtype=np.float64, order="F", copy=True)
+        c = d / prm.dt
+        a[...] = 0.2 * c * (2.0 * rng.random(a.shape) - 1.0)
+        hi = (nx, ny, nz)
+        ix = lambda i, j, k: (i - b[0][0], j - b[1][0], k - b[2][0])
+        a[ix(*lo)] = 0.45 * c
+        a[ix(*hi)] = -0.4 * c
+        for ax in range(3):
+            for edge, val in ((lo[ax] - 1, 0.6 * c), (hi[ax] + 1, -0.6 * c)):
+                if b[ax][0] <= edge <= b[ax][1]:
+                    sl = [slice(None)] * 3
+                    sl[ax] = edge - b[ax][0]
+                    a[tuple(sl)] = val
+        out[name] = a
+    return out
+
+
+def worker_cfl(ref, seed):
+    """The reference's CFL_VALUES on its initial state with planted
+    velocities: returns (state, six Courant numbers as it printed them)."""
+    ref.init_profiles(PROFILE)
+    ref.call("density")
+    ref.call("initialize")
+    ref.call("sponge_maker")
+    st = {n: ref.get(n) for n in ("u", "v", "w")}
+    st = plant_cfl_extremes(st, ref.prm, seed)
+    for n in ("u", "v", "w"):
+        ref.set(n, st[n])
+    ref.call("cfl_values")
+    with open("fort.81") as fh:
+        line = fh.read().splitlines()[-1]
+    vals = [float(line[15 * q:15 * (q + 1)]) for q in range(6)]
+    return st, np.array(vals)

@@ -119,3 +119,42 @@ def test_gpu_tracers_match_oracle(m):
         else:
             err = G.compare(a, b)
             assert err <= 1e-10, (f, err)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_gpu_cfl_values_match_oracle(seed):
+    """CFL_VALUES bit for bit against the oracle, on velocities with planted
+    range extremes; then the two bailouts: a Courant number above 0.75
+    (HC_ECFL, as the oracle signals) and a NaN velocity (HC_ENAN, which the
+    min/max scan alone would pass over)."""
+    import refcases as R
+    path = [p for p in GOLDEN if "16x12x10_50" in p][0]
+    prm, before, _, _, _ = load_case(path)
+    print("cfl seed", seed, flush=True)
+    state = R.plant_cfl_extremes(before, prm, seed)
+    o = T.Oracle(prm)
+    for k, v in state.items():
+        o.set(k, v)
+    ctx = G.context_from_state(prm, state)
+    try:
+        assert o.call("cfl_values") == 0
+        assert np.array_equal(ctx.cfl_values(), o.scalars()[17:23])
+        # bailout: one interior w over 0.75 Courant
+        w = state["w"].copy()
+        w[3, 4, 5] = 0.8 * prm.dz / prm.dt
+        o.set("w", w)
+        ctx.upload("w", w)
+        assert o.call("cfl_values") == 1
+        with pytest.raises(G.H.HcError) as e:
+            ctx.cfl_values()
+        assert e.value.code == -4
+        # NaN: the oracle's min/max pass over it; the library stops
+        u = state["u"].copy()
+        u[5, 3, 2] = np.nan
+        ctx.upload("w", state["w"])
+        ctx.upload("u", u)
+        with pytest.raises(G.H.HcError) as e:
+            ctx.cfl_values()
+        assert e.value.code == -5 and "NaN" in str(e.value)
+    finally:
+        ctx.close()

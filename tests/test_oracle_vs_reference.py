@@ -54,3 +54,21 @@ def test_initial_state_matches_reference():
     o.call("sponge_maker")
     for n in ("t_bkg", "s_bkg", "rho_bkg", "azzmin", "alphau", "alphav", "alphae", "alphaw", "alphaee"):
         assert np.array_equal(o.get(n), before[n]) or T.rel_err(o.get(n), before[n]) < 1e-15, n
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_cfl_values_match_reference(seed):
+    """CFL_VALUES: the oracle's Courant numbers against the reference's
+    printed ones (format 6e15.7) on velocities whose extremes sit on the
+    scanned ranges' first and last cells, with larger values just outside
+    every range face."""
+    prm = T.reference_params(16, 12, 10)
+    st, printed = T.run_in_ref(R.worker_cfl, prm, seed=seed)
+    o = T.Oracle(prm)
+    for n, a in st.items():
+        o.set(n, a)
+    assert o.call("cfl_values") == 0
+    vals = o.scalars()[17:23]
+    assert np.allclose(vals, printed, rtol=1e-6, atol=0), (vals, printed)
+    # the planted in-range extremes, not the larger values outside the ranges
+    assert np.allclose(vals, [0.45, -0.4] * 3, rtol=1e-12, atol=0), vals
