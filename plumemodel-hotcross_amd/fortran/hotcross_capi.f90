@@ -71,6 +71,19 @@ module hotcross_capi
       integer(c_int), value :: field
       integer(c_int32_t), intent(in) :: lo(3), hi(3)
     end function
+    integer(c_int) function hc_download_f32(ctx, field, host, lo, hi, scale) bind(C, name="hc_download_f32")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: field
+      integer(c_int32_t), intent(in) :: lo(3), hi(3)
+      real(c_double), value :: scale
+    end function
+    integer(c_int) function hc_clock(ctx, time, mtime, next_step) bind(C, name="hc_clock")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: time
+      integer(c_int32_t), intent(out) :: mtime, next_step
+    end function
     integer(c_int) function hc_set_profiles(ctx, t_bkg, s_bkg, rho_bkg, azzmin) bind(C, name="hc_set_profiles")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

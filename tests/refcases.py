@@ -189,3 +189,10 @@ def worker_cfl(ref, seed):
         line = fh.read().splitlines()[-1]
     vals = [float(line[15 * q:15 * (q + 1)]) for q in range(6)]
     return st, np.array(vals)
+
+
+def worker_profiles(ref, path):
+    """INITIALIZE_PROFILES of the reference on the table at path: its
+    t_bkg and s_bkg."""
+    ref.init_profiles(path)
+    return ref.get("t_bkg"), ref.get("s_bkg")
