@@ -145,7 +145,8 @@ HC_HD inline void forces_z_at(const Layout &L, int i, int j, int k, const StepAr
   // alphaee(i,j,k) = max(alphae(i,j), (alphaw(k)+alphaw(k-1))/2), sponge80.f:115-119
   // (alphae here is the mask before the periodic copies of :128-138)
   double ae = fmax(alphae[i + (L.nx + 2) * j], (alphaw[k + 1] + alphaw[k]) / 2.0);
-  T r = dis + (T)ph.bit3 * (T)ph.g * (rhop(i, j, k) + rhop(i, j, k + 1)) / (T)ph.rho0 / (T)2.0 -
+  const UDiv<T> rho0 = udiv_t<T>(ph.rho0, s.rrho0, s.rrho0f);
+  T r = dis + rho0((T)ph.bit3 * (T)ph.g * (rhop(i, j, k) + rhop(i, j, k + 1))) / (T)2.0 -
         (T)0.0 * (T)alphaw[k + 1] * wmc - (T)0.5 * (T)ae * wmc - coe_4z * (c1 * c1 - c2 * c2) -
         coe_4x * ((w(i + 1, j, k) + wc_) * (u(i, j, k) + u(i, j, k + 1)) -
                   (wc_ + w(i - 1, j, k)) * (u(i - 1, j, k) + u(i - 1, j, k + 1))) -
@@ -196,8 +197,11 @@ HC_HD inline T ssv_val(const Layout &L, const Phys &ph, const StepArgs &s, const
   // ~1e-4 of that
   double rkm = (double)rhop(i, j, k - 1) + rho_bkg[k - 1];
   double rkp = (double)rhop(i, j, k + 1) + rho_bkg[k + 1];
-  double ri = -(ph.g * (rkm - rkp) / 2.0 / Sp<double, P2>(s.dz, s.idz) / ph.rho0 / ((double)sv + 1.0e-30));
-  double arg = 1.0 + ph.eps - ri / 3.0;
+  // the uniform divisors through their reciprocals (UDiv) and the
+  // data-dependent one through ddiv: the same correctly rounded quotients
+  const UDiv<double> rho0(ph.rho0, s.rrho0), three(3.0, 1.0 / 3.0);
+  double ri = -ddiv(rho0(ph.g * (rkm - rkp) / 2.0 / Sp<double, P2>(s.dz, s.idz)), (double)sv + 1.0e-30);
+  double arg = 1.0 + ph.eps - three(ri);
   return coef * sqrt(sv + (T)1.0e-30) * (T)sqrt(fmax(0.0, arg));
 }
 
@@ -267,21 +271,23 @@ template <typename T, class V>
 HC_HD inline T pass1_at(const Layout &L, const StepArgs &s, const Phys &ph, const V &c, const V &uc,
                              const V &vc, const V &wc, const V &ss, const V &ssv, const double *azzmin,
                              const double *cbkg, int i, int j, int k) {
-  const double pr = ph.prandtl;
+  // the divisions by the Prandtl number through its reciprocal (UDiv: the
+  // same correctly rounded quotients)
+  const UDiv<double> prd(ph.prandtl, s.rpr);
+  const UDiv<T> tpr = udiv_t<T>(ph.prandtl, s.rpr, s.rprf);
   const T xfac = (T)((double)(1 * ph.bit5k) / s.dx / s.dx * s.tau3 * 1);
   const T yfac = (T)((double)(1 * ph.bit5k) / s.dy / s.dy * s.tau3 * 1);
   const T zfac = (T)((double)ph.bit2k / s.dz / s.dz * s.tau3 * 1);
   const T b5 = (T)ph.bit5k, b2 = (T)ph.bit2k;
-  const T axp = (T)(ph.axmin / pr), ayp = (T)(ph.aymin / pr), tpr = (T)pr;
+  const T axp = (T)prd(ph.axmin), ayp = (T)prd(ph.aymin);
   T cp = c(i, j, k);
-  T kx1 = b5 * (((T).5 * (ss(i, j, k) + ss(i + 1, j, k))) / tpr + axp);
-  T kx0 = b5 * (((T).5 * (ss(i - 1, j, k) + ss(i, j, k))) / tpr + axp);
-  T ky1 = b5 * (((T).5 * (ss(i, j, k) + ss(i, j + 1, k))) / tpr + ayp);
-  T ky0 = b5 * (((T).5 * (ss(i, j - 1, k) + ss(i, j, k))) / tpr + ayp);
+  T kx1 = b5 * (tpr((T).5 * (ss(i, j, k) + ss(i + 1, j, k))) + axp);
+  T kx0 = b5 * (tpr((T).5 * (ss(i - 1, j, k) + ss(i, j, k))) + axp);
+  T ky1 = b5 * (tpr((T).5 * (ss(i, j, k) + ss(i, j + 1, k))) + ayp);
+  T ky0 = b5 * (tpr((T).5 * (ss(i, j - 1, k) + ss(i, j, k))) + ayp);
   // kzt(k) = 0 at k = 0 and k = nz (mixing80-revised2014.f:49-59)
-  T kz1 = (k == L.nz) ? (T)0 : b2 * (((T)0.5 * (ssv(i, j, k) + ssv(i, j, k + 1))) / tpr + (T)(azzmin[k] / pr));
-  T kz0 = (k == 1) ? (T)0
-                   : b2 * (((T)0.5 * (ssv(i, j, k - 1) + ssv(i, j, k))) / tpr + (T)(azzmin[k - 1] / pr));
+  T kz1 = (k == L.nz) ? (T)0 : b2 * (tpr((T)0.5 * (ssv(i, j, k) + ssv(i, j, k + 1))) + (T)prd(azzmin[k]));
+  T kz0 = (k == 1) ? (T)0 : b2 * (tpr((T)0.5 * (ssv(i, j, k - 1) + ssv(i, j, k))) + (T)prd(azzmin[k - 1]));
   T cx = xfac * (kx1 * (c(i + 1, j, k) - cp) - kx0 * (cp - c(i - 1, j, k)));
   T cy = yfac * (ky1 * (c(i, j + 1, k) - cp) - ky0 * (cp - c(i, j - 1, k)));
   T bk = (T)cbkg[k], bkm = (T)cbkg[k - 1], bkp = (T)cbkg[k + 1];

@@ -163,6 +163,8 @@ static inline StepArgs step_args(hc_ctx *ctx) {
   StepArgs s{};
   s.dx = ctx->P.dx; s.dy = ctx->P.dy; s.dz = ctx->P.dz; s.dt = ctx->P.dt; s.tau3 = ctx->P.dt;
   s.idx = 1.0 / s.dx; s.idy = 1.0 / s.dy; s.idz = 1.0 / s.dz;
+  s.rpr = 1.0 / ctx->P.prandtl; s.rrho0 = 1.0 / ctx->P.rho0;
+  s.rprf = 1.0f / (float)ctx->P.prandtl; s.rrho0f = 1.0f / (float)ctx->P.rho0;
   s.p2 = ctx->prec == HC_FP64
              ? (pow2_spacing<double>(s.dx) && pow2_spacing<double>(s.dy) && pow2_spacing<double>(s.dz))
              : (pow2_spacing<float>(s.dx) && pow2_spacing<float>(s.dy) && pow2_spacing<float>(s.dz));

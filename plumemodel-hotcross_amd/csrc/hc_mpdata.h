@@ -28,6 +28,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "hc_common.h"
+
 #ifndef HC_HD
 #define HC_HD __host__ __device__
 #endif
@@ -83,14 +85,17 @@ struct MpPass {
 HC_HD inline double mp_fabs(double x) { return fabs(x); }
 HC_HD inline float mp_fabs(float x) { return fabsf(x); }
 
-// the antidiffusive ratios (a - b) / (a + b + eps): an IEEE division in FP64
-// (the reference's rounding); in the FP32 form the device uses the quotient
-// through the hardware reciprocal (within 2 ulp), which the form's stated
-// error against FP64 absorbs
-HC_HD inline double mp_ratio(double a, double b) { return a / b; }
+// the antidiffusive ratios (a - b) / (a + b + eps), nine per face triple.
+// FP64: ddiv, the correctly rounded quotient (the reference's rounding)
+// without the division's scaling and fix-up, which the ratios never need:
+// b = a' + b' + 1e-30 of bounded concentrations is a normal number far from
+// the overflow range.  FP32 form: the quotient through the hardware
+// reciprocal (v_rcp_f32, 1 ulp; the product within 2 ulp), which the form's
+// stated error against FP64 absorbs.
+HC_HD inline double mp_ratio(double a, double b) { return ddiv(a, b); }
 HC_HD inline float mp_ratio(float a, float b) {
 #ifdef __HIP_DEVICE_COMPILE__
-  return __fdividef(a, b);
+  return a * __builtin_amdgcn_rcpf(b);
 #else
   return a / b;
 #endif
