@@ -16,8 +16,10 @@
 //   c  : q-2 .. q+1  (4)     U, V : q, q+1 (2)     W : q-1, q (2)
 //   ud, vd : q (1)           wd, flags : q-1, q (2)
 // The values the finishing step of plane q-1 takes from the cell's own
-// column (its six face velocities and the flags above and below) roll in
-// registers from one plane to the next.
+// column (its six face velocities, the flags above and below, and c below)
+// roll in registers from one plane to the next.  Because the finishing step
+// reads only planes q-1 and q of the c ring, the next plane's staging (which
+// overwrites plane q-2's slot) needs no barrier after it.
 //
 // The face and donor formulas keep the reference's evaluation order term by
 // term, so with -ffp-contract=off the FP64 form rounds as the reference.
@@ -64,6 +66,7 @@ struct MpRegs {
   T u0, u1, v0, v1;  // ud(i-1), ud(i), vd(j-1), vd(j) at the plane being finished
   T wA, wB;          // wd at planes q-2, q-1 (the finished plane's bottom/top faces)
   int fA, fB;        // flags of the cell at planes q-2, q-1
+  T cB;              // the cell's own c at plane q-2 (below the finished plane)
 };
 
 // Arguments of one pass.
@@ -437,7 +440,7 @@ struct MpTile {
       const unsigned char *f0 = sh.fl[kk & 1];
       T cc = c0[p];
       T ce = mp_donor<T>(cc, c0[HC_MP_CHECK(p - 1)], c0[HC_MP_CHECK(p + 1)], c0[HC_MP_CHECK(p - SI)],
-                         c0[HC_MP_CHECK(p + SI)], sh.c[(kk - 1) & 3][p], sh.c[(kk + 1) & 3][p], r.u0, r.u1, r.v0,
+                         c0[HC_MP_CHECK(p + SI)], r.cB, sh.c[(kk + 1) & 3][p], r.u0, r.u1, r.v0,
                          r.v1, r.wA, r.wB, true, r.fB, f0[HC_MP_CHECKC(p - 1)], f0[HC_MP_CHECKC(p + 1)],
                          f0[HC_MP_CHECKC(p - SI)], f0[HC_MP_CHECKC(p + SI)], r.fA, nf);
       T cn = cc + ce;
@@ -463,6 +466,7 @@ struct MpTile {
     r.u0 = nu0; r.u1 = nu1; r.v0 = nv0; r.v1 = nv1;
     r.wA = r.wB; r.wB = nw;
     r.fA = r.fB; r.fB = nf;
+    r.cB = sh.c[kk & 3][p];
   }
 };
 

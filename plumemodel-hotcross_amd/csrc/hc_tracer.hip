@@ -90,7 +90,10 @@ __global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, i
   Tl::stage_initial(g, tid, pl, sh, a, qs);
   Tl::fetch(g, pl, a, qs, pf);
   for (int q = qs; q <= g.kb + 1; ++q) {
-    __syncthreads();  // the previous plane's finishing reads are done
+    // no barrier before the staging: it overwrites the slots of c(q-3),
+    // U(q-1), V(q-1), W(q-2), which the previous plane's finishing step does
+    // not read and its faces and flags phases finished reading before the
+    // barrier ahead of that step
     Tl::commit(tid, sh, q, pf);
     __syncthreads();
     if (q <= g.kb) Tl::fetch(g, pl, a, q + 1, pf);  // in flight during this plane
