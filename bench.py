@@ -6,7 +6,7 @@ roofline and the reference's OpenMP CPU path timed beside it.
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload les256]
                 [--precision fp64|fp32] [--no-cpu] [--no-fp32] [--no-adjoint]
                 [--only-adjoint] [--adjoint-workload les128] [--adjoint-steps 400]
-                [--adjoint-seg 20]
+                [--adjoint-seg 20] [--adjoint-precision fp64]
 
 A step is one pass of the reference's time loop body
 (hotcross80-revised2014.f:264-350: body force, forces, pressure solve,
@@ -99,15 +99,15 @@ def gpu_leg(args, precision=None, workload=None):
     return p, sec, cells, phases, nprof
 
 
-def adjoint_leg(workload="les128", nsteps=400, seg=20):
+def adjoint_leg(workload="les128", nsteps=400, seg=20, precision="fp64"):
     """The adjoint window of the headline adjoint configuration: nsteps steps
-    of the FP64 timestep, 25 probes on the plane 20 m above the vent sampled
+    of the timestep (FP64 unless precision is fp32), 25 probes on the plane 20 m above the vent sampled
     at 1 Hz, a checkpoint every seg steps, then the reverse sweep (segments
     recomputed into an HBM tape, adjoint of every step).  Timed beside the
     same number of plain forward steps (hc_step) from the same state."""
     import hotcross as H
     import adjoint as AD
-    p, ctx = init_context(H, workload, H.FP64)
+    p, ctx = init_context(H, workload, H.FP64 if precision == "fp64" else H.FP32)
     ctx.step(2)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -123,7 +123,8 @@ def adjoint_leg(workload="les128", nsteps=400, seg=20):
     ctx.synchronize()
     win = time.perf_counter() - t0
     ctx.close()
-    return {"workload": "%s, %d steps, checkpoint every %d, %d probes at 1 Hz" % (workload, nsteps, seg, len(probes)),
+    return {"workload": "%s %s, %d steps, checkpoint every %d, %d probes at 1 Hz" % (workload, precision, nsteps, seg,
+                                                                                  len(probes)),
             "forward_ms_per_step": round(fwd / nsteps * 1e3, 4),
             "window_ms_per_step": round(win / nsteps * 1e3, 4),
             "ratio": round((win - fwd) / fwd, 3),
@@ -196,6 +197,7 @@ def main():
     ap.add_argument("--adjoint-workload", default="les128")
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-seg", type=int, default=20)
+    ap.add_argument("--adjoint-precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-worker", nargs=5)
     args = ap.parse_args()
@@ -208,7 +210,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.only_adjoint:
-        print(json.dumps(adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg)))
+        print(json.dumps(adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg,
+                                     args.adjoint_precision)))
         return
     p, sec, cells, phases, nprof = gpu_leg(args)
     value = cells * args.steps / sec / 1e6
@@ -247,7 +250,7 @@ def main():
                 "err_vs_fp64": "tests/test_gpu_fp32.py: max error relative to max|field| after 50 steps"}
     adj = None
     if not args.no_adjoint:
-        adj = adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg)
+        adj = adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg, args.adjoint_precision)
     out = {
         "metric": "Mcell-steps/s FP64 vs FP32 at stated err vs FP64; adjoint/forward time ratio",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
