@@ -847,6 +847,13 @@ static const int NTAPE = 14;
 static const int TAPE_FIELDS[NTAPE] = {HC_U,    HC_V,   HC_W, HC_UM, HC_VM, HC_WM, HC_SS,
                                        HC_SSV,  HC_RHOP, HC_T, HC_S,  HC_UP, HC_VP, HC_WP};
 
+// a step's taped tracer intermediates: the results of passes 1 and 2 and
+// the antidiffusive velocities of passes 2 and 3
+static const int TTAPE = 8;
+struct TracerSlot {
+  void *c1, *c2, *ud2[3], *ud3[3];
+};
+
 struct HostState {
   double time;
   double acc[8];
@@ -873,6 +880,11 @@ struct AdjCtx {
   // window
   void *tape = nullptr;       // (nck + seg + 1) states of NTAPE fields
   size_t tape_slots = 0;
+  // the tracers' intermediates of the recomputed steps of a segment (steps
+  // a .. b-2), TTAPE fields per tracer (T, S) per step, so that their
+  // adjoints need not run the tracer passes once more
+  void *ttape = nullptr;
+  size_t ttape_steps = 0;
   int nprobe = 0, every = 1;
   double wgt_w = 1.0, wgt_t = 1.0;
   int *d_iw = nullptr, *d_it = nullptr;
@@ -889,7 +901,7 @@ void adj_free(hc_ctx *ctx) {
   for (void *p : A->luc) ps.push_back(p);
   for (int q = 0; q < 3; ++q) { ps.push_back(A->ud2[q]); ps.push_back(A->ud3[q]); ps.push_back(A->lud2[q]); ps.push_back(A->lud3[q]); }
   void *rest[] = {A->lamb, A->lcnew, A->cw, A->c1, A->c2, A->cpre, A->lc1, A->lc2, A->lpre, A->fl2, A->fl3, A->g,
-                  A->d_sc, A->d_part, A->d_gq, A->tape, A->d_iw, A->d_it, A->d_samp, A->d_obs};
+                  A->d_sc, A->d_part, A->d_gq, A->tape, A->ttape, A->d_iw, A->d_it, A->d_samp, A->d_obs};
   for (void *p : rest) ps.push_back(p);
   for (void *p : ps)
     if (p) hipFree(p);
@@ -1048,10 +1060,12 @@ static void adj_courant_t(hc_ctx *ctx, double tau3) {
 }
 
 // TRACER31 for field `field` at the current state: lam[field] holds the
-// adjoint of the call's output and receives the adjoint of its input
+// adjoint of the call's output and receives the adjoint of its input.  The
+// passes' intermediates come from `taped` (a window's segment recompute) or
+// are recomputed here on a copy of the entry value.
 template <typename T>
 static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3,
-                         int vent, const double *qpatch) {
+                         int vent, const double *qpatch, const TracerSlot *taped = nullptr) {
   AdjCtx *A = ctx->adj;
   const Layout &L = ctx->L;
   hipStream_t st = ctx->st;
@@ -1061,23 +1075,26 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   VentArgs va = vent_args(ctx);
   FieldPtrs<T> f = ptrs<T>(ctx);
   const T *c = (const T *)ctx->f[field];
-  // recompute the passes' intermediates on a copy of the entry value
-  hipMemcpyAsync(A->cw, c, fb, hipMemcpyDeviceToDevice, st);
-  TracerWork tw{};
-  tw.uc = ctx->tw.uc; tw.vc = ctx->tw.vc; tw.wc = ctx->tw.wc;
-  tw.ud2 = A->ud2[0]; tw.vd2 = A->ud2[1]; tw.wd2 = A->ud2[2];
-  tw.c1 = A->c1; tw.c2 = A->c2; tw.cpre = A->cpre;
-  tw.ud3 = A->ud3[0]; tw.vd3 = A->ud3[1]; tw.wd3 = A->ud3[2];
-  tw.budget_part = ctx->tw.budget_part; tw.budget = ctx->tw.budget;
-  TracerArgs ta{cmin, cmax, tau3, 0, 0.0, 0};
-  launch_tracer<T>(st, L, s, ctx->P, va, f, ctx->aux, tw, (T *)A->cw, d_cbkg, qpatch, ta);
+  TracerSlot is{A->c1, A->c2, {A->ud2[0], A->ud2[1], A->ud2[2]}, {A->ud3[0], A->ud3[1], A->ud3[2]}};
+  if (taped) {
+    is = *taped;
+  } else {
+    hipMemcpyAsync(A->cw, c, fb, hipMemcpyDeviceToDevice, st);
+    TracerWork tw{};
+    tw.uc = ctx->tw.uc; tw.vc = ctx->tw.vc; tw.wc = ctx->tw.wc;
+    tw.ud2 = is.ud2[0]; tw.vd2 = is.ud2[1]; tw.wd2 = is.ud2[2];
+    tw.c1 = is.c1; tw.c2 = is.c2; tw.cpre = A->cpre;
+    tw.ud3 = is.ud3[0]; tw.vd3 = is.ud3[1]; tw.wd3 = is.ud3[2];
+    tw.budget_part = ctx->tw.budget_part; tw.budget = ctx->tw.budget;
+    TracerArgs ta{cmin, cmax, tau3, 0, 0.0, 0};
+    launch_tracer<T>(st, L, s, ctx->P, va, f, ctx->aux, tw, (T *)A->cw, d_cbkg, qpatch, ta);
+  }
+  const T *c1 = (const T *)is.c1, *c2 = (const T *)is.c2;
   const Box3 ib = interior(L);
-  const T *ud2 = (const T *)A->ud2[0], *vd2 = (const T *)A->ud2[1], *wd2 = (const T *)A->ud2[2];
-  T *ud3 = (T *)A->ud3[0], *vd3 = (T *)A->ud3[1], *wd3 = (T *)A->ud3[2];
-  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, (const T *)A->c1, ud2, vd2, wd2, cmin,
-                     cmax, A->fl2);
-  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, (const T *)A->c2, (const T *)ud3,
-                     (const T *)vd3, (const T *)wd3, cmin, cmax, A->fl3);
+  const T *ud2 = (const T *)is.ud2[0], *vd2 = (const T *)is.ud2[1], *wd2 = (const T *)is.ud2[2];
+  const T *ud3 = (const T *)is.ud3[0], *vd3 = (const T *)is.ud3[1], *wd3 = (const T *)is.ud3[2];
+  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, c1, ud2, vd2, wd2, cmin, cmax, A->fl2);
+  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, c2, ud3, vd3, wd3, cmin, cmax, A->fl3);
   // adjoint buffers
   void *zs[] = {A->lcnew, A->lpre, A->lc1, A->lc2, A->lud2[0], A->lud2[1], A->lud2[2], A->lud3[0], A->lud3[1], A->lud3[2]};
   for (void *p : zs) zero(ctx, p);
@@ -1106,18 +1123,17 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   };
   const Box3 full{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
   hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, dl, (const T *)lout,
-                     (const T *)A->lpre, (const T *)A->c2, (const T *)ud3, (const T *)vd3, (const T *)wd3,
-                     (const uint8_t *)A->fl3, (T *)A->lc2, (T *)A->lud3[0], (T *)A->lud3[1], (T *)A->lud3[2],
-                     (T *)A->lcnew);
-  anti((const T *)A->c2, ud2, vd2, wd2, (const T *)A->lud3[0], (const T *)A->lud3[1], (const T *)A->lud3[2],
-       (T *)A->lc2, (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2]);
+                     (const T *)A->lpre, c2, ud3, vd3, wd3, (const uint8_t *)A->fl3, (T *)A->lc2, (T *)A->lud3[0],
+                     (T *)A->lud3[1], (T *)A->lud3[2], (T *)A->lcnew);
+  anti(c2, ud2, vd2, wd2, (const T *)A->lud3[0], (const T *)A->lud3[1], (const T *)A->lud3[2], (T *)A->lc2,
+       (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2]);
   hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc2, (T *)A->lc2);
   // pass 2
   DonorLast d2{};
   hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, d2, (const T *)A->lc2,
-                     (const T *)nullptr, (const T *)A->c1, ud2, vd2, wd2, (const uint8_t *)A->fl2, (T *)A->lc1,
+                     (const T *)nullptr, c1, ud2, vd2, wd2, (const uint8_t *)A->fl2, (T *)A->lc1,
                      (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2], (T *)nullptr);
-  anti((const T *)A->c1, (const T *)ctx->tw.uc, (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)A->lud2[0],
+  anti(c1, (const T *)ctx->tw.uc, (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)A->lud2[0],
        (const T *)A->lud2[1], (const T *)A->lud2[2], (T *)A->lc1, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2]);
   hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc1, (T *)A->lc1);
   // pass 1
@@ -1395,12 +1411,46 @@ static void step_head(hc_ctx *ctx, int m) {
   hc_work_velocity(ctx, ctx->P.dt);
 }
 
+// the taped intermediates of tracer `tr` (0: T, 1: S) of the segment's step
+// index `q`
+static TracerSlot tracer_slot(hc_ctx *ctx, int q, int tr) {
+  const size_t fb = ctx->L.n * ctx->esz;
+  char *b = (char *)ctx->adj->ttape + ((size_t)q * 2 + tr) * TTAPE * fb;
+  TracerSlot t;
+  t.c1 = b;
+  t.c2 = b + fb;
+  for (int d = 0; d < 3; ++d) {
+    t.ud2[d] = b + (2 + d) * fb;
+    t.ud3[d] = b + (5 + d) * fb;
+  }
+  return t;
+}
+
+// a tracer of the native loop whose passes leave their intermediates in
+// slot `t` (and the pre-sponge boundary values in the adjoint's scratch)
+static void taped_tracer(hc_ctx *ctx, int field, const TracerSlot &t) {
+  const TracerWork keep = ctx->tw;
+  ctx->tw.c1 = t.c1; ctx->tw.c2 = t.c2; ctx->tw.cpre = ctx->adj->cpre;
+  ctx->tw.ud2 = t.ud2[0]; ctx->tw.vd2 = t.ud2[1]; ctx->tw.wd2 = t.ud2[2];
+  ctx->tw.ud3 = t.ud3[0]; ctx->tw.vd3 = t.ud3[1]; ctx->tw.wd3 = t.ud3[2];
+  native_tracer(ctx, field);
+  ctx->tw = keep;
+}
+
 // one forward step of the window (the native loop without the passive
-// tracer c, which no loss here depends on, and without the CFL check)
-static void fwd_step(hc_ctx *ctx, int m) {
+// tracer c, which no loss here depends on, and without the CFL check);
+// tq >= 0 tapes the tracers' intermediates at segment step index tq
+static void fwd_step(hc_ctx *ctx, int m, int tq = -1) {
   step_head(ctx, m);
-  if (ctx->P.toggle_t == 1) native_tracer(ctx, HC_T);
-  if (ctx->P.toggle_s == 1) native_tracer(ctx, HC_S);
+  const bool tape = tq >= 0 && ctx->adj && ctx->adj->ttape;
+  if (ctx->P.toggle_t == 1) {
+    if (tape) taped_tracer(ctx, HC_T, tracer_slot(ctx, tq, 0));
+    else native_tracer(ctx, HC_T);
+  }
+  if (ctx->P.toggle_s == 1) {
+    if (tape) taped_tracer(ctx, HC_S, tracer_slot(ctx, tq, 1));
+    else native_tracer(ctx, HC_S);
+  }
   if (ctx->P.bit3 == 1) hc_density(ctx);
   hc_refresh(ctx);
   ctx->next_step = m + 1;
@@ -1410,7 +1460,7 @@ static void fwd_step(hc_ctx *ctx, int m) {
 // recomputed head (step_head from its taped state); t1/s1 are T and S after
 // the step, ss0/ssv0 the viscosities at its start
 template <typename T>
-static void adj_step(hc_ctx *ctx, int m, const T *t1, const T *s1, const T *ss0, const T *ssv0) {
+static void adj_step(hc_ctx *ctx, int m, const T *t1, const T *s1, const T *ss0, const T *ssv0, int tq = -1) {
   (void)m;
   const hc_params &P = ctx->P;
   adj_refresh_t<T>(ctx);
@@ -1418,10 +1468,14 @@ static void adj_step(hc_ctx *ctx, int m, const T *t1, const T *s1, const T *ss0,
   double tmin, smax;
   native_bounds(ctx, &tmin, &smax);
   const int nz = P.nz;
+  const bool taped = tq >= 0 && ctx->adj->ttape;
+  TracerSlot ts[2];
+  if (taped)
+    for (int tr = 0; tr < 2; ++tr) ts[tr] = tracer_slot(ctx, tq, tr);
   if (P.toggle_s == 1)
-    adj_tracer_t<T>(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, ctx->d_q[1]);
+    adj_tracer_t<T>(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, ctx->d_q[1], taped ? &ts[1] : nullptr);
   if (P.toggle_t == 1)
-    adj_tracer_t<T>(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, ctx->d_q[0]);
+    adj_tracer_t<T>(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, ctx->d_q[0], taped ? &ts[0] : nullptr);
   (void)nz;
   adj_courant_t<T>(ctx, P.dt);
   adj_viscosity_t<T>(ctx, (const T *)ctx->f[HC_SS], (const T *)ctx->f[HC_SSV]);
@@ -1471,6 +1525,23 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
     A->nsamp_cap = ns;
   }
   if (ns) HIPCHK(hipMemcpyAsync(A->d_obs, obs, ns * 8, hipMemcpyHostToDevice, ctx->st));
+  // the tracers' intermediates of a segment's recomputed steps; without room
+  // for them the adjoint recomputes the tracer passes step by step
+  const size_t tsteps = (size_t)std::max(seg - 1, 0);
+  if (A->ttape_steps < tsteps) {
+    if (A->ttape) { hipStreamSynchronize(ctx->st); hipFree(A->ttape); A->ttape = nullptr; A->ttape_steps = 0; }
+    // zeroed once: the passes leave the ghost faces of the antidiffusive
+    // velocities unwritten, and the adjoint kernels read them (with zero
+    // weight), so they must hold finite values
+    const size_t tb = tsteps * 2 * TTAPE * ctx->L.n * ctx->esz;
+    if (hipMalloc(&A->ttape, tb) == hipSuccess) {
+      A->ttape_steps = tsteps;
+      HIPCHK(hipMemsetAsync(A->ttape, 0, tb, ctx->st));
+    } else {
+      A->ttape = nullptr;
+      (void)hipGetLastError();
+    }
+  }
   const size_t tape0 = (size_t)nck + 1;  // first slot of the segment tape
   std::vector<HostState> ck(nck + 1), tp(seg + 1);
   // forward with checkpoints
@@ -1508,7 +1579,7 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
     set_host_state(ctx, ck[sg]);
     tp[0] = ck[sg];
     for (int n = a; n < b - 1; ++n) {
-      fwd_step(ctx, m0 + n);
+      fwd_step(ctx, m0 + n, n - a);
       save_slot(ctx, tape0 + (n + 1 - a));
       tp[n + 1 - a] = host_state(ctx);
     }
@@ -1524,8 +1595,10 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
       set_host_state(ctx, tp[n - a]);
       step_head(ctx, m0 + n);
       const size_t s1 = slot_of(n + 1), s0 = slot_of(n);
+      // step b-1 is not recomputed (its end state is X_b): its tracers'
+      // intermediates are recomputed in the adjoint
       adj_step<T>(ctx, m0 + n, (const T *)slot_ptr(ctx, s1, 9), (const T *)slot_ptr(ctx, s1, 10),
-                  (const T *)slot_ptr(ctx, s0, 6), (const T *)slot_ptr(ctx, s0, 7));
+                  (const T *)slot_ptr(ctx, s0, 6), (const T *)slot_ptr(ctx, s0, 7), n < b - 1 ? n - a : -1);
     }
   }
   // the window's end state (to continue the run) or its initial state (to
