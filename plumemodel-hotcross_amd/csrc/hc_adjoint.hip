@@ -1355,8 +1355,8 @@ extern "C" int hc_adj_set_probes(hc_ctx *ctx, int32_t nprobe, const int32_t *ijk
   int rc = 0;
   if (!(A->d_iw = (int *)dalloc(ctx, (size_t)nprobe * 4, &rc))) return rc;
   if (!(A->d_it = (int *)dalloc(ctx, (size_t)nprobe * 4, &rc))) return rc;
-  HIPCHK(hipMemcpy(A->d_iw, iw.data(), (size_t)nprobe * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(A->d_it, it.data(), (size_t)nprobe * 4, hipMemcpyHostToDevice));
+  HIPCHK(h2d(ctx, A->d_iw, iw.data(), (size_t)nprobe * 4));
+  HIPCHK(h2d(ctx, A->d_it, it.data(), (size_t)nprobe * 4));
   A->nprobe = nprobe;
   A->every = every;
   A->wgt_w = wgt_w;

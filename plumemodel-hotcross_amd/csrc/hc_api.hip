@@ -198,7 +198,7 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
       for (int m = 0; m < N; ++m) { tw[o++] = cos(2 * pi * m / N); tw[o++] = -sin(2 * pi * m / N); }
       for (int k = 0; k < N; ++k) { tw[o++] = cos(pi * k / (2.0 * N)); tw[o++] = sin(pi * k / (2.0 * N)); }
     }
-    HIPCHK(hipMemcpy(ctx->d_tw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(h2d(ctx, ctx->d_tw, tw.data(), tw.size() * 8));
     ctx->pw.tw_x = ctx->d_tw;
     ctx->pw.tw_y = ctx->d_tw + 4 * P.nx;
     // pivots of a*x(k-1) + (b_k + lambda)*x(k) + a*x(k+1) = g(k), b = -2a, b_1 = b_nz = -a,
@@ -220,7 +220,7 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
           piv[((size_t)k * P.ny + ky) * P.nx + kx] = d;
         }
       }
-    HIPCHK(hipMemcpy(ctx->pw.piv, piv.data(), nc * 8, hipMemcpyHostToDevice));
+    HIPCHK(h2d(ctx, ctx->pw.piv, piv.data(), nc * 8));
   }
   double pi = 4.0 * atan2(1.0, 1.0);
   ctx->fc = 2.0 * P.omega * sin(pi * P.lat / 180.0);
@@ -354,10 +354,10 @@ extern "C" int hc_set_profiles(hc_ctx *ctx, const double *t_bkg, const double *s
   ctx->h_sbkg.assign(s_bkg, s_bkg + nz2);
   ctx->h_rho_bkg.assign(rho_bkg, rho_bkg + nz2);
   ctx->h_azzmin.assign(azzmin, azzmin + nz2);
-  HIPCHK(hipMemcpy(ctx->d_tbkg, t_bkg, nz2 * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_sbkg, s_bkg, nz2 * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->aux.rho_bkg, rho_bkg, nz2 * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->aux.azzmin, azzmin, nz2 * 8, hipMemcpyHostToDevice));
+  HIPCHK(h2d(ctx, ctx->d_tbkg, t_bkg, nz2 * 8));
+  HIPCHK(h2d(ctx, ctx->d_sbkg, s_bkg, nz2 * 8));
+  HIPCHK(h2d(ctx, ctx->aux.rho_bkg, rho_bkg, nz2 * 8));
+  HIPCHK(h2d(ctx, ctx->aux.azzmin, azzmin, nz2 * 8));
   return HC_OK;
 }
 
@@ -378,11 +378,11 @@ extern "C" int hc_set_sponge(hc_ctx *ctx, const double *alphau, const double *al
       ctx->h_alphae0[i + (nx + 2) * j] =
           fmax((alphau[(i + 1) + (nx + 3) * j] + alphau[i + (nx + 3) * j]),
                (alphav[i + (nx + 2) * (j + 1)] + alphav[i + (nx + 2) * j])) / 2.0;
-  HIPCHK(hipMemcpy(ctx->aux.alphau, alphau, nu * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->aux.alphav, alphav, nv * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->aux.alphae, alphae, ne * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->aux.alphae0, ctx->h_alphae0.data(), ne * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->aux.alphaw, alphaw, (nz + 3) * 8, hipMemcpyHostToDevice));
+  HIPCHK(h2d(ctx, ctx->aux.alphau, alphau, nu * 8));
+  HIPCHK(h2d(ctx, ctx->aux.alphav, alphav, nv * 8));
+  HIPCHK(h2d(ctx, ctx->aux.alphae, alphae, ne * 8));
+  HIPCHK(h2d(ctx, ctx->aux.alphae0, ctx->h_alphae0.data(), ne * 8));
+  HIPCHK(h2d(ctx, ctx->aux.alphaw, alphaw, (nz + 3) * 8));
   update_sums(ctx);
   return HC_OK;
 }

@@ -215,6 +215,17 @@ static inline void fill(hc_ctx *ctx, void *p, double v) {
   else launch_fill<float>(ctx, p, v);
 }
 
+// host -> device copy ordered on the context stream, complete on return.
+// dalloc clears new buffers with an asynchronous memset on that stream, and
+// the stream does not wait for the null stream (nor the null stream for
+// it), so a synchronous hipMemcpy could land before a pending clear of the
+// same buffer and be wiped.
+static inline hipError_t h2d(hc_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->st);
+}
+
 static inline void *dalloc(hc_ctx *ctx, size_t bytes, int *rc) {
   void *p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) {
