@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(256) k_dct_y(Layout L, FftPlan pl, int npair, 
 // with all its threads
 constexpr int TRI_MAXZ = 1024;
 
-template <typename T>
+template <typename T, int KB>
 __global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const double *__restrict__ piv,
                                                  T *__restrict__ buf, double *__restrict__ y00) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -421,7 +421,6 @@ __global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const doub
   if (q == 0 || q >= plane) return;
   // the sweeps are sequential in k; loads are issued in batches of KB levels
   // ahead of the recurrence so their latency overlaps
-  constexpr int KB = 8;
   T y = (T)0;
   for (int k0 = 0; k0 < nz; k0 += KB) {
     T g[KB], d[KB];
@@ -820,8 +819,21 @@ void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const P
   int gy = nz * ((nx + 2 * npy - 1) / (2 * npy));
   if (!(fast && launch_dcty_fast<T, 0>(st, L, twy, cty, r)))
     hipLaunchKernelGGL((k_dct_y<T, 0>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
-  hipLaunchKernelGGL(k_tridiag<T>, dim3((unsigned)(((size_t)nx * ny + 255) / 256)), dim3(256), 0, st, L, s.dz,
-                     pw.piv, r, pw.part);
+  {
+    // levels per batch of loads in the z sweeps: 32 in FP64 (pressure 0.77
+    // -> 0.69 ms per step at 256^3 against 8), 16 in FP32 (no difference
+    // measured); HC_TRI_KB = 8, 16 or 32 overrides
+    static const int kb_env = [] {
+      const char *e = getenv("HC_TRI_KB");
+      const int v = e ? atoi(e) : 0;
+      return (v == 8 || v == 16 || v == 32) ? v : 0;
+    }();
+    const int kb = kb_env ? kb_env : (sizeof(T) == 8 ? 32 : 16);
+    const dim3 tg((unsigned)(((size_t)nx * ny + 255) / 256));
+    if (kb == 8) hipLaunchKernelGGL((k_tridiag<T, 8>), tg, dim3(256), 0, st, L, s.dz, pw.piv, r, pw.part);
+    else if (kb == 32) hipLaunchKernelGGL((k_tridiag<T, 32>), tg, dim3(256), 0, st, L, s.dz, pw.piv, r, pw.part);
+    else hipLaunchKernelGGL((k_tridiag<T, 16>), tg, dim3(256), 0, st, L, s.dz, pw.piv, r, pw.part);
+  }
   if (!(fast && launch_dcty_fast<T, 1>(st, L, twy, cty, r)))
     hipLaunchKernelGGL((k_dct_y<T, 1>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
   if (!(fast && launch_dctx_fast<T, 1>(st, L, twx, ctx, r, (T *)nullptr, pw.delta, s, ph, f.p, f.fx, f.w, f.wm,
