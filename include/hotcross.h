@@ -276,6 +276,12 @@ int hc_adj_gradients(hc_ctx *ctx, double *gq, double out[3]);
  * window again); gq and gs as hc_adj_gradients. */
 int hc_adj_set_probes(hc_ctx *ctx, int32_t nprobe, const int32_t *ijk, int32_t every, double wgt_w,
                       double wgt_t);
+/* HBM (bytes) the checkpoints and tapes of hc_adj_window(nsteps, seg) take
+ * beyond the context's state and adjoint fields; for choosing seg under a
+ * memory budget (the recompute is one forward step per step for any seg,
+ * so the smallest footprint, near seg = sqrt(nsteps * 14 / 30), is the
+ * usual choice). */
+int hc_adj_window_bytes(hc_ctx *ctx, int32_t nsteps, int32_t seg, uint64_t *bytes);
 int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples, double *loss,
                   double *gq, double gs[3], int32_t end_state);
 

@@ -77,3 +77,18 @@ def plane_probes(p, n=5, spacing=2, height_m=20.0):
 def samples_every(p):
     """Steps between 1 Hz samples."""
     return max(1, int(round(1.0 / p.dt)))
+
+
+def plan_segment(ctx, nsteps, budget_bytes):
+    """The checkpoint interval whose checkpoints and tapes take the least
+    HBM, and that amount; raises if even that exceeds budget_bytes.  Every
+    interval recomputes one forward step per step, so the smallest
+    footprint costs nothing in time (with memory for every step's tape the
+    recompute could be dropped; at the sizes here it cannot)."""
+    best = min(range(1, nsteps + 1), key=lambda s: ctx.adj_window_bytes(nsteps, s))
+    need = ctx.adj_window_bytes(nsteps, best)
+    if need > budget_bytes:
+        raise ValueError("the window needs %.1f GB at best (checkpoint every %d steps), over the %.1f GB budget"
+                         % (need / 1e9, best, budget_bytes / 1e9))
+    return best, need
+

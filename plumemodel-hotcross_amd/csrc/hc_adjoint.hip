@@ -1611,6 +1611,17 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   return hc_adj_gradients(ctx, gq, gs);
 }
 
+// HBM the window's checkpoints and tapes take (beyond the context's own
+// state and the adjoint fields): (nck + 1 + seg) states of NTAPE fields and
+// (seg - 1) steps of TTAPE fields for each of the two tracers
+extern "C" int hc_adj_window_bytes(hc_ctx *ctx, int32_t nsteps, int32_t seg, uint64_t *bytes) {
+  if (!ctx || !bytes || nsteps < 1 || seg < 1) return fail(ctx, HC_EINVAL, "nsteps and seg must be positive");
+  const uint64_t fb = (uint64_t)ctx->L.n * ctx->esz;
+  const uint64_t nck = (uint64_t)((nsteps + seg - 1) / seg);
+  *bytes = (nck + 1 + (uint64_t)seg) * NTAPE * fb + (uint64_t)(seg - 1) * 2 * TTAPE * fb;
+  return HC_OK;
+}
+
 extern "C" int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples,
                              double *loss, double *gq, double gs[3], int32_t end_state) {
   int r = adj_need(ctx);

@@ -279,6 +279,12 @@ module hotcross_capi
       integer(c_int32_t), intent(in) :: ijk(3, *)
       real(c_double), value :: wgt_w, wgt_t
     end function
+    integer(c_int) function hc_adj_window_bytes(ctx, nsteps, seg, bytes) bind(C, name="hc_adj_window_bytes")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps, seg
+      integer(c_int64_t), intent(out) :: bytes
+    end function
     integer(c_int) function hc_adj_window(ctx, nsteps, seg, obs, samples, loss, gq, gs, end_state) &
         bind(C, name="hc_adj_window")
       import :: c_int, c_ptr, c_int32_t, c_double

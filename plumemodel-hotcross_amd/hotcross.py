@@ -115,6 +115,7 @@ def lib():
             "hc_adj_gradients": [vp, dp, dp],
             "hc_adj_set_probes": [vp, i32, dp, i32, dbl, dbl],
             "hc_adj_window": [vp, i32, i32, dp, dp, dp, dp, dp, i32],
+            "hc_adj_window_bytes": [vp, i32, i32, dp],
         }
         for n, a in sig.items():
             f = getattr(L, n)
@@ -445,6 +446,12 @@ class Context:
         self._probes = len(a)
         self._every = int(every)
         self._check(self.L.hc_adj_set_probes(self.h, len(a), _arr(a), int(every), float(wgt_w), float(wgt_t)))
+
+    def adj_window_bytes(self, nsteps, seg):
+        """HBM bytes of the window's checkpoints and tapes."""
+        b = np.zeros(1, dtype=np.uint64)
+        self._check(self.L.hc_adj_window_bytes(self.h, nsteps, seg, _arr(b)))
+        return int(b[0])
 
     def adj_window(self, nsteps, seg, obs, end_state=True):
         """Forward nsteps with a checkpoint every seg steps, probe-misfit loss
