@@ -242,6 +242,8 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
                 ctx->pw.delta, ctx->d_tw};
   for (void *p : ps)
     if (p) hipFree(p);
+  for (void *p : ctx->c1x)
+    if (p) hipFree(p);
   for (void *p : ctx->spare)
     if (p) hipFree(p);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
@@ -631,20 +633,61 @@ static void mark_pass(void *arg, int end) {
   else phase_end(ctx, ctx->mark_q);
 }
 
+// c1: if set, holds the tracer's pass-1 result already (launch_pass1_multi)
 static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3, int vent,
-                  double vent_value, int want_budget) {
+                  double vent_value, int want_budget, void *c1 = nullptr) {
   PhaseScope ps(ctx, "tracer31");
   int which = field == HC_T ? 0 : field == HC_S ? 1 : 2;
   StepArgs s = step_args(ctx);
   VentArgs va = vent_args(ctx);
-  TracerArgs ta{cmin, cmax, tau3, vent, vent_value, want_budget};
+  TracerArgs ta{cmin, cmax, tau3, vent, vent_value, want_budget, c1 != nullptr};
+  TracerWork tw = ctx->tw;
+  if (c1) tw.c1 = c1;
   MarkFn mk = ctx->timing ? mark_pass : nullptr;
   if (ctx->prec == HC_FP64)
-    launch_tracer<double>(ctx->st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, ctx->tw,
+    launch_tracer<double>(ctx->st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, tw,
                           (double *)ctx->f[field], d_cbkg, ctx->d_q[which], ta, mk, ctx);
   else
-    launch_tracer<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, ctx->tw,
+    launch_tracer<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, tw,
                          (float *)ctx->f[field], d_cbkg, ctx->d_q[which], ta, mk, ctx);
+  return HC_OK;
+}
+
+// pass 1 of the native loop's tracers (T, S, C as switched on) in one launch;
+// c1[field - HC_T...] receives where each result went, nullptr for the rest
+static int tracers_pass1(hc_ctx *ctx, double tau3, void *c1[3]) {
+  const hc_params &P = ctx->P;
+  const int on[3] = {P.toggle_t == 1, P.toggle_s == 1, P.toggle_c == 1};
+  const int fid[3] = {HC_T, HC_S, HC_C};
+  const double *bk[3] = {ctx->d_tbkg, ctx->d_sbkg, ctx->d_zero};
+  void *dst[3] = {ctx->tw.c1, nullptr, nullptr};
+  for (int q = 1; q < 3; ++q) {
+    if (!on[q]) continue;
+    if (!ctx->c1x[q - 1]) {
+      int rc = 0;
+      if (!(ctx->c1x[q - 1] = dalloc(ctx, ctx->L.n * ctx->esz, &rc))) return rc;
+    }
+    dst[q] = ctx->c1x[q - 1];
+  }
+  void *c[3], *o[3];
+  const double *b[3];
+  int n = 0;
+  for (int q = 0; q < 3; ++q) {
+    c1[q] = on[q] ? dst[q] : nullptr;
+    if (!on[q]) continue;
+    c[n] = ctx->f[fid[q]]; b[n] = bk[q]; o[n] = dst[q];
+    ++n;
+  }
+  if (!n) return HC_OK;
+  PhaseScope ps(ctx, "tracer31");
+  StepArgs s = step_args(ctx);
+  VentArgs va = vent_args(ctx);
+  if (ctx->prec == HC_FP64)
+    launch_pass1_multi<double>(ctx->st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, ctx->tw, n,
+                               (double *const *)c, b, (double *const *)o, tau3);
+  else
+    launch_pass1_multi<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, ctx->tw, n,
+                              (float *const *)c, b, (float *const *)o, tau3);
   return HC_OK;
 }
 
@@ -802,13 +845,22 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
       hc_source_terms(ctx, nullptr);
     }
     int budgets = (m % 100 == 0);
+    // with fusion, pass 1 of T, S and C runs first, in one launch: it reads
+    // the tracers at the step's start and the shared Courant numbers and
+    // viscosities, none of which the T and S calls or DENSITY change
+    void *c1[3] = {nullptr, nullptr, nullptr};
+    if (fuse) {
+      int r = tracers_pass1(ctx, P.dt, c1);
+      if (r) return r;
+    }
     if (P.toggle_t == 1) {
       if (!fuse) hc_work_velocity(ctx, P.dt);
-      tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], budgets);
+      tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], budgets, c1[0]);
     }
-    if (P.toggle_s == 1) tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], budgets);
+    if (P.toggle_s == 1)
+      tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], budgets, c1[1]);
     if (P.bit3 == 1) hc_density(ctx);
-    if (P.toggle_c == 1) tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, budgets);
+    if (P.toggle_c == 1) tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, budgets, c1[2]);
     if (fuse) {
       PhaseScope ps(ctx, "refresh");
       refresh_rotate(ctx);

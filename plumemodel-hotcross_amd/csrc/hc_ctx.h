@@ -73,6 +73,7 @@ struct hc_ctx {
   // filtered fields (into spare[c]); HC_NO_FUSE=1 at creation turns it off
   bool fuse = true;
   void *spare[3] = {nullptr, nullptr, nullptr};
+  void *c1x[2] = {nullptr, nullptr};  // pass-1 results of S and C in the native loop
   // controls that the adjoint differentiates: a per-cell multiplier of the
   // vent heat flux (null: 1 everywhere), the crossflow amplitude (a factor on
   // the series of site_forcing.inc) and the Smagorinsky constants in P

@@ -82,6 +82,7 @@ struct TracerArgs {
   int vent;          // apply the driver's vent ghost override after the call
   double vent_value;
   int budgets;       // mod(mtime,100) == 0
+  int pass1_done;    // tw.c1 already holds pass 1's result with its boundary images
 };
 template <typename T>
 void launch_courant(hipStream_t st, const Layout &L, const StepArgs &s, const FieldPtrs<T> &f, TracerWork &tw);
@@ -92,6 +93,12 @@ template <typename T>
 void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const VentArgs &va,
                    const FieldPtrs<T> &f, const Aux &aux, TracerWork &tw, T *c, const double *c_bkg,
                    const double *qpatch, const TracerArgs &ta, MarkFn mark = nullptr, void *mark_arg = nullptr);
+// pass 1 of n <= 3 tracers c[q] (backgrounds c_bkg[q]) into c1[q], boundary
+// images included, in one launch that stages the shared inputs once
+template <typename T>
+void launch_pass1_multi(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const VentArgs &va,
+                        const FieldPtrs<T> &f, const Aux &aux, const TracerWork &tw, int n, T *const *c,
+                        const double *const *c_bkg, T *const *c1, double tau3);
 template <typename T>
 void launch_vent_ghost(hipStream_t st, const Layout &L, const VentArgs &va, T *c, double value);
 template <typename T>

@@ -67,6 +67,105 @@ __global__ void __launch_bounds__(TI *TJ) k_pass1_tile(Layout L, StepArgs s, Phy
   });
 }
 
+// Pass 1 of NC tracers in one k-march.  The tracers of a native step read the
+// same Courant numbers and viscosities, so uc, vc, wc, ss and ss_v are staged
+// once for all of them: 5 + NC field moves in and NC out per cell instead of
+// 7 per tracer.  LDS holds planes k and k+1 of every field with a one-cell
+// halo in i and j; pass 1 reads plane k-1 only in the cell's own column
+// (c, wc, ss_v), so that value rolls in a register.
+template <typename T, int TI, int TJ, int NF>
+struct Ring2 {
+  static constexpr int SI = TI + 2, S = SI * (TJ + 2), NT = TI * TJ, RS = (S + NT - 1) / NT;
+  T f[NF][2][S];
+};
+
+// planes k (p0) and k+1 (pp) in LDS; m = the own column's value at plane k-1
+// (a read at plane k-1 must be at the own column)
+template <typename T, int SI>
+struct R2View {
+  const T *p0, *pp;
+  T m;
+  int k, ib, jb;
+  __device__ T operator()(int i, int j, int kk) const {
+    if (kk < k) return m;
+    return (kk > k ? pp : p0)[(i - ib) + SI * (j - jb)];
+  }
+};
+
+template <typename T>
+struct Pass1Set {
+  const T *src[8];       // uc, vc, wc, ss, ss_v, then the tracers
+  const double *cbkg[3];
+  T *out[3];
+};
+
+template <typename T, int TI, int TJ, int NC>
+__global__ void __launch_bounds__(TI *TJ) k_pass1_multi(Layout L, StepArgs s, Phys ph, Pass1Set<T> set,
+                                                        const double *__restrict__ azzmin, int kc, int remap) {
+  constexpr int NF = 5 + NC;
+  typedef Ring2<T, TI, TJ, NF> R;
+  constexpr int SI = R::SI, S = R::S, NT = R::NT, RS = R::RS;
+  __shared__ R sh;
+  unsigned bx, by, bz;
+  box_block_coords(remap, bx, by, bz);
+  const int i0 = 1 + (int)bx * TI, j0 = 1 + (int)by * TJ;
+  const int ka = 1 + (int)bz * kc, kb = min(L.nz, ka + kc - 1);
+  const int tid = (int)threadIdx.x;
+  int so[RS];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int e = tid + r * NT;
+    const int i = i0 - 1 + e % SI, j = j0 - 1 + e / SI;
+    so[r] = (e < S && i <= L.nx + 1 && j <= L.ny + 1) ? (int)IX(L, i, j, -1) : -1;
+  }
+  const long pstride = (long)L.px * L.py;
+  T pf[NF][RS];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) pf[f][r] = so[r] >= 0 ? set.src[f][so[r] + (k + 1) * pstride] : (T)0;
+  };
+  auto commit = [&](int k) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) {
+        const int e = tid + r * NT;
+        if (e < S) sh.f[f][k & 1][e] = pf[f][r];
+      }
+  };
+  const int i = i0 + tid % TI, j = j0 + tid / TI;
+  const bool own = i <= L.nx && j <= L.ny;
+  const int cp = (tid % TI + 1) + SI * (tid / TI + 1);  // own cell in the LDS plane
+  T pm[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) pm[f] = own ? set.src[f][IX(L, i, j, ka - 1)] : (T)0;
+  fetch(ka);
+  commit(ka);
+  fetch(ka + 1);  // ka + 1 <= nz + 1
+  commit(ka + 1);
+  if (ka + 2 <= kb + 1) fetch(ka + 2);
+  __syncthreads();
+  for (int k = ka; k <= kb; ++k) {
+    if (own) {
+      auto view = [&](int f) { return R2View<T, SI>{sh.f[f][k & 1], sh.f[f][(k + 1) & 1], pm[f], k, i0 - 1, j0 - 1}; };
+#pragma unroll
+      for (int q = 0; q < NC; ++q)
+        F3(set.out[q], i, j, k) = pass1_at<T>(L, s, ph, view(5 + q), view(0), view(1), view(2), view(3), view(4),
+                                              azzmin, set.cbkg[q], i, j, k);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) pm[f] = sh.f[f][k & 1][cp];
+    }
+    if (k < kb) {
+      __syncthreads();  // every read of plane k's slot is done
+      commit(k + 2);    // k + 2 <= kb + 1
+      __syncthreads();
+      if (k + 3 <= kb + 1) fetch(k + 3);  // in flight during plane k + 1
+    }
+  }
+}
+
 // one corrective pass (antidiffusive velocities, limiter flags, limited
 // donor-cell step; sponge and source on the last pass) as a k-marching tile
 template <typename T, int TI, int TJ>
@@ -254,14 +353,14 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
   const unsigned nbc = (unsigned)((nghost + 255) / 256);
   T *c1 = (T *)tw.c1, *c2 = (T *)tw.c2;
-  {
+  if (!ta.pass1_done) {
     constexpr int TI = 32, TJ = 8;
     const int kc = L.nz < 32 ? L.nz : 32;
     dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
     hipLaunchKernelGGL((k_pass1_tile<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s2, ph, c, (T *)tw.uc,
                        (T *)tw.vc, (T *)tw.wc, f.ss, f.ssv, aux.azzmin, c_bkg, c1, kc, tile_remap());
   }
-  hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c1, c1, 0, 0.0);
+  if (!ta.pass1_done) hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c1, c1, 0, 0.0);
   if (ph.ismol != 1) {
     // no corrective passes: sponge/source straight after pass 1 through the
     // limited kernel with zero antidiffusive velocities would change nothing;
@@ -298,6 +397,39 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
     hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, st, nb, tw.budget_part, tw.budget,
                        s.dx * s.dy * s.dz, s.dx * s.dy * s.dz * ta.tau3);
   }
+}
+
+template <typename T, int NC>
+static void launch_pass1_set(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph,
+                             const Pass1Set<T> &set, const double *azzmin) {
+  constexpr int TI = 32, TJ = 8;
+  const int kc = L.nz < 32 ? L.nz : 32;
+  dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
+  hipLaunchKernelGGL((k_pass1_multi<T, TI, TJ, NC>), grid, dim3(TI * TJ), 0, st, L, s, ph, set, azzmin, kc,
+                     tile_remap());
+}
+
+template <typename T>
+void launch_pass1_multi(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const VentArgs &va,
+                        const FieldPtrs<T> &f, const Aux &aux, const TracerWork &tw, int n, T *const *c,
+                        const double *const *c_bkg, T *const *c1, double tau3) {
+  if (n < 1 || n > 3) return;
+  StepArgs s2 = s;
+  s2.tau3 = tau3;
+  Pass1Set<T> set{};
+  set.src[0] = (const T *)tw.uc; set.src[1] = (const T *)tw.vc; set.src[2] = (const T *)tw.wc;
+  set.src[3] = f.ss; set.src[4] = f.ssv;
+  for (int q = 0; q < n; ++q) {
+    set.src[5 + q] = c[q];
+    set.cbkg[q] = c_bkg[q];
+    set.out[q] = c1[q];
+  }
+  if (n == 1) launch_pass1_set<T, 1>(st, L, s2, ph, set, aux.azzmin);
+  else if (n == 2) launch_pass1_set<T, 2>(st, L, s2, ph, set, aux.azzmin);
+  else launch_pass1_set<T, 3>(st, L, s2, ph, set, aux.azzmin);
+  const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
+  const unsigned nbc = (unsigned)((nghost + 255) / 256);
+  for (int q = 0; q < n; ++q) hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c1[q], c1[q], 0, 0.0);
 }
 
 template <typename T>
@@ -391,6 +523,9 @@ void launch_cfl(hipStream_t st, const Layout &L, const FieldPtrs<T> &f, double *
   template void launch_tracer<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,                 \
                                  const VentArgs &, const FieldPtrs<T> &, const Aux &, TracerWork &, T *,      \
                                  const double *, const double *, const TracerArgs &, MarkFn, void *);         \
+  template void launch_pass1_multi<T>(hipStream_t, const Layout &, const StepArgs &, const Phys &,            \
+                                      const VentArgs &, const FieldPtrs<T> &, const Aux &, const TracerWork &,   \
+                                      int, T *const *, const double *const *, T *const *, double);             \
   template void launch_vent_ghost<T>(hipStream_t, const Layout &, const VentArgs &, T *, double);             \
   template void launch_cfl<T>(hipStream_t, const Layout &, const FieldPtrs<T> &, double *, double *);
 HC_INST_T(double)

@@ -42,12 +42,30 @@ def test_headline_size_projection_and_finiteness():
     assert abs(pr.mean()) <= 1e-12 * max(np.abs(pr).max(), 1e-300)
 
 
-def test_native_loop_equals_subroutine_path():
+def _grid(name):
     p = H.workload_params("les64")
+    if name == "ragged":
+        # tiles of 32 x 8 cells and 32-plane chunks left partly empty
+        p.nx, p.ny, p.nz = 72, 36, 40
+        p.zend = p.zstart + p.nz * p.dz
+    elif name == "no_c":
+        p.toggle_c = 0  # two transported tracers
+    return p
+
+
+# the native loop runs pass 1 of T, S and C in one launch and fuses the work
+# velocities and REFRESH's filter into MOMENTUM; the subroutine path runs
+# every entry on its own: the two agree bit for bit
+@pytest.mark.parametrize("grid,prec", [("les64", "fp64"), ("les64", "fp32"), ("ragged", "fp64"),
+                                       ("no_c", "fp32")])
+def test_native_loop_equals_subroutine_path(grid, prec):
+    p = _grid(grid)
+    print("native loop vs subroutine path:", grid, prec, flush=True)
+    pr = H.FP64 if prec == "fp64" else H.FP32
     z, t, s = H.linear_stratification(p)
-    a = H.Context(p, H.FP64)
+    a = H.Context(p, pr)
     a.init_case(z, t, s, t_noise=1.0e-4, seed=5)
-    b = H.Context(p, H.FP64)
+    b = H.Context(p, pr)
     b.init_case(z, t, s, t_noise=1.0e-4, seed=5)
     b._t_bkg = None
     # the subroutine path needs the host profiles: rebuild them as hc_init_case does
@@ -61,3 +79,5 @@ def test_native_loop_equals_subroutine_path():
         b.driver_step(m, a.source_noise(m))
     for f in ("u", "v", "w", "p", "t", "s", "c", "ss_v", "rhop"):
         assert np.array_equal(a.download(f), b.download(f)), f
+    a.close()
+    b.close()
