@@ -1411,6 +1411,30 @@ static void step_head(hc_ctx *ctx, int m) {
   hc_work_velocity(ctx, ctx->P.dt);
 }
 
+// The head of reverse step n without running it again.  Its inputs are the
+// state X_n (tape slot s0), and the adjoint reads of its outputs only those
+// that X_n+1 (slot s1) holds: VISCOSITY3D's ss and ss_v, which nothing later
+// in the step changes, and MOMENTUM's up, vp, wp, which REFRESH made X_n+1's
+// u, v, w (the buffers themselves).  The work velocities are formed from the
+// two states.  The fields point into the tape for the step's adjoint (which
+// writes no forward field); the caller restores the pointers afterwards.
+// Host state advances as the head would advance it: step counters, body
+// force, FORCES' entrance-ramp accumulators and the source terms.
+static void adj_head(hc_ctx *ctx, int m, size_t s0, size_t s1) {
+  hc_begin_step(ctx, m, ctx->time + ctx->P.dt);
+  hc_body_force(ctx);
+  for (int i = 0; i <= 7; ++i) ctx->acc[i] = ctx->acc[i] + (double)(7 - i) * .01 / 7.0;
+  for (int f = 0; f < NTAPE; ++f) ctx->f[TAPE_FIELDS[f]] = slot_ptr(ctx, s0, f);
+  ctx->f[HC_SS] = slot_ptr(ctx, s1, 6);
+  ctx->f[HC_SSV] = slot_ptr(ctx, s1, 7);
+  for (int c = 0; c < 3; ++c) {
+    ctx->f[HC_UP + c] = slot_ptr(ctx, s1, c);
+    ctx->up_alias[c] = false;
+  }
+  hc_source_terms(ctx, nullptr);
+  hc_work_velocity(ctx, ctx->P.dt);
+}
+
 // the taped intermediates of tracer `tr` (0: T, 1: S) of the segment's step
 // index `q`
 static TracerSlot tracer_slot(hc_ctx *ctx, int q, int tr) {
@@ -1568,6 +1592,8 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   // reverse sweep
   int r = hc_adj_reset(ctx);
   if (r) return r;
+  void *own[HC_NFIELDS];
+  for (int f = 0; f < HC_NFIELDS; ++f) own[f] = ctx->f[f];
   for (int sg = nck - 1; sg >= 0; --sg) {
     const int a = sg * seg, b = std::min(nsteps, a + seg);
     // the segment's states: X_a is checkpoint sg, X_b checkpoint sg + 1,
@@ -1591,14 +1617,14 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
                            A->d_it, A->d_samp + 2 * A->nprobe * q, A->d_obs + 2 * A->nprobe * q, A->wgt_w, A->wgt_t,
                            lam.w, lam.t);
       }
-      if (n != b - 1 || n == a) load_slot(ctx, slot_of(n));  // X_b-1 is live after the recompute
       set_host_state(ctx, tp[n - a]);
-      step_head(ctx, m0 + n);
       const size_t s1 = slot_of(n + 1), s0 = slot_of(n);
+      adj_head(ctx, m0 + n, s0, s1);
       // step b-1 is not recomputed (its end state is X_b): its tracers'
       // intermediates are recomputed in the adjoint
       adj_step<T>(ctx, m0 + n, (const T *)slot_ptr(ctx, s1, 9), (const T *)slot_ptr(ctx, s1, 10),
                   (const T *)slot_ptr(ctx, s0, 6), (const T *)slot_ptr(ctx, s0, 7), n < b - 1 ? n - a : -1);
+      for (int f = 0; f < HC_NFIELDS; ++f) ctx->f[f] = own[f];
     }
   }
   // the window's end state (to continue the run) or its initial state (to
