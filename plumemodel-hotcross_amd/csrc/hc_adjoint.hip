@@ -1451,13 +1451,14 @@ static TracerSlot tracer_slot(hc_ctx *ctx, int q, int tr) {
 }
 
 // a tracer of the native loop whose passes leave their intermediates in
-// slot `t` (and the pre-sponge boundary values in the adjoint's scratch)
-static void taped_tracer(hc_ctx *ctx, int field, const TracerSlot &t) {
+// slot `t` (and the pre-sponge boundary values in the adjoint's scratch);
+// pass1_done: t.c1 holds pass 1's result already
+static void taped_tracer(hc_ctx *ctx, int field, const TracerSlot &t, bool pass1_done = false) {
   const TracerWork keep = ctx->tw;
   ctx->tw.c1 = t.c1; ctx->tw.c2 = t.c2; ctx->tw.cpre = ctx->adj->cpre;
   ctx->tw.ud2 = t.ud2[0]; ctx->tw.vd2 = t.ud2[1]; ctx->tw.wd2 = t.ud2[2];
   ctx->tw.ud3 = t.ud3[0]; ctx->tw.vd3 = t.ud3[1]; ctx->tw.wd3 = t.ud3[2];
-  native_tracer(ctx, field);
+  native_tracer(ctx, field, pass1_done ? t.c1 : nullptr);
   ctx->tw = keep;
 }
 
@@ -1465,18 +1466,58 @@ static void taped_tracer(hc_ctx *ctx, int field, const TracerSlot &t) {
 // tracer c, which no loss here depends on, and without the CFL check);
 // tq >= 0 tapes the tracers' intermediates at segment step index tq
 static void fwd_step(hc_ctx *ctx, int m, int tq = -1) {
-  step_head(ctx, m);
+  const hc_params &P = ctx->P;
   const bool tape = tq >= 0 && ctx->adj && ctx->adj->ttape;
-  if (ctx->P.toggle_t == 1) {
-    if (tape) taped_tracer(ctx, HC_T, tracer_slot(ctx, tq, 0));
-    else native_tracer(ctx, HC_T);
+  if (!ctx->fuse) {
+    step_head(ctx, m);
+    if (P.toggle_t == 1) {
+      if (tape) taped_tracer(ctx, HC_T, tracer_slot(ctx, tq, 0));
+      else native_tracer(ctx, HC_T);
+    }
+    if (P.toggle_s == 1) {
+      if (tape) taped_tracer(ctx, HC_S, tracer_slot(ctx, tq, 1));
+      else native_tracer(ctx, HC_S);
+    }
+    if (P.bit3 == 1) hc_density(ctx);
+    hc_refresh(ctx);
+    ctx->next_step = m + 1;
+    return;
   }
-  if (ctx->P.toggle_s == 1) {
-    if (tape) taped_tracer(ctx, HC_S, tracer_slot(ctx, tq, 1));
-    else native_tracer(ctx, HC_S);
+  // the native loop's fused path: MOMENTUM with the work velocities and
+  // REFRESH's filter, pass 1 of T and S in one launch, REFRESH as a buffer
+  // rotation (the same values as the path above)
+  hc_begin_step(ctx, m, ctx->time + P.dt);
+  hc_body_force(ctx);
+  hc_forces(ctx);
+  hc_pressure(ctx);
+  for (int c = 0; c < 3; ++c) momentum_fused(ctx, c, true, P.dt);
+  hc_viscosity3d(ctx);
+  hc_source_terms(ctx, nullptr);
+  const int on[2] = {P.toggle_t == 1, P.toggle_s == 1}, fid[2] = {HC_T, HC_S};
+  TracerSlot ts[2];
+  void *c1[2] = {nullptr, nullptr}, *dst[2];
+  int fields[2], n = 0;
+  for (int q = 0; q < 2; ++q) {
+    if (!on[q]) continue;
+    int rc = 0;
+    if (tape) {
+      ts[q] = tracer_slot(ctx, tq, q);
+      c1[q] = ts[q].c1;
+    } else {
+      c1[q] = native_c1(ctx, q, &rc);
+    }
+    if (!c1[q]) break;  // no memory for S's pass-1 buffer: the step's call runs its own pass 1
+    fields[n] = fid[q];
+    dst[n++] = c1[q];
   }
-  if (ctx->P.bit3 == 1) hc_density(ctx);
-  hc_refresh(ctx);
+  native_pass1(ctx, n, fields, dst);
+  for (int q = 0; q < 2; ++q) {
+    if (!on[q]) continue;
+    if (tape) taped_tracer(ctx, fid[q], ts[q], c1[q] != nullptr);
+    else native_tracer(ctx, fid[q], c1[q]);
+  }
+  if (P.bit3 == 1) hc_density(ctx);
+  refresh_rotate(ctx);
   ctx->next_step = m + 1;
 }
 
@@ -1592,8 +1633,7 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   // reverse sweep
   int r = hc_adj_reset(ctx);
   if (r) return r;
-  void *own[HC_NFIELDS];
-  for (int f = 0; f < HC_NFIELDS; ++f) own[f] = ctx->f[f];
+
   for (int sg = nck - 1; sg >= 0; --sg) {
     const int a = sg * seg, b = std::min(nsteps, a + seg);
     // the segment's states: X_a is checkpoint sg, X_b checkpoint sg + 1,
@@ -1619,6 +1659,8 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
       }
       set_host_state(ctx, tp[n - a]);
       const size_t s1 = slot_of(n + 1), s0 = slot_of(n);
+      void *own[HC_NFIELDS];  // the field buffers (the fused forward rotates them)
+      for (int f = 0; f < HC_NFIELDS; ++f) own[f] = ctx->f[f];
       adj_head(ctx, m0 + n, s0, s1);
       // step b-1 is not recomputed (its end state is X_b): its tracers'
       // intermediates are recomputed in the adjoint

@@ -653,42 +653,55 @@ static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, dou
   return HC_OK;
 }
 
-// pass 1 of the native loop's tracers (T, S, C as switched on) in one launch;
-// c1[field - HC_T...] receives where each result went, nullptr for the rest
-static int tracers_pass1(hc_ctx *ctx, double tau3, void *c1[3]) {
-  const hc_params &P = ctx->P;
-  const int on[3] = {P.toggle_t == 1, P.toggle_s == 1, P.toggle_c == 1};
-  const int fid[3] = {HC_T, HC_S, HC_C};
-  const double *bk[3] = {ctx->d_tbkg, ctx->d_sbkg, ctx->d_zero};
-  void *dst[3] = {ctx->tw.c1, nullptr, nullptr};
-  for (int q = 1; q < 3; ++q) {
-    if (!on[q]) continue;
-    if (!ctx->c1x[q - 1]) {
-      int rc = 0;
-      if (!(ctx->c1x[q - 1] = dalloc(ctx, ctx->L.n * ctx->esz, &rc))) return rc;
-    }
-    dst[q] = ctx->c1x[q - 1];
-  }
-  void *c[3], *o[3];
+// pass 1 of the native loop's tracers fields[0..n) (T, S or C) in one
+// launch, results (boundary images included) into dst[q]
+int native_pass1(hc_ctx *ctx, int n, const int *fields, void *const *dst) {
+  if (n < 1) return HC_OK;
+  if (n > 3) return fail(ctx, HC_EINVAL, "at most three tracers per pass-1 launch");
+  void *c[3];
   const double *b[3];
-  int n = 0;
-  for (int q = 0; q < 3; ++q) {
-    c1[q] = on[q] ? dst[q] : nullptr;
-    if (!on[q]) continue;
-    c[n] = ctx->f[fid[q]]; b[n] = bk[q]; o[n] = dst[q];
-    ++n;
+  for (int q = 0; q < n; ++q) {
+    c[q] = ctx->f[fields[q]];
+    b[q] = fields[q] == HC_T ? ctx->d_tbkg : fields[q] == HC_S ? ctx->d_sbkg : ctx->d_zero;
   }
-  if (!n) return HC_OK;
   PhaseScope ps(ctx, "tracer31");
   StepArgs s = step_args(ctx);
   VentArgs va = vent_args(ctx);
   if (ctx->prec == HC_FP64)
     launch_pass1_multi<double>(ctx->st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, ctx->tw, n,
-                               (double *const *)c, b, (double *const *)o, tau3);
+                               (double *const *)c, b, (double *const *)dst, ctx->P.dt);
   else
     launch_pass1_multi<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, ctx->tw, n,
-                              (float *const *)c, b, (float *const *)o, tau3);
+                              (float *const *)c, b, (float *const *)dst, ctx->P.dt);
   return HC_OK;
+}
+
+// the pass-1 buffer of S (q = 1) or C (q = 2) in the native loop; T uses
+// the tracer scratch's own
+void *native_c1(hc_ctx *ctx, int q, int *rc) {
+  if (q == 0) return ctx->tw.c1;
+  if (!ctx->c1x[q - 1]) ctx->c1x[q - 1] = dalloc(ctx, ctx->L.n * ctx->esz, rc);
+  return ctx->c1x[q - 1];
+}
+
+// pass 1 of the native loop's tracers (T, S, C as switched on) in one launch;
+// c1[q] (q = 0 T, 1 S, 2 C) receives where each result went, nullptr for the
+// tracers that are off
+static int tracers_pass1(hc_ctx *ctx, void *c1[3]) {
+  const hc_params &P = ctx->P;
+  const int on[3] = {P.toggle_t == 1, P.toggle_s == 1, P.toggle_c == 1};
+  const int fid[3] = {HC_T, HC_S, HC_C};
+  int fields[3], n = 0;
+  void *dst[3];
+  for (int q = 0; q < 3; ++q) {
+    c1[q] = nullptr;
+    if (!on[q]) continue;
+    int rc = 0;
+    if (!(c1[q] = native_c1(ctx, q, &rc))) return rc;
+    fields[n] = fid[q];
+    dst[n++] = c1[q];
+  }
+  return native_pass1(ctx, n, fields, dst);
 }
 
 // cmin of T and cmax of S in the native loop (hotcross80-revised2014.f:307,
@@ -703,15 +716,17 @@ void native_bounds(hc_ctx *ctx, double *tmin, double *smax) {
 }
 
 // TRACER31 for T, S or C as the native loop calls it (bounds, background,
-// vent ghost cells of T and S), without budgets
-int native_tracer(hc_ctx *ctx, int field) {
+// vent ghost cells of T and S), without budgets; c1, if set, holds the
+// call's pass-1 result already (native_pass1)
+int native_tracer(hc_ctx *ctx, int field, void *c1) {
   const hc_params &P = ctx->P;
   double tmin, smax;
   native_bounds(ctx, &tmin, &smax);
   const int nz = P.nz;
-  if (field == HC_T) return tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], 0);
-  if (field == HC_S) return tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], 0);
-  return tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, 0);
+  if (field == HC_T)
+    return tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], 0, c1);
+  if (field == HC_S) return tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], 0, c1);
+  return tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, 0, c1);
 }
 
 extern "C" int hc_tracer31(hc_ctx *ctx, int field, const double *c_bkg, double cmin, double cmax, double tau3,
@@ -788,7 +803,7 @@ extern "C" int hc_cfl_values(hc_ctx *ctx, double out[6]) {
 // MOMENTUM of the native loop with the work velocity (tau3) and REFRESH's
 // filter fused in; the filtered field waits in spare[comp] until the loop's
 // REFRESH point, where the buffers rotate
-static int momentum_fused(hc_ctx *ctx, int comp, bool cour, double tau3) {
+int momentum_fused(hc_ctx *ctx, int comp, bool cour, double tau3) {
   PhaseScope ps(ctx, "momentum");
   ctx->up_alias[comp] = false;
   if (!ctx->spare[comp]) {
@@ -810,7 +825,7 @@ static int momentum_fused(hc_ctx *ctx, int comp, bool cour, double tau3) {
 
 // REFRESH of the native loop after momentum_fused: u <- up (buffer swap),
 // um <- the filtered field, the old um buffer becomes the spare
-static void refresh_rotate(hc_ctx *ctx) {
+void refresh_rotate(hc_ctx *ctx) {
   int tri[3][3] = {{HC_U, HC_UP, HC_UM}, {HC_V, HC_VP, HC_VM}, {HC_W, HC_WP, HC_WM}};
   for (int c = 0; c < 3; ++c) {
     void *u = ctx->f[tri[c][0]], *up = ctx->f[tri[c][1]], *um = ctx->f[tri[c][2]];
@@ -850,7 +865,7 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
     // viscosities, none of which the T and S calls or DENSITY change
     void *c1[3] = {nullptr, nullptr, nullptr};
     if (fuse) {
-      int r = tracers_pass1(ctx, P.dt, c1);
+      int r = tracers_pass1(ctx, c1);
       if (r) return r;
     }
     if (P.toggle_t == 1) {

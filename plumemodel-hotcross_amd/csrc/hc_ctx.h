@@ -98,7 +98,13 @@ int box_copy(hc_ctx *ctx, void *dev, int bounds_field, double *host, const int32
              bool up);
 void update_sums(hc_ctx *ctx);
 void native_bounds(hc_ctx *ctx, double *tmin, double *smax);
-int native_tracer(hc_ctx *ctx, int field);
+int native_tracer(hc_ctx *ctx, int field, void *c1 = nullptr);
+int native_pass1(hc_ctx *ctx, int n, const int *fields, void *const *dst);
+void *native_c1(hc_ctx *ctx, int q, int *rc);
+// MOMENTUM with the work velocity and REFRESH's filter fused in, and the
+// buffer rotation that completes REFRESH (the native loop's fused path)
+int momentum_fused(hc_ctx *ctx, int comp, bool cour, double tau3);
+void refresh_rotate(hc_ctx *ctx);
 
 #define HIPCHK(x)                                                                    \
   do {                                                                               \
