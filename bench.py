@@ -19,6 +19,8 @@ The same line carries the FP32 form's throughput on the same workload
 128^3 adjoint configuration (400 steps, a checkpoint every 20, 25 probes at
 1 Hz) timed beside the same number of plain forward steps; its ratio is
 (window - forward) / forward, the reverse sweep per forward step.
+"adjoint_256" is the same measurement at the headline size in the FP32 form
+(256^3, 100 steps, a checkpoint every 10).
 """
 import argparse
 import ctypes
@@ -248,9 +250,11 @@ def main():
         fp32 = {"value": round(v32, 3), "unit": "Mcell-steps/s", "ms_per_step": round(sec32 * 1e3 / args.steps, 4),
                 "speedup_vs_fp64": round(v32 / value, 3), "dtype": "f32 (FP64 islands)",
                 "err_vs_fp64": "tests/test_gpu_fp32.py: max error relative to max|field| after 50 steps"}
-    adj = None
+    adj = adj256 = None
     if not args.no_adjoint:
         adj = adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg, args.adjoint_precision)
+        # the headline size's forward + adjoint configuration: 256^3 in the FP32 form
+        adj256 = adjoint_leg("les256", 100, 10, "fp32")
     out = {
         "metric": "Mcell-steps/s FP64 vs FP32 at stated err vs FP64; adjoint/forward time ratio",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -266,6 +270,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32": fp32,
         "adjoint": adj,
+        "adjoint_256": adj256,
         "phases_ms_per_step": {k: round(v[0] / nprof, 4) for k, v in phases.items()},
     }
     print(json.dumps(out))
