@@ -420,32 +420,49 @@ __global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const doub
   }
   if (q == 0 || q >= plane) return;
   // the sweeps are sequential in k; loads are issued in batches of KB levels
-  // ahead of the recurrence so their latency overlaps
+  // ahead of the recurrence so their latency overlaps.  The pivots stay in
+  // FP64 registers until the recurrence uses them (a conversion next to each
+  // load would wait for it, one level at a time), and whole batches load
+  // without a per-level test.
+  auto load = [&](int k, int dir, int kn, T *g, double *d) {
+    if (kn == KB) {
+#pragma unroll
+      for (int t = 0; t < KB; ++t) {
+        g[t] = buf[(size_t)(k + dir * t) * plane + q];
+        d[t] = piv[(size_t)(k + dir * t) * plane + q];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < KB; ++t)
+        if (t < kn) {
+          g[t] = buf[(size_t)(k + dir * t) * plane + q];
+          d[t] = piv[(size_t)(k + dir * t) * plane + q];
+        }
+    }
+  };
   T y = (T)0;
   for (int k0 = 0; k0 < nz; k0 += KB) {
-    T g[KB], d[KB];
+    T g[KB];
+    double d[KB];
     const int kn = min(KB, nz - k0);
-#pragma unroll
-    for (int t = 0; t < KB; ++t)
-      if (t < kn) { g[t] = buf[(size_t)(k0 + t) * plane + q]; d[t] = (T)piv[(size_t)(k0 + t) * plane + q]; }
+    load(k0, 1, kn, g, d);
 #pragma unroll
     for (int t = 0; t < KB; ++t)
       if (t < kn) {
-        y = (k0 + t == 0) ? g[t] * d[t] : (g[t] - (T)a * y) * d[t];
+        y = (k0 + t == 0) ? g[t] * (T)d[t] : (g[t] - (T)a * y) * (T)d[t];
         buf[(size_t)(k0 + t) * plane + q] = y;
       }
   }
   T x = y;
   for (int k1 = nz - 2; k1 >= 0; k1 -= KB) {
-    T g[KB], d[KB];
+    T g[KB];
+    double d[KB];
     const int kn = min(KB, k1 + 1);
-#pragma unroll
-    for (int t = 0; t < KB; ++t)
-      if (t < kn) { g[t] = buf[(size_t)(k1 - t) * plane + q]; d[t] = (T)piv[(size_t)(k1 - t) * plane + q]; }
+    load(k1, -1, kn, g, d);
 #pragma unroll
     for (int t = 0; t < KB; ++t)
       if (t < kn) {
-        x = g[t] - (T)a * d[t] * x;
+        x = g[t] - (T)a * (T)d[t] * x;
         buf[(size_t)(k1 - t) * plane + q] = x;
       }
   }
