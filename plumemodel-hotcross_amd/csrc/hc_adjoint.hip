@@ -52,9 +52,11 @@ __device__ inline void adj_add(double *p, double v) {
 __device__ inline void adj_add(float *p, float v) {
   if (adj_keep(v)) __builtin_amdgcn_global_atomic_fadd_f32((gfloat *)p, v);
 }
+// LDS ring accumulation in FP64: every contribution is added, without a
+// test (a branch per contribution cost more than the adds)
 template <typename T>
 __device__ inline void adj_ladd(double *p, T v) {
-  if (adj_keep(v)) atomicAdd(p, (double)v);
+  atomicAdd(p, (double)v);
 }
 
 // accumulator: hardware FP64 / FP32 atomic add into a padded field
@@ -134,7 +136,7 @@ struct LAcc {
   double *ring;  // [3][S] of one field
   int ib, jb;
   __device__ void operator()(int i, int j, int k, T v) const {
-    adj_ladd(&ring[(k % 3) * S + (i - ib) + SI * (j - jb)], v);
+    adj_ladd(&ring[((unsigned)(k + 3) % 3u) * S + (i - ib) + SI * (j - jb)], v);  // k >= -3
   }
 };
 // the same, relative to a face (MPDATA face formulas)
@@ -143,7 +145,7 @@ struct LRel {
   double *ring;
   int ib, jb, i, j, k;
   __device__ void operator()(int a, int b, int c, T v) const {
-    adj_ladd(&ring[((k + c) % 3) * S + (i + a - ib) + SI * (j + b - jb)], v);
+    adj_ladd(&ring[((unsigned)(k + c + 3) % 3u) * S + (i + a - ib) + SI * (j + b - jb)], v);  // k + c >= -3
   }
 };
 
