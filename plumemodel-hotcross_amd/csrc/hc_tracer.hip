@@ -291,12 +291,13 @@ __global__ void __launch_bounds__(256) k_sum3(int n, const double *__restrict__ 
   }
 }
 
-// planes per block of the corrective-pass kernel (HC_MP_KC overrides)
+// planes per block of the corrective-pass kernel (HC_MP_KC overrides; 64
+// measured 1-2% faster than 32 at 256^3 in both forms: fewer chunk overlaps)
 static int mp_chunk(int nz) {
   static int kc = [] {
     const char *e = getenv("HC_MP_KC");
     int v = e ? atoi(e) : 0;
-    return v > 0 ? v : 32;
+    return v > 0 ? v : 64;
   }();
   return kc < nz ? kc : nz;
 }
