@@ -88,19 +88,27 @@ __device__ inline void box_block_coords(int remap, unsigned &bx, unsigned &by, u
   bz = lid / (gx * gy);
 }
 
+// Cell-parallel kernels over a box: 256 threads per block take 256
+// consecutive cells of one (i,j) plane in memory order (i fastest, rows
+// wrapping), so a padded row of nx+2 or nx+3 cells does not leave most of a
+// fifth 64-wide block idle; blockIdx.z is the plane.
 inline dim3 box_grid(const Box &b) {
-  int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
-  return dim3((ni + 63) / 64, (nj + 3) / 4, nk);
+  long ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+  int nk = b.k1 - b.k0 + 1;
+  return dim3((unsigned)((ni * nj + 255) / 256), 1, nk);
 }
 inline dim3 box_block() { return dim3(64, 4, 1); }
 
 #define HC_BOX_IJK(b)                                                 \
   unsigned bx_, by_, bz_;                                             \
   box_block_coords((b).remap, bx_, by_, bz_);                         \
-  const int i = (b).i0 + (int)(bx_ * 64 + threadIdx.x);               \
-  const int j = (b).j0 + (int)(by_ * 4 + threadIdx.y);                \
+  const int ni_ = (b).i1 - (b).i0 + 1;                                \
+  const int t_ = (int)(bx_ * 256u + threadIdx.y * 64u + threadIdx.x); \
+  const int jr_ = t_ / ni_;                                           \
+  const int i = (b).i0 + (t_ - jr_ * ni_);                            \
+  const int j = (b).j0 + jr_;                                         \
   const int k = (b).k0 + (int)bz_;                                    \
-  if (i > (b).i1 || j > (b).j1) return;
+  if (j > (b).j1) return;
 
 constexpr double MISSING = -1.0e+34;
 
