@@ -50,6 +50,9 @@ struct StepArgs {
   // and FP32, for the divisions by them (UDiv)
   double rpr, rrho0;
   float rprf, rrho0f;
+  // the same for dt (FP64, FP32) and for dx, dy, dz in FP32
+  double rdt;
+  float rdtf, idxf, idyf, idzf;
   int p2;                // dx, dy, dz are powers of two (exact reciprocals)
   int leap, delta, mtime;
   double ubkg_m, ubkg, vbkg_m, vbkg, fbx, fby;
@@ -65,7 +68,7 @@ inline int &box_remap_flag() {
   return flag;
 }
 
-// Box-wise index for 3-D launches: block (64,4,1) over (i, j), grid.z = k.
+// Box of cells for the cell-parallel launches (box_grid, HC_BOX_IJK).
 struct Box {
   int i0, i1, j0, j1, k0, k1;            // inclusive Fortran ranges
   int remap = box_remap_flag();          // XCD-aware block order (box_block_coords)

@@ -172,6 +172,8 @@ static inline StepArgs step_args(hc_ctx *ctx) {
   s.idx = 1.0 / s.dx; s.idy = 1.0 / s.dy; s.idz = 1.0 / s.dz;
   s.rpr = 1.0 / ctx->P.prandtl; s.rrho0 = 1.0 / ctx->P.rho0;
   s.rprf = 1.0f / (float)ctx->P.prandtl; s.rrho0f = 1.0f / (float)ctx->P.rho0;
+  s.rdt = 1.0 / s.dt; s.rdtf = 1.0f / (float)s.dt;
+  s.idxf = 1.0f / (float)s.dx; s.idyf = 1.0f / (float)s.dy; s.idzf = 1.0f / (float)s.dz;
   s.p2 = ctx->prec == HC_FP64
              ? (pow2_spacing<double>(s.dx) && pow2_spacing<double>(s.dy) && pow2_spacing<double>(s.dz))
              : (pow2_spacing<float>(s.dx) && pow2_spacing<float>(s.dy) && pow2_spacing<float>(s.dz));

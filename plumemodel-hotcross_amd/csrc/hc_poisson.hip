@@ -156,12 +156,18 @@ __global__ void __launch_bounds__(256) k_rhs(Layout L, StepArgs s, Phys ph, cons
   const int j = 1 + (int)blockIdx.x % ny, k = 1 + (int)blockIdx.x / ny;
   const double dx = s.dx, dy = s.dy, dz = s.dz, dt = s.dt;
   const double dlx = 1.0 / ((nx * dx - 0.0) / nx), dly = 1.0 / ((ny * dy - 0.0) / ny), dlz = 1.0 / ((nz * dz - 0.0) / nz);
+  // the divisions by dx, dy, dz and dt through host reciprocals (UDiv: the
+  // same correctly rounded quotients); leap is 1 or 2, so x / leap equals
+  // x * (1 / leap) exactly
+  const UDiv<T> ux = udiv_t<T>(dx, s.idx, s.idxf), uy = udiv_t<T>(dy, s.idy, s.idyf),
+                uz = udiv_t<T>(dz, s.idz, s.idzf), ut = udiv_t<T>(dt, s.rdt, s.rdtf);
+  const T rleap = (T)(1.0 / s.leap);
   double acc = 0.0;
   for (int i = 1 + (int)threadIdx.x; i <= nx; i += blockDim.x) {
-    T v = ((F3(um, i, j, k) - F3(um, i - 1, j, k)) / (T)dx + (F3(vm, i, j, k) - F3(vm, i, j - 1, k)) / (T)dy +
-           (F3(wm, i, j, k) - F3(wm, i, j, k - 1)) / (T)dz) / (T)dt / (T)s.leap +
-          ((F3(fx, i, j, k) - F3(fx, i - 1, j, k)) / (T)dx + (F3(fy, i, j, k) - F3(fy, i, j - 1, k)) / (T)dy +
-           (F3(fz, i, j, k) - F3(fz, i, j, k - 1)) / (T)dz);
+    T v = ut(ux(F3(um, i, j, k) - F3(um, i - 1, j, k)) + uy(F3(vm, i, j, k) - F3(vm, i, j - 1, k)) +
+             uz(F3(wm, i, j, k) - F3(wm, i, j, k - 1))) * rleap +
+          (ux(F3(fx, i, j, k) - F3(fx, i - 1, j, k)) + uy(F3(fy, i, j, k) - F3(fy, i, j - 1, k)) +
+           uz(F3(fz, i, j, k) - F3(fz, i, j, k - 1)));
     if (i == 1) {
       T bd = (T)(-(s.ubkg - s.ubkg_m) / dt - pgradx[k]) + F3(fx, 0, j, k);
       v = v + bd * (T)dlx;
