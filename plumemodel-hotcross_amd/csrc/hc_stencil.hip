@@ -50,8 +50,10 @@ __global__ void __launch_bounds__(256) k_forces(Layout L, Box b, StepArgs s, Phy
 // k+1 of the eight stencil inputs in LDS rings with a one-cell halo in i and
 // j.  Each step reads one new plane per field from HBM (prefetched into
 // registers a plane ahead), and the three force components of the cell are
-// formed from LDS with the same expressions as k_forces.  The faces i = 0
-// (fx), j = 0 (fy) and k = 0 (fz) are left to k_forces.
+// formed from LDS with the same expressions as k_forces.  The blocks of the
+// first chunk start one plane higher, at k = 0, where only fz is formed (the
+// k = 0 face over i, j >= 1); the faces i = 0 (fx) and j = 0 (fy) are left to
+// k_forces.
 template <typename T, bool P2, int TI, int TJ>
 __global__ void __launch_bounds__(TI *TJ) k_forces_tile(Layout L, StepArgs s, Phys ph, FieldPtrs<T> fp, int kc,
                                                         int remap, const double *__restrict__ azzmin,
@@ -65,12 +67,14 @@ __global__ void __launch_bounds__(TI *TJ) k_forces_tile(Layout L, StepArgs s, Ph
   const int i0 = 1 + (int)bx * TI, j0 = 1 + (int)by * TJ;
   const int ka = 1 + (int)bz * kc, kb = min(L.nz, ka + kc - 1);
   const T *src[8] = {fp.u, fp.v, fp.w, fp.um, fp.vm, fp.wm, fp.ss, fp.ssv};
-  ring3_march(sh, L, src, i0, j0, ka, kb, [&](int i, int j, int k, auto &view) {
+  ring3_march(sh, L, src, i0, j0, ka == 1 ? 0 : ka, kb, [&](int i, int j, int k, auto &view) {
     const auto U = view(0), Vv = view(1), W = view(2), UM = view(3), VM = view(4), WM = view(5), SS = view(6),
                SV = view(7);
     const GView<T> RH{fp.rhop, L};
-    forces_x_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphau, fp.fx);
-    forces_y_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphav, fp.fy);
+    if (k >= 1) {
+      forces_x_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphau, fp.fx);
+      forces_y_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, azzmin, alphav, fp.fy);
+    }
     forces_z_at<T, P2>(L, i, j, k, s, ph, U, Vv, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, fp.fz);
   });
 }
@@ -336,8 +340,9 @@ void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
                  f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy, f.fz);
     return;
   }
-  // faces i = 0, j = 0 and k = 0 with the cell-parallel kernel
-  const Box faces[3] = {{0, 0, 0, L.ny, 0, L.nz}, {1, L.nx, 0, 0, 0, L.nz}, {1, L.nx, 1, L.ny, 0, 0}};
+  // faces i = 0 and j = 0 with the cell-parallel kernel (the tile forms the
+  // k = 0 face)
+  const Box faces[2] = {{0, 0, 0, L.ny, 0, L.nz}, {1, L.nx, 0, 0, 0, L.nz}};
   for (const Box &b : faces)
     HC_P2_LAUNCH(k_forces, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm, f.ss,
                  f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy, f.fz);

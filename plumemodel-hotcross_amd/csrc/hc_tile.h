@@ -40,8 +40,9 @@ struct Ring3 {
 };
 
 // body(i, j, k, view) for every cell of the tile inside the grid (i <= nx,
-// j <= ny), planes ka..kb in order; view(f) is the LView of field f.  All
-// threads of the block must call this (it synchronizes the block).
+// j <= ny), planes ka..kb in order (0 <= ka, kb <= nz); view(f) is the
+// LView of field f.  All threads of the block must call this (it
+// synchronizes the block).
 template <typename T, int TI, int TJ, int NF, class Body>
 __device__ inline void ring3_march(Ring3<T, TI, TJ, NF> &sh, const Layout &L, const T *const *src, int i0, int j0,
                                    int ka, int kb, Body &&body) {
@@ -64,7 +65,7 @@ __device__ inline void ring3_march(Ring3<T, TI, TJ, NF> &sh, const Layout &L, co
       for (int r = 0; r < RS; ++r) pf[f][r] = so[r] >= 0 ? src[f][so[r] + (k + 1) * pstride] : (T)0;
   };
   auto commit = [&](int k) {
-    const int slot = k % 3;
+    const int slot = (k + 3) % 3;  // k >= -1
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
