@@ -234,15 +234,6 @@ def main():
                 "avg_ms": round(per_launch * 1e3, 4), "launches": k_n,
                 "alg_bytes_per_launch": alg}
     step_bytes = FIELD_MOVES * esz * cells
-    cpu = None
-    if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
-        r = cpu_leg(p.nx, p.ny, p.nz, args.cpu_steps, threads)
-        if r:
-            cpu = {"value": round(cells * r["steps"] / r["sec"] / 1e6, 4), "unit": "Mcell-steps/s",
-                   "cores": threads, "kind": "reference",
-                   "sample": "%d steps of the reference's time loop body at %dx%dx%d (amdflang -O2 OpenMP, "
-                             "reference constants), after one untimed step" % (r["steps"], p.nx, p.ny, p.nz)}
     fp32 = None
     if args.precision == "fp64" and not args.no_fp32:
         p32, sec32, _, ph32, n32 = gpu_leg(args, "fp32")
@@ -255,6 +246,17 @@ def main():
         adj = adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg, args.adjoint_precision)
         # the headline size's forward + adjoint configuration: 256^3 in the FP32 form
         adj256 = adjoint_leg("les256", 100, 10, "fp32")
+    # the CPU baseline last: run right after it, the launch-bound 128^3
+    # adjoint leg once measured 1.5x slower
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+        r = cpu_leg(p.nx, p.ny, p.nz, args.cpu_steps, threads)
+        if r:
+            cpu = {"value": round(cells * r["steps"] / r["sec"] / 1e6, 4), "unit": "Mcell-steps/s",
+                   "cores": threads, "kind": "reference",
+                   "sample": "%d steps of the reference's time loop body at %dx%dx%d (amdflang -O2 OpenMP, "
+                             "reference constants), after one untimed step" % (r["steps"], p.nx, p.ny, p.nz)}
     out = {
         "metric": "Mcell-steps/s FP64 vs FP32 at stated err vs FP64; adjoint/forward time ratio",
         "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,

@@ -206,13 +206,11 @@ __global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, i
 
 // ghost cells of a scalar field from their interior images.  One thread per
 // ghost cell: the x faces (with edges), then the y faces, then the z faces.
-template <typename T>
-__global__ void __launch_bounds__(256) k_bc(Layout L, Phys ph, VentArgs va, T *__restrict__ dst,
-                                            const T *__restrict__ src, int vent, double vent_value) {
+// ghost cell number t: the x faces (with edges), then the y faces, then the
+// z faces; false past the last
+__device__ inline bool bc_ghost(const Layout &L, long t, int &i, int &j, int &k) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long na = 2L * (ny + 2) * (nz + 2), nb = 2L * nx * (nz + 2), nc = 2L * nx * ny;
-  int i, j, k;
   if (t < na) {
     j = (int)(t % (ny + 2)); long r = t / (ny + 2);
     k = (int)(r % (nz + 2)); i = (r / (nz + 2)) ? nx + 1 : 0;
@@ -223,8 +221,31 @@ __global__ void __launch_bounds__(256) k_bc(Layout L, Phys ph, VentArgs va, T *_
     i = 1 + (int)(t % nx); long r = t / nx;
     j = 1 + (int)(r % ny); k = (r / ny) ? nz + 1 : 0;
   } else {
-    return;
+    return false;
   }
+  return true;
+}
+
+// the ghost cells of up to three fields in place, field blockIdx.y
+template <typename T>
+struct BcSet {
+  T *f[3];
+};
+template <typename T>
+__global__ void __launch_bounds__(256) k_bc_set(Layout L, Phys ph, BcSet<T> set) {
+  int i, j, k, si, sj, sk;
+  if (!bc_ghost(L, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  bc_source(L, ph, i, j, k, si, sj, sk);
+  T *c = set.f[blockIdx.y];
+  c[IX(L, i, j, k)] = c[IX(L, si, sj, sk)];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bc(Layout L, Phys ph, VentArgs va, T *__restrict__ dst,
+                                            const T *__restrict__ src, int vent, double vent_value) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  int i, j, k;
+  if (!bc_ghost(L, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
   bool gi = (i == 0 || i == nx + 1), gj = (j == 0 || j == ny + 1);
   int si, sj, sk;
   bc_source(L, ph, i, j, k, si, sj, sk);
@@ -430,7 +451,10 @@ void launch_pass1_multi(hipStream_t st, const Layout &L, const StepArgs &s, cons
   else launch_pass1_set<T, 3>(st, L, s2, ph, set, aux.azzmin);
   const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
   const unsigned nbc = (unsigned)((nghost + 255) / 256);
-  for (int q = 0; q < n; ++q) hipLaunchKernelGGL(k_bc<T>, dim3(nbc), dim3(256), 0, st, L, ph, va, c1[q], c1[q], 0, 0.0);
+  (void)va;
+  BcSet<T> bs{};
+  for (int q = 0; q < n; ++q) bs.f[q] = c1[q];
+  hipLaunchKernelGGL(k_bc_set<T>, dim3(nbc, (unsigned)n), dim3(256), 0, st, L, ph, bs);
 }
 
 template <typename T>
