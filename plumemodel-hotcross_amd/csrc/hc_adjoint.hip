@@ -974,7 +974,7 @@ static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
     hipLaunchKernelGGL(k_adj_forces<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, s, ctx->P, fs, ptrs<T>(ctx), ss0,
                        ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav, ctx->aux.alphae0,
                        ctx->aux.alphaw, A->d_sc);
-  const int kc = std::min(L.nz, AKC);
+  const int kc = tile_chunk(L.nz, (long)((L.nx + ATI - 1) / ATI) * ((L.ny + ATJ - 1) / ATJ), AKC, 512);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
   hipLaunchKernelGGL(k_adj_forces_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
@@ -1022,7 +1022,7 @@ static void adj_viscosity_t(hc_ctx *ctx, const T *ss1, const T *ssv1) {
                      (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
                      (const double *)ctx->aux.rho_bkg, ss1, ssv1, (const T *)lam.ss, (const T *)lam.ssv, lam.um,
                      lam.vm, lam.wm, lam.rhop, A->d_sc + ADJ_SLOTS);
-  const int kc = std::min(L.nz, AKC);
+  const int kc = tile_chunk(L.nz, (long)((L.nx + ATI - 1) / ATI) * ((L.ny + ATJ - 1) / ATJ), AKC, 512);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
   hipLaunchKernelGGL(k_adj_visc_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
@@ -1112,7 +1112,7 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   dl.gq = (field == HC_T) ? A->d_gq : nullptr;
   dl.map_valid = (field == HC_T && ctx->d_qbase) ? va.map_valid : 0;
   dl.nxs = va.nxs; dl.nxe = va.nxe; dl.nys = va.nys; dl.nye = va.nye;
-  const int kc = std::min(L.nz, AKC);
+  const int kc = tile_chunk(L.nz, (long)((L.nx + ATI - 1) / ATI) * ((L.ny + ATJ - 1) / ATJ), AKC, 512);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc), tb(ATI * ATJ);
   // the antidiffusive velocities' adjoint: boundary faces cell by cell,
   // the interior faces as tiles

@@ -113,6 +113,15 @@ inline dim3 box_block() { return dim3(64, 4, 1); }
   const int k = (b).k0 + (int)bz_;                                    \
   if (j > (b).j1) return;
 
+// Planes per block of a k-marching tile kernel: kmax, halved (not below 4)
+// until the launch has at least min_blocks blocks, so that a small grid
+// still fills the chip's 256 CUs.
+inline int tile_chunk(int nz, long tiles, int kmax, long min_blocks) {
+  int kc = kmax;
+  while (kc > 4 && tiles * ((nz + kc - 1) / kc) < min_blocks) kc /= 2;
+  return kc < nz ? kc : nz;
+}
+
 constexpr double MISSING = -1.0e+34;
 
 // A grid spacing as a divisor.  x / d for a power of two d equals x * (1/d)

@@ -997,18 +997,56 @@ __global__ void __launch_bounds__(256) k_tridiag_adj(Layout L, double dz, const 
     for (int k = (int)threadIdx.x; k < nz; k += blockDim.x) buf[k * plane] = (T)g[k];
   }
   if (q == 0 || q >= plane) return;
+  // as k_tridiag: loads in batches of KB levels ahead of the recurrence,
+  // pivots in FP64 registers until used.  Each batch loads the levels it
+  // reads before it stores any, and every level is read before the sweep
+  // stores to it, as in the level-by-level form.
+  constexpr int KB = sizeof(T) == 8 ? 32 : 16;
+  // levels kg + dir*t of buf and kd + dir*t of the pivots, t < kn
+  auto load = [&](int kg, int kd, int dir, int kn, T *g, double *d) {
+    if (kn == KB) {
+#pragma unroll
+      for (int t = 0; t < KB; ++t) {
+        g[t] = buf[(size_t)(kg + dir * t) * plane + q];
+        d[t] = piv[(size_t)(kd + dir * t) * plane + q];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < KB; ++t)
+        if (t < kn) {
+          g[t] = buf[(size_t)(kg + dir * t) * plane + q];
+          d[t] = piv[(size_t)(kd + dir * t) * plane + q];
+        }
+    }
+  };
   T lx = buf[q];
-  for (int k = 0; k < nz - 1; ++k) {
-    const T ly = lx;
-    buf[(size_t)k * plane + q] = ly;
-    lx = buf[(size_t)(k + 1) * plane + q] - (T)a * (T)piv[(size_t)k * plane + q] * ly;
+  for (int k0 = 0; k0 < nz - 1; k0 += KB) {
+    const int kn = min(KB, nz - 1 - k0);
+    T g[KB];
+    double d[KB];
+    load(k0 + 1, k0, 1, kn, g, d);
+#pragma unroll
+    for (int t = 0; t < KB; ++t)
+      if (t < kn) {
+        const T ly = lx;
+        buf[(size_t)(k0 + t) * plane + q] = ly;
+        lx = g[t] - (T)a * (T)d[t] * ly;
+      }
   }
   buf[(size_t)(nz - 1) * plane + q] = lx;
   T ly = lx;
-  for (int k = nz - 1; k >= 1; --k) {
-    const T lg = (T)piv[(size_t)k * plane + q] * ly;
-    buf[(size_t)k * plane + q] = lg;
-    ly = buf[(size_t)(k - 1) * plane + q] - (T)a * lg;
+  for (int k1 = nz - 1; k1 >= 1; k1 -= KB) {
+    const int kn = min(KB, k1);
+    T g[KB];
+    double d[KB];
+    load(k1 - 1, k1, -1, kn, g, d);
+#pragma unroll
+    for (int t = 0; t < KB; ++t)
+      if (t < kn) {
+        const T lg = (T)d[t] * ly;
+        buf[(size_t)(k1 - t) * plane + q] = lg;
+        ly = g[t] - (T)a * lg;
+      }
   }
   buf[q] = (T)piv[q] * ly;
 }

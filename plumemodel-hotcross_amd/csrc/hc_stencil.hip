@@ -347,7 +347,7 @@ void launch_forces(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
     HC_P2_LAUNCH(k_forces, box_grid(b), box_block(), 0, st, L, b, s, ph, f.u, f.v, f.w, f.um, f.vm, f.wm, f.ss,
                  f.ssv, f.rhop, aux.azzmin, aux.alphau, aux.alphav, aux.alphae0, aux.alphaw, f.fx, f.fy, f.fz);
   constexpr int TI = 32, TJ = 8;
-  const int kc = std::min(L.nz, 32);
+  const int kc = tile_chunk(L.nz, (long)((L.nx + TI - 1) / TI) * ((L.ny + TJ - 1) / TJ), 32, 768);
   dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
   if (s.p2)
     hipLaunchKernelGGL((k_forces_tile<T, true, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f, kc, tile_remap(), aux.azzmin,
@@ -400,7 +400,7 @@ void launch_viscosity(hipStream_t st, const Layout &L, const StepArgs &s, const 
   HC_P2_LAUNCH(k_visc_shell, dim3((unsigned)((nshell + 255) / 256)), dim3(256), 0, st, L, s, ph, f.um, f.vm, f.wm,
                f.rhop, aux.rho_bkg, f.ss, f.ssv);
   constexpr int TI = 32, TJ = 8;
-  const int kc = std::min(L.nz, 32);
+  const int kc = tile_chunk(L.nz, (long)((L.nx + TI - 1) / TI) * ((L.ny + TJ - 1) / TJ), 32, 768);
   dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
   if (s.p2)
     hipLaunchKernelGGL((k_visc_tile<T, true, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s, ph, f.um, f.vm, f.wm,

@@ -312,15 +312,17 @@ __global__ void __launch_bounds__(256) k_sum3(int n, const double *__restrict__ 
   }
 }
 
-// planes per block of the corrective-pass kernel (HC_MP_KC overrides; 64
-// measured 1-2% faster than 32 at 256^3 in both forms: fewer chunk overlaps)
-static int mp_chunk(int nz) {
-  static int kc = [] {
+// planes per block of the corrective-pass kernel (HC_MP_KC overrides): 64
+// (1-2% faster than 32 at 256^3 in both forms: fewer chunk overlaps), fewer
+// where the grid has too few tiles to give two 512-thread blocks per CU
+static int mp_chunk(const Layout &L, int TI, int TJ) {
+  static int kc_env = [] {
     const char *e = getenv("HC_MP_KC");
-    int v = e ? atoi(e) : 0;
-    return v > 0 ? v : 64;
+    return e ? atoi(e) : 0;
   }();
-  return kc < nz ? kc : nz;
+  if (kc_env > 0) return kc_env < L.nz ? kc_env : L.nz;
+  const long tiles = (long)((L.nx + TI - 1) / TI) * ((L.ny + TJ - 1) / TJ);
+  return tile_chunk(L.nz, tiles, 64, 512);
 }
 
 // XCD-aware tile order for the LDS-tiled kernels (HC_TILE_XCD=0 turns it off)
@@ -344,7 +346,7 @@ static int mp_tile() {
 
 template <typename T, int TI, int TJ>
 static void launch_mpdata_tile(hipStream_t st, const Layout &L, const MpPass<T> &a) {
-  const int kc = mp_chunk(L.nz);
+  const int kc = mp_chunk(L, TI, TJ);
   dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
   hipLaunchKernelGGL((k_mpdata_pass<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc, tile_remap());
 }
@@ -377,7 +379,7 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   T *c1 = (T *)tw.c1, *c2 = (T *)tw.c2;
   if (!ta.pass1_done) {
     constexpr int TI = 32, TJ = 8;
-    const int kc = L.nz < 32 ? L.nz : 32;
+    const int kc = tile_chunk(L.nz, (long)((L.nx + TI - 1) / TI) * ((L.ny + TJ - 1) / TJ), 32, 768);
     dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
     hipLaunchKernelGGL((k_pass1_tile<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, s2, ph, c, (T *)tw.uc,
                        (T *)tw.vc, (T *)tw.wc, f.ss, f.ssv, aux.azzmin, c_bkg, c1, kc, tile_remap());
@@ -425,7 +427,7 @@ template <typename T, int NC>
 static void launch_pass1_set(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph,
                              const Pass1Set<T> &set, const double *azzmin) {
   constexpr int TI = 32, TJ = 8;
-  const int kc = L.nz < 32 ? L.nz : 32;
+  const int kc = tile_chunk(L.nz, (long)((L.nx + TI - 1) / TI) * ((L.ny + TJ - 1) / TJ), 32, 768);
   dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
   hipLaunchKernelGGL((k_pass1_multi<T, TI, TJ, NC>), grid, dim3(TI * TJ), 0, st, L, s, ph, set, azzmin, kc,
                      tile_remap());
