@@ -208,7 +208,7 @@ def main():
         cpu_worker(lib, int(nx), int(ny), int(nz), int(steps))
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+        print("bench.py runs on one GPU: the timestep has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.only_adjoint:
