@@ -1611,13 +1611,23 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   }
   const size_t tape0 = (size_t)nck + 1;  // first slot of the segment tape
   std::vector<HostState> ck(nck + 1), tp(seg + 1);
-  // forward with checkpoints
+  // forward with checkpoints; the last segment, which the reverse sweep
+  // takes first, is taped here as its recompute would tape it (its steps
+  // but the last, with the tracers' intermediates), so it is not recomputed
+  const int alast = (nck - 1) * seg;
   for (int n = 0; n < nsteps; ++n) {
     if (n % seg == 0) {
       save_slot(ctx, n / seg);
       ck[n / seg] = host_state(ctx);
     }
-    fwd_step(ctx, m0 + n);
+    if (n >= alast && n < nsteps - 1) {
+      if (n == alast) tp[0] = ck[nck - 1];
+      fwd_step(ctx, m0 + n, n - alast);
+      save_slot(ctx, tape0 + (n + 1 - alast));
+      tp[n + 1 - alast] = host_state(ctx);
+    } else {
+      fwd_step(ctx, m0 + n);
+    }
     if ((n + 1) % A->every == 0) probe_sample<T>(ctx, (size_t)((n + 1) / A->every - 1));
   }
   save_slot(ctx, nck);
@@ -1643,13 +1653,15 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
     auto slot_of = [&](int n) -> size_t {
       return n == a ? (size_t)sg : (n == b ? (size_t)sg + 1 : tape0 + (size_t)(n - a));
     };
-    load_slot(ctx, sg);
-    set_host_state(ctx, ck[sg]);
-    tp[0] = ck[sg];
-    for (int n = a; n < b - 1; ++n) {
-      fwd_step(ctx, m0 + n, n - a);
-      save_slot(ctx, tape0 + (n + 1 - a));
-      tp[n + 1 - a] = host_state(ctx);
+    if (sg != nck - 1) {  // the last segment was taped by the forward sweep
+      load_slot(ctx, sg);
+      set_host_state(ctx, ck[sg]);
+      tp[0] = ck[sg];
+      for (int n = a; n < b - 1; ++n) {
+        fwd_step(ctx, m0 + n, n - a);
+        save_slot(ctx, tape0 + (n + 1 - a));
+        tp[n + 1 - a] = host_state(ctx);
+      }
     }
     for (int n = b - 1; n >= a; --n) {
       FieldPtrs<T> lam = lptrs<T>(ctx);
