@@ -8,8 +8,8 @@
 // right-hand side's total delta from the plane i = nx, delta/(ny*nz) per cell;
 // the null-space constant is then removed by PRESSURE's mean subtraction.
 //
-// This build keeps the operator and that compatibility rule but transforms in
-// the two directions that map onto LDS best and solves along z:
+// The solve here keeps the operator and that compatibility rule but transforms
+// in the two directions that map onto LDS best and solves along z:
 //   k_rhs       RHS + boundary data (HS3CRT :38-106), FP64 block sums of it
 //   k_sum1      delta = total of the RHS (FP64)
 //   k_dct_x<0>  rows along x: delta correction at i=nx, DCT-II   (LDS FFT)
@@ -31,6 +31,24 @@
 namespace hc {
 
 #define F3(a, i, j, k) (a)[IX(L, (i), (j), (k))]
+
+// Sum over a block of one double per thread (blockDim.x a multiple of 64, at
+// most 1024): shuffles within each wavefront, then the wavefronts' sums
+// through LDS and shuffles in the first wavefront.  The total is valid in
+// thread 0; every thread must call it, once per kernel.
+__device__ inline double block_sum(double v) {
+  __shared__ double ws[16];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+  if (lane == 0) ws[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (w == 0) {
+    t = lane < nw ? ws[lane] : 0.0;
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  }
+  return t;
+}
 
 template <typename T>
 struct cpx {
@@ -195,14 +213,8 @@ __global__ void __launch_bounds__(256) k_rhs(Layout L, StepArgs s, Phys ph, cons
     r[((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1)] = v;
     acc += (double)v;
   }
-  __shared__ double sh[256];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 // total of n block partials (one block of SUM1_THREADS threads).  Each thread
@@ -212,7 +224,6 @@ __global__ void __launch_bounds__(256) k_rhs(Layout L, StepArgs s, Phys ph, cons
 constexpr int SUM1_THREADS = 1024, SUM1_ACC = 8;
 __global__ void __launch_bounds__(SUM1_THREADS) k_sum1(int n, const double *__restrict__ part,
                                                        double *__restrict__ out) {
-  __shared__ double sh[SUM1_THREADS];
   double a[SUM1_ACC] = {};
   const int stride = SUM1_THREADS * SUM1_ACC;
   for (int q0 = (int)threadIdx.x; q0 < n; q0 += stride) {
@@ -225,13 +236,8 @@ __global__ void __launch_bounds__(SUM1_THREADS) k_sum1(int n, const double *__re
   double v = 0.0;
 #pragma unroll
   for (int t = 0; t < SUM1_ACC; ++t) v += a[t];
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = SUM1_THREADS / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = sh[0];
+  v = block_sum(v);
+  if (threadIdx.x == 0) out[0] = v;
 }
 
 // ---------------------------------------------------------------------------
