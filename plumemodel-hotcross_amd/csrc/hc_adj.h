@@ -501,11 +501,39 @@ HC_HD inline void adj_mp_donor(T cc, T cxm, T cxp, T cym, T cyp, T czm, T czp, T
   lv[5] = lam * (dm[5] * cc + dp[5] * czp);
 }
 
-// d r / d ca and d r / d cb of r = (ca - cb) / (ca + cb + eps)
+// 1 / x for the antidiffusive ratios' reverse (their values and
+// derivatives to an ulp or two, which the adjoint's coefficients need no more
+// precisely): the hardware reciprocal refined by two Newton steps in FP64,
+// the hardware reciprocal in FP32; one reciprocal per ratio replaces the two
+// divisions of r and of its derivative
 template <typename T>
-HC_HD inline void adj_ratio(T ca, T cb, T lr, T &lca, T &lcb) {
-  const T sden = ca + cb + (T)1.0e-30;
-  const T q = lr / (sden * sden);
+HC_HD inline T adj_recip(T x);
+template <>
+HC_HD inline double adj_recip<double>(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+template <>
+HC_HD inline float adj_recip<float>(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+// d r / d ca and d r / d cb of r = (ca - cb) / (ca + cb + eps), with
+// rs = 1 / (ca + cb + eps)
+template <typename T>
+HC_HD inline void adj_ratio(T ca, T cb, T rs, T lr, T &lca, T &lcb) {
+  const T q = lr * rs * rs;
   lca = q * ((T)2 * cb + (T)1.0e-30);
   lcb = -q * ((T)2 * ca + (T)1.0e-30);
 }
@@ -518,11 +546,11 @@ HC_HD inline void adj_anti_u(const FC &C, const FU &U, const FV &V, const FW &W,
   if (lam == (T)0) return;
   const T Uc = U(0, 0, 0);
   const T ca0 = C(1, 0, 0), cb0 = C(0, 0, 0);
-  const T A0 = (ca0 - cb0) / (ca0 + cb0 + (T)1.0e-30);
+  const T r0 = adj_recip<T>(ca0 + cb0 + (T)1.0e-30), A0 = (ca0 - cb0) * r0;
   const T ca1 = C(1, 1, 0) + C(0, 1, 0), cb1 = C(1, -1, 0) + C(0, -1, 0);
-  const T A1 = (ca1 - cb1) / (ca1 + cb1 + (T)1.0e-30);
+  const T r1 = adj_recip<T>(ca1 + cb1 + (T)1.0e-30), A1 = (ca1 - cb1) * r1;
   const T ca2 = C(1, 0, -1) + C(0, 0, -1), cb2 = C(1, 0, 1) + C(0, 0, 1);
-  const T A2 = (ca2 - cb2) / (ca2 + cb2 + (T)1.0e-30);
+  const T r2 = adj_recip<T>(ca2 + cb2 + (T)1.0e-30), A2 = (ca2 - cb2) * r2;
   const T Vb = (T)0.25 * (V(1, 0, 0) + V(0, 0, 0) + V(1, -1, 0) + V(0, -1, 0));
   const T Wb = (T)0.25 * (W(1, 0, 0) + W(0, 0, 0) + W(1, 0, -1) + W(0, 0, -1));
   aU(0, 0, 0, lam * ((adj_sign(Uc) - (T)2 * Uc) * A0 - (T)0.5 * A1 * Vb - (T)0.5 * A2 * Wb));
@@ -530,11 +558,11 @@ HC_HD inline void adj_anti_u(const FC &C, const FU &U, const FV &V, const FW &W,
   aV(1, 0, 0, lvb); aV(0, 0, 0, lvb); aV(1, -1, 0, lvb); aV(0, -1, 0, lvb);
   aW(1, 0, 0, lwb); aW(0, 0, 0, lwb); aW(1, 0, -1, lwb); aW(0, 0, -1, lwb);
   T la, lb;
-  adj_ratio<T>(ca0, cb0, lam * (mp_fabs(Uc) - Uc * Uc), la, lb);
+  adj_ratio<T>(ca0, cb0, r0, lam * (mp_fabs(Uc) - Uc * Uc), la, lb);
   ac(1, 0, 0, la); ac(0, 0, 0, lb);
-  adj_ratio<T>(ca1, cb1, -(T)0.5 * lam * Uc * Vb, la, lb);
+  adj_ratio<T>(ca1, cb1, r1, -(T)0.5 * lam * Uc * Vb, la, lb);
   ac(1, 1, 0, la); ac(0, 1, 0, la); ac(1, -1, 0, lb); ac(0, -1, 0, lb);
-  adj_ratio<T>(ca2, cb2, -(T)0.5 * lam * Uc * Wb, la, lb);
+  adj_ratio<T>(ca2, cb2, r2, -(T)0.5 * lam * Uc * Wb, la, lb);
   ac(1, 0, -1, la); ac(0, 0, -1, la); ac(1, 0, 1, lb); ac(0, 0, 1, lb);
 }
 
@@ -545,11 +573,11 @@ HC_HD inline void adj_anti_v(const FC &C, const FU &U, const FV &V, const FW &W,
   if (lam == (T)0) return;
   const T Vc = V(0, 0, 0);
   const T ca0 = C(0, 1, 0), cb0 = C(0, 0, 0);
-  const T A0 = (ca0 - cb0) / (ca0 + cb0 + (T)1.0e-30);
+  const T r0 = adj_recip<T>(ca0 + cb0 + (T)1.0e-30), A0 = (ca0 - cb0) * r0;
   const T ca1 = C(1, 1, 0) + C(1, 0, 0), cb1 = C(-1, 1, 0) + C(-1, 0, 0);
-  const T A1 = (ca1 - cb1) / (ca1 + cb1 + (T)1.0e-30);
+  const T r1 = adj_recip<T>(ca1 + cb1 + (T)1.0e-30), A1 = (ca1 - cb1) * r1;
   const T ca2 = C(0, 1, -1) + C(0, 0, -1), cb2 = C(0, 1, 1) + C(0, 0, 1);
-  const T A2 = (ca2 - cb2) / (ca2 + cb2 + (T)1.0e-30);
+  const T r2 = adj_recip<T>(ca2 + cb2 + (T)1.0e-30), A2 = (ca2 - cb2) * r2;
   const T Ub = (T)0.25 * (U(0, 1, 0) + U(0, 0, 0) + U(-1, 1, 0) + U(-1, 0, 0));
   const T Wb = (T)0.25 * (W(0, 0, 0) + W(0, 1, 0) + W(0, 0, -1) + W(0, 1, -1));
   aV(0, 0, 0, lam * ((adj_sign(Vc) - (T)2 * Vc) * A0 - (T)0.5 * A1 * Ub - (T)0.5 * A2 * Wb));
@@ -557,11 +585,11 @@ HC_HD inline void adj_anti_v(const FC &C, const FU &U, const FV &V, const FW &W,
   aU(0, 1, 0, lub); aU(0, 0, 0, lub); aU(-1, 1, 0, lub); aU(-1, 0, 0, lub);
   aW(0, 0, 0, lwb); aW(0, 1, 0, lwb); aW(0, 0, -1, lwb); aW(0, 1, -1, lwb);
   T la, lb;
-  adj_ratio<T>(ca0, cb0, lam * (mp_fabs(Vc) - Vc * Vc), la, lb);
+  adj_ratio<T>(ca0, cb0, r0, lam * (mp_fabs(Vc) - Vc * Vc), la, lb);
   ac(0, 1, 0, la); ac(0, 0, 0, lb);
-  adj_ratio<T>(ca1, cb1, -(T)0.5 * lam * Vc * Ub, la, lb);
+  adj_ratio<T>(ca1, cb1, r1, -(T)0.5 * lam * Vc * Ub, la, lb);
   ac(1, 1, 0, la); ac(1, 0, 0, la); ac(-1, 1, 0, lb); ac(-1, 0, 0, lb);
-  adj_ratio<T>(ca2, cb2, -(T)0.5 * lam * Vc * Wb, la, lb);
+  adj_ratio<T>(ca2, cb2, r2, -(T)0.5 * lam * Vc * Wb, la, lb);
   ac(0, 1, -1, la); ac(0, 0, -1, la); ac(0, 1, 1, lb); ac(0, 0, 1, lb);
 }
 
@@ -572,11 +600,11 @@ HC_HD inline void adj_anti_w(const FC &C, const FU &U, const FV &V, const FW &W,
   if (lam == (T)0) return;
   const T Wc = W(0, 0, 0);
   const T ca0 = C(0, 0, 0), cb0 = C(0, 0, 1);
-  const T A0 = (ca0 - cb0) / (ca0 + cb0 + (T)1.0e-30);
+  const T r0 = adj_recip<T>(ca0 + cb0 + (T)1.0e-30), A0 = (ca0 - cb0) * r0;
   const T ca1 = C(1, 0, 0) + C(1, 0, 1), cb1 = C(-1, 0, 0) + C(-1, 0, 1);
-  const T A1 = (ca1 - cb1) / (ca1 + cb1 + (T)1.0e-30);
+  const T r1 = adj_recip<T>(ca1 + cb1 + (T)1.0e-30), A1 = (ca1 - cb1) * r1;
   const T ca2 = C(0, 1, 1) + C(0, 1, 0), cb2 = C(0, -1, 1) + C(0, -1, 0);
-  const T A2 = (ca2 - cb2) / (ca2 + cb2 + (T)1.0e-30);
+  const T r2 = adj_recip<T>(ca2 + cb2 + (T)1.0e-30), A2 = (ca2 - cb2) * r2;
   const T Ub = (T)0.25 * (U(0, 0, 0) + U(0, 0, 1) + U(-1, 0, 1) + U(-1, 0, 0));
   const T Vb = (T)0.25 * (V(0, 0, 0) + V(0, 0, 1) + V(0, -1, 1) + V(0, -1, 0));
   aW(0, 0, 0, lam * ((adj_sign(Wc) - (T)2 * Wc) * A0 - (T)0.5 * A1 * Ub - (T)0.5 * A2 * Vb));
@@ -584,11 +612,11 @@ HC_HD inline void adj_anti_w(const FC &C, const FU &U, const FV &V, const FW &W,
   aU(0, 0, 0, lub); aU(0, 0, 1, lub); aU(-1, 0, 1, lub); aU(-1, 0, 0, lub);
   aV(0, 0, 0, lvb); aV(0, 0, 1, lvb); aV(0, -1, 1, lvb); aV(0, -1, 0, lvb);
   T la, lb;
-  adj_ratio<T>(ca0, cb0, lam * (mp_fabs(Wc) - Wc * Wc), la, lb);
+  adj_ratio<T>(ca0, cb0, r0, lam * (mp_fabs(Wc) - Wc * Wc), la, lb);
   ac(0, 0, 0, la); ac(0, 0, 1, lb);
-  adj_ratio<T>(ca1, cb1, -(T)0.5 * lam * Wc * Ub, la, lb);
+  adj_ratio<T>(ca1, cb1, r1, -(T)0.5 * lam * Wc * Ub, la, lb);
   ac(1, 0, 0, la); ac(1, 0, 1, la); ac(-1, 0, 0, lb); ac(-1, 0, 1, lb);
-  adj_ratio<T>(ca2, cb2, -(T)0.5 * lam * Wc * Vb, la, lb);
+  adj_ratio<T>(ca2, cb2, r2, -(T)0.5 * lam * Wc * Vb, la, lb);
   ac(0, 1, 1, la); ac(0, 1, 0, la); ac(0, -1, 1, lb); ac(0, -1, 0, lb);
 }
 
