@@ -628,21 +628,23 @@ HC_HD inline void adj_pass1(const Layout &L, const StepArgs &s, const Phys &ph, 
                             int j, int k, T lam, const A &acc, const A &auc, const A &avc, const A &awc,
                             const A &ass, const A &assv) {
   if (lam == (T)0) return;
-  const double pr = ph.prandtl;
-  const T xfac = (T)((double)(1 * ph.bit5k) / s.dx / s.dx * s.tau3 * 1);
-  const T yfac = (T)((double)(1 * ph.bit5k) / s.dy / s.dy * s.tau3 * 1);
-  const T zfac = (T)((double)ph.bit2k / s.dz / s.dz * s.tau3 * 1);
+  // the divisions by dx, dy, dz and by the Prandtl number through host
+  // reciprocals (UDiv: the same correctly rounded quotients as pass1_at's)
+  const UDiv<double> ux(s.dx, s.idx), uy(s.dy, s.idy), uz(s.dz, s.idz), prd(ph.prandtl, s.rpr);
+  const UDiv<T> tpr = udiv_t<T>(ph.prandtl, s.rpr, s.rprf);
+  const T xfac = (T)(ux(ux((double)(1 * ph.bit5k))) * s.tau3 * 1);
+  const T yfac = (T)(uy(uy((double)(1 * ph.bit5k))) * s.tau3 * 1);
+  const T zfac = (T)(uz(uz((double)ph.bit2k)) * s.tau3 * 1);
   const T b5 = (T)ph.bit5k, b2 = (T)ph.bit2k;
-  const T axp = (T)(ph.axmin / pr), ayp = (T)(ph.aymin / pr), tpr = (T)pr;
-  const T h5 = b5 * (T)0.5 / tpr, h2 = b2 * (T)0.5 / tpr;
+  const T axp = (T)prd(ph.axmin), ayp = (T)prd(ph.aymin);
+  const T h5 = tpr(b5 * (T)0.5), h2 = tpr(b2 * (T)0.5);
   const T cp = c(i, j, k);
-  const T kx1 = b5 * (((T).5 * (ss(i, j, k) + ss(i + 1, j, k))) / tpr + axp);
-  const T kx0 = b5 * (((T).5 * (ss(i - 1, j, k) + ss(i, j, k))) / tpr + axp);
-  const T ky1 = b5 * (((T).5 * (ss(i, j, k) + ss(i, j + 1, k))) / tpr + ayp);
-  const T ky0 = b5 * (((T).5 * (ss(i, j - 1, k) + ss(i, j, k))) / tpr + ayp);
-  const T kz1 = (k == L.nz) ? (T)0 : b2 * (((T)0.5 * (ssv(i, j, k) + ssv(i, j, k + 1))) / tpr + (T)(azzmin[k] / pr));
-  const T kz0 =
-      (k == 1) ? (T)0 : b2 * (((T)0.5 * (ssv(i, j, k - 1) + ssv(i, j, k))) / tpr + (T)(azzmin[k - 1] / pr));
+  const T kx1 = b5 * (tpr((T).5 * (ss(i, j, k) + ss(i + 1, j, k))) + axp);
+  const T kx0 = b5 * (tpr((T).5 * (ss(i - 1, j, k) + ss(i, j, k))) + axp);
+  const T ky1 = b5 * (tpr((T).5 * (ss(i, j, k) + ss(i, j + 1, k))) + ayp);
+  const T ky0 = b5 * (tpr((T).5 * (ss(i, j - 1, k) + ss(i, j, k))) + ayp);
+  const T kz1 = (k == L.nz) ? (T)0 : b2 * (tpr((T)0.5 * (ssv(i, j, k) + ssv(i, j, k + 1))) + (T)prd(azzmin[k]));
+  const T kz0 = (k == 1) ? (T)0 : b2 * (tpr((T)0.5 * (ssv(i, j, k - 1) + ssv(i, j, k))) + (T)prd(azzmin[k - 1]));
   const T bk = (T)cbkg[k], bkm = (T)cbkg[k - 1], bkp = (T)cbkg[k + 1];
   T lcp = lam;  // cd = cp + ...
   {  // cx = xfac (kx1 (c(i+1)-cp) - kx0 (cp-c(i-1)))

@@ -271,13 +271,14 @@ template <typename T, class V>
 HC_HD inline T pass1_at(const Layout &L, const StepArgs &s, const Phys &ph, const V &c, const V &uc,
                              const V &vc, const V &wc, const V &ss, const V &ssv, const double *azzmin,
                              const double *cbkg, int i, int j, int k) {
-  // the divisions by the Prandtl number through its reciprocal (UDiv: the
-  // same correctly rounded quotients)
+  // the divisions by the Prandtl number and by dx, dy, dz through their
+  // reciprocals (UDiv: the same correctly rounded quotients)
   const UDiv<double> prd(ph.prandtl, s.rpr);
   const UDiv<T> tpr = udiv_t<T>(ph.prandtl, s.rpr, s.rprf);
-  const T xfac = (T)((double)(1 * ph.bit5k) / s.dx / s.dx * s.tau3 * 1);
-  const T yfac = (T)((double)(1 * ph.bit5k) / s.dy / s.dy * s.tau3 * 1);
-  const T zfac = (T)((double)ph.bit2k / s.dz / s.dz * s.tau3 * 1);
+  const UDiv<double> ux(s.dx, s.idx), uy(s.dy, s.idy), uz(s.dz, s.idz);
+  const T xfac = (T)(ux(ux((double)(1 * ph.bit5k))) * s.tau3 * 1);
+  const T yfac = (T)(uy(uy((double)(1 * ph.bit5k))) * s.tau3 * 1);
+  const T zfac = (T)(uz(uz((double)ph.bit2k)) * s.tau3 * 1);
   const T b5 = (T)ph.bit5k, b2 = (T)ph.bit2k;
   const T axp = (T)prd(ph.axmin), ayp = (T)prd(ph.aymin);
   T cp = c(i, j, k);
