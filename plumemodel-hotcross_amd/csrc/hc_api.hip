@@ -221,6 +221,16 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
         }
       }
     HIPCHK(h2d(ctx, ctx->pw.piv, piv.data(), nc * 8));
+    if (ctx->prec == HC_FP64) {
+      ctx->pw.pivt = ctx->pw.piv;
+    } else {
+      // the pivots the FP32 sweeps use, each rounded once (the value a
+      // conversion at the point of use gives), half the bytes per sweep
+      std::vector<float> pf(nc);
+      for (size_t q = 0; q < nc; ++q) pf[q] = (float)piv[q];
+      if (!(ctx->pw.pivt = dalloc(ctx, nc * 4, &rc))) return rc;
+      HIPCHK(h2d(ctx, ctx->pw.pivt, pf.data(), nc * 4));
+    }
   }
   double pi = 4.0 * atan2(1.0, 1.0);
   ctx->fc = 2.0 * P.omega * sin(pi * P.lat / 180.0);
@@ -244,6 +254,7 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
     if (p) hipFree(p);
   for (void *p : ctx->c1x)
     if (p) hipFree(p);
+  if (ctx->pw.pivt && ctx->pw.pivt != (void *)ctx->pw.piv) hipFree(ctx->pw.pivt);
   for (void *p : ctx->spare)
     if (p) hipFree(p);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);

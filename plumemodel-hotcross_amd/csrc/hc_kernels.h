@@ -109,6 +109,8 @@ struct PoissonWork {
   void *r;         // compact (nx,ny,nz) right-hand side / transform work
   void *rh;        // second compact buffer
   double *piv;     // pivots of the z tridiagonal systems, (nx,ny,nz)
+  void *pivt;      // the same in the context's precision (piv itself in FP64); the
+                   // singular (0,0) column reads piv in either form
   double *lam;     // lambda_x(kx) + lambda_y(ky), (nx,ny)
   const double *tw_x, *tw_y;  // FFT twiddles exp(-2 pi i m/N), complex (2N)
   double *part;    // per-block partial sums of the right-hand side

@@ -396,7 +396,8 @@ constexpr int TRI_MAXZ = 1024;
 
 template <typename T, int KB>
 __global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const double *__restrict__ piv,
-                                                 T *__restrict__ buf, double *__restrict__ y00) {
+                                                 const T *__restrict__ pivt, T *__restrict__ buf,
+                                                 double *__restrict__ y00) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -432,49 +433,46 @@ __global__ void __launch_bounds__(256) k_tridiag(Layout L, double dz, const doub
   }
   if (q == 0 || q >= plane) return;
   // the sweeps are sequential in k; loads are issued in batches of KB levels
-  // ahead of the recurrence so their latency overlaps.  The pivots stay in
-  // FP64 registers until the recurrence uses them (a conversion next to each
-  // load would wait for it, one level at a time), and whole batches load
-  // without a per-level test.
-  auto load = [&](int k, int dir, int kn, T *g, double *d) {
+  // ahead of the recurrence so their latency overlaps, whole batches without
+  // a per-level test.  The pivots come in the context's precision (pivt, the
+  // FP64 values rounded once on the host), so no conversion waits on a load.
+  auto load = [&](int k, int dir, int kn, T *g, T *d) {
     if (kn == KB) {
 #pragma unroll
       for (int t = 0; t < KB; ++t) {
         g[t] = buf[(size_t)(k + dir * t) * plane + q];
-        d[t] = piv[(size_t)(k + dir * t) * plane + q];
+        d[t] = pivt[(size_t)(k + dir * t) * plane + q];
       }
     } else {
 #pragma unroll
       for (int t = 0; t < KB; ++t)
         if (t < kn) {
           g[t] = buf[(size_t)(k + dir * t) * plane + q];
-          d[t] = piv[(size_t)(k + dir * t) * plane + q];
+          d[t] = pivt[(size_t)(k + dir * t) * plane + q];
         }
     }
   };
   T y = (T)0;
   for (int k0 = 0; k0 < nz; k0 += KB) {
-    T g[KB];
-    double d[KB];
+    T g[KB], d[KB];
     const int kn = min(KB, nz - k0);
     load(k0, 1, kn, g, d);
 #pragma unroll
     for (int t = 0; t < KB; ++t)
       if (t < kn) {
-        y = (k0 + t == 0) ? g[t] * (T)d[t] : (g[t] - (T)a * y) * (T)d[t];
+        y = (k0 + t == 0) ? g[t] * d[t] : (g[t] - (T)a * y) * d[t];
         buf[(size_t)(k0 + t) * plane + q] = y;
       }
   }
   T x = y;
   for (int k1 = nz - 2; k1 >= 0; k1 -= KB) {
-    T g[KB];
-    double d[KB];
+    T g[KB], d[KB];
     const int kn = min(KB, k1 + 1);
     load(k1, -1, kn, g, d);
 #pragma unroll
     for (int t = 0; t < KB; ++t)
       if (t < kn) {
-        x = g[t] - (T)a * (T)d[t] * x;
+        x = g[t] - (T)a * d[t] * x;
         buf[(size_t)(k1 - t) * plane + q] = x;
       }
   }
@@ -859,9 +857,10 @@ void launch_pressure(hipStream_t st, const Layout &L, const StepArgs &s, const P
     }();
     const int kb = kb_env ? kb_env : (sizeof(T) == 8 ? 32 : 16);
     const dim3 tg((unsigned)(((size_t)nx * ny + 255) / 256));
-    if (kb == 8) hipLaunchKernelGGL((k_tridiag<T, 8>), tg, dim3(256), 0, st, L, s.dz, pw.piv, r, pw.part);
-    else if (kb == 32) hipLaunchKernelGGL((k_tridiag<T, 32>), tg, dim3(256), 0, st, L, s.dz, pw.piv, r, pw.part);
-    else hipLaunchKernelGGL((k_tridiag<T, 16>), tg, dim3(256), 0, st, L, s.dz, pw.piv, r, pw.part);
+    const T *pt = (const T *)pw.pivt;
+    if (kb == 8) hipLaunchKernelGGL((k_tridiag<T, 8>), tg, dim3(256), 0, st, L, s.dz, pw.piv, pt, r, pw.part);
+    else if (kb == 32) hipLaunchKernelGGL((k_tridiag<T, 32>), tg, dim3(256), 0, st, L, s.dz, pw.piv, pt, r, pw.part);
+    else hipLaunchKernelGGL((k_tridiag<T, 16>), tg, dim3(256), 0, st, L, s.dz, pw.piv, pt, r, pw.part);
   }
   if (!(fast && launch_dcty_fast<T, 1>(st, L, twy, cty, r)))
     hipLaunchKernelGGL((k_dct_y<T, 1>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, r);
@@ -964,7 +963,7 @@ __global__ void __launch_bounds__(256) k_padj_ghost(Layout L, StepArgs s, Phys p
 // transposed Thomas recurrences of k_tridiag, per (kx,ky) column
 template <typename T>
 __global__ void __launch_bounds__(256) k_tridiag_adj(Layout L, double dz, const double *__restrict__ piv,
-                                                     T *__restrict__ buf) {
+                                                     const T *__restrict__ pivt, T *__restrict__ buf) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1004,57 +1003,55 @@ __global__ void __launch_bounds__(256) k_tridiag_adj(Layout L, double dz, const 
   }
   if (q == 0 || q >= plane) return;
   // as k_tridiag: loads in batches of KB levels ahead of the recurrence,
-  // pivots in FP64 registers until used.  Each batch loads the levels it
-  // reads before it stores any, and every level is read before the sweep
-  // stores to it, as in the level-by-level form.
+  // pivots in the context's precision.  Each batch loads the levels it reads
+  // before it stores any, and every level is read before the sweep stores to
+  // it, as in the level-by-level form.
   constexpr int KB = sizeof(T) == 8 ? 32 : 16;
   // levels kg + dir*t of buf and kd + dir*t of the pivots, t < kn
-  auto load = [&](int kg, int kd, int dir, int kn, T *g, double *d) {
+  auto load = [&](int kg, int kd, int dir, int kn, T *g, T *d) {
     if (kn == KB) {
 #pragma unroll
       for (int t = 0; t < KB; ++t) {
         g[t] = buf[(size_t)(kg + dir * t) * plane + q];
-        d[t] = piv[(size_t)(kd + dir * t) * plane + q];
+        d[t] = pivt[(size_t)(kd + dir * t) * plane + q];
       }
     } else {
 #pragma unroll
       for (int t = 0; t < KB; ++t)
         if (t < kn) {
           g[t] = buf[(size_t)(kg + dir * t) * plane + q];
-          d[t] = piv[(size_t)(kd + dir * t) * plane + q];
+          d[t] = pivt[(size_t)(kd + dir * t) * plane + q];
         }
     }
   };
   T lx = buf[q];
   for (int k0 = 0; k0 < nz - 1; k0 += KB) {
     const int kn = min(KB, nz - 1 - k0);
-    T g[KB];
-    double d[KB];
+    T g[KB], d[KB];
     load(k0 + 1, k0, 1, kn, g, d);
 #pragma unroll
     for (int t = 0; t < KB; ++t)
       if (t < kn) {
         const T ly = lx;
         buf[(size_t)(k0 + t) * plane + q] = ly;
-        lx = g[t] - (T)a * (T)d[t] * ly;
+        lx = g[t] - (T)a * d[t] * ly;
       }
   }
   buf[(size_t)(nz - 1) * plane + q] = lx;
   T ly = lx;
   for (int k1 = nz - 1; k1 >= 1; k1 -= KB) {
     const int kn = min(KB, k1);
-    T g[KB];
-    double d[KB];
+    T g[KB], d[KB];
     load(k1 - 1, k1, -1, kn, g, d);
 #pragma unroll
     for (int t = 0; t < KB; ++t)
       if (t < kn) {
-        const T lg = (T)d[t] * ly;
+        const T lg = d[t] * ly;
         buf[(size_t)(k1 - t) * plane + q] = lg;
         ly = g[t] - (T)a * lg;
       }
   }
-  buf[q] = (T)piv[q] * ly;
+  buf[q] = pivt[q] * ly;
 }
 
 // sum of the adjoint over the plane i = nx (the delta rule's transpose)
@@ -1169,7 +1166,7 @@ void launch_pressure_adj(hipStream_t st, const Layout &L, const StepArgs &s, con
   if (!(fast && launch_dcty_fast<T, 0>(st, L, twy, cty, g)))
     hipLaunchKernelGGL((k_dct_y<T, 0>), dim3(gy), dim3(256), shy, st, L, py, npy, twy, cty, g);
   hipLaunchKernelGGL(k_tridiag_adj<T>, dim3((unsigned)(((size_t)nx * ny + 255) / 256)), dim3(256), 0, st, L, s.dz,
-                     pw.piv, g);
+                     pw.piv, (const T *)pw.pivt, g);
   // Dx^-1 Dy^-1 into the interior of lam.p (its ghosts get values that are
   // not used and cleared with the rest of lam.p afterwards)
   if (!(fast && launch_dcty_fast<T, 1>(st, L, twy, cty, g)))
