@@ -81,3 +81,24 @@ def test_native_loop_equals_subroutine_path(grid, prec):
         assert np.array_equal(a.download(f), b.download(f)), f
     a.close()
     b.close()
+
+
+def test_native_loop_fused_equals_unfused(monkeypatch):
+    """HC_NO_FUSE=1 (read when a context is created) runs the native loop
+    with separate work-velocity, REFRESH-copy and per-tracer pass-1 kernels;
+    both forms give the same fields bit for bit."""
+    p = H.workload_params("les64")
+    z, t, s = H.linear_stratification(p)
+    ctxs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("HC_NO_FUSE", flag)
+        c = H.Context(p, H.FP64)
+        c.init_case(z, t, s, t_noise=1.0e-4, seed=9)
+        ctxs.append(c)
+    print("native loop, fused and unfused, 12 steps", flush=True)
+    for c in ctxs:
+        c.step(12)
+    for f in ("u", "v", "w", "p", "t", "s", "c", "ss_v", "rhop"):
+        assert np.array_equal(ctxs[0].download(f), ctxs[1].download(f)), f
+    for c in ctxs:
+        c.close()
