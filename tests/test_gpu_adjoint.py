@@ -253,6 +253,23 @@ def test_window_control_gradients(window_obs):
         assert abs(ad - fd) <= 1e-4 * abs(fd), (ad, fd)
 
 
+def test_window_fused_equals_unfused(window_obs, monkeypatch):
+    """The window's forward steps through the native loop's fused path and,
+    with HC_NO_FUSE=1, through the separate kernels: the same loss, and the
+    same gradients up to the order of the adjoint's atomic additions."""
+    prm, st, obs, npatch = window_obs
+    print("window, fused and unfused forward", flush=True)
+    q0 = np.ones(npatch)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("HC_NO_FUSE", flag)
+        out.append(window_loss(prm, st, obs, qmap=q0, flow=1.0, cs=0.2))
+    (l0, s0, q0g, g0, _), (l1, s1, q1g, g1, _) = out
+    assert l0 == l1 and np.array_equal(s0, s1)
+    assert np.allclose(q0g, q1g, rtol=1e-10, atol=1e-14 * np.abs(q0g).max())
+    assert np.allclose(g0, g1, rtol=1e-9, atol=0.0)
+
+
 def test_window_initial_state_adjoint(window_obs):
     prm, st, obs, npatch = window_obs
     print("window initial-state adjoint", flush=True)
