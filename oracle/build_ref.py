@@ -8,7 +8,7 @@ array_size.inc (array_size.inc:1-6 fixes nx, ny, nz as parameters; the recipe
 writes the one line that selects the grid, as the reference's own commented
 alternatives at array_size.inc:4-6 do), and compiles with amdflang:
 
-  amdflang -c -O2 -ffixed-form -ffixed-line-length=132 -fdefault-real-8
+  amdflang -c -O3 -ffixed-form -ffixed-line-length=132 -fdefault-real-8
            -fopenmp -fd-lines-as-comments -fPIC -mcmodel=medium
 
 (-fdefault-real-8 is flang's spelling of PGI's -r8, Hotcross-SAPELO2.sh:15;
@@ -20,11 +20,21 @@ INITIALIZE together with `program hotcross`, whose body includes the run's case
 file (case.inc:1 -> DanteExp1.case) and its output include write.inc
 (hotcross80-revised2014.f:118,257,354,361).  Neither file is in the reference;
 both are read only inside `program hotcross`, which the harness never runs,
-and the recipe writes them empty.  smagorinsky_split_mixing-revised2014.f does
-not compile under flang (csqrt of a REAL, :112), so VISCOSITY3D is restated in
-oracle/ref_harness.f90; the netCDF writers are not built.  The library is
-loaded with RTLD_LAZY so the unresolved writer symbols referenced by
-`program hotcross` are never bound.
+and the recipe writes them empty.  The netCDF writers are not built.
+
+smagorinsky_split_mixing-revised2014.f (VISCOSITY3D) applies the COMPLEX
+intrinsic csqrt to a REAL argument at :112, which PGI accepts by promoting the
+argument to COMPLEX and flang rejects.  The file is compiled in place with the
+preprocessor on and one definition,
+
+  -cpp -D'csqrt(x)=sqrt((x)+(0d0,0d0))'
+
+which is that promotion written out: the generic sqrt of a COMPLEX(8) is the
+principal root, and the assignment to the REAL ss_v keeps its real part, as
+under PGI -r8 (whose precision for the promoted csqrt is not documented).
+
+The library is loaded with RTLD_LAZY so the unresolved writer symbols
+referenced by `program hotcross` are never bound.
 
 usage: python oracle/build_ref.py 16 12 10 [nx ny nz ...]
 """
@@ -47,9 +57,12 @@ SOURCES = [
     "body_force80.f",
     "sponge80.f",
     "mpdata-hotcross80.f",
+    "smagorinsky_split_mixing-revised2014.f",
 ]
+# per-file additions to FFLAGS (see the module docstring)
+EXTRA = {"smagorinsky_split_mixing-revised2014.f": ["-cpp", "-Dcsqrt(x)=sqrt((x)+(0d0,0d0))"]}
 INCLUDES = ["case.inc", "site_forcing.inc", "source80.inc"]
-FFLAGS = ["-O2", "-ffixed-form", "-ffixed-line-length=132", "-fdefault-real-8",
+FFLAGS = ["-O3", "-ffixed-form", "-ffixed-line-length=132", "-fdefault-real-8",
           "-fopenmp", "-fd-lines-as-comments", "-fPIC", "-mcmodel=medium"]
 
 
@@ -77,7 +90,7 @@ def build(nx, ny, nz, quiet=True):
                                      stdout=subprocess.PIPE if quiet else None,
                                      stderr=subprocess.STDOUT if quiet else None)
     for f in SOURCES:
-        run(["amdflang", "-c"] + FFLAGS + [f])
+        run(["amdflang", "-c"] + FFLAGS + EXTRA.get(f, []) + [f])
         objs.append(f.replace(".f", ".o"))
     run(["amdflang", "-c", "-O2", "-fdefault-real-8", "-fopenmp", "-fPIC",
          "-mcmodel=medium", "-I.", os.path.join(HERE, "ref_harness.f90"), "-o", "ref_harness.o"])

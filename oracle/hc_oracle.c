@@ -14,10 +14,12 @@
  *
  * Parity status per routine (how each is pinned) is recorded in the tests:
  * tests/test_oracle_vs_reference.py compares every routine against the
- * reference compiled with amdflang (oracle/build_ref.py), except
- * VISCOSITY3D, whose reference source does not compile under flang (csqrt of
- * a REAL argument, smagorinsky_split_mixing-revised2014.f:112) and is pinned
- * by closed forms in tests/test_oracle_math.py.
+ * reference compiled with amdflang (oracle/build_ref.py; VISCOSITY3D with a
+ * preprocessor definition that promotes the REAL argument of its csqrt,
+ * smagorinsky_split_mixing-revised2014.f:112, to COMPLEX as PGI does), and
+ * tests/test_oracle_math.py adds closed forms.  The linear equation of state
+ * (eos = 1) has no reference counterpart (the reference's DENSITY is the
+ * UNESCO polynomial); it is pinned by closed forms only.
  */
 #include <math.h>
 #include <stddef.h>

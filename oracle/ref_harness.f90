@@ -6,10 +6,14 @@
 ! subroutines one at a time or as the driver's step sequence
 ! (hotcross80-revised2014.f:264-350), and copies module arrays in and out.
 !
-! VISCOSITY3D is restated here (href_viscosity3d) because its reference source
-! does not compile under flang: csqrt is applied to a REAL argument at
-! smagorinsky_split_mixing-revised2014.f:112.  The restatement keeps the real
-! part of that complex root, sqrt(max(0,x)).
+! Every reference subroutine runs as compiled from its own source, including
+! VISCOSITY3D (smagorinsky_split_mixing-revised2014.f), which oracle/build_ref.py
+! compiles with one preprocessor definition for the csqrt of a REAL at :112.
+!
+! The reference's parallel loops in VISCOSITY3D keep the strain components
+! s11..s23 shared between threads (declared at
+! smagorinsky_split_mixing-revised2014.f:16, loops at :26-65 and :72-122), so its results depend on thread timing whenever more than one thread
+! runs; href_set_threads(1) makes the reference deterministic for checks.
 
 module href
   use iso_c_binding
@@ -248,59 +252,18 @@ subroutine href_tracer(which) bind(C, name="href_tracer")
   endif
 end subroutine
 
-! Restatement of VISCOSITY3D (see file header)
 subroutine href_viscosity3d() bind(C, name="href_viscosity3d")
-  use viscosity_module
-  use grid_module
-  use velocity_module
-  use tracer_module
-  use constants_module
-  use signals_module
+  call viscosity3d
+end subroutine
+
+! OpenMP thread count of the reference's parallel regions (the reference's
+! main program fixes 8, hotcross80-revised2014.f:88-90, which never runs here)
+subroutine href_set_threads(n) bind(C, name="href_set_threads")
+  use iso_c_binding
+  use omp_lib
   implicit none
-  real(8) :: coef, Rf, eps30, s11, s12, s22, s33, s13, s23
-  integer :: i, j, k
-  ls = sqrt(dx * dy)
-  coef = ismag_h * (cs_h * ls)**2
-  eps30 = 1.0e-30
-  ss = 0
-  do k = 1, nz; do j = 1, ny; do i = 1, nx
-    s11 = ( um(i,j,k) - um(i-1,j,k) ) / dx
-    s22 = ( vm(i,j,k) - vm(i,j-1,k) ) / dy
-    s33 = ( wm(i,j,k) - wm(i,j, k-1) ) / dz
-    s12 = 0.25 * ( ( um(i,j+1,k) + um(i-1,j+1,k) - um(i,j-1,k) - um(i-1,j-1,k) ) / dy &
-                 + ( vm(i+1,j,k) + vm(i+1,j-1,k) - vm(i-1,j,k) - vm(i-1,j-1,k) ) / dx ) / 2.0d0
-    ss(i,j,k) = 2.0d0*(s11**2) + 2.0d0*(s22**2) + 4.0d0*(s12**2) + 2.0d0*(s33**2)
-    epsh(i,j,k) = ss(i,j,k)
-    ss(i,j,k) = coef * sqrt( ss(i,j,k) + eps30)
-    epsh(i,j,k) = epsh(i,j,k) * (ss(i,j,k) + axmin)
-  enddo; enddo; enddo
-  ls = dz
-  coef = ismag_v * (cs_v * ls)**2
-  do k = 1, nz; do j = 1, ny; do i = 1, nx
-    s13 = 0.25 * ( ( um(i,j,k-1) + um(i-1,j,k-1) - um(i,j,k+1) - um(i-1,j,k+1) ) / dz &
-                 + ( wm(i+1,j,k) + wm(i+1,j,k-1) - wm(i-1,j,k) - wm(i-1,j,k-1) ) / dx ) / 2.0d0
-    s23 = 0.25 * ( ( vm(i,j,k-1) + vm(i,j-1,k-1) - vm(i,j,k+1) - vm(i,j-1,k+1) ) / dz &
-                 + ( wm(i,j+1,k) + wm(i,j+1,k-1) - wm(i,j-1,k) - wm(i,j-1,k-1) ) / dy ) / 2.0d0
-    ss_v(i,j,k) = 4.0d0*(s13**2) + 4.0d0*(s23**2)
-    Ri(i,j,k) = - g * ( ( rhop(i,j,k-1) + rho_bkg(k-1) ) - ( rhop(i,j,k+1) + rho_bkg(k+1) ) ) / 2.0d0 &
-                / dz / rho0 / (ss_v(i,j,k) + eps30)
-    Rf = 3
-    epsv(i,j,k) = ss_v(i,j,k)
-    ss_v(i,j,k) = coef * sqrt( ss_v(i,j,k)+eps30 ) * sqrt( max(0.0d0, 1.0 + eps - Ri(i,j,k)/Rf) )
-    epsv(i,j,k) = epsv(i,j,k) * (ss_v(i,j,k)+azmin)
-  enddo; enddo; enddo
-  do k = 1, nz; do j = 1, ny
-    ss(0,j,k)    = ss(1,j,k) * bcondx + ( 1 - bcondx ) * ss(nx,j,k)
-    ss(nx+1,j,k) = ss(nx,j,k) * bcondx + ( 1 - bcondx ) * ss(1,j,k)
-  enddo; enddo
-  do k = 1, nz; do i = 1 , nx
-    ss(i,0,k)    = ss(i,1,k) * bcondy + ( 1 - bcondy ) * ss(i,ny,k)
-    ss(i,ny+1,k) = ss(i,ny,k) * bcondy + ( 1 - bcondy ) * ss(i,1,k)
-  enddo; enddo
-  do j = 1, ny; do i = 1, nx
-    ss_v(i,j,0) = ss_v(i,j,1)
-    ss_v(i,j,nz+1) = ss_v(i,j,nz)
-  enddo; enddo
+  integer(c_int), value :: n
+  call omp_set_num_threads(n)
 end subroutine
 
 ! one pass of the time loop body, hotcross80-revised2014.f:264-350, without
@@ -317,7 +280,7 @@ subroutine href_step(m) bind(C, name="href_step")
   call momentum_u
   call momentum_v
   call momentum_w
-  call href_viscosity3d
+  call viscosity3d
   call href_source_terms
   call mixing
   if (toggle(6) .eq. 1) then

@@ -334,8 +334,12 @@ class RefLib:
     """The reference's subroutines through oracle/ref_harness.f90.  Module
     state is per process: use one instance per fresh process."""
 
-    def __init__(self, path, prm):
+    def __init__(self, path, prm, threads=1):
         self.L = ctypes.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        # one thread unless asked: the reference's VISCOSITY3D shares its
+        # strain scalars between threads (a race, see oracle/ref_harness.f90)
+        self.L.href_set_threads.argtypes = [ctypes.c_int]
+        self.L.href_set_threads(threads)
         self.prm = prm
         sz = (ctypes.c_int * 3)()
         self.L.href_sizes(sz)

@@ -82,10 +82,10 @@ def dump(ref):
     return {n: ref.get(n) for n in DUMP}
 
 
-def worker_sequence(ref, seed, steps, calls):
+def worker_sequence(ref, seed, steps, calls, amp=0.02):
     """Initial state, then for each step the given calls; returns the dump
     before the first call, the dump after, and the recorded source noise."""
-    ref_init(ref, seed)
+    ref_init(ref, seed, amp=amp)
     before = dump(ref)
     noises = []
     for m in steps:

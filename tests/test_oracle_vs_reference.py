@@ -19,6 +19,7 @@ SEQS = {
     "momentum": (["body_force", "forces", "pressure", "momentum_u", "momentum_v", "momentum_w"],
                  ["up", "vp", "wp"], 1e-14),
     "source_terms": (["body_force", "source_terms"], ["q_t", "q_s", "q_c", "isource_map"], 0.0),
+    "viscosity3d": (["viscosity3d"], ["ss", "ss_v", "ri", "epsh", "epsv"], 1e-15),
     "mixing": (["body_force", "viscosity3d", "mixing"], ["kxt", "kyt", "kzt"], 1e-15),
     "tracers+density": (["body_force", "viscosity3d", "source_terms", "mixing", "work_velocity", "tracer1",
                          "tracer2", "density", "tracer3"], ["t", "s", "c", "rhop", "ud", "vd", "wd"], 1e-11),
@@ -41,6 +42,28 @@ def test_subroutine_matches_reference(name, steps):
         assert np.array_equal(a[~m], b[~m]), n
         err = T.rel_err(a, b, m)
         assert err <= max(tol, 3e-15), (n, err)
+
+
+@pytest.mark.parametrize("bcondy", [1, 0])
+def test_viscosity3d_richardson_cutoff_matches_reference(bcondy):
+    """VISCOSITY3D of the reference (its csqrt at
+    smagorinsky_split_mixing-revised2014.f:112 promoted to COMPLEX, as PGI
+    does) on a quiet flow in the reference's stratification, where the
+    Richardson number passes the cutoff Ri/3 > 1 + eps in some cells and not
+    in others, so both branches of the complex root are compared."""
+    prm = T.reference_params(16, 12, 10, bcondy=bcondy)
+    before, after, _ = T.run_in_ref(R.worker_sequence, prm, seed=3, steps=[0], calls=["viscosity3d"],
+                                    amp=1e-4)
+    ri = after["ri"][1:17, 1:13, 1:11]
+    cut = ri / 3.0 > 1.0 + prm.eps
+    assert 0 < cut.sum() < cut.size, cut.sum()
+    assert np.all(after["ss_v"][1:17, 1:13, 1:11][cut] == 0.0)
+    o = R.oracle_from(prm, before)
+    o.set_time(-prm.dt, 0)
+    R.oracle_sequence(o, [0], ["viscosity3d"], [])
+    for n in ("ss", "ss_v", "ri", "epsh", "epsv"):
+        a, b = o.get(n), after[n]
+        assert T.rel_err(a, b) <= 3e-15, (n, T.rel_err(a, b))
 
 
 def test_initial_state_matches_reference():
