@@ -1466,8 +1466,10 @@ static void taped_tracer(hc_ctx *ctx, int field, const TracerSlot &t, bool pass1
 
 // one forward step of the window (the native loop without the passive
 // tracer c, which no loss here depends on, and without the CFL check);
-// tq >= 0 tapes the tracers' intermediates at segment step index tq
-static void fwd_step(hc_ctx *ctx, int m, int tq = -1) {
+// tq >= 0 tapes the tracers' intermediates at segment step index tq.
+// Returns HC_ENOMEM when the pass-1 buffer of S cannot be allocated (as
+// hc_step does), else 0.
+static int fwd_step(hc_ctx *ctx, int m, int tq = -1) {
   const hc_params &P = ctx->P;
   const bool tape = tq >= 0 && ctx->adj && ctx->adj->ttape;
   if (!ctx->fuse) {
@@ -1483,7 +1485,7 @@ static void fwd_step(hc_ctx *ctx, int m, int tq = -1) {
     if (P.bit3 == 1) hc_density(ctx);
     hc_refresh(ctx);
     ctx->next_step = m + 1;
-    return;
+    return HC_OK;
   }
   // the native loop's fused path: MOMENTUM with the work velocities and
   // REFRESH's filter, pass 1 of T and S in one launch, REFRESH as a buffer
@@ -1508,7 +1510,7 @@ static void fwd_step(hc_ctx *ctx, int m, int tq = -1) {
     } else {
       c1[q] = native_c1(ctx, q, &rc);
     }
-    if (!c1[q]) break;  // no memory for S's pass-1 buffer: the step's call runs its own pass 1
+    if (!c1[q]) return rc ? rc : fail(ctx, HC_ENOMEM, "pass-1 buffer: allocation failed");
     fields[n] = fid[q];
     dst[n++] = c1[q];
   }
@@ -1521,6 +1523,7 @@ static void fwd_step(hc_ctx *ctx, int m, int tq = -1) {
   if (P.bit3 == 1) hc_density(ctx);
   refresh_rotate(ctx);
   ctx->next_step = m + 1;
+  return HC_OK;
 }
 
 // the adjoint of step m, with the live state holding the step's
@@ -1620,18 +1623,22 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
       save_slot(ctx, n / seg);
       ck[n / seg] = host_state(ctx);
     }
+    int rs;
     if (n >= alast && n < nsteps - 1) {
-      if (n == alast) tp[0] = ck[nck - 1];
-      fwd_step(ctx, m0 + n, n - alast);
+      rs = fwd_step(ctx, m0 + n, n - alast);
       save_slot(ctx, tape0 + (n + 1 - alast));
       tp[n + 1 - alast] = host_state(ctx);
     } else {
-      fwd_step(ctx, m0 + n);
+      rs = fwd_step(ctx, m0 + n);
     }
+    if (rs) return rs;
     if ((n + 1) % A->every == 0) probe_sample<T>(ctx, (size_t)((n + 1) / A->every - 1));
   }
   save_slot(ctx, nck);
   ck[nck] = host_state(ctx);
+  // the last segment's first state, also when that segment is a single step
+  // (nsteps % seg == 1), whose forward step the taping branch above skips
+  tp[0] = ck[nck - 1];
   std::vector<double> samp(ns);
   if (ns) HIPCHK(hipMemcpyAsync(samp.data(), A->d_samp, ns * 8, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1658,7 +1665,8 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
       set_host_state(ctx, ck[sg]);
       tp[0] = ck[sg];
       for (int n = a; n < b - 1; ++n) {
-        fwd_step(ctx, m0 + n, n - a);
+        const int rs = fwd_step(ctx, m0 + n, n - a);
+        if (rs) return rs;
         save_slot(ctx, tape0 + (n + 1 - a));
         tp[n + 1 - a] = host_state(ctx);
       }
