@@ -181,6 +181,26 @@ int hc_refresh(hc_ctx *ctx);
  * is, for the caller's emergency snapshot (the reference's :407-411). */
 int hc_cfl_values(hc_ctx *ctx, double out[6]);
 
+/* CFL_VALUES of hc_cfl_values over caller-owned device velocity fields,
+ * allocating nothing.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc), which also cannot create a
+ * context.  u, v, w (FP64 or FP32 by precision) are device buffers laid out
+ * as the library's fields: one padded box per field over i=-1..nx+1,
+ * j=-1..ny+1, k=-1..nz+1 with element (i,j,k) at
+ *   (i+16) + px*((j+1) + (ny+3)*(k+1)),  px = 16*ceil((nx+18)/16),
+ * whatever the field's staggering (only the ranges CFL_VALUES scans are
+ * read).  The reduction's partial results live in one static device array
+ * of 57,400 bytes per GPU, whatever the grid; calls that overlap on one GPU
+ * (other host threads, other streams) share it and race.  Grids whose padded
+ * box holds 2^31 elements or more are rejected (HC_EINVAL).  Runs on
+ * `stream` (a hipStream_t, NULL for the null stream) and synchronises it;
+ * out and the return code as hc_cfl_values (HC_ECFL, HC_ENAN), with no
+ * message (there is no context). */
+int hc_cfl_values_noalloc(int32_t nx, int32_t ny, int32_t nz, int precision, const void *u, const void *v,
+                          const void *w, double dx, double dy, double dz, double dt, void *stream,
+                          double out[6]);
+
 /* The whole reference time loop body (hotcross80-revised2014.f:264-350) for
  * nsteps steps starting at the context's next step, with the source noise
  * from the counter-based generator.  CFL every 10 steps as in the
