@@ -1195,6 +1195,74 @@ static void o_mpdata(oracle *o, A3 *cb, A3 *ca, A3 *uca, A3 *vca, A3 *wca,
   free(upi.d); free(umi.d); free(vpj.d); free(vmj.d); free(wpk.d); free(wmk.d);
 }
 
+/* One corrective pass of TRACER31 (iord = 2 or 3 of mpdata-hotcross80.f:196-350):
+ * the antidiffusive velocities ud, vd, wd from the field c (ghosts set) and
+ * the Courant numbers in uc, vc, wc (:218-326), the limited donor step
+ * (MPDATA), the boundary images, and ud, vd, wd copied into uc, vc, wc for the
+ * next pass (:345-347).  ce is work space over the scalar bounds. */
+static void o_corrective_pass(oracle *o, A3 *c, A3 *ce, const double *c_bkg, int iord, double cmin,
+                              double cmax) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  size_t nc = a3_size(c);
+  double epsa = 1.0e-30;
+  memcpy(o->cc.d, c->d, nc * sizeof(double));
+  A3 cc = o->cc, uc = o->uc, vc = o->vc, wc = o->wc;
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        double ca = X3(cc, i + 1, j, k), cb = X3(cc, i, j, k);
+        double U = X3(uc, i, j, k);
+        double ud = (fabs(U) - U * U / 1.0) * (ca - cb) / (ca + cb + epsa);
+        ca = (X3(cc, i + 1, j + 1, k) + X3(cc, i, j + 1, k));
+        cb = (X3(cc, i + 1, j - 1, k) + X3(cc, i, j - 1, k));
+        ud = ud - 1 * 0.5 * U / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
+                      (X3(vc, i + 1, j, k) + X3(vc, i, j, k) + X3(vc, i + 1, j - 1, k) + X3(vc, i, j - 1, k)) * 1;
+        ca = (X3(cc, i + 1, j, k - 1) + X3(cc, i, j, k - 1));
+        cb = (X3(cc, i + 1, j, k + 1) + X3(cc, i, j, k + 1));
+        ud = ud - 0.5 * U / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
+                      (X3(wc, i + 1, j, k) + X3(wc, i, j, k) + X3(wc, i + 1, j, k - 1) + X3(wc, i, j, k - 1)) * 1;
+        X3(o->ud, i, j, k) = 1 * 1 * ud * 1;
+      }
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        double ca = X3(cc, i, j + 1, k), cb = X3(cc, i, j, k);
+        double V = X3(vc, i, j, k);
+        double vd = (fabs(V) - V * V / 1.0) * 1 * (ca - cb) / (ca + cb + epsa);
+        ca = (X3(cc, i + 1, j + 1, k) + X3(cc, i + 1, j, k));
+        cb = (X3(cc, i - 1, j + 1, k) + X3(cc, i - 1, j, k));
+        vd = vd - 0.5 * V / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
+                      (X3(uc, i, j + 1, k) + X3(uc, i, j, k) + X3(uc, i - 1, j + 1, k) + X3(uc, i - 1, j, k)) * 1;
+        ca = (X3(cc, i, j + 1, k - 1) + X3(cc, i, j, k - 1));
+        cb = (X3(cc, i, j + 1, k + 1) + X3(cc, i, j, k + 1));
+        vd = vd - 0.5 * V / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
+                      (X3(wc, i, j, k) + X3(wc, i, j + 1, k) + X3(wc, i, j, k - 1) + X3(wc, i, j + 1, k - 1)) * 1;
+        X3(o->vd, i, j, k) = 1 * 1 * vd * 1;
+      }
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        double ca = X3(cc, i, j, k), cb = X3(cc, i, j, k + 1);
+        double W = X3(wc, i, j, k);
+        double wd = (fabs(W) - W * W / 1.0) * (ca - cb) / (ca + cb + epsa);
+        ca = (X3(cc, i + 1, j, k) + X3(cc, i + 1, j, k + 1));
+        cb = (X3(cc, i - 1, j, k) + X3(cc, i - 1, j, k + 1));
+        wd = wd - 1 * 0.5 * W / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
+                      (X3(uc, i, j, k) + X3(uc, i, j, k + 1) + X3(uc, i - 1, j, k + 1) + X3(uc, i - 1, j, k)) * 1;
+        ca = (X3(cc, i, j + 1, k + 1) + X3(cc, i, j + 1, k));
+        cb = (X3(cc, i, j - 1, k + 1) + X3(cc, i, j - 1, k));
+        wd = wd - 1 * 0.5 * W / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
+                      (X3(vc, i, j, k) + X3(vc, i, j, k + 1) + X3(vc, i, j - 1, k + 1) + X3(vc, i, j - 1, k)) * 1;
+        X3(o->wd, i, j, k) = 1 * wd * 1;
+      }
+  o_mpdata(o, ce, &o->cc, &o->ud, &o->vd, &o->wd, iord, cmin, cmax);
+  for (size_t q = 0; q < nc; ++q) c->d[q] = (c->d[q] + ce->d[q]);
+  o_boundary_conditions(o, c, c_bkg);
+  memcpy(o->uc.d, o->ud.d, a3_size(&o->uc) * sizeof(double));
+  memcpy(o->vc.d, o->vd.d, a3_size(&o->vc) * sizeof(double));
+  memcpy(o->wc.d, o->wd.d, a3_size(&o->wc) * sizeof(double));
+}
+
 /* TRACER31, mpdata-hotcross80.f:16-433 (iord_number = 3, ismol from the
  * case).  budgets receives c_total, sum_sponge, sum_source when
  * mod(mtime,100)==0. */
@@ -1206,7 +1274,6 @@ void o_tracer31(oracle *o, A3 *c, A3 *q_c, const double *c_bkg /* offset 1 */,
   a3_fill(&o->uc, 0); a3_fill(&o->vc, 0); a3_fill(&o->wc, 0);
   a3_fill(&o->cp, 0); a3_fill(&o->cc, 0);
   a3_fill(&o->ud, 0); a3_fill(&o->vd, 0); a3_fill(&o->wd, 0);
-  double epsa = 1.0e-30;
   /* :94-104 */
   for (int k = 0; k <= nz + 1; ++k)
     for (int j = 0; j <= ny + 1; ++j)
@@ -1240,66 +1307,8 @@ void o_tracer31(oracle *o, A3 *c, A3 *q_c, const double *c_bkg /* offset 1 */,
   size_t nc = a3_size(c);
   for (size_t q = 0; q < nc; ++q) c->d[q] = (c->d[q] + ce.d[q]);
   o_boundary_conditions(o, c, c_bkg);
-  if (P->ismol == 1) {
-    for (int iord = 2; iord <= 3; ++iord) {
-      memcpy(o->cc.d, c->d, nc * sizeof(double));
-      A3 cc = o->cc, uc = o->uc, vc = o->vc, wc = o->wc;
-      for (int k = 1; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j)
-          for (int i = 0; i <= nx; ++i) {
-            double ca = X3(cc, i + 1, j, k), cb = X3(cc, i, j, k);
-            double U = X3(uc, i, j, k);
-            double ud = (fabs(U) - U * U / 1.0) * (ca - cb) / (ca + cb + epsa);
-            ca = (X3(cc, i + 1, j + 1, k) + X3(cc, i, j + 1, k));
-            cb = (X3(cc, i + 1, j - 1, k) + X3(cc, i, j - 1, k));
-            ud = ud - 1 * 0.5 * U / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
-                          (X3(vc, i + 1, j, k) + X3(vc, i, j, k) + X3(vc, i + 1, j - 1, k) + X3(vc, i, j - 1, k)) * 1;
-            ca = (X3(cc, i + 1, j, k - 1) + X3(cc, i, j, k - 1));
-            cb = (X3(cc, i + 1, j, k + 1) + X3(cc, i, j, k + 1));
-            ud = ud - 0.5 * U / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
-                          (X3(wc, i + 1, j, k) + X3(wc, i, j, k) + X3(wc, i + 1, j, k - 1) + X3(wc, i, j, k - 1)) * 1;
-            X3(o->ud, i, j, k) = 1 * 1 * ud * 1;
-          }
-      for (int k = 1; k <= nz; ++k)
-        for (int j = 0; j <= ny; ++j)
-          for (int i = 1; i <= nx; ++i) {
-            double ca = X3(cc, i, j + 1, k), cb = X3(cc, i, j, k);
-            double V = X3(vc, i, j, k);
-            double vd = (fabs(V) - V * V / 1.0) * 1 * (ca - cb) / (ca + cb + epsa);
-            ca = (X3(cc, i + 1, j + 1, k) + X3(cc, i + 1, j, k));
-            cb = (X3(cc, i - 1, j + 1, k) + X3(cc, i - 1, j, k));
-            vd = vd - 0.5 * V / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
-                          (X3(uc, i, j + 1, k) + X3(uc, i, j, k) + X3(uc, i - 1, j + 1, k) + X3(uc, i - 1, j, k)) * 1;
-            ca = (X3(cc, i, j + 1, k - 1) + X3(cc, i, j, k - 1));
-            cb = (X3(cc, i, j + 1, k + 1) + X3(cc, i, j, k + 1));
-            vd = vd - 0.5 * V / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
-                          (X3(wc, i, j, k) + X3(wc, i, j + 1, k) + X3(wc, i, j, k - 1) + X3(wc, i, j + 1, k - 1)) * 1;
-            X3(o->vd, i, j, k) = 1 * 1 * vd * 1;
-          }
-      for (int k = 0; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j)
-          for (int i = 1; i <= nx; ++i) {
-            double ca = X3(cc, i, j, k), cb = X3(cc, i, j, k + 1);
-            double W = X3(wc, i, j, k);
-            double wd = (fabs(W) - W * W / 1.0) * (ca - cb) / (ca + cb + epsa);
-            ca = (X3(cc, i + 1, j, k) + X3(cc, i + 1, j, k + 1));
-            cb = (X3(cc, i - 1, j, k) + X3(cc, i - 1, j, k + 1));
-            wd = wd - 1 * 0.5 * W / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
-                          (X3(uc, i, j, k) + X3(uc, i, j, k + 1) + X3(uc, i - 1, j, k + 1) + X3(uc, i - 1, j, k)) * 1;
-            ca = (X3(cc, i, j + 1, k + 1) + X3(cc, i, j + 1, k));
-            cb = (X3(cc, i, j - 1, k + 1) + X3(cc, i, j - 1, k));
-            wd = wd - 1 * 0.5 * W / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
-                          (X3(vc, i, j, k) + X3(vc, i, j, k + 1) + X3(vc, i, j - 1, k + 1) + X3(vc, i, j - 1, k)) * 1;
-            X3(o->wd, i, j, k) = 1 * wd * 1;
-          }
-      o_mpdata(o, &ce, &o->cc, &o->ud, &o->vd, &o->wd, iord, cmin, cmax);
-      for (size_t q = 0; q < nc; ++q) c->d[q] = (c->d[q] + ce.d[q]);
-      o_boundary_conditions(o, c, c_bkg);
-      memcpy(o->uc.d, o->ud.d, a3_size(&o->uc) * sizeof(double));
-      memcpy(o->vc.d, o->vd.d, a3_size(&o->vc) * sizeof(double));
-      memcpy(o->wc.d, o->wd.d, a3_size(&o->wc) * sizeof(double));
-    }
-  }
+  if (P->ismol == 1)
+    for (int iord = 2; iord <= 3; ++iord) o_corrective_pass(o, c, &ce, c_bkg, iord, cmin, cmax);
   free(ce.d);
   /* :356-381 sponge + source */
   for (int k = 1; k <= nz; ++k)
@@ -1463,6 +1472,31 @@ int o_tracer(oracle *o, const char *name, double cmin, double cmax, double tau3)
   A3 *q = !strcmp(name, "t") ? &o->q_t : !strcmp(name, "s") ? &o->q_s : &o->q_c;
   double *bkg = !strcmp(name, "t") ? o->t_bkg : !strcmp(name, "s") ? o->s_bkg : o->c_bkg;
   o_tracer31(o, c, q, bkg, cmin, cmax, tau3);
+  return 0;
+}
+
+/* One corrective pass (iord 2 or 3) of TRACER31 on the named tracer, from
+ * the Courant numbers or antidiffusive velocities held in uc, vc, wc (tests
+ * of the tiled corrective-pass kernel); with finish = 1 also the sponge and
+ * source of :356-381, relative to the entry value held in cp. */
+int o_corrective(oracle *o, const char *name, int iord, double cmin, double cmax, double tau3, int finish) {
+  A3 *c = find3(o, name);
+  if (!c || (iord != 2 && iord != 3)) return -1;
+  const hc_params *P = &o->P;
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  A3 *q = !strcmp(name, "t") ? &o->q_t : !strcmp(name, "s") ? &o->q_s : &o->q_c;
+  double *bkg = !strcmp(name, "t") ? o->t_bkg : !strcmp(name, "s") ? o->s_bkg : o->c_bkg;
+  A3 ce;
+  a3_alloc(&ce, 0, nx + 1, 0, ny + 1, 0, nz + 1);
+  o_corrective_pass(o, c, &ce, bkg, iord, cmin, cmax);
+  free(ce.d);
+  if (finish)
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          X3(*c, i, j, k) = X3(*c, i, j, k) - 1 * 0.5 * X3(o->alphaee, i, j, k) * tau3 * (X3(o->cp, i, j, k) - Z1(bkg, k));
+          X3(*c, i, j, k) = X3(*c, i, j, k) + tau3 * X3(*q, i, j, k) / (P->dx * P->dy * P->dz) / 1.0 * 1;
+        }
   return 0;
 }
 

@@ -44,6 +44,10 @@
 #ifndef HC_MP_CHECKC
 #define HC_MP_CHECKC(x) (x)
 #endif
+// and HC_MP_GIDX(x) to range-check the element index of every staged load
+#ifndef HC_MP_GIDX
+#define HC_MP_GIDX(x) (x)
+#endif
 
 namespace hc {
 
@@ -88,7 +92,10 @@ struct MpPass {
 HC_HD inline double mp_fabs(double x) { return fabs(x); }
 HC_HD inline float mp_fabs(float x) { return fabsf(x); }
 
-// the antidiffusive ratios (a - b) / (a + b + eps), nine per face triple.
+// the quotients of the antidiffusive velocities, nine per face triple: each
+// term is (coefficient * (a - b)) / (a + b + eps), multiplied before it is
+// divided, as the reference evaluates it left to right
+// (mpdata-hotcross80.f:228-229, 236-237, 245-246 and the vd and wd lines).
 // FP64: ddiv, the correctly rounded quotient (the reference's rounding)
 // without the division's scaling and fix-up, which the ratios never need:
 // b = a' + b' + 1e-30 of bounded concentrations is a normal number far from
@@ -124,14 +131,14 @@ HC_HD inline T mp_anti_u(const FC &C, const FU &U, const FV &V, const FW &W) {
   const T epsa = (T)1.0e-30;
   T ca = C(1, 0, 0), cb = C(0, 0, 0);
   T Uc = U(0, 0, 0);
-  T r = (mp_fabs(Uc) - Uc * Uc / (T)1.0) * mp_ratio(ca - cb, ca + cb + epsa);
+  T r = mp_ratio((mp_fabs(Uc) - Uc * Uc / (T)1.0) * (ca - cb), ca + cb + epsa);
   ca = (C(1, 1, 0) + C(0, 1, 0));
   cb = (C(1, -1, 0) + C(0, -1, 0));
-  r = r - (T)1 * (T)0.5 * Uc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
+  r = r - mp_ratio((T)1 * (T)0.5 * Uc / (T)1.0 * (ca - cb), ca + cb + epsa) * (T)0.25 *
               (V(1, 0, 0) + V(0, 0, 0) + V(1, -1, 0) + V(0, -1, 0)) * (T)1;
   ca = (C(1, 0, -1) + C(0, 0, -1));
   cb = (C(1, 0, 1) + C(0, 0, 1));
-  r = r - (T)0.5 * Uc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
+  r = r - mp_ratio((T)0.5 * Uc / (T)1.0 * (ca - cb), ca + cb + epsa) * (T)0.25 *
               (W(1, 0, 0) + W(0, 0, 0) + W(1, 0, -1) + W(0, 0, -1)) * (T)1;
   return r;
 }
@@ -142,14 +149,14 @@ HC_HD inline T mp_anti_v(const FC &C, const FU &U, const FV &V, const FW &W) {
   const T epsa = (T)1.0e-30;
   T ca = C(0, 1, 0), cb = C(0, 0, 0);
   T Vc = V(0, 0, 0);
-  T r = (mp_fabs(Vc) - Vc * Vc / (T)1.0) * (T)1 * mp_ratio(ca - cb, ca + cb + epsa);
+  T r = mp_ratio((mp_fabs(Vc) - Vc * Vc / (T)1.0) * (T)1 * (ca - cb), ca + cb + epsa);
   ca = (C(1, 1, 0) + C(1, 0, 0));
   cb = (C(-1, 1, 0) + C(-1, 0, 0));
-  r = r - (T)0.5 * Vc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
+  r = r - mp_ratio((T)0.5 * Vc / (T)1.0 * (ca - cb), ca + cb + epsa) * (T)0.25 *
               (U(0, 1, 0) + U(0, 0, 0) + U(-1, 1, 0) + U(-1, 0, 0)) * (T)1;
   ca = (C(0, 1, -1) + C(0, 0, -1));
   cb = (C(0, 1, 1) + C(0, 0, 1));
-  r = r - (T)0.5 * Vc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
+  r = r - mp_ratio((T)0.5 * Vc / (T)1.0 * (ca - cb), ca + cb + epsa) * (T)0.25 *
               (W(0, 0, 0) + W(0, 1, 0) + W(0, 0, -1) + W(0, 1, -1)) * (T)1;
   return r;
 }
@@ -160,14 +167,14 @@ HC_HD inline T mp_anti_w(const FC &C, const FU &U, const FV &V, const FW &W) {
   const T epsa = (T)1.0e-30;
   T ca = C(0, 0, 0), cb = C(0, 0, 1);
   T Wc = W(0, 0, 0);
-  T r = (mp_fabs(Wc) - Wc * Wc / (T)1.0) * mp_ratio(ca - cb, ca + cb + epsa);
+  T r = mp_ratio((mp_fabs(Wc) - Wc * Wc / (T)1.0) * (ca - cb), ca + cb + epsa);
   ca = (C(1, 0, 0) + C(1, 0, 1));
   cb = (C(-1, 0, 0) + C(-1, 0, 1));
-  r = r - (T)1 * (T)0.5 * Wc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
+  r = r - mp_ratio((T)1 * (T)0.5 * Wc / (T)1.0 * (ca - cb), ca + cb + epsa) * (T)0.25 *
               (U(0, 0, 0) + U(0, 0, 1) + U(-1, 0, 1) + U(-1, 0, 0)) * (T)1;
   ca = (C(0, 1, 1) + C(0, 1, 0));
   cb = (C(0, -1, 1) + C(0, -1, 0));
-  r = r - (T)1 * (T)0.5 * Wc / (T)1.0 * mp_ratio(ca - cb, ca + cb + epsa) * (T)0.25 *
+  r = r - mp_ratio((T)1 * (T)0.5 * Wc / (T)1.0 * (ca - cb), ca + cb + epsa) * (T)0.25 *
               (V(0, 0, 0) + V(0, 0, 1) + V(0, -1, 1) + V(0, -1, 0)) * (T)1;
   return r;
 }
@@ -305,17 +312,21 @@ struct MpTile {
     T c[RS], u[RS], v[RS], w[RS];
   };
 
+  // elements outside the box (so < 0, or planes beyond -1 .. nz+1: the
+  // prefetch of the last plane step asks for nz+2) stage as zero without a
+  // load (loading a clamped in-box address instead measured 10% slower:
+  // a third of the second staging round has no element)
   HC_HD static T fetch1(const MpGeom &g, const T *src, int so, int k) {
-    return (so >= 0 && k >= -1 && k <= g.nz + 1) ? (src + plane(g, k))[so] : (T)0;
+    return (so >= 0 && k >= -1 && k <= g.nz + 1) ? src[HC_MP_GIDX(plane(g, k) + so)] : (T)0;
   }
 
-  HC_HD static void fetch(const MpGeom &g, const Plan &pl, const MpPass<T> &a, int q, Prefetch &pf) {
+  HC_HD static void fetch(const MpGeom &g, const int (&so)[RS], const MpPass<T> &a, int q, Prefetch &pf) {
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
-      pf.c[r] = fetch1(g, a.cc, pl.so[r], q + 1);
-      pf.u[r] = fetch1(g, a.U, pl.so[r], q + 1);
-      pf.v[r] = fetch1(g, a.V, pl.so[r], q + 1);
-      pf.w[r] = fetch1(g, a.W, pl.so[r], q);
+      pf.c[r] = fetch1(g, a.cc, so[r], q + 1);
+      pf.u[r] = fetch1(g, a.U, so[r], q + 1);
+      pf.v[r] = fetch1(g, a.V, so[r], q + 1);
+      pf.w[r] = fetch1(g, a.W, so[r], q);
     }
   }
 
@@ -335,15 +346,16 @@ struct MpTile {
   // the planes the first step needs besides its own prefetch:
   // c(qs-1), c(qs), U(qs), V(qs), W(qs-1); all loads are issued before the
   // first LDS store
-  HC_HD static void stage_initial(const MpGeom &g, int tid, const Plan &pl, Sh &sh, const MpPass<T> &a, int qs) {
+  HC_HD static void stage_initial(const MpGeom &g, int tid, const int (&so)[RS], Sh &sh, const MpPass<T> &a,
+                                  int qs) {
     T v[5][RS];
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
-      v[0][r] = fetch1(g, a.cc, pl.so[r], qs - 1);
-      v[1][r] = fetch1(g, a.cc, pl.so[r], qs);
-      v[2][r] = fetch1(g, a.U, pl.so[r], qs);
-      v[3][r] = fetch1(g, a.V, pl.so[r], qs);
-      v[4][r] = fetch1(g, a.W, pl.so[r], qs - 1);
+      v[0][r] = fetch1(g, a.cc, so[r], qs - 1);
+      v[1][r] = fetch1(g, a.cc, so[r], qs);
+      v[2][r] = fetch1(g, a.U, so[r], qs);
+      v[3][r] = fetch1(g, a.V, so[r], qs);
+      v[4][r] = fetch1(g, a.W, so[r], qs - 1);
     }
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
@@ -430,12 +442,18 @@ struct MpTile {
   // phase C: after the flags of plane q are in LDS, take the own column's
   // values of plane q into registers and finish plane kk = q-1
   HC_HD static void finish(const MpGeom &g, const Plan &pl, Sh &sh, const MpPass<T> &a, int q, MpRegs<T> &r) {
-    const int p = pl.cp;
+    finish_cell(g, pl.cp, pl.co, pl.ci, pl.cj, sh, a, q, r);
+  }
+
+  // the finishing step of the cell (ci, cj) at LDS index p and global (i,j)
+  // offset co (-1: outside the box)
+  HC_HD static void finish_cell(const MpGeom &g, int p, int co, int ci, int cj, Sh &sh, const MpPass<T> &a, int q,
+                                MpRegs<T> &r) {
     const T nu0 = sh.ud[HC_MP_CHECKC(p - 1)], nu1 = sh.ud[HC_MP_CHECKC(p)], nv0 = sh.vd[HC_MP_CHECKC(p - SI)], nv1 = sh.vd[p];
     const T nw = sh.wd[q & 1][p];
     const int nf = sh.fl[q & 1][p];
     const int kk = q - 1;
-    if (kk >= g.ka && kk <= g.kb && pl.co >= 0) {
+    if (kk >= g.ka && kk <= g.kb && co >= 0) {
       const T *c0 = sh.c[kk & 3];
       const unsigned char *f0 = sh.fl[kk & 1];
       T cc = c0[p];
@@ -446,21 +464,21 @@ struct MpTile {
       T cn = cc + ce;
       const long pk = plane(g, kk);
       if (!a.last) {
-        (a.cout + pk)[pl.co] = cn;
+        (a.cout + pk)[co] = cn;
       } else {
-        const int i = pl.ci, j = pl.cj;
+        const int i = ci, j = cj;
         // the boundary images take the pre-sponge value of the cells next
         // to a boundary; only those are stored
-        if (i == 1 || i == g.nx || j == 1 || j == g.ny || kk == 1 || kk == g.nz) (a.cpre + pk)[pl.co] = cn;
+        if (i == 1 || i == g.nx || j == 1 || j == g.ny || kk == 1 || kk == g.nz) (a.cpre + pk)[co] = cn;
         // sponge: c = c - 0.5*alphaee*tau3*(cp - c_bkg(k)), cp = the field at entry
         double ae = fmax(a.alphae0[i + (g.nx + 2) * j], (a.alphaw[kk + 1] + a.alphaw[kk]) / 2.0);
-        T cp = (a.cout + pk)[pl.co];
+        T cp = (a.cout + pk)[co];
         T rr = cn - (T)1 * (T)0.5 * (T)ae * (T)a.tau3 * (cp - (T)a.cbkg[kk]);
         double qv = 0.0;
         if (kk == g.nz && a.map_valid && i >= a.nxs && i <= a.nxe && j >= a.nys && j <= a.nye)
           qv = a.qpatch[(i - a.nxs) + (a.nxe - a.nxs + 1) * (j - a.nys)];
         rr = rr + (T)a.tau3 * (T)qv / (T)a.dxdydz / (T)1.0 * (T)1;
-        (a.cout + pk)[pl.co] = rr;
+        (a.cout + pk)[co] = rr;
       }
     }
     r.u0 = nu0; r.u1 = nu1; r.v0 = nv0; r.v1 = nv1;

@@ -26,6 +26,7 @@
 #include "hc_kernels.h"
 #include "hc_cell.h"
 #include "hc_mpdata.h"
+#include "hc_mpdata2.h"
 #include "hc_tile.h"
 
 namespace hc {
@@ -186,8 +187,8 @@ __global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, i
   typename Tl::Plan pl;
   typename Tl::Prefetch pf;
   Tl::plan(g, tid, (int)(bx + by + bz), pl);
-  Tl::stage_initial(g, tid, pl, sh, a, qs);
-  Tl::fetch(g, pl, a, qs, pf);
+  Tl::stage_initial(g, tid, pl.so, sh, a, qs);
+  Tl::fetch(g, pl.so, a, qs, pf);
   for (int q = qs; q <= g.kb + 1; ++q) {
     // no barrier before the staging: it overwrites the slots of c(q-3),
     // U(q-1), V(q-1), W(q-2), which the previous plane's finishing step does
@@ -195,13 +196,41 @@ __global__ void __launch_bounds__(TI *TJ) k_mpdata_pass(Layout L, MpPass<T> a, i
     // barrier ahead of that step
     Tl::commit(tid, sh, q, pf);
     __syncthreads();
-    if (q <= g.kb) Tl::fetch(g, pl, a, q + 1, pf);  // in flight during this plane
+    if (q <= g.kb) Tl::fetch(g, pl.so, a, q + 1, pf);  // in flight during this plane
     Tl::faces(g, pl, sh, a, q, q >= g.ka - 1);
     __syncthreads();
     if (q >= g.ka - 1) Tl::flags(g, pl, sh, a, q);
     __syncthreads();
     Tl::finish(g, pl, sh, a, q, r);
   }
+}
+
+// the block executor of mp2_block on the device: a phase is each thread's
+// own call, the barrier is __syncthreads
+template <typename T, int TI, int TJ>
+struct DevExec {
+  Mp2State<T, TI, TJ> st;
+  template <class F>
+  __device__ void each(F f) { f((int)threadIdx.x, st); }
+  __device__ void sync() { __syncthreads(); }
+};
+
+// the same pass with the fixed per-thread assignment of MpTile2; at least
+// four waves per SIMD (two 512-thread blocks per CU, what the LDS of a
+// 32 x 16 FP64 tile allows), which caps the registers at 128
+template <typename T, int TI, int TJ>
+__global__ void __launch_bounds__(TI *TJ, 4) k_mpdata_pass2(Layout L, MpPass<T> a, int kc, int remap) {
+  __shared__ MpShared<T, TI, TJ> sh;
+  unsigned bx, by, bz;
+  box_block_coords(remap, bx, by, bz);  // XCD x gets the x-th slice of tiles
+  MpGeom g;
+  g.nx = L.nx; g.ny = L.ny; g.nz = L.nz; g.px = L.px; g.py = L.py;
+  g.i0 = 1 + (int)bx * TI;
+  g.j0 = 1 + (int)by * TJ;
+  g.ka = 1 + (int)bz * kc;
+  g.kb = min(L.nz, g.ka + kc - 1);
+  DevExec<T, TI, TJ> x;
+  mp2_block<T, TI, TJ>(x, g, sh, a);
 }
 
 // ghost cells of a scalar field from their interior images.  One thread per
@@ -334,12 +363,16 @@ int tile_remap() {
   return r;
 }
 
-// tile shape of the corrective-pass kernel (HC_MP_TILE = 0: 32x8, 1: 16x16,
-// 2: 32x16, the default: two 512-thread blocks of 79 KiB LDS per CU)
+// tile of the corrective-pass kernel (HC_MP_TILE = 0, 1, 2: MpTile at
+// 32x8, 16x16, 32x16; 3, the default, and 4: the fixed-assignment MpTile2 at
+// 32x16 and 16x16).  All give the same values bit for bit; at 256^3 the
+// default took 0.403 ms (FP64) and 0.240 ms (FP32) per launch against
+// 0.461 and 0.304 for MpTile 32x16 (two 512-thread blocks of 79 KiB LDS per
+// CU in FP64).
 static int mp_tile() {
   static int t = [] {
     const char *e = getenv("HC_MP_TILE");
-    return e ? atoi(e) : 2;
+    return e ? atoi(e) : 3;
   }();
   return t;
 }
@@ -351,11 +384,20 @@ static void launch_mpdata_tile(hipStream_t st, const Layout &L, const MpPass<T> 
   hipLaunchKernelGGL((k_mpdata_pass<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc, tile_remap());
 }
 
+template <typename T, int TI, int TJ>
+static void launch_mpdata_tile2(hipStream_t st, const Layout &L, const MpPass<T> &a) {
+  const int kc = mp_chunk(L, TI, TJ);
+  dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
+  hipLaunchKernelGGL((k_mpdata_pass2<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc, tile_remap());
+}
+
 template <typename T>
 static void launch_mpdata_pass(hipStream_t st, const Layout &L, const MpPass<T> &a) {
   switch (mp_tile()) {
     case 1: launch_mpdata_tile<T, 16, 16>(st, L, a); break;
     case 2: launch_mpdata_tile<T, 32, 16>(st, L, a); break;
+    case 3: launch_mpdata_tile2<T, 32, 16>(st, L, a); break;
+    case 4: launch_mpdata_tile2<T, 16, 16>(st, L, a); break;
     default: launch_mpdata_tile<T, 32, 8>(st, L, a); break;
   }
 }

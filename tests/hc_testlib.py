@@ -186,6 +186,8 @@ class Oracle:
             L.o_tracer.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double]
             L.o_vent.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
+            L.o_corrective.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int]
             L.o_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
             L.o_step.restype = ctypes.c_int
             L.o_cfl_values.argtypes = [ctypes.c_void_p]
