@@ -31,9 +31,43 @@
 #include "../include/hotcross.h"
 
 /* ------------------------------------------------------------------------ */
+/* Arithmetic type of the state.  Built with -DHC_CS the oracle is also a
+ * tangent-linear model by complex-step differentiation (Squire & Trapp,
+ * SIAM Review 40, 1998; Martins, Sturdza & Alonso, ACM TOMS 29, 2003): every
+ * state value carries x + i*h*dx with h = 1e-30, the arithmetic is complex,
+ * and Im(f)/h is the derivative of f along dx to roundoff, with no
+ * subtraction of nearby values.  The operations that are not analytic
+ * follow the branch the real part takes, as the discrete adjoint does:
+ * |x| -> x*sign(Re x) (0 at 0), max/min pick by the real part, comparisons
+ * compare real parts.  Without HC_CS, R is double and every routine is the
+ * plain restatement. */
+#ifdef HC_CS
+#include <complex.h>
+#undef I /* the oracle names indexes I; the imaginary unit is _Complex_I */
+typedef double _Complex R;
+static inline double re(R x) { return creal(x); }
+static inline double im(R x) { return cimag(x); }
+static inline R rabs(R x) { return creal(x) > 0 ? x : (creal(x) < 0 ? -x : (R)0.0); }
+static inline R rmax(R a, R b) { return creal(a) >= creal(b) ? a : b; }
+static inline R rmin(R a, R b) { return creal(a) <= creal(b) ? a : b; }
+static inline R rsqrt(R x) { return creal(x) == 0.0 && cimag(x) == 0.0 ? (R)0.0 : csqrt(x); }
+static inline R rpow(R x, double y) { return cpow(x, y); }
+#define CS_H 1.0e-30
+#else
+typedef double R;
+static inline double re(R x) { return x; }
+static inline double im(R x) { (void)x; return 0.0; }
+static inline R rabs(R x) { return fabs(x); }
+static inline R rmax(R a, R b) { return fmax(a, b); }
+static inline R rmin(R a, R b) { return fmin(a, b); }
+static inline R rsqrt(R x) { return sqrt(x); }
+static inline R rpow(R x, double y) { return pow(x, y); }
+#endif
+
+/* ------------------------------------------------------------------------ */
 /* Fortran-bounded arrays                                                    */
 typedef struct {
-  double *d;
+  R *d;
   int lo[3], hi[3], n[3];
 } A3;
 typedef struct {
@@ -49,7 +83,7 @@ static void a3_alloc(A3 *a, int l0, int h0, int l1, int h1, int l2, int h2) {
   a->lo[0] = l0; a->hi[0] = h0; a->lo[1] = l1; a->hi[1] = h1;
   a->lo[2] = l2; a->hi[2] = h2;
   for (int q = 0; q < 3; ++q) a->n[q] = a->hi[q] - a->lo[q] + 1;
-  a->d = (double *)calloc((size_t)a->n[0] * a->n[1] * a->n[2], sizeof(double));
+  a->d = (R *)calloc((size_t)a->n[0] * a->n[1] * a->n[2], sizeof(R));
 }
 static void a2_alloc(A2 *a, int l0, int h0, int l1, int h1) {
   a->lo[0] = l0; a->hi[0] = h0; a->lo[1] = l1; a->hi[1] = h1;
@@ -62,7 +96,7 @@ static void i2_alloc(I2 *a, int l0, int h0, int l1, int h1) {
   a->d = (int *)calloc((size_t)a->n[0] * a->n[1], sizeof(int));
 }
 static size_t a3_size(const A3 *a) { return (size_t)a->n[0] * a->n[1] * a->n[2]; }
-static void a3_fill(A3 *a, double v) {
+static void a3_fill(A3 *a, R v) {
   size_t n = a3_size(a);
   for (size_t q = 0; q < n; ++q) a->d[q] = v;
 }
@@ -245,7 +279,10 @@ static double *find1(oracle *o, const char *name) {
 /* copy a whole named array (Fortran order over its full bounds) */
 int o_get(oracle *o, const char *name, double *buf) {
   A3 *a = find3(o, name);
-  if (a) { memcpy(buf, a->d, a3_size(a) * sizeof(double)); return 0; }
+  if (a) {
+    for (size_t q = 0; q < a3_size(a); ++q) buf[q] = re(a->d[q]);
+    return 0;
+  }
   A2 *b = NULL;
   if (!strcmp(name, "alphau")) b = &o->alphau;
   if (!strcmp(name, "alphav")) b = &o->alphav;
@@ -263,7 +300,10 @@ int o_get(oracle *o, const char *name, double *buf) {
 
 int o_set(oracle *o, const char *name, const double *buf) {
   A3 *a = find3(o, name);
-  if (a) { memcpy(a->d, buf, a3_size(a) * sizeof(double)); return 0; }
+  if (a) {
+    for (size_t q = 0; q < a3_size(a); ++q) a->d[q] = buf[q];
+    return 0;
+  }
   A2 *b = NULL;
   if (!strcmp(name, "alphau")) b = &o->alphau;
   if (!strcmp(name, "alphav")) b = &o->alphav;
@@ -277,6 +317,32 @@ int o_set(oracle *o, const char *name, const double *buf) {
   double *z = find1(o, name);
   if (z) { memcpy(z, buf, (size_t)(o->nz + 2 + (!strcmp(name, "alphaw"))) * sizeof(double)); return 0; }
   return -1;
+}
+
+/* The tangent of a state field (the complex-step build): get returns
+ * Im(field)/h, set gives the field the tangent buf (its value unchanged).
+ * In the plain build there is no tangent: get returns zeros, set fails. */
+int o_get_tan(oracle *o, const char *name, double *buf) {
+  A3 *a = find3(o, name);
+  if (!a) return -1;
+#ifdef HC_CS
+  for (size_t q = 0; q < a3_size(a); ++q) buf[q] = im(a->d[q]) / CS_H;
+#else
+  for (size_t q = 0; q < a3_size(a); ++q) buf[q] = 0.0;
+#endif
+  return 0;
+}
+
+int o_set_tan(oracle *o, const char *name, const double *buf) {
+  A3 *a = find3(o, name);
+  if (!a) return -1;
+#ifdef HC_CS
+  for (size_t q = 0; q < a3_size(a); ++q) a->d[q] = re(a->d[q]) + CS_H * buf[q] * _Complex_I;
+  return 0;
+#else
+  (void)buf;
+  return -2;
+#endif
 }
 
 /* scalars: time, mtime, leap, delta, ubkg..., fbx, fby, budgets, cfl */
@@ -296,20 +362,21 @@ void o_set_time(oracle *o, double time, int mtime) { o->time = time; o->mtime = 
  * one-atmosphere polynomial, Gill pp. 599-600).  Integer powers t**n are
  * evaluated as the flang-compiled reference evaluates them (repeated
  * products, pinned bit-for-bit by the reference tests); s**1.5 is pow(). */
-double o_rho_rho(const hc_params *P, double s, double t) {
+static R rho_rho_r(const hc_params *P, R s, R t) {
   if (P->eos == 1)
     return P->eos_rho_ref *
            (1.0 - P->eos_alpha * (t - P->eos_t0) + P->eos_beta * (s - P->eos_s0));
-  double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t4 * t;
-  double rho0 = 999.842594 + 6.793952e-2 * t - 9.095290e-3 * t * t +
-                1.001685e-4 * t3 - 1.120083e-6 * t4 + 6.536332e-9 * t5;
-  double rho1 = rho0 +
-                s * (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t * t -
-                     8.2467e-7 * t3 + 5.3875e-9 * t4) +
-                pow(s, 1.5) * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t * t) +
-                4.8314e-4 * s * s;
+  R t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t4 * t;
+  R rho0 = 999.842594 + 6.793952e-2 * t - 9.095290e-3 * t * t +
+           1.001685e-4 * t3 - 1.120083e-6 * t4 + 6.536332e-9 * t5;
+  R rho1 = rho0 +
+           s * (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t * t -
+                8.2467e-7 * t3 + 5.3875e-9 * t4) +
+           rpow(s, 1.5) * (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t * t) +
+           4.8314e-4 * s * s;
   return rho1;
 }
+double o_rho_rho(const hc_params *P, double s, double t) { return re(rho_rho_r(P, s, t)); }
 
 /* ------------------------------------------------------------------------ */
 /* Host-side set-up (run once): not on the timed path.                       */
@@ -501,12 +568,12 @@ void o_body_force(oracle *o) {
 void o_forces(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dx = P->dx, dy = P->dy, dz = P->dz;
+  R dx = P->dx, dy = P->dy, dz = P->dz;
   int bit2 = P->bit2, bit3 = P->bit3, bit5 = P->bit5;
-  double coe_4x = P->bit4 / 4.0 / dx, coe_4y = P->bit4 / 4.0 / dy, coe_4z = P->bit4 / 4.0 / dz;
-  double coe_7 = P->bit7 * o->fc / 4.0, coe_9 = P->bit9 * o->fc_y / 4.0;
-  double coe_10 = P->bit10;
-  double axmin = P->axmin, aymin = P->aymin;
+  R coe_4x = P->bit4 / 4.0 / dx, coe_4y = P->bit4 / 4.0 / dy, coe_4z = P->bit4 / 4.0 / dz;
+  R coe_7 = P->bit7 * o->fc / 4.0, coe_9 = P->bit9 * o->fc_y / 4.0;
+  R coe_10 = P->bit10;
+  R axmin = P->axmin, aymin = P->aymin;
   A3 ss = o->ss, ssv = o->ss_v;
   /* :922-933 coefficient arrays (axw/ayw are never read after the 2014
    * revision and are not formed) */
@@ -519,7 +586,7 @@ void o_forces(oracle *o) {
   for (int k = 0; k <= nz + 1; ++k)
     for (int j = 0; j <= ny; ++j)
       for (int i = 0; i <= nx; ++i) {
-        double q = X3(ss, i, j, k) + X3(ss, i + 1, j, k) + X3(ss, i, j + 1, k) + X3(ss, i + 1, j + 1, k);
+        R q = X3(ss, i, j, k) + X3(ss, i + 1, j, k) + X3(ss, i, j + 1, k) + X3(ss, i + 1, j + 1, k);
         X3(o->ayu, i, j, k) = .25 * q + aymin;
         X3(o->axv, i, j, k) = .25 * q + axmin;
       }
@@ -539,7 +606,7 @@ void o_forces(oracle *o) {
   /* :953-956 entrance intensification (accumulates on entries that :937-944
    * do not recompute) */
   for (int i = 0; i <= 7; ++i) {
-    double a = (double)(7 - i) * .01 / 7.0;
+    R a = (R)(7 - i) * .01 / 7.0;
     for (int k = 1; k <= nz + 1; ++k) {
       for (int j = 1; j <= ny; ++j) X3(o->azu, i, j, k) = X3(o->azu, i, j, k) + a;
       for (int j = 0; j <= ny; ++j) X3(o->azv, i, j, k) = X3(o->azv, i, j, k) + a;
@@ -553,7 +620,7 @@ void o_forces(oracle *o) {
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 0; i <= nx; ++i) {
-        double dis =
+        R dis =
             2.0 * bit5 * (X3(axu, i + 1, j, k) * (X3(um, i + 1, j, k) - X3(um, i, j, k)) -
                           X3(axu, i, j, k) * (X3(um, i, j, k) - X3(um, i - 1, j, k))) / dx / dx +
             bit5 * (X3(ayu, i, j, k) * (X3(um, i, j + 1, k) - X3(um, i, j, k)) -
@@ -564,7 +631,7 @@ void o_forces(oracle *o) {
                     X3(azu, i, j, k - 1) * (X3(um, i, j, k) - X3(um, i, j, k - 1))) / dz / dz +
             bit2 * (X3(azu, i, j, k) * (X3(wm, i + 1, j, k) - X3(wm, i, j, k)) -
                     X3(azu, i, j, k - 1) * (X3(wm, i + 1, j, k - 1) - X3(wm, i, j, k - 1))) / dx / dz;
-        double a1 = X3(u, i + 1, j, k) + X3(u, i, j, k), a2 = X3(u, i, j, k) + X3(u, i - 1, j, k);
+        R a1 = X3(u, i + 1, j, k) + X3(u, i, j, k), a2 = X3(u, i, j, k) + X3(u, i - 1, j, k);
         X3(o->fx, i, j, k) =
             dis + bit3 * 0.0 - 0.5 * X2(o->alphau, i, j) * (X3(um, i, j, k) - o->ubkg_m) -
             coe_4x * (a1 * a1 - a2 * a2) -
@@ -580,7 +647,7 @@ void o_forces(oracle *o) {
   for (int k = 1; k <= nz; ++k)
     for (int j = 0; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double dis =
+        R dis =
             +bit5 * (X3(axv, i, j, k) * (X3(vm, i + 1, j, k) - X3(vm, i, j, k)) -
                      X3(axv, i - 1, j, k) * (X3(vm, i, j, k) - X3(vm, i - 1, j, k))) / dx / dx +
             bit5 * (X3(axv, i, j, k) * (X3(um, i, j + 1, k) - X3(um, i, j, k)) -
@@ -591,7 +658,7 @@ void o_forces(oracle *o) {
                     X3(azv, i, j, k - 1) * (X3(vm, i, j, k) - X3(vm, i, j, k - 1))) / dz / dz +
             bit2 * (X3(azv, i, j, k) * (X3(wm, i, j + 1, k) - X3(wm, i, j, k)) -
                     X3(azv, i, j, k - 1) * (X3(wm, i, j + 1, k - 1) - X3(wm, i, j, k - 1))) / dy / dz;
-        double b1 = X3(v, i, j + 1, k) + X3(v, i, j, k), b2 = X3(v, i, j, k) + X3(v, i, j - 1, k);
+        R b1 = X3(v, i, j + 1, k) + X3(v, i, j, k), b2 = X3(v, i, j, k) + X3(v, i, j - 1, k);
         X3(o->fy, i, j, k) =
             dis + bit3 * 0.0 - 0.5 * X2(o->alphav, i, j) * (X3(vm, i, j, k) - o->vbkg_m) -
             coe_4y * (b1 * b1 - b2 * b2) -
@@ -606,19 +673,19 @@ void o_forces(oracle *o) {
   for (int k = 0; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double ahp = X3(ss, i, j, k + 1) + axmin, ahm = X3(ss, i, j, k) + axmin;
-        double zk = Z1(o->azzmin, k);
-        double avxp = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i + 1, j, k + 1) + X3(ssv, i + 1, j, k)) + zk;
-        double avxm = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i - 1, j, k + 1) + X3(ssv, i - 1, j, k)) + zk;
-        double avyp = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i, j + 1, k + 1) + X3(ssv, i, j + 1, k)) + zk;
-        double avym = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i, j - 1, k + 1) + X3(ssv, i, j - 1, k)) + zk;
-        double dis =
+        R ahp = X3(ss, i, j, k + 1) + axmin, ahm = X3(ss, i, j, k) + axmin;
+        R zk = Z1(o->azzmin, k);
+        R avxp = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i + 1, j, k + 1) + X3(ssv, i + 1, j, k)) + zk;
+        R avxm = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i - 1, j, k + 1) + X3(ssv, i - 1, j, k)) + zk;
+        R avyp = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i, j + 1, k + 1) + X3(ssv, i, j + 1, k)) + zk;
+        R avym = 0.25 * (X3(ssv, i, j, k + 1) + X3(ssv, i, j, k) + X3(ssv, i, j - 1, k + 1) + X3(ssv, i, j - 1, k)) + zk;
+        R dis =
             2.0 * bit5 * (ahp * (X3(wm, i, j, k + 1) - X3(wm, i, j, k)) - ahm * (X3(wm, i, j, k) - X3(wm, i, j, k - 1))) / dz / dz +
             bit2 * (avxp * (X3(wm, i + 1, j, k) - X3(wm, i, j, k)) - avxm * (X3(wm, i, j, k) - X3(wm, i - 1, j, k))) / dx / dx +
             bit2 * (avxp * (X3(um, i, j, k) - X3(um, i, j, k + 1)) - avxm * (X3(um, i - 1, j, k) - X3(um, i - 1, j, k + 1))) / dz / dx +
             bit2 * (avyp * (X3(wm, i, j + 1, k) - X3(wm, i, j, k)) - avym * (X3(wm, i, j, k) - X3(wm, i, j - 1, k))) / dy / dy +
             bit2 * (avyp * (X3(vm, i, j, k) - X3(vm, i, j, k + 1)) - avym * (X3(vm, i, j - 1, k) - X3(vm, i, j - 1, k + 1))) / dz / dy;
-        double c1 = X3(w, i, j, k + 1) + X3(w, i, j, k), c2 = X3(w, i, j, k) + X3(w, i, j, k - 1);
+        R c1 = X3(w, i, j, k + 1) + X3(w, i, j, k), c2 = X3(w, i, j, k) + X3(w, i, j, k - 1);
         X3(o->fz, i, j, k) =
             dis + bit3 * P->g * (X3(o->rhop, i, j, k) + X3(o->rhop, i, j, k + 1)) / P->rho0 / 2.0 -
             0.0 * Z1(o->alphaw, k) * X3(wm, i, j, k) -
@@ -705,10 +772,10 @@ static void ps_init(oracle *o) {
 }
 
 /* F is (L,M,N) Fortran order, 0-based here */
-static void dct_axis(double *F, int L, int M, int N, int axis, int inverse,
+static void dct_axis(R *F, int L, int M, int N, int axis, int inverse,
                      const double *cs, const double *wt) {
   int len = axis == 1 ? M : N;
-  double *in = malloc(len * sizeof(double)), *out = malloc(len * sizeof(double));
+  R *in = malloc(len * sizeof(R)), *out = malloc(len * sizeof(R));
   int nb1 = L, nb2 = axis == 1 ? N : M;
   for (int b2 = 0; b2 < nb2; ++b2)
     for (int b1 = 0; b1 < nb1; ++b1) {
@@ -719,13 +786,13 @@ static void dct_axis(double *F, int L, int M, int N, int axis, int inverse,
       }
       if (!inverse) {
         for (int k = 0; k < len; ++k) {
-          double acc = 0;
+          R acc = 0;
           for (int j = 0; j < len; ++j) acc += in[j] * cs[(size_t)k * len + j];
           out[k] = wt[k] * acc;
         }
       } else {
         for (int j = 0; j < len; ++j) {
-          double acc = 0;
+          R acc = 0;
           for (int k = 0; k < len; ++k) acc += wt[k] * in[k] * cs[(size_t)k * len + j];
           out[j] = acc;
         }
@@ -740,15 +807,14 @@ static void dct_axis(double *F, int L, int M, int N, int axis, int inverse,
 }
 
 /* HS3CRT, hs3crt_composite80.f:4-143 (types 3/3/3).  F is r(1:nx,1:ny,1:nz). */
-void o_hs3crt(oracle *o, const double *bdxs, const double *bdxf,
-              const double *bdys, const double *bdyf, const double *bdzs,
-              const double *bdzf, double *F) {
+static void o_hs3crt(oracle *o, const R *bdxs, const R *bdxf, const R *bdys, const R *bdyf,
+                     const R *bdzs, const R *bdzf, R *F) {
   if (!o->ps_ready) ps_init(o);
   const hc_params *P = &o->P;
   int L = o->nx, M = o->ny, N = o->nz;
-  double deltax = (L * P->dx - 0.0) / L, deltay = (M * P->dy - 0.0) / M,
+  R deltax = (L * P->dx - 0.0) / L, deltay = (M * P->dy - 0.0) / M,
          deltaz = (N * P->dz - 0.0) / N;
-  double dlxrcp = 1.0 / deltax, dlyrcp = 1.0 / deltay, dlzrcp = 1.0 / deltaz;
+  R dlxrcp = 1.0 / deltax, dlyrcp = 1.0 / deltay, dlzrcp = 1.0 / deltaz;
 #define FF(I, J, K) F[((I) - 1) + (size_t)L * (((J) - 1) + (size_t)M * ((K) - 1))]
   for (int K = 1; K <= N; ++K)
     for (int J = 1; J <= M; ++J)
@@ -772,7 +838,7 @@ void o_hs3crt(oracle *o, const double *bdxs, const double *bdxf,
   dct_axis(F, L, M, N, 1, 0, o->cosy, o->wy);
   dct_axis(F, L, M, N, 2, 0, o->cosz, o->wz);
   /* x solve, :408-431 */
-  double *A = o->psA, *C = o->psC, *FT = o->fctrd;
+  const double *A = o->psA, *C = o->psC, *FT = o->fctrd;
 #define FC(J, I, K) FT[((J) - 1) + (size_t)M * (((I) - 1) + (size_t)L * ((K) - 1))]
   for (int K = 1; K <= N; ++K)
     for (int J = 1; J <= M; ++J) FF(1, J, K) = FF(1, J, K) * FC(J, 1, K);
@@ -794,11 +860,11 @@ void o_hs3crt(oracle *o, const double *bdxs, const double *bdxf,
 void o_pressure(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dx = P->dx, dy = P->dy, dz = P->dz, dt = P->dt;
-  double *bdxs = malloc((size_t)ny * nz * sizeof(double)), *bdxf = malloc((size_t)ny * nz * sizeof(double));
-  double *bdys = malloc((size_t)nx * nz * sizeof(double)), *bdyf = malloc((size_t)nx * nz * sizeof(double));
-  double *bdzs = malloc((size_t)nx * ny * sizeof(double)), *bdzf = malloc((size_t)nx * ny * sizeof(double));
-  double *r = malloc((size_t)nx * ny * nz * sizeof(double));
+  R dx = P->dx, dy = P->dy, dz = P->dz, dt = P->dt;
+  R *bdxs = malloc((size_t)ny * nz * sizeof(R)), *bdxf = malloc((size_t)ny * nz * sizeof(R));
+  R *bdys = malloc((size_t)nx * nz * sizeof(R)), *bdyf = malloc((size_t)nx * nz * sizeof(R));
+  R *bdzs = malloc((size_t)nx * ny * sizeof(R)), *bdzf = malloc((size_t)nx * ny * sizeof(R));
+  R *r = malloc((size_t)nx * ny * nz * sizeof(R));
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j) {
       bdxs[(j - 1) + (size_t)ny * (k - 1)] = -(o->ubkg - o->ubkg_m) / dt - Z1(o->pgradx, k) + X3(o->fx, 0, j, k);
@@ -824,16 +890,16 @@ void o_pressure(oracle *o) {
             ((X3(o->fx, i, j, k) - X3(o->fx, i - 1, j, k)) / dx + (X3(o->fy, i, j, k) - X3(o->fy, i, j - 1, k)) / dy +
              (X3(o->fz, i, j, k) - X3(o->fz, i, j, k - 1)) / dz);
   o_hs3crt(o, bdxs, bdxf, bdys, bdyf, bdzs, bdzf, r);
-  double r111 = RR(1, 1, 1);
+  R r111 = RR(1, 1, 1);
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) X3(o->p, i, j, k) = RR(i, j, k) - r111;
 #undef RR
-  double sumr = 0.0;
+  R sumr = 0.0;
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) sumr = X3(o->p, i, j, k) + sumr;
-  double ave = sumr / (double)nx / (double)ny / (double)nz;
+  R ave = sumr / (R)nx / (R)ny / (R)nz;
   size_t np = a3_size(&o->p);
   for (size_t q = 0; q < np; ++q) o->p.d[q] = o->p.d[q] - ave;
   for (int j = 1; j <= ny; ++j)
@@ -844,7 +910,7 @@ void o_pressure(oracle *o) {
   int bx = P->bcondx, by = P->bcondy;
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j) {
-      double xs = bdxs[(j - 1) + (size_t)ny * (k - 1)], xf = bdxf[(j - 1) + (size_t)ny * (k - 1)];
+      R xs = bdxs[(j - 1) + (size_t)ny * (k - 1)], xf = bdxf[(j - 1) + (size_t)ny * (k - 1)];
       if (P->open_box == 1) {
         X3(o->p, 0, j, k) = (X3(o->p, 1, j, k) - xs * dx);
         X3(o->p, nx + 1, j, k) = (X3(o->p, nx, j, k) + xf * dx);
@@ -855,7 +921,7 @@ void o_pressure(oracle *o) {
     }
   for (int i = 1; i <= nx; ++i)
     for (int k = 1; k <= nz; ++k) {
-      double ys = bdys[(i - 1) + (size_t)nx * (k - 1)], yf = bdyf[(i - 1) + (size_t)nx * (k - 1)];
+      R ys = bdys[(i - 1) + (size_t)nx * (k - 1)], yf = bdyf[(i - 1) + (size_t)nx * (k - 1)];
       X3(o->p, i, 0, k) = (X3(o->p, i, 1, k) - ys * dy) * by + (1 - by) * X3(o->p, i, ny, k);
       X3(o->p, i, ny + 1, k) = (X3(o->p, i, ny, k) + yf * dy) * by + (1 - by) * X3(o->p, i, 1, k);
     }
@@ -866,7 +932,7 @@ void o_pressure(oracle *o) {
 void o_momentum_u(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dt = P->dt, dx = P->dx;
+  R dt = P->dt, dx = P->dx;
   int delta = o->delta, leap = o->leap;
   a3_fill(&o->up, MISSING);
   for (int k = 1; k <= nz; ++k)
@@ -896,7 +962,7 @@ void o_momentum_u(oracle *o) {
 void o_momentum_v(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dt = P->dt, dy = P->dy;
+  R dt = P->dt, dy = P->dy;
   int delta = o->delta, leap = o->leap;
   a3_fill(&o->vp, MISSING);
   for (int k = 1; k <= nz; ++k)
@@ -926,21 +992,21 @@ void o_momentum_v(oracle *o) {
 void o_momentum_w(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dt = P->dt, dz = P->dz;
+  R dt = P->dt, dz = P->dz;
   int delta = o->delta, leap = o->leap;
   for (int k = 1; k <= nz - 1; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i)
         X3(o->wp, i, j, k) = (1 - delta) * X3(o->w, i, j, k) + delta * X3(o->wm, i, j, k) +
                              leap * dt * (-(X3(o->p, i, j, k + 1) - X3(o->p, i, j, k)) / dz + X3(o->fz, i, j, k));
-  double sumup = 0.0, sumdown = 0.0;
+  R sumup = 0.0, sumdown = 0.0;
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i)
       if (X2(o->alphae, i, j) == 0) {
         if (X2(o->isource_map, i, j) == 1) sumup = sumup + 1.0; else sumdown = sumdown + 1.0;
       }
   double wramp_hrs = P->wramp_hrs;
-  double wup = P->wsource * fmin(1.0, (o->mtime * dt) / (wramp_hrs * 3600.0));
+  double wup = P->wsource * fmin(1.0, (o->mtime * P->dt) / (wramp_hrs * 3600.0));
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
       X3(o->wp, i, j, -1) = 0.0; X3(o->wp, i, j, nz + 1) = 0.0;
@@ -972,23 +1038,23 @@ void o_momentum_w(oracle *o) {
 void o_viscosity3d(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dx = P->dx, dy = P->dy, dz = P->dz;
-  double ls = sqrt(dx * dy);
-  double coef = P->ismag_h * ((P->cs_h * ls) * (P->cs_h * ls));
-  double eps30 = 1.0e-30;
+  R dx = P->dx, dy = P->dy, dz = P->dz;
+  R ls = rsqrt(dx * dy);
+  R coef = P->ismag_h * ((P->cs_h * ls) * (P->cs_h * ls));
+  R eps30 = 1.0e-30;
   a3_fill(&o->ss, 0.0);
   A3 um = o->um, vm = o->vm, wm = o->wm;
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double s11 = (X3(um, i, j, k) - X3(um, i - 1, j, k)) / dx;
-        double s22 = (X3(vm, i, j, k) - X3(vm, i, j - 1, k)) / dy;
-        double s33 = (X3(wm, i, j, k) - X3(wm, i, j, k - 1)) / dz;
-        double s12 = 0.25 * ((X3(um, i, j + 1, k) + X3(um, i - 1, j + 1, k) - X3(um, i, j - 1, k) - X3(um, i - 1, j - 1, k)) / dy +
+        R s11 = (X3(um, i, j, k) - X3(um, i - 1, j, k)) / dx;
+        R s22 = (X3(vm, i, j, k) - X3(vm, i, j - 1, k)) / dy;
+        R s33 = (X3(wm, i, j, k) - X3(wm, i, j, k - 1)) / dz;
+        R s12 = 0.25 * ((X3(um, i, j + 1, k) + X3(um, i - 1, j + 1, k) - X3(um, i, j - 1, k) - X3(um, i - 1, j - 1, k)) / dy +
                              (X3(vm, i + 1, j, k) + X3(vm, i + 1, j - 1, k) - X3(vm, i - 1, j, k) - X3(vm, i - 1, j - 1, k)) / dx) / 2.0;
-        double q = 2.0 * (s11 * s11) + 2.0 * (s22 * s22) + 4.0 * (s12 * s12) + 2.0 * (s33 * s33);
+        R q = 2.0 * (s11 * s11) + 2.0 * (s22 * s22) + 4.0 * (s12 * s12) + 2.0 * (s33 * s33);
         X3(o->epsh, i, j, k) = q;
-        double a = coef * sqrt(q + eps30);
+        R a = coef * rsqrt(q + eps30);
         X3(o->ss, i, j, k) = a;
         X3(o->epsh, i, j, k) = X3(o->epsh, i, j, k) * (a + P->axmin);
       }
@@ -997,19 +1063,19 @@ void o_viscosity3d(oracle *o) {
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double s13 = 0.25 * ((X3(um, i, j, k - 1) + X3(um, i - 1, j, k - 1) - X3(um, i, j, k + 1) - X3(um, i - 1, j, k + 1)) / dz +
+        R s13 = 0.25 * ((X3(um, i, j, k - 1) + X3(um, i - 1, j, k - 1) - X3(um, i, j, k + 1) - X3(um, i - 1, j, k + 1)) / dz +
                              (X3(wm, i + 1, j, k) + X3(wm, i + 1, j, k - 1) - X3(wm, i - 1, j, k) - X3(wm, i - 1, j, k - 1)) / dx) / 2.0;
-        double s23 = 0.25 * ((X3(vm, i, j, k - 1) + X3(vm, i, j - 1, k - 1) - X3(vm, i, j, k + 1) - X3(vm, i, j - 1, k + 1)) / dz +
+        R s23 = 0.25 * ((X3(vm, i, j, k - 1) + X3(vm, i, j - 1, k - 1) - X3(vm, i, j, k + 1) - X3(vm, i, j - 1, k + 1)) / dz +
                              (X3(wm, i, j + 1, k) + X3(wm, i, j + 1, k - 1) - X3(wm, i, j - 1, k) - X3(wm, i, j - 1, k - 1)) / dy) / 2.0;
-        double sv = 4.0 * (s13 * s13) + 4.0 * (s23 * s23);
-        double ri = -(P->g * ((X3(o->rhop, i, j, k - 1) + Z1(o->rho_bkg, k - 1)) -
+        R sv = 4.0 * (s13 * s13) + 4.0 * (s23 * s23);
+        R ri = -(P->g * ((X3(o->rhop, i, j, k - 1) + Z1(o->rho_bkg, k - 1)) -
                               (X3(o->rhop, i, j, k + 1) + Z1(o->rho_bkg, k + 1))) /
                       2.0 / dz / P->rho0 / (sv + eps30));
         X3(o->ri, i, j, k) = ri;
-        double rf = 3;
+        R rf = 3;
         X3(o->epsv, i, j, k) = sv;
-        double arg = 1.0 + P->eps - ri / rf;
-        double a = coef * sqrt(sv + eps30) * sqrt(fmax(0.0, arg));
+        R arg = 1.0 + P->eps - ri / rf;
+        R a = coef * rsqrt(sv + eps30) * rsqrt(rmax(0.0, arg));
         X3(o->ss_v, i, j, k) = a;
         X3(o->epsv, i, j, k) = X3(o->epsv, i, j, k) * (a + P->azmin);
       }
@@ -1071,7 +1137,7 @@ void o_mixing(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
   a3_fill(&o->kxt, 0.0); a3_fill(&o->kyt, 0.0); a3_fill(&o->kzt, 0.0);
-  double pr = P->prandtl;
+  R pr = P->prandtl;
   for (int k = 1; k <= nz; ++k)
     for (int j = 0; j <= ny + 1; ++j)
       for (int i = 0; i <= nx; ++i)
@@ -1131,7 +1197,7 @@ static void o_boundary_conditions(oracle *o, A3 *c, const double *c_bkg) {
 
 /* MPDATA, mpdata-hotcross80.f:497-726 (capG = hzmn = 1) */
 static void o_mpdata(oracle *o, A3 *cb, A3 *ca, A3 *uca, A3 *vca, A3 *wca,
-                     int iord, double cmin, double cmax) {
+                     int iord, R cmin, R cmax) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   A3 upi, umi, vpj, vmj, wpk, wmk;
   a3_alloc(&upi, -1, nx + 1, 0, ny + 1, 0, nz + 1); a3_alloc(&umi, -1, nx + 1, 0, ny + 1, 0, nz + 1);
@@ -1140,21 +1206,21 @@ static void o_mpdata(oracle *o, A3 *cb, A3 *ca, A3 *uca, A3 *vca, A3 *wca,
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 0; i <= nx; ++i) {
-        double a = X3(*uca, i, j, k), wa = fabs(a);
+        R a = X3(*uca, i, j, k), wa = rabs(a);
         X3(upi, i, j, k) = 0.5 * (a + wa);
         X3(umi, i, j, k) = 0.5 * (a - wa);
       }
   for (int k = 1; k <= nz; ++k)
     for (int j = 0; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double a = X3(*vca, i, j, k), wa = fabs(a);
+        R a = X3(*vca, i, j, k), wa = rabs(a);
         X3(vpj, i, j, k) = 0.5 * (a + wa);
         X3(vmj, i, j, k) = 0.5 * (a - wa);
       }
   for (int k = 0; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double a = X3(*wca, i, j, k), wa = fabs(a);
+        R a = X3(*wca, i, j, k), wa = rabs(a);
         X3(wpk, i, j, k) = 0.5 * (a + wa);
         X3(wmk, i, j, k) = 0.5 * (a - wa);
       }
@@ -1176,12 +1242,12 @@ static void o_mpdata(oracle *o, A3 *cb, A3 *ca, A3 *uca, A3 *vca, A3 *wca,
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i) {
-          double x = X3(*ca, i, j, k) + X3(*cb, i, j, k);
-          if (x <= cmin) {
+          R x = X3(*ca, i, j, k) + X3(*cb, i, j, k);
+          if (re(x) <= re(cmin)) {
             X3(umi, i - 1, j, k) = 0.0; X3(upi, i, j, k) = 0.0;
             X3(vmj, i, j - 1, k) = 0.0; X3(vpj, i, j, k) = 0.0;
             X3(wpk, i, j, k - 1) = 0.0; X3(wmk, i, j, k) = 0.0;
-          } else if (x >= cmax) {
+          } else if (re(x) >= re(cmax)) {
             X3(upi, i - 1, j, k) = 0.0; X3(umi, i, j, k) = 0.0;
             X3(vpj, i, j - 1, k) = 0.0; X3(vmj, i, j, k) = 0.0;
             X3(wpk, i, j, k) = 0.0; X3(wmk, i, j, k - 1) = 0.0;
@@ -1200,19 +1266,19 @@ static void o_mpdata(oracle *o, A3 *cb, A3 *ca, A3 *uca, A3 *vca, A3 *wca,
  * the Courant numbers in uc, vc, wc (:218-326), the limited donor step
  * (MPDATA), the boundary images, and ud, vd, wd copied into uc, vc, wc for the
  * next pass (:345-347).  ce is work space over the scalar bounds. */
-static void o_corrective_pass(oracle *o, A3 *c, A3 *ce, const double *c_bkg, int iord, double cmin,
-                              double cmax) {
+static void o_corrective_pass(oracle *o, A3 *c, A3 *ce, const double *c_bkg, int iord, R cmin,
+                              R cmax) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   size_t nc = a3_size(c);
-  double epsa = 1.0e-30;
-  memcpy(o->cc.d, c->d, nc * sizeof(double));
+  R epsa = 1.0e-30;
+  memcpy(o->cc.d, c->d, nc * sizeof(R));
   A3 cc = o->cc, uc = o->uc, vc = o->vc, wc = o->wc;
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 0; i <= nx; ++i) {
-        double ca = X3(cc, i + 1, j, k), cb = X3(cc, i, j, k);
-        double U = X3(uc, i, j, k);
-        double ud = (fabs(U) - U * U / 1.0) * (ca - cb) / (ca + cb + epsa);
+        R ca = X3(cc, i + 1, j, k), cb = X3(cc, i, j, k);
+        R U = X3(uc, i, j, k);
+        R ud = (rabs(U) - U * U / 1.0) * (ca - cb) / (ca + cb + epsa);
         ca = (X3(cc, i + 1, j + 1, k) + X3(cc, i, j + 1, k));
         cb = (X3(cc, i + 1, j - 1, k) + X3(cc, i, j - 1, k));
         ud = ud - 1 * 0.5 * U / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
@@ -1226,9 +1292,9 @@ static void o_corrective_pass(oracle *o, A3 *c, A3 *ce, const double *c_bkg, int
   for (int k = 1; k <= nz; ++k)
     for (int j = 0; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double ca = X3(cc, i, j + 1, k), cb = X3(cc, i, j, k);
-        double V = X3(vc, i, j, k);
-        double vd = (fabs(V) - V * V / 1.0) * 1 * (ca - cb) / (ca + cb + epsa);
+        R ca = X3(cc, i, j + 1, k), cb = X3(cc, i, j, k);
+        R V = X3(vc, i, j, k);
+        R vd = (rabs(V) - V * V / 1.0) * 1 * (ca - cb) / (ca + cb + epsa);
         ca = (X3(cc, i + 1, j + 1, k) + X3(cc, i + 1, j, k));
         cb = (X3(cc, i - 1, j + 1, k) + X3(cc, i - 1, j, k));
         vd = vd - 0.5 * V / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
@@ -1242,9 +1308,9 @@ static void o_corrective_pass(oracle *o, A3 *c, A3 *ce, const double *c_bkg, int
   for (int k = 0; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double ca = X3(cc, i, j, k), cb = X3(cc, i, j, k + 1);
-        double W = X3(wc, i, j, k);
-        double wd = (fabs(W) - W * W / 1.0) * (ca - cb) / (ca + cb + epsa);
+        R ca = X3(cc, i, j, k), cb = X3(cc, i, j, k + 1);
+        R W = X3(wc, i, j, k);
+        R wd = (rabs(W) - W * W / 1.0) * (ca - cb) / (ca + cb + epsa);
         ca = (X3(cc, i + 1, j, k) + X3(cc, i + 1, j, k + 1));
         cb = (X3(cc, i - 1, j, k) + X3(cc, i - 1, j, k + 1));
         wd = wd - 1 * 0.5 * W / 1.0 * (ca - cb) / (ca + cb + epsa) * 0.25 *
@@ -1258,19 +1324,19 @@ static void o_corrective_pass(oracle *o, A3 *c, A3 *ce, const double *c_bkg, int
   o_mpdata(o, ce, &o->cc, &o->ud, &o->vd, &o->wd, iord, cmin, cmax);
   for (size_t q = 0; q < nc; ++q) c->d[q] = (c->d[q] + ce->d[q]);
   o_boundary_conditions(o, c, c_bkg);
-  memcpy(o->uc.d, o->ud.d, a3_size(&o->uc) * sizeof(double));
-  memcpy(o->vc.d, o->vd.d, a3_size(&o->vc) * sizeof(double));
-  memcpy(o->wc.d, o->wd.d, a3_size(&o->wc) * sizeof(double));
+  memcpy(o->uc.d, o->ud.d, a3_size(&o->uc) * sizeof(R));
+  memcpy(o->vc.d, o->vd.d, a3_size(&o->vc) * sizeof(R));
+  memcpy(o->wc.d, o->wd.d, a3_size(&o->wc) * sizeof(R));
 }
 
 /* TRACER31, mpdata-hotcross80.f:16-433 (iord_number = 3, ismol from the
  * case).  budgets receives c_total, sum_sponge, sum_source when
  * mod(mtime,100)==0. */
 void o_tracer31(oracle *o, A3 *c, A3 *q_c, const double *c_bkg /* offset 1 */,
-                double cmin, double cmax, double tau3) {
+                R cmin, R cmax, R tau3) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
-  double dx = P->dx, dy = P->dy, ds = P->dz;
+  R dx = P->dx, dy = P->dy, ds = P->dz;
   a3_fill(&o->uc, 0); a3_fill(&o->vc, 0); a3_fill(&o->wc, 0);
   a3_fill(&o->cp, 0); a3_fill(&o->cc, 0);
   a3_fill(&o->ud, 0); a3_fill(&o->vd, 0); a3_fill(&o->wd, 0);
@@ -1284,20 +1350,20 @@ void o_tracer31(oracle *o, A3 *c, A3 *q_c, const double *c_bkg /* offset 1 */,
   for (int k = 0; k <= nz; ++k)
     for (int j = 0; j <= ny + 1; ++j)
       for (int i = 0; i <= nx + 1; ++i) X3(o->wc, i, j, k) = -X3(o->workw, i, j, k) * tau3 / ds;
-  memcpy(o->cp.d, c->d, a3_size(c) * sizeof(double));
-  double xfac = (double)(1 * P->bit5k) / dx / dx * tau3 * 1;
-  double yfac = (double)(1 * P->bit5k) / dy / dy * tau3 * 1;
-  double zfac = (double)P->bit2k / ds / ds * tau3 * 1;
+  memcpy(o->cp.d, c->d, a3_size(c) * sizeof(R));
+  R xfac = (R)(1 * P->bit5k) / dx / dx * tau3 * 1;
+  R yfac = (R)(1 * P->bit5k) / dy / dy * tau3 * 1;
+  R zfac = (R)P->bit2k / ds / ds * tau3 * 1;
   A3 cp = o->cp, kxt = o->kxt, kyt = o->kyt, kzt = o->kzt;
   /* :141-174 explicit diffusion */
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
-        double cx = xfac * (X3(kxt, i, j, k) * (X3(cp, i + 1, j, k) - X3(cp, i, j, k)) -
+        R cx = xfac * (X3(kxt, i, j, k) * (X3(cp, i + 1, j, k) - X3(cp, i, j, k)) -
                             X3(kxt, i - 1, j, k) * (X3(cp, i, j, k) - X3(cp, i - 1, j, k)));
-        double cy = yfac * (X3(kyt, i, j, k) * (X3(cp, i, j + 1, k) - X3(cp, i, j, k)) -
+        R cy = yfac * (X3(kyt, i, j, k) * (X3(cp, i, j + 1, k) - X3(cp, i, j, k)) -
                             X3(kyt, i, j - 1, k) * (X3(cp, i, j, k) - X3(cp, i, j - 1, k)));
-        double cz = zfac * (X3(kzt, i, j, k - 1) * ((X3(cp, i, j, k - 1) + Z1(c_bkg, k)) - (Z1(c_bkg, k - 1) + X3(cp, i, j, k))) -
+        R cz = zfac * (X3(kzt, i, j, k - 1) * ((X3(cp, i, j, k - 1) + Z1(c_bkg, k)) - (Z1(c_bkg, k - 1) + X3(cp, i, j, k))) -
                             X3(kzt, i, j, k) * ((X3(cp, i, j, k) + Z1(c_bkg, k + 1)) - (Z1(c_bkg, k) + X3(cp, i, j, k + 1))));
         X3(*c, i, j, k) = X3(*c, i, j, k) + (+cx + cy + cz);
       }
@@ -1319,22 +1385,22 @@ void o_tracer31(oracle *o, A3 *c, A3 *q_c, const double *c_bkg /* offset 1 */,
       }
   /* :384-423 budgets */
   if (o->mtime % 100 == 0) {
-    double sum = 0.0;
+    R sum = 0.0;
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i) sum = sum + (X3(*c, i, j, k) - Z1(c_bkg, k)) * 1.0 * 1;
-    o->budgets[0] = sum * dx * dy * ds;
+    o->budgets[0] = re(sum * dx * dy * ds);
     sum = 0.0;
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i)
           sum = sum - 1 * X2(o->alphae, i, j) * (X3(*c, i, j, k) - Z1(c_bkg, k)) * 1.0 * 1;
-    o->budgets[1] = sum * dx * dy * ds * tau3;
+    o->budgets[1] = re(sum * dx * dy * ds * tau3);
     sum = 0.0;
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i) sum = sum + tau3 * X3(*q_c, i, j, k) * 1;
-    o->budgets[2] = sum;
+    o->budgets[2] = re(sum);
   }
 }
 
@@ -1344,7 +1410,7 @@ void o_density(oracle *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
 #define RHO_AT(i, j, k)                                                                          \
   do {                                                                                           \
-    X3(o->rhop, i, j, k) = o_rho_rho(P, X3(o->s, i, j, k), X3(o->t, i, j, k)) - Z1(o->rho_bkg, k); \
+    X3(o->rhop, i, j, k) = rho_rho_r(P, X3(o->s, i, j, k), X3(o->t, i, j, k)) - Z1(o->rho_bkg, k); \
     X3(o->rrho, i, j, k) = X3(o->rhop, i, j, k) + Z1(o->rho_bkg, k);                             \
   } while (0)
   for (int k = 1; k <= nz; ++k)
@@ -1361,7 +1427,7 @@ void o_density(oracle *o) {
 
 /* REFRESH, hotcross80-revised2014.f:1411-1493 */
 void o_refresh(oracle *o) {
-  double assel = o->P.assel;
+  R assel = o->P.assel;
   A3 *tri[3][3] = {{&o->u, &o->up, &o->um}, {&o->v, &o->vp, &o->vm}, {&o->w, &o->wp, &o->wm}};
   for (int q = 0; q < 3; ++q) {
     A3 *a = tri[q][0], *ap = tri[q][1], *am = tri[q][2];
@@ -1385,13 +1451,13 @@ int o_cfl_values(oracle *o) {
   double mnu = 1e300, mxu = -1e300, mnv = 1e300, mxv = -1e300, mnw = 1e300, mxw = -1e300;
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
-      for (int i = 0; i <= nx; ++i) { double a = X3(o->u, i, j, k); mnu = fmin(mnu, a); mxu = fmax(mxu, a); }
+      for (int i = 0; i <= nx; ++i) { double a = re(X3(o->u, i, j, k)); mnu = fmin(mnu, a); mxu = fmax(mxu, a); }
   for (int k = 1; k <= nz; ++k)
     for (int j = 0; j <= ny; ++j)
-      for (int i = 1; i <= nx; ++i) { double a = X3(o->v, i, j, k); mnv = fmin(mnv, a); mxv = fmax(mxv, a); }
+      for (int i = 1; i <= nx; ++i) { double a = re(X3(o->v, i, j, k)); mnv = fmin(mnv, a); mxv = fmax(mxv, a); }
   for (int k = 0; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
-      for (int i = 1; i <= nx; ++i) { double a = X3(o->w, i, j, k); mnw = fmin(mnw, a); mxw = fmax(mxw, a); }
+      for (int i = 1; i <= nx; ++i) { double a = re(X3(o->w, i, j, k)); mnw = fmin(mnw, a); mxw = fmax(mxw, a); }
   o->cfl[0] = mxu * P->dt / P->dx; o->cfl[1] = mnu * P->dt / P->dx;
   o->cfl[2] = mxv * P->dt / P->dy; o->cfl[3] = mnv * P->dt / P->dy;
   o->cfl[4] = mxw * P->dt / P->dz; o->cfl[5] = mnw * P->dt / P->dz;
@@ -1511,5 +1577,20 @@ int o_vent(oracle *o, const char *name, double value) {
 void o_hs3crt_c(oracle *o, const double *bdxs, const double *bdxf,
                 const double *bdys, const double *bdyf, const double *bdzs,
                 const double *bdzf, double *F) {
-  o_hs3crt(o, bdxs, bdxf, bdys, bdyf, bdzs, bdzf, F);
+  const int nx = o->nx, ny = o->ny, nz = o->nz;
+  const double *in[6] = {bdxs, bdxf, bdys, bdyf, bdzs, bdzf};
+  const size_t len[6] = {(size_t)ny * nz, (size_t)ny * nz, (size_t)nx * nz, (size_t)nx * nz, (size_t)nx * ny,
+                         (size_t)nx * ny};
+  R *bd[6];
+  for (int q = 0; q < 6; ++q) {
+    bd[q] = malloc(len[q] * sizeof(R));
+    for (size_t e = 0; e < len[q]; ++e) bd[q][e] = in[q][e];
+  }
+  const size_t nf = (size_t)nx * ny * nz;
+  R *f = malloc(nf * sizeof(R));
+  for (size_t e = 0; e < nf; ++e) f[e] = F[e];
+  o_hs3crt(o, bd[0], bd[1], bd[2], bd[3], bd[4], bd[5], f);
+  for (size_t e = 0; e < nf; ++e) F[e] = re(f[e]);
+  for (int q = 0; q < 6; ++q) free(bd[q]);
+  free(f);
 }

@@ -16,6 +16,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_CS_SO = os.path.join(ROOT, "oracle", "liboracle_cs.so")
 REF_DIR = "/root/reference"
 
 
@@ -164,10 +165,12 @@ class Oracle:
     """ctypes wrapper of oracle/liboracle.so (the C restatement)."""
     _lib = None
 
+    _path = ORACLE_SO
+
     @classmethod
     def lib(cls):
         if cls._lib is None:
-            L = ctypes.CDLL(ORACLE_SO)
+            L = ctypes.CDLL(cls._path)
             L.o_create.restype = ctypes.c_void_p
             L.o_create.argtypes = [ctypes.POINTER(Params)]
             for f in ("o_destroy", "o_sponge_maker", "o_initialize", "o_body_force",
@@ -188,6 +191,8 @@ class Oracle:
             L.o_vent.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
             L.o_corrective.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int]
+            L.o_get_tan.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+            L.o_set_tan.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
             L.o_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
             L.o_step.restype = ctypes.c_int
             L.o_cfl_values.argtypes = [ctypes.c_void_p]
@@ -271,6 +276,36 @@ class Oracle:
         v = [ctypes.c_int() for _ in range(4)]
         self.L.o_source_patch(ctypes.byref(self.prm), *[ctypes.byref(x) for x in v])
         return tuple(x.value for x in v)
+
+
+class OracleCS(Oracle):
+    """The oracle built with -DHC_CS (oracle/liboracle_cs.so): every state
+    value carries a tangent by complex-step differentiation, so the oracle's
+    routines are also their tangent-linear model.  get/set read and write
+    the values, get_tan/set_tan the tangents (set_tan keeps the value)."""
+    _lib = None
+    _path = ORACLE_CS_SO
+
+
+def _get_tan(self, name):
+    b = self.bounds(name)
+    buf = np.zeros(shape_of(b), dtype=np.float64, order="F")
+    if self.L.o_get_tan(self.h, name.encode(), buf.ctypes.data) != 0:
+        raise KeyError(name)
+    return buf
+
+
+def _set_tan(self, name, arr):
+    b = self.bounds(name)
+    buf = np.asfortranarray(np.asarray(arr, dtype=np.float64).reshape(shape_of(b), order="F"))
+    if self.L.o_set_tan(self.h, name.encode(), buf.ctypes.data) != 0:
+        raise KeyError(name)
+
+
+# tangents of the state (only the complex-step build carries them; the plain
+# build reads zeros and refuses to set one)
+Oracle.get_tan = _get_tan
+Oracle.set_tan = _set_tan
 
 
 def rho_rho(prm, s, t):
