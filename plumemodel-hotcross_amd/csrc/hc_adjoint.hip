@@ -85,7 +85,8 @@ struct RelA {
 // sum of one double per thread into one of the 256 slots at `slots` (one
 // atomic per block, spread over the slots so that blocks do not queue on one
 // address; the slots are summed on the host); every thread of the block must
-// call it
+// call it.  The closing barrier keeps a following call (which reuses `sh`)
+// from overwriting the partial sums before thread 0 has read them.
 __device__ inline void block_add(double v, double *slots) {
   const int tid = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
   const int nt = (int)(blockDim.x * blockDim.y * blockDim.z);
@@ -99,6 +100,7 @@ __device__ inline void block_add(double v, double *slots) {
     const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     if (s != 0.0) atomicAdd(slots + (b & (ADJ_SLOTS - 1)), s);
   }
+  __syncthreads();
 }
 
 struct Box3 {
@@ -404,6 +406,14 @@ __global__ void __launch_bounds__(256) k_adj_visc_shell(Layout L, StepArgs s, Ph
   }
   block_add(sh, gcs);
   block_add(sv, gcs + ADJ_SLOTS);
+}
+
+// zero the adjoint of a field over a box of cells (those the forward
+// overwrote); the adjoint of the cells it left alone passes through
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_zero_box(Layout L, Box3 b, T *__restrict__ p) {
+  ADJ_IJK(b);
+  if (in) F3(p, i, j, k) = (T)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1028,7 +1038,11 @@ static void adj_viscosity_t(hc_ctx *ctx, const T *ss1, const T *ssv1) {
                      (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
                      (const double *)ctx->aux.rho_bkg, ss1, ssv1, (const T *)lam.ss, (const T *)lam.ssv, lam.um,
                      lam.vm, lam.wm, lam.rhop, A->d_sc + ADJ_SLOTS, kc);
-  zero(ctx, lam.ss); zero(ctx, lam.ssv);
+  // VISCOSITY3D writes every cell of ss (constants on the edges and the k
+  // ghosts) but leaves ss_v on the i and j ghost columns as it was
+  zero(ctx, lam.ss);
+  const Box3 vb{1, L.nx, 1, L.ny, 0, L.nz + 1};
+  hipLaunchKernelGGL(k_adj_zero_box<T>, grid3(vb), dim3(64, 4), 0, ctx->st, L, vb, lam.ssv);
 }
 
 template <typename T>
