@@ -6,7 +6,8 @@ roofline and the reference's OpenMP CPU path timed beside it.
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload les256]
                 [--precision fp64|fp32] [--no-cpu] [--no-fp32] [--no-adjoint]
                 [--only-adjoint] [--adjoint-workload les128] [--adjoint-steps 400]
-                [--adjoint-seg 20] [--adjoint-precision fp64] [--c5-steps 1800]
+                [--adjoint-seg 20] [--adjoint-precision fp64] [--adjoint-budget-gb G]
+                [--c5-steps 1800] [--c5-budget-gb 200]
 
 A step is one pass of the reference's time loop body
 (hotcross80-revised2014.f:264-350: body force, forces, pressure solve,
@@ -18,8 +19,9 @@ The same line carries the FP32 form's throughput on the same workload and its
 error against the FP64 form after 50 steps from the same state ("fp32"), and
 the adjoint's cost: "adjoint" is the 128^3 FP64 adjoint configuration (400
 steps, a checkpoint every 20, 25 probes at 1 Hz); "adjoint_256" the 256^3
-FP32 forward + adjoint configuration (1800 steps, the checkpoint interval
-plan_segment chooses under 200 GB).  Each window is timed beside the same
+FP32 forward + adjoint configuration (1800 steps, the checkpointing
+plan_window chooses under 200 GB: a fixed interval where one fits, binomial
+checkpointing otherwise).  Each window is timed beside the same
 number of forward steps of the step body the window runs (T and S, no passive
 tracer), and reports the reverse sweep per forward step, (window - forward) /
 forward, and the whole window per forward step.
@@ -43,7 +45,7 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 # algorithmic HBM bytes per cell-step: 130.3 field moves (compulsory reads and
 # writes of every reference subroutine, counted once each) x 8 B in FP64
 FIELD_MOVES = 130.3
-# MPDATA corrective pass, one launch of k_mpdata_pass: pass 2 reads c and the
+# MPDATA corrective pass, one launch of k_mpdata_pass2: pass 2 reads c and the
 # three Courant numbers and writes the three antidiffusive velocities and the
 # result (8 field moves per cell); pass 3 reads c, the three antidiffusive
 # velocities and the field's entry value and writes the result (6 moves).
@@ -52,9 +54,9 @@ MPDATA_PASS_MOVES = 7.0
 
 
 def profiled_traffic(workload, precision):
-    """HBM bytes per launch of k_mpdata_pass from the committed rocprofv3
+    """HBM bytes per launch of k_mpdata_pass2 from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload (None if there is none)."""
-    path = os.path.join(ROOT, "profiles", "k_mpdata_pass_%s_%s_hbm.md" % (workload, precision))
+    path = os.path.join(ROOT, "profiles", "k_mpdata_pass2_%s_%s_hbm.md" % (workload, precision))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -107,8 +109,10 @@ def adjoint_leg(workload="les128", nsteps=400, seg=20, precision="fp64", budget=
     """The adjoint window of an adjoint configuration: nsteps steps of the
     timestep (FP64 unless precision is fp32), 25 probes on the plane 20 m
     above the vent sampled at 1 Hz, a checkpoint every seg steps (or the
-    interval plan_segment chooses under `budget` bytes), then the reverse
-    sweep (segments recomputed into an HBM tape, adjoint of every step).
+    interval plan_window chooses under `budget` bytes, or binomial
+    checkpointing with as many snapshots as fit where no interval does),
+    then the reverse sweep (segments recomputed into an HBM tape, adjoint of
+    every step).
     Timed beside the same number of forward steps (hc_step) of the body the
     window runs: T and S without the passive tracer, which no loss reads."""
     import hotcross as H
@@ -124,20 +128,25 @@ def adjoint_leg(workload="les128", nsteps=400, seg=20, precision="fp64", budget=
     ctx.step(nsteps)
     ctx.synchronize()
     fwd = time.perf_counter() - t0
-    need = None
+    need, kind = None, "fixed"
     if budget:
-        seg, need = AD.plan_segment(ctx, nsteps, budget)
+        kind, seg, need = AD.plan_window(ctx, nsteps, budget)
     probes = AD.plane_probes(p)
     every = AD.samples_every(p)
     ctx.set_probes(probes, every, 1.0e4, 1.0)
     obs = np.zeros((nsteps // every, len(probes), 2))
     t0 = time.perf_counter()
-    loss, _, gq, gs = ctx.adj_window(nsteps, seg, obs)
+    if kind == "fixed":
+        loss, _, gq, gs = ctx.adj_window(nsteps, seg, obs)
+    else:
+        loss, _, gq, gs = ctx.adj_window_binomial(nsteps, seg, obs)
     ctx.synchronize()
     win = time.perf_counter() - t0
     ctx.close()
-    out = {"workload": "%s %s, %d steps, checkpoint every %d, %d probes at 1 Hz" % (workload, precision, nsteps, seg,
-                                                                                 len(probes)),
+    how = ("checkpoint every %d" % seg) if kind == "fixed" else ("binomial checkpointing, %d snapshot slots, %d "
+                                                                "forward steps" % (seg, H.binomial_schedule(nsteps, seg)[1]))
+    out = {"workload": "%s %s, %d steps, %s, %d probes at 1 Hz" % (workload, precision, nsteps, how, len(probes)),
+           "checkpointing": kind,
            "forward_ms_per_step": round(fwd / nsteps * 1e3, 4),
            "window_ms_per_step": round(win / nsteps * 1e3, 4),
            "ratio": round((win - fwd) / fwd, 3),
@@ -310,7 +319,12 @@ def main():
     ap.add_argument("--adjoint-seg", type=int, default=20)
     ap.add_argument("--adjoint-precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--adjoint-budget-gb", type=float, default=0.0,
+                    help="with --only-adjoint: plan the checkpoints under this HBM budget (GB) instead of "
+                         "--adjoint-seg")
     ap.add_argument("--c5-steps", type=int, default=1800, help="steps of the 256^3 FP32 adjoint window")
+    ap.add_argument("--c5-budget-gb", type=float, default=200.0,
+                    help="HBM budget of the 256^3 FP32 adjoint window's checkpoints (GB)")
     ap.add_argument("--cpu-worker", nargs=7)
     ap.add_argument("--cpu-full", action="store_true",
                     help="only the whole CPU baseline plan (64^3 and 256^3, all physical cores and 8 threads)")
@@ -329,22 +343,23 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.only_adjoint:
+        budget = args.adjoint_budget_gb * 1e9 if args.adjoint_budget_gb else None
         print(json.dumps(adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg,
-                                     args.adjoint_precision)))
+                                     args.adjoint_precision, budget=budget)))
         return
     p, sec, cells, phases, nprof = gpu_leg(args)
     value = cells * args.steps / sec / 1e6
     ms_step = sec * 1e3 / args.steps
     esz = 8 if args.precision == "fp64" else 4
     # roofline of the dominant kernel: the fused MPDATA corrective pass
-    # (k_mpdata_pass), timed per launch with HIP events on the context stream
+    # (k_mpdata_pass2), timed per launch with HIP events on the context stream
     roof = None
     k_ms, k_n = phases.get("mpdata_pass", (0.0, 0))
     if k_n:
         per_launch = k_ms / k_n * 1e-3
         alg = MPDATA_PASS_MOVES * esz * cells
         achieved = alg / per_launch / 1e9
-        roof = {"bound": "hbm", "kernel": "k_mpdata_pass (MPDATA corrective pass, mean of passes 2 and 3)",
+        roof = {"bound": "hbm", "kernel": "k_mpdata_pass2 (MPDATA corrective pass, mean of passes 2 and 3)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved / (HBM_PEAK / 1e9), 4),
                 "traffic": profiled_traffic(args.workload, args.precision),
@@ -364,7 +379,7 @@ def main():
         adj = adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg, args.adjoint_precision)
         # the headline size's forward + adjoint configuration: 256^3 in the
         # FP32 form, 1800 steps, checkpoints under a 200 GB HBM budget
-        adj256 = adjoint_leg("les256", args.c5_steps, None, "fp32", budget=200e9)
+        adj256 = adjoint_leg("les256", args.c5_steps, None, "fp32", budget=args.c5_budget_gb * 1e9)
     # the CPU baseline last: run right after it, the launch-bound 128^3
     # adjoint leg once measured 1.5x slower
     cpu = None

@@ -305,6 +305,26 @@ int hc_adj_window_bytes(hc_ctx *ctx, int32_t nsteps, int32_t seg, uint64_t *byte
 int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples, double *loss,
                   double *gq, double gs[3], int32_t end_state);
 
+/* The same window under binomial checkpointing (Griewank & Walther's
+ * revolve schedule): nsnap snapshot slots (slot 0 holds the window's start)
+ * and the fewest forward steps for them; for budgets the fixed-interval
+ * window does not fit (it needs about 2 sqrt(nsteps) states, this nsnap + 2
+ * states and one step of tracer intermediates).  Every reversed step is run
+ * once more just before its adjoint, with its tracers' intermediates taped.
+ * Arguments and results as hc_adj_window.  No reference counterpart (the
+ * reference has no adjoint); it serves the forward + adjoint configuration
+ * at 256^3 over 1800 steps under a fixed HBM budget. */
+int hc_adj_window_binomial_bytes(hc_ctx *ctx, int32_t nsteps, int32_t nsnap, uint64_t *bytes);
+int hc_adj_window_binomial(hc_ctx *ctx, int32_t nsteps, int32_t nsnap, const double *obs, double *samples,
+                           double *loss, double *gq, double gs[3], int32_t end_state);
+/* The binomial schedule itself, computed on the host (no context, no device):
+ * up to cap actions of four int32 (kind, step, slot, 0) into acts (may be
+ * null; kind 0 advance the live state by step `step`, 1 snapshot of the live
+ * state X_step into slot, 2 restore slot (X_step), 3 turn step `step` from
+ * X_step held in slot), their number into nacts and the forward steps run
+ * (advances and turns) into nfwd. */
+int hc_binomial_schedule(int32_t nsteps, int32_t nsnap, int32_t *acts, int64_t cap, int64_t *nacts, int64_t *nfwd);
+
 /* Per-phase device time accumulated since the last reset (ms), measured with
  * HIP events on the context stream when enabled.  names/ms have room for
  * `cap` entries; returns the number of phases. */

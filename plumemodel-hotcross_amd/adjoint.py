@@ -64,6 +64,37 @@ class Window:
         return ProbeMisfit.apply(qmap, flow_scale, cs, self)
 
 
+def binomial_forward_steps(nsteps, nsnap):
+    """Forward steps the binomial window runs for nsnap snapshot slots: the
+    advances of the schedule (t l - beta(s+2, t-1) for s = nsnap - 1 free
+    slots, t the least with beta(s+1, t) >= l) plus one turn per step."""
+    import hotcross as H
+    return H.binomial_schedule(nsteps, nsnap)[1]
+
+
+def plan_window(ctx, nsteps, budget_bytes):
+    """The window's checkpointing under an HBM budget: ("fixed", seg, bytes)
+    when a fixed-interval window fits (about two forward steps per step: the
+    forward sweep and one recompute), else ("binomial", nsnap, bytes) with
+    as many snapshot slots as fit (the fewest forward steps for that
+    memory); raises if not even one snapshot fits."""
+    try:
+        seg, need = plan_segment(ctx, nsteps, budget_bytes)
+        return "fixed", seg, need
+    except ValueError:
+        pass
+    lo, hi = 1, nsteps
+    if ctx.adj_window_binomial_bytes(nsteps, 1) > budget_bytes:
+        raise ValueError("not even one snapshot fits the %.1f GB budget" % (budget_bytes / 1e9))
+    while lo < hi:  # the largest nsnap that fits
+        mid = (lo + hi + 1) // 2
+        if ctx.adj_window_binomial_bytes(nsteps, mid) <= budget_bytes:
+            lo = mid
+        else:
+            hi = mid - 1
+    return "binomial", lo, ctx.adj_window_binomial_bytes(nsteps, lo)
+
+
 def plane_probes(p, n=5, spacing=2, height_m=20.0):
     """n x n probes on the plane `height_m` above the vent, centred over the
     source patch (nx/3, ny/2, source80.inc:10-27), `spacing` cells apart; k

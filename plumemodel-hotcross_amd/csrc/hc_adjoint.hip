@@ -1436,19 +1436,28 @@ static void step_head(hc_ctx *ctx, int m) {
 // writes no forward field); the caller restores the pointers afterwards.
 // Host state advances as the head would advance it: step counters, body
 // force, FORCES' entrance-ramp accumulators and the source terms.
-static void adj_head(hc_ctx *ctx, int m, size_t s0, size_t s1) {
+// x0, x1: the NTAPE fields (TAPE_FIELDS order) of X_n and X_n+1
+static void adj_head_p(hc_ctx *ctx, int m, void *const x0[NTAPE], void *const x1[NTAPE]) {
   hc_begin_step(ctx, m, ctx->time + ctx->P.dt);
   hc_body_force(ctx);
   for (int i = 0; i <= 7; ++i) ctx->acc[i] = ctx->acc[i] + (double)(7 - i) * .01 / 7.0;
-  for (int f = 0; f < NTAPE; ++f) ctx->f[TAPE_FIELDS[f]] = slot_ptr(ctx, s0, f);
-  ctx->f[HC_SS] = slot_ptr(ctx, s1, 6);
-  ctx->f[HC_SSV] = slot_ptr(ctx, s1, 7);
+  for (int f = 0; f < NTAPE; ++f) ctx->f[TAPE_FIELDS[f]] = x0[f];
+  ctx->f[HC_SS] = x1[6];
+  ctx->f[HC_SSV] = x1[7];
   for (int c = 0; c < 3; ++c) {
-    ctx->f[HC_UP + c] = slot_ptr(ctx, s1, c);
+    ctx->f[HC_UP + c] = x1[c];
     ctx->up_alias[c] = false;
   }
   hc_source_terms(ctx, nullptr);
   hc_work_velocity(ctx, ctx->P.dt);
+}
+static void adj_head(hc_ctx *ctx, int m, size_t s0, size_t s1) {
+  void *x0[NTAPE], *x1[NTAPE];
+  for (int f = 0; f < NTAPE; ++f) {
+    x0[f] = slot_ptr(ctx, s0, f);
+    x1[f] = slot_ptr(ctx, s1, f);
+  }
+  adj_head_p(ctx, m, x0, x1);
 }
 
 // the taped intermediates of tracer `tr` (0: T, 1: S) of the segment's step
@@ -1715,6 +1724,240 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   return hc_adj_gradients(ctx, gq, gs);
 }
 
+// ---------------------------------------------------------------------------
+// binomial checkpointing (Griewank 1992; Griewank & Walther 2000, "Algorithm
+// 799: revolve"): the window's reversal under a fixed number of snapshot
+// slots with the fewest forward steps.  reverse(a, b, s) reverses steps
+// a .. b-1 from X_a, which a slot holds and the live state equals, with s
+// free slots: one step is a turn (the step run again with its tracers'
+// intermediates taped, then its adjoint); more steps advance to the split
+// point m, take a snapshot of X_m, reverse(m, b, s-1), restore X_a and
+// reverse(a, m, s).  With C(l, s) the advances this takes for l steps,
+//   C(1, s) = 0,  C(l, 0) = l(l-1)/2,
+//   C(l, s) = min over 1 <= m < l of m + C(l-m, s-1) + C(m, s)
+//           = t l - beta(s+2, t-1),  t the least with beta(s+1, t) >= l,
+// beta(s, t) = (s+t)! / (s! t!); the split is a minimiser of the bracket.
+// Without free slots the steps are turned last to first, each from X_a.
+
+// beta(s, t) capped at cap (the counts only need comparing with l)
+static long long binom_beta(long long s, long long t, long long cap) {
+  long long b = 1;
+  for (long long i = 1; i <= t; ++i) {
+    b = b * (s + i) / i;  // exact: b is C(s+i, i) after each step
+    if (b > cap) return cap + 1;
+  }
+  return b;
+}
+
+static long long binom_cost(long long l, long long s) {
+  if (l <= 1) return 0;
+  if (s <= 0) return l * (l - 1) / 2;
+  long long t = 1;
+  while (binom_beta(s + 1, t, l) < l) ++t;
+  return t * l - binom_beta(s + 2, t - 1, (long long)4e18);
+}
+
+static long long binom_split(long long l, long long s) {
+  long long best = 1, bc = -1;
+  for (long long m = 1; m < l; ++m) {
+    const long long c = m + binom_cost(l - m, s - 1) + binom_cost(m, s);
+    if (bc < 0 || c < bc) { bc = c; best = m; }
+  }
+  return best;
+}
+
+// actions: {kind, step, slot, 0}; kind 0: advance the live state by one step
+// (step = the step run), 1: snapshot of the live state (X_step) into slot,
+// 2: restore slot (X_step), 3: turn step `step` from X_step held in slot
+enum { BA_ADVANCE = 0, BA_SNAP = 1, BA_RESTORE = 2, BA_TURN = 3 };
+struct BinomAct {
+  int kind, step, slot;
+};
+
+// slots: 0 .. nsnap-1 snapshots (slot 0 holds X_0), nsnap the temporary of
+// the turns without a free slot
+static void binom_reverse(std::vector<BinomAct> &out, int a, int b, int s, int slot_a, int next, int temp) {
+  if (b - a == 1) {
+    out.push_back({BA_TURN, a, slot_a});
+    return;
+  }
+  if (s == 0) {
+    for (int n = b - 1; n >= a; --n) {
+      if (n != b - 1) out.push_back({BA_RESTORE, a, slot_a});
+      for (int q = a; q < n; ++q) out.push_back({BA_ADVANCE, q, -1});
+      if (n == a) {
+        out.push_back({BA_TURN, a, slot_a});
+      } else {
+        out.push_back({BA_SNAP, n, temp});
+        out.push_back({BA_TURN, n, temp});
+      }
+    }
+    return;
+  }
+  const int m = a + (int)binom_split(b - a, s);
+  for (int q = a; q < m; ++q) out.push_back({BA_ADVANCE, q, -1});
+  out.push_back({BA_SNAP, m, next});
+  binom_reverse(out, m, b, s - 1, next, next + 1, temp);
+  out.push_back({BA_RESTORE, a, slot_a});
+  binom_reverse(out, a, m, s, slot_a, next, temp);
+}
+
+static std::vector<BinomAct> binom_schedule(int nsteps, int nsnap) {
+  std::vector<BinomAct> out;
+  out.push_back({BA_SNAP, 0, 0});
+  binom_reverse(out, 0, nsteps, nsnap - 1, 0, 1, nsnap);
+  return out;
+}
+
+// The schedule for nsteps steps and nsnap snapshot slots (host only, no
+// device): up to cap actions of 4 int32 (kind, step, slot, 0) into acts
+// (may be null), their number into nacts and the forward steps it runs
+// (advances and turns) into nfwd.
+extern "C" int hc_binomial_schedule(int32_t nsteps, int32_t nsnap, int32_t *acts, int64_t cap, int64_t *nacts,
+                                    int64_t *nfwd) {
+  if (nsteps < 1 || nsnap < 1) return HC_EINVAL;
+  const std::vector<BinomAct> v = binom_schedule(nsteps, nsnap);
+  int64_t nf = 0;
+  for (size_t q = 0; q < v.size(); ++q) {
+    if (v[q].kind == BA_ADVANCE || v[q].kind == BA_TURN) ++nf;
+    if (acts && (int64_t)q < cap) {
+      acts[4 * q] = v[q].kind;
+      acts[4 * q + 1] = v[q].step;
+      acts[4 * q + 2] = v[q].slot;
+      acts[4 * q + 3] = 0;
+    }
+  }
+  if (nacts) *nacts = (int64_t)v.size();
+  if (nfwd) *nfwd = nf;
+  return HC_OK;
+}
+
+template <typename T>
+static int window_binomial_t(hc_ctx *ctx, int nsteps, int nsnap, const double *obs, double *samples, double *loss,
+                             double *gq, double gs[3], int end_state) {
+  AdjCtx *A = ctx->adj;
+  const int m0 = ctx->next_step;
+  // slots: nsnap snapshots, the turns' temporary, the window's end state
+  const size_t slots = (size_t)nsnap + 2, temp = (size_t)nsnap, endslot = (size_t)nsnap + 1;
+  const size_t slot_bytes = (size_t)NTAPE * ctx->L.n * ctx->esz;
+  if (A->tape_slots < slots) {
+    if (A->tape) { hipStreamSynchronize(ctx->st); hipFree(A->tape); A->tape = nullptr; A->tape_slots = 0; }
+    if (hipMalloc(&A->tape, slots * slot_bytes) != hipSuccess) {
+      A->tape = nullptr;
+      return fail(ctx, HC_ENOMEM, "snapshot slots: hipMalloc of " + std::to_string(slots * slot_bytes) +
+                                      " bytes failed (use fewer snapshots)");
+    }
+    A->tape_slots = slots;
+  }
+  const int nsamp = nsteps / A->every;
+  const size_t ns = (size_t)nsamp * 2 * A->nprobe;
+  if (A->nsamp_cap < ns) {
+    if (A->d_samp) { hipFree(A->d_samp); hipFree(A->d_obs); }
+    int rc = 0;
+    if (!(A->d_samp = (double *)dalloc(ctx, ns * 8, &rc))) return rc;
+    if (!(A->d_obs = (double *)dalloc(ctx, ns * 8, &rc))) return rc;
+    A->nsamp_cap = ns;
+  }
+  if (ns) HIPCHK(hipMemcpyAsync(A->d_obs, obs, ns * 8, hipMemcpyHostToDevice, ctx->st));
+  if (A->ttape_steps < 1) {  // one step of the tracers' intermediates, for the turns
+    if (A->ttape) { hipStreamSynchronize(ctx->st); hipFree(A->ttape); A->ttape = nullptr; A->ttape_steps = 0; }
+    const size_t tb = 2 * TTAPE * ctx->L.n * ctx->esz;
+    if (hipMalloc(&A->ttape, tb) == hipSuccess) {
+      A->ttape_steps = 1;
+      HIPCHK(hipMemsetAsync(A->ttape, 0, tb, ctx->st));  // finite ghost faces, as in the fixed-interval window
+    } else {
+      A->ttape = nullptr;
+      (void)hipGetLastError();
+    }
+  }
+  const std::vector<BinomAct> plan = binom_schedule(nsteps, nsnap);
+  std::vector<HostState> hs(slots);
+  HostState h_end{};
+  int done = 0;        // states X_1 .. X_done computed (and sampled) so far
+  int live = 0;        // the step of the live state
+  bool reset = false;  // the adjoint fields zeroed (at the first turn, after every sample exists)
+  auto sample = [&](int n1) {  // X_n1 was computed just now
+    if (n1 > done) {
+      if (n1 % A->every == 0) probe_sample<T>(ctx, (size_t)(n1 / A->every - 1));
+      done = n1;
+    }
+  };
+  for (const BinomAct &ac : plan) {
+    if (ac.kind == BA_ADVANCE) {
+      const int rs = fwd_step(ctx, m0 + ac.step);
+      if (rs) return rs;
+      live = ac.step + 1;
+      sample(live);
+      if (live == nsteps) {
+        save_slot(ctx, endslot);
+        h_end = host_state(ctx);
+      }
+    } else if (ac.kind == BA_SNAP) {
+      save_slot(ctx, (size_t)ac.slot);
+      hs[ac.slot] = host_state(ctx);
+    } else if (ac.kind == BA_RESTORE) {
+      load_slot(ctx, (size_t)ac.slot);
+      set_host_state(ctx, hs[ac.slot]);
+      live = ac.step;
+    } else {  // turn: step n from X_n (slot), the adjoint reads X_n+1 from the live buffers
+      const int n = ac.step;
+      const HostState h0 = hs[ac.slot];
+      const int rs = fwd_step(ctx, m0 + n, 0);
+      if (rs) return rs;
+      sample(n + 1);
+      if (n + 1 == nsteps) {
+        save_slot(ctx, endslot);
+        h_end = host_state(ctx);
+      }
+      if (!reset) {  // the last step is turned first: every sample exists now
+        const int r = hc_adj_reset(ctx);
+        if (r) return r;
+        reset = true;
+      }
+      FieldPtrs<T> lam = lptrs<T>(ctx);
+      if ((n + 1) % A->every == 0) {
+        const size_t q = (size_t)((n + 1) / A->every - 1);
+        hipLaunchKernelGGL(k_probe_seed<T>, dim3((A->nprobe + 255) / 256), dim3(256), 0, ctx->st, A->nprobe, A->d_iw,
+                           A->d_it, A->d_samp + 2 * A->nprobe * q, A->d_obs + 2 * A->nprobe * q, A->wgt_w, A->wgt_t,
+                           lam.w, lam.t);
+      }
+      void *own[HC_NFIELDS];  // X_n+1 in the field buffers (the fused forward rotates them)
+      for (int f = 0; f < HC_NFIELDS; ++f) own[f] = ctx->f[f];
+      void *x0[NTAPE], *x1[NTAPE];
+      for (int f = 0; f < NTAPE; ++f) {
+        x0[f] = slot_ptr(ctx, (size_t)ac.slot, f);
+        x1[f] = own[TAPE_FIELDS[f]];
+      }
+      set_host_state(ctx, h0);
+      adj_head_p(ctx, m0 + n, x0, x1);
+      adj_step<T>(ctx, m0 + n, (const T *)x1[9], (const T *)x1[10], (const T *)x0[6], (const T *)x0[7], 0);
+      for (int f = 0; f < HC_NFIELDS; ++f) ctx->f[f] = own[f];
+      live = -1;  // the live buffers are X_n+1, but the host state is X_n's: restore before use
+    }
+  }
+  std::vector<double> samp(ns);
+  if (ns) HIPCHK(hipMemcpyAsync(samp.data(), A->d_samp, ns * 8, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  double L2 = 0.0;
+  for (size_t q = 0; q < ns; q += 2) {
+    const double dw = samp[q] - obs[q], dtt = samp[q + 1] - obs[q + 1];
+    L2 += 0.5 * (A->wgt_w * dw * dw + A->wgt_t * dtt * dtt);
+  }
+  *loss = L2;
+  if (samples && ns) memcpy(samples, samp.data(), ns * 8);
+  (void)live;
+  if (end_state) {
+    load_slot(ctx, endslot);
+    set_host_state(ctx, h_end);
+  } else {
+    load_slot(ctx, 0);
+    set_host_state(ctx, hs[0]);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, HC_EDEVICE, std::string("adjoint window: ") + hipGetErrorString(e));
+  return hc_adj_gradients(ctx, gq, gs);
+}
+
 // HBM the window's checkpoints and tapes take (beyond the context's own
 // state and the adjoint fields): (nck + 1 + seg) states of NTAPE fields and
 // (seg - 1) steps of TTAPE fields for each of the two tracers
@@ -1724,6 +1967,29 @@ extern "C" int hc_adj_window_bytes(hc_ctx *ctx, int32_t nsteps, int32_t seg, uin
   const uint64_t nck = (uint64_t)((nsteps + seg - 1) / seg);
   *bytes = (nck + 1 + (uint64_t)seg) * NTAPE * fb + (uint64_t)(seg - 1) * 2 * TTAPE * fb;
   return HC_OK;
+}
+
+// HBM the binomial window takes: nsnap + 2 states of NTAPE fields (the
+// snapshots, the turns' temporary, the end state) and one step of TTAPE
+// fields for each of the two tracers
+extern "C" int hc_adj_window_binomial_bytes(hc_ctx *ctx, int32_t nsteps, int32_t nsnap, uint64_t *bytes) {
+  if (!ctx || !bytes || nsteps < 1 || nsnap < 1) return fail(ctx, HC_EINVAL, "nsteps and nsnap must be positive");
+  const uint64_t fb = (uint64_t)ctx->L.n * ctx->esz;
+  *bytes = ((uint64_t)nsnap + 2) * NTAPE * fb + 2 * TTAPE * fb;
+  return HC_OK;
+}
+
+extern "C" int hc_adj_window_binomial(hc_ctx *ctx, int32_t nsteps, int32_t nsnap, const double *obs,
+                                      double *samples, double *loss, double *gq, double gs[3], int32_t end_state) {
+  int r = adj_need(ctx);
+  if (r) return r;
+  if (ctx->h_tbkg.empty()) return fail(ctx, HC_EINVAL, "profiles not set (hc_init_case or hc_set_profiles)");
+  if (!ctx->adj->d_iw) return fail(ctx, HC_EINVAL, "no probes (hc_adj_set_probes)");
+  if (nsteps < 1 || nsnap < 1) return fail(ctx, HC_EINVAL, "nsteps and nsnap must be positive");
+  if (ctx->P.ismol != 1) return fail(ctx, HC_EINVAL, "only ismol = 1 (two corrective passes) is supported");
+  PhaseScope ps(ctx, "adj_window");
+  if (ctx->prec == HC_FP64) return window_binomial_t<double>(ctx, nsteps, nsnap, obs, samples, loss, gq, gs, end_state);
+  return window_binomial_t<float>(ctx, nsteps, nsnap, obs, samples, loss, gq, gs, end_state);
 }
 
 extern "C" int hc_adj_window(hc_ctx *ctx, int32_t nsteps, int32_t seg, const double *obs, double *samples,

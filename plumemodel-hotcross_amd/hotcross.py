@@ -116,6 +116,9 @@ def lib():
             "hc_adj_set_probes": [vp, i32, dp, i32, dbl, dbl],
             "hc_adj_window": [vp, i32, i32, dp, dp, dp, dp, dp, i32],
             "hc_adj_window_bytes": [vp, i32, i32, dp],
+            "hc_adj_window_binomial": [vp, i32, i32, dp, dp, dp, dp, dp, i32],
+            "hc_adj_window_binomial_bytes": [vp, i32, i32, dp],
+            "hc_binomial_schedule": [i32, i32, vp, ctypes.c_int64, vp, vp],
         }
         for n, a in sig.items():
             f = getattr(L, n)
@@ -469,6 +472,24 @@ class Context:
                                          1 if end_state else 0))
         return float(loss[0]), samp.reshape(ns, self._probes, 2), gq, gs
 
+    def adj_window_binomial_bytes(self, nsteps, nsnap):
+        """HBM bytes of the binomial window's snapshots and tapes."""
+        b = np.zeros(1, dtype=np.uint64)
+        self._check(self.L.hc_adj_window_binomial_bytes(self.h, nsteps, nsnap, _arr(b)))
+        return int(b[0])
+
+    def adj_window_binomial(self, nsteps, nsnap, obs, end_state=True):
+        """adj_window under binomial checkpointing with nsnap snapshot slots."""
+        ns = nsteps // self._every
+        o = np.ascontiguousarray(np.asarray(obs, dtype=np.float64).reshape(ns * self._probes * 2))
+        samp = np.zeros_like(o)
+        loss = np.zeros(1)
+        gq = np.zeros(self.npatch())
+        gs = np.zeros(3)
+        self._check(self.L.hc_adj_window_binomial(self.h, nsteps, nsnap, _arr(o), _arr(samp), _arr(loss), _arr(gq),
+                                                  _arr(gs), 1 if end_state else 0))
+        return float(loss[0]), samp.reshape(ns, self._probes, 2), gq, gs
+
     def phase_timing(self, enable=True):
         self._check(self.L.hc_phase_timing(self.h, 1 if enable else 0))
 
@@ -479,6 +500,19 @@ class Context:
         cnt = np.zeros(cap, dtype=np.int32)
         n = self.L.hc_phase_times(self.h, names, _arr(ms), _arr(cnt), cap)
         return {names[q].decode(): (float(ms[q]), int(cnt[q])) for q in range(min(n, cap))}
+
+
+def binomial_schedule(nsteps, nsnap):
+    """The binomial window's schedule, computed on the host (no device):
+    (actions as rows (kind, step, slot) with kind 0 advance, 1 snapshot,
+    2 restore, 3 turn; the forward steps it runs)."""
+    L = lib()
+    na, nf = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
+    if L.hc_binomial_schedule(nsteps, nsnap, None, 0, _arr(na), _arr(nf)) != 0:
+        raise ValueError("nsteps and nsnap must be positive")
+    acts = np.zeros((int(na[0]), 4), dtype=np.int32)
+    L.hc_binomial_schedule(nsteps, nsnap, _arr(acts), int(na[0]), _arr(na), _arr(nf))
+    return acts[:, :3], int(nf[0])
 
 
 # --------------------------------------------------------------------------

@@ -195,7 +195,10 @@ NSTEPS, SEG, EVERY = 8, 3, 2
 
 
 def window_loss(prm, st, obs, qmap=None, flow=1.0, cs=0.2, dstate=None, eps=0.0, grads=False, nsteps=NSTEPS,
-                seg=SEG, every=EVERY):
+                seg=SEG, every=EVERY, nsnap=None, end=None):
+    """The window through hc_adj_window (a checkpoint every seg steps) or,
+    with nsnap, hc_adj_window_binomial; end: a dict that receives the
+    context's fields after the call (the window's end state)."""
     s2 = dict(st)
     if dstate:
         for k, d in dstate.items():
@@ -204,8 +207,14 @@ def window_loss(prm, st, obs, qmap=None, flow=1.0, cs=0.2, dstate=None, eps=0.0,
     try:
         ctx.set_controls(qmap, flow, cs)
         ctx.set_probes(PROBES, every, 1.0e4, 1.0)
-        loss, samp, gq, gs = ctx.adj_window(nsteps, seg, obs)
+        if nsnap is None:
+            loss, samp, gq, gs = ctx.adj_window(nsteps, seg, obs)
+        else:
+            loss, samp, gq, gs = ctx.adj_window_binomial(nsteps, nsnap, obs)
         lam = {k: ctx.adj_download(k) for k in STATE} if grads else None
+        if end is not None:
+            for k in STATE:
+                end[k] = ctx.download(k)
         return loss, samp, gq, gs, lam
     finally:
         ctx.close()
@@ -321,3 +330,27 @@ def test_window_short_last_segment(nsteps, seg):
         # a control the probes cannot see yet (one step) gives fd = 0: the
         # adjoint must then be zero to roundoff of the loss
         assert abs(ad - fd) <= 1e-4 * abs(fd) + 1e-12 * loss, (name, ad, fd)
+
+
+@pytest.mark.parametrize("nsnap", [1, 2, 3, 9])
+def test_window_binomial_equals_fixed_interval(window_obs, nsnap):
+    """Binomial checkpointing (hc_adj_window_binomial) against the
+    fixed-interval window on the same 8 steps: the same forward (loss,
+    samples and end state bit for bit) and the same gradients and
+    initial-state adjoint up to the order of the adjoint's atomic additions.
+    One slot turns every step from X_0 (28 advances), nine hold every
+    state (7 advances)."""
+    prm, st, obs, npatch = window_obs
+    print("binomial window, %d snapshot slots" % nsnap, flush=True)
+    q0 = np.ones(npatch)
+    e0, e1 = {}, {}
+    l0, s0, gq0, gs0, lam0 = window_loss(prm, st, obs, qmap=q0, grads=True, end=e0)
+    l1, s1, gq1, gs1, lam1 = window_loss(prm, st, obs, qmap=q0, grads=True, nsnap=nsnap, end=e1)
+    assert l0 == l1 and np.array_equal(s0, s1)
+    for k in STATE:
+        assert np.array_equal(e0[k], e1[k]), k
+    assert np.allclose(gq1, gq0, rtol=1e-10, atol=1e-14 * np.abs(gq0).max())
+    assert np.allclose(gs1, gs0, rtol=1e-9, atol=1e-14 * np.abs(gs0).max())
+    for k in STATE:
+        a, b = lam1[k], lam0[k]
+        assert np.abs(a - b).max() <= 1e-10 * max(np.abs(b).max(), 1e-300), k
