@@ -122,6 +122,24 @@ int hc_set_profiles(hc_ctx *ctx, const double *t_bkg, const double *s_bkg,
  * alphav(0:nx+1,-1:ny+1), alphae(0:nx+1,0:ny+1), alphaw(-1:nz+1). */
 int hc_set_sponge(hc_ctx *ctx, const double *alphau, const double *alphav,
                   const double *alphae, const double *alphaw);
+/* The profiles and sponge masks the context holds (set by hc_set_profiles /
+ * hc_set_sponge or hc_init_case), in the layouts of the setters; any pointer
+ * may be null.  No reference counterpart (the reference reads its module
+ * arrays directly); for output writers and drivers that need them. */
+int hc_get_profiles(hc_ctx *ctx, double *t_bkg, double *s_bkg, double *rho_bkg, double *azzmin);
+int hc_get_sponge(hc_ctx *ctx, double *alphau, double *alphav, double *alphae, double *alphaw);
+/* Output diagnostics of VISCOSITY3D and MIXING: with on = 1, every later
+ * hc_viscosity3d (also inside hc_step) also forms the Richardson number Ri
+ * (smagorinsky_split_mixing-revised2014.f:101-102), epsh = S_H^2 (A_H +
+ * axmin) (:54-59), epsv = S_V^2 (A_V + azmin) (:108-114), and chi_v, chi_h
+ * (mixing80-revised2014.f:64-94, from T and ss, ss_v as they are then, which
+ * is what MIXING sees in the reference's call order), over the interior; the
+ * ghost cells are zero.  Off by default (one more pass over the state per
+ * step).  hc_get_diagnostic copies diagnostic `which` (0 Ri, 1 epsh, 2 epsv,
+ * 3 chi_v, 4 chi_h) of the last step as FP64 over (0:nx+1, 0:ny+1, 0:nz+1),
+ * x fastest. */
+int hc_set_diagnostics(hc_ctx *ctx, int32_t on);
+int hc_get_diagnostic(hc_ctx *ctx, int32_t which, double *host);
 /* Host-side case set-up in the library: profiles by LOCATOR interpolation of
  * a (z, T, S) table (initialize_profiles80.f:46-64,164-197), sponge masks
  * (sponge80.f), and the initial state of INITIALIZE

@@ -127,7 +127,7 @@ typedef struct oracle {
   double *t_bkg, *s_bkg, *c_bkg, *rho_bkg; /* (0:nz+1), offset 1 */
   I2 isource_map;
   /* viscosity_module */
-  A3 ss, ss_v, ri, epsh, epsv, axu, ayu, axv, ayv, azu, azv;
+  A3 ss, ss_v, ri, epsh, epsv, chi_v, chi_h, axu, ayu, axv, ayv, azu, azv;
   double *azzmin; /* (0:nz+1) */
   /* tracer_smol_module + driver work arrays */
   A3 uc, vc, wc, ud, vd, wd, cp, cc, worku, workv, workw;
@@ -162,7 +162,7 @@ oracle *o_create(const hc_params *P) {
   UCL(fx); VCL(fy); WCL(fz);
   SCL(t); SCL(s); SCL(c); SCL(q_t); SCL(q_s); SCL(q_c);
   UCL(kxt); VCL(kyt); WCL(kzt); SCL(rhop); SCL(rrho);
-  SCL(ss); SCL(ss_v); SCL(ri); SCL(epsh); SCL(epsv);
+  SCL(ss); SCL(ss_v); SCL(ri); SCL(epsh); SCL(epsv); SCL(chi_v); SCL(chi_h);
   a3_alloc(&o->axu, 0, nx + 1, 0, ny + 1, 0, nz + 1);
   a3_alloc(&o->ayu, 0, nx, 0, ny, 0, nz + 1);
   a3_alloc(&o->axv, 0, nx, 0, ny, 0, nz + 1);
@@ -202,7 +202,7 @@ void o_destroy(oracle *o) {
                &o->wm, &o->p, &o->fx, &o->fy, &o->fz, &o->t, &o->s, &o->c,
                &o->q_t, &o->q_s, &o->q_c, &o->kxt, &o->kyt, &o->kzt,
                &o->rhop, &o->rrho, &o->ss, &o->ss_v, &o->ri, &o->epsh,
-               &o->epsv, &o->axu, &o->ayu, &o->axv, &o->ayv, &o->azu,
+               &o->epsv, &o->chi_v, &o->chi_h, &o->axu, &o->ayu, &o->axv, &o->ayv, &o->azu,
                &o->azv, &o->uc, &o->vc, &o->wc, &o->ud, &o->vd, &o->wd,
                &o->cp, &o->cc, &o->worku, &o->workv, &o->workw, &o->alphaee};
   for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]->d);
@@ -226,7 +226,7 @@ static const fentry FIELDS[] = {
     FE(u), FE(up), FE(um), FE(v), FE(vp), FE(vm), FE(w), FE(wp), FE(wm),
     FE(p), FE(fx), FE(fy), FE(fz), FE(t), FE(s), FE(c), FE(q_t), FE(q_s),
     FE(q_c), FE(kxt), FE(kyt), FE(kzt), FE(rhop), FE(rrho), FE(ss),
-    FE(ss_v), FE(ri), FE(epsh), FE(epsv), FE(axu), FE(ayu), FE(axv),
+    FE(ss_v), FE(ri), FE(epsh), FE(epsv), FE(chi_v), FE(chi_h), FE(axu), FE(ayu), FE(axv),
     FE(ayv), FE(azu), FE(azv), FE(uc), FE(vc), FE(wc), FE(ud), FE(vd),
     FE(wd), FE(cp), FE(cc), FE(worku), FE(workv), FE(workw), FE(alphaee)};
 
@@ -1131,8 +1131,9 @@ void o_source_terms(oracle *o, const double *noise) {
     }
 }
 
-/* MIXING, mixing80-revised2014.f:14-61 (the chi diagnostics :64-94 are not
- * formed: they feed output only) */
+/* MIXING, mixing80-revised2014.f:14-94: the diffusivities (:22-59) and the
+ * temperature-variance dissipation diagnostics chi_v, chi_h (:64-94), which
+ * feed output only */
 void o_mixing(oracle *o) {
   const hc_params *P = &o->P;
   int nx = o->nx, ny = o->ny, nz = o->nz;
@@ -1150,6 +1151,25 @@ void o_mixing(oracle *o) {
     for (int j = 0; j <= ny + 1; ++j)
       for (int i = 0; i <= nx + 1; ++i)
         X3(o->kzt, i, j, k) = P->bit2k * ((0.5 * (X3(o->ss_v, i, j, k) + X3(o->ss_v, i, j, k + 1))) / pr + Z1(o->azzmin, k) / pr);
+  /* temperature-variance dissipation, :60-94: tprime = t - t_bkg(k), the
+   * centred differences over two cells */
+  a3_fill(&o->chi_v, 0.0);
+  a3_fill(&o->chi_h, 0.0);
+#define TPR(i, j, k) (X3(o->t, i, j, k) - Z1(o->t_bkg, k))
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        R d = (TPR(i, j, k - 1) - TPR(i, j, k + 1)) / (2.0 * P->dz);
+        X3(o->chi_v, i, j, k) = X3(o->kzt, i, j, k) * (d * d);
+      }
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        R a = (TPR(i + 1, j, k) - TPR(i - 1, j, k)) / (2.0 * P->dx);
+        R b = (TPR(i, j + 1, k) - TPR(i, j - 1, k)) / (2.0 * P->dy);
+        X3(o->chi_h, i, j, k) = X3(o->kxt, i, j, k) * (a * a) + X3(o->kyt, i, j, k) * (b * b);
+      }
+#undef TPR
 }
 
 /* work velocities, hotcross80-revised2014.f:303-305 */

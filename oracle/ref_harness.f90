@@ -416,6 +416,8 @@ function href_copy(cname, n, buf, dir) bind(C, name="href_copy") result(cnt)
   case ('ri'); call cp3(ri, buf, dir); cnt = size(ri)
   case ('epsh'); call cp3(epsh, buf, dir); cnt = size(epsh)
   case ('epsv'); call cp3(epsv, buf, dir); cnt = size(epsv)
+  case ('chi_v'); call cp3(chi_v, buf, dir); cnt = size(chi_v)
+  case ('chi_h'); call cp3(chi_h, buf, dir); cnt = size(chi_h)
   case ('axu'); call cp3(axu, buf, dir); cnt = size(axu)
   case ('ayu'); call cp3(ayu, buf, dir); cnt = size(ayu)
   case ('axv'); call cp3(axv, buf, dir); cnt = size(axv)

@@ -254,6 +254,8 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
     if (p) hipFree(p);
   for (void *p : ctx->c1x)
     if (p) hipFree(p);
+  for (void *p : ctx->d_diag)
+    if (p) hipFree(p);
   if (ctx->pw.pivt && ctx->pw.pivt != (void *)ctx->pw.piv) hipFree(ctx->pw.pivt);
   for (void *p : ctx->spare)
     if (p) hipFree(p);
@@ -371,6 +373,27 @@ extern "C" int hc_set_profiles(hc_ctx *ctx, const double *t_bkg, const double *s
   HIPCHK(h2d(ctx, ctx->d_sbkg, s_bkg, nz2 * 8));
   HIPCHK(h2d(ctx, ctx->aux.rho_bkg, rho_bkg, nz2 * 8));
   HIPCHK(h2d(ctx, ctx->aux.azzmin, azzmin, nz2 * 8));
+  return HC_OK;
+}
+
+extern "C" int hc_get_profiles(hc_ctx *ctx, double *t_bkg, double *s_bkg, double *rho_bkg, double *azzmin) {
+  if (!ctx) return HC_EINVAL;
+  if (ctx->h_tbkg.empty()) return fail(ctx, HC_EINVAL, "profiles not set (hc_init_case or hc_set_profiles)");
+  const size_t n = ctx->h_tbkg.size() * 8;
+  if (t_bkg) memcpy(t_bkg, ctx->h_tbkg.data(), n);
+  if (s_bkg) memcpy(s_bkg, ctx->h_sbkg.data(), n);
+  if (rho_bkg) memcpy(rho_bkg, ctx->h_rho_bkg.data(), n);
+  if (azzmin) memcpy(azzmin, ctx->h_azzmin.data(), n);
+  return HC_OK;
+}
+
+extern "C" int hc_get_sponge(hc_ctx *ctx, double *alphau, double *alphav, double *alphae, double *alphaw) {
+  if (!ctx) return HC_EINVAL;
+  if (ctx->h_alphau.empty()) return fail(ctx, HC_EINVAL, "sponge not set (hc_init_case or hc_set_sponge)");
+  if (alphau) memcpy(alphau, ctx->h_alphau.data(), ctx->h_alphau.size() * 8);
+  if (alphav) memcpy(alphav, ctx->h_alphav.data(), ctx->h_alphav.size() * 8);
+  if (alphae) memcpy(alphae, ctx->h_alphae.data(), ctx->h_alphae.size() * 8);
+  if (alphaw) memcpy(alphaw, ctx->h_alphaw.data(), ctx->h_alphaw.size() * 8);
   return HC_OK;
 }
 
@@ -605,7 +628,38 @@ extern "C" int hc_viscosity3d(hc_ctx *ctx) {
   StepArgs s = step_args(ctx);
   if (ctx->prec == HC_FP64) launch_viscosity<double>(ctx->st, ctx->L, s, ctx->P, ptrs<double>(ctx), ctx->aux);
   else launch_viscosity<float>(ctx->st, ctx->L, s, ctx->P, ptrs<float>(ctx), ctx->aux);
+  if (ctx->diag) {
+    if (ctx->prec == HC_FP64)
+      launch_diag<double>(ctx->st, ctx->L, s, ctx->P, ptrs<double>(ctx), ctx->aux, ctx->d_tbkg, ctx->d_diag);
+    else
+      launch_diag<float>(ctx->st, ctx->L, s, ctx->P, ptrs<float>(ctx), ctx->aux, ctx->d_tbkg, ctx->d_diag);
+  }
   return HC_OK;
+}
+
+extern "C" int hc_set_diagnostics(hc_ctx *ctx, int32_t on) {
+  if (!ctx) return HC_EINVAL;
+  if (on) {
+    if (ctx->h_tbkg.empty()) return fail(ctx, HC_EINVAL, "profiles not set (hc_init_case or hc_set_profiles)");
+    for (void *&p : ctx->d_diag) {
+      if (!p) {
+        int rc = 0;
+        if (!(p = dalloc(ctx, ctx->L.n * ctx->esz, &rc))) return rc;
+      }
+      HIPCHK(hipMemsetAsync(p, 0, ctx->L.n * ctx->esz, ctx->st));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+  }
+  ctx->diag = on != 0;
+  return HC_OK;
+}
+
+extern "C" int hc_get_diagnostic(hc_ctx *ctx, int32_t which, double *host) {
+  if (!ctx || !host) return HC_EINVAL;
+  if (which < 0 || which > 4) return fail(ctx, HC_EINVAL, "diagnostic 0..4 (ri, epsh, epsv, chi_v, chi_h)");
+  if (!ctx->d_diag[which]) return fail(ctx, HC_EINVAL, "diagnostics are off (hc_set_diagnostics)");
+  const int32_t lo[3] = {0, 0, 0}, hi[3] = {ctx->P.nx + 1, ctx->P.ny + 1, ctx->P.nz + 1};
+  return box_copy(ctx, ctx->d_diag[which], HC_T, host, lo, hi, false);
 }
 
 extern "C" int hc_source_terms(hc_ctx *ctx, const double *noise) {

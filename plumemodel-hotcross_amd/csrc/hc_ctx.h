@@ -72,6 +72,9 @@ struct hc_ctx {
   // native loop: MOMENTUM also forms the Courant numbers and REFRESH's
   // filtered fields (into spare[c]); HC_NO_FUSE=1 at creation turns it off
   bool fuse = true;
+  // output diagnostics (hc_set_diagnostics): Ri, epsh, epsv, chi_v, chi_h
+  bool diag = false;
+  void *d_diag[5] = {};
   void *spare[3] = {nullptr, nullptr, nullptr};
   void *c1x[2] = {nullptr, nullptr};  // pass-1 results of S and C in the native loop
   // controls that the adjoint differentiates: a per-cell multiplier of the

@@ -64,6 +64,11 @@ void launch_viscosity(hipStream_t st, const Layout &L, const StepArgs &s, const 
                       const Aux &aux);
 template <typename T>
 void launch_density(hipStream_t st, const Layout &L, const Phys &ph, const FieldPtrs<T> &f, const Aux &aux);
+// output diagnostics after VISCOSITY3D (hc_diag.hip): out = Ri, epsh, epsv,
+// chi_v, chi_h, scalar-shaped fields of type T
+template <typename T>
+void launch_diag(hipStream_t st, const Layout &L, const StepArgs &s, const Phys &ph, const FieldPtrs<T> &f,
+                 const Aux &aux, const double *t_bkg, void *const out[5]);
 template <typename T>
 void launch_refresh(hipStream_t st, const Layout &L, int leap, double assel, const T *u, const T *up, T *um);
 

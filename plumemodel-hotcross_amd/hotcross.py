@@ -86,6 +86,10 @@ def lib():
             "hc_download": [vp, ctypes.c_int, dp, dp, dp],
             "hc_set_profiles": [vp, dp, dp, dp, dp],
             "hc_set_sponge": [vp, dp, dp, dp, dp],
+            "hc_get_profiles": [vp, dp, dp, dp, dp],
+            "hc_get_sponge": [vp, dp, dp, dp, dp],
+            "hc_set_diagnostics": [vp, i32],
+            "hc_get_diagnostic": [vp, i32, dp],
             "hc_init_case": [vp, dp, dp, dp, i32, dbl, ctypes.c_uint64],
             "hc_begin_step": [vp, i32, dbl],
             "hc_body_force": [vp], "hc_forces": [vp], "hc_pressure": [vp],
@@ -131,6 +135,10 @@ def lib():
     return _lib
 
 
+# hc_get_diagnostic's order
+DIAGNOSTICS = ("ri", "epsh", "epsv", "chi_v", "chi_h")
+
+
 def _arr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
@@ -172,6 +180,7 @@ class Context:
         self._check(r)
         self.time = -prm.dt
         self.m = 0
+        self.diagnostics_on = False
 
     def _check(self, r):
         if r != 0:
@@ -235,6 +244,37 @@ class Context:
         arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (t_bkg, s_bkg, rho_bkg, azzmin)]
         self._t_bkg, self._s_bkg = arrs[0].copy(), arrs[1].copy()
         self._check(self.L.hc_set_profiles(self.h, *[_arr(x) for x in arrs]))
+
+    def profiles(self):
+        """(t_bkg, s_bkg, rho_bkg, azzmin), each over k = 0..nz+1."""
+        a = [np.zeros(self.prm.nz + 2) for _ in range(4)]
+        self._check(self.L.hc_get_profiles(self.h, *[_arr(x) for x in a]))
+        return tuple(a)
+
+    def sponge(self):
+        """(alphau, alphav, alphae, alphaw): alphau over (-1:nx+1, 0:ny+1),
+        alphav over (0:nx+1, -1:ny+1), alphae over (0:nx+1, 0:ny+1) (x
+        first), alphaw over -1..nz+1."""
+        p = self.prm
+        a = [np.zeros((p.nx + 3) * (p.ny + 2)), np.zeros((p.nx + 2) * (p.ny + 3)), np.zeros((p.nx + 2) * (p.ny + 2)),
+             np.zeros(p.nz + 3)]
+        self._check(self.L.hc_get_sponge(self.h, *[_arr(x) for x in a]))
+        return (a[0].reshape((p.nx + 3, p.ny + 2), order="F"), a[1].reshape((p.nx + 2, p.ny + 3), order="F"),
+                a[2].reshape((p.nx + 2, p.ny + 2), order="F"), a[3])
+
+    def set_diagnostics(self, on=True):
+        """Form Ri, epsh, epsv, chi_v and chi_h at every later VISCOSITY3D."""
+        self._check(self.L.hc_set_diagnostics(self.h, 1 if on else 0))
+        self.diagnostics_on = bool(on)
+
+    def diagnostic(self, name):
+        """Diagnostic `name` (ri, epsh, epsv, chi_v, chi_h) of the last step,
+        FP64 over (0:nx+1, 0:ny+1, 0:nz+1), x first."""
+        p = self.prm
+        which = DIAGNOSTICS.index(name)
+        a = np.zeros((p.nx + 2) * (p.ny + 2) * (p.nz + 2))
+        self._check(self.L.hc_get_diagnostic(self.h, which, _arr(a)))
+        return a.reshape((p.nx + 2, p.ny + 2, p.nz + 2), order="F")
 
     def set_sponge(self, alphau, alphav, alphae, alphaw):
         arrs = [np.asfortranarray(x, dtype=np.float64) for x in (alphau, alphav, alphae, alphaw)]

@@ -9,7 +9,7 @@ import hc_testlib as T
 
 DUMP = ("u", "up", "um", "v", "vp", "vm", "w", "wp", "wm", "p", "fx", "fy", "fz",
         "t", "s", "c", "q_t", "q_s", "q_c", "kxt", "kyt", "kzt", "rhop", "rrho",
-        "ss", "ss_v", "ri", "epsh", "epsv", "axu", "ayu", "axv", "ayv", "azu", "azv",
+        "ss", "ss_v", "ri", "epsh", "epsv", "chi_v", "chi_h", "axu", "ayu", "axv", "ayv", "azu", "azv",
         "uc", "vc", "wc", "ud", "vd", "wd", "cp", "cc", "worku", "workv", "workw",
         "alphaee", "alphau", "alphav", "alphae", "isource_map", "alphaw", "t_bkg",
         "s_bkg", "c_bkg", "rho_bkg", "azzmin", "pgradx", "pgrady")

@@ -103,3 +103,90 @@ def test_cfl_bailout_writes_emergency_snapshot(tmp_path):
             f.close()
     finally:
         ctx.close()
+
+
+DIAG = ("ri", "epsh", "epsv", "chi_v", "chi_h")
+
+
+@pytest.mark.parametrize("path", ["subroutines", "native"])
+def test_diagnostics_match_oracle(path):
+    """hc_set_diagnostics: Ri, epsh, epsv (VISCOSITY3D) and chi_v, chi_h
+    (MIXING) of the last step against the oracle's after the same three
+    steps of the golden run (the oracle's are pinned against the reference
+    compiled in place), through the per-subroutine entries and through the
+    native loop (the oracle replaying its noise draws); the state is the one
+    a run without diagnostics reaches, bit for bit."""
+    prm, before, _, noise, _ = load_case(GOLDEN)
+    print("diagnostics,", path, flush=True)
+    o = T.Oracle(prm)
+    for k, v in before.items():
+        o.set(k, v)
+    o.set_time(-prm.dt, 0)
+    ctx = G.context_from_state(prm, before)
+    plain = G.context_from_state(prm, before)
+    try:
+        ctx.set_diagnostics(True)
+        for m in range(3):
+            nz = noise[m] if path == "subroutines" else ctx.source_noise(m)
+            o.step(m, nz)
+            if path == "subroutines":
+                ctx.driver_step(m, nz)
+                plain.driver_step(m, nz)
+        if path == "native":
+            ctx.step(3)
+            plain.step(3)
+        for n in DIAG:
+            a, b = ctx.diagnostic(n), o.get(n)
+            assert np.abs(b).max() > 0, n
+            err = float(np.abs(a - b).max() / np.abs(b).max())
+            # rho' enters Ri as rho' + rho_bkg: the oracle's rhop differs
+            # from the device's by its s**1.5 rounding (a few ulps of 1030)
+            assert err <= (1e-8 if n == "ri" else 1e-10), (n, err)
+        for k in ("u", "v", "w", "t", "s", "ss", "ss_v", "rhop"):
+            assert np.array_equal(ctx.download(k), plain.download(k)), k
+    finally:
+        ctx.close()
+        plain.close()
+
+
+def test_full_block_and_box_writers(tmp_path):
+    """PUT_full_block and PUT_full_block2 files from the device state, read
+    back: the diagnostics, rrho and the slab/box extents as the device holds
+    them, rounded once to FP32."""
+    prm, before, _, noise, _ = load_case(GOLDEN)
+    print("full-block writers", flush=True)
+    ctx = G.context_from_state(prm, before)
+    try:
+        ctx.set_diagnostics(True)
+        for m in range(2):
+            ctx.driver_step(m, noise[m])
+        nys, nye = 3, 8
+        path = str(tmp_path / "block.nc")
+        with output.FullBlockWriter(ctx, path, nys, nye) as w:
+            w.append()
+        f = netcdf_file(path, "r", mmap=False)
+        try:
+            for n, d in (("epsh", "epsh"), ("chiv", "chi_v"), ("Ri", "ri")):
+                want = ctx.diagnostic(d)[:, nys:nye + 1, :].astype(np.float32).transpose(2, 1, 0)
+                assert np.array_equal(f.variables[n][0], want), n
+            want = expected(ctx, "t")[:, nys:nye + 1, :].transpose(2, 1, 0)
+            assert np.array_equal(f.variables["temp"][0], want)
+            rr = f.variables["rrho"][0]
+            assert np.all((rr[1:-1, 1:-1, 1:-1] > 1000.0))
+        finally:
+            f.close()
+        box = ((2, 10), (1, 5), (3, 9))
+        path = str(tmp_path / "box.nc")
+        with output.BoxWriter(ctx, path, box) as w:
+            w.append()
+        f = netcdf_file(path, "r", mmap=False)
+        try:
+            (xs, xe), (ys, ye), (zs, ze) = box
+            want = expected(ctx, "w")[xs:xe + 1, ys:ye + 1, zs:ze + 2].transpose(2, 1, 0)  # w bounds start at -1
+            assert np.array_equal(f.variables["w"][0], want)
+            want = ctx.diagnostic("epsv")[xs:xe + 1, ys:ye + 1, zs:ze + 1].astype(np.float32).transpose(2, 1, 0)
+            assert np.array_equal(f.variables["epsv"][0], want)
+        finally:
+            f.close()
+    finally:
+        ctx.close()

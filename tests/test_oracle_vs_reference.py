@@ -20,7 +20,7 @@ SEQS = {
                  ["up", "vp", "wp"], 1e-14),
     "source_terms": (["body_force", "source_terms"], ["q_t", "q_s", "q_c", "isource_map"], 0.0),
     "viscosity3d": (["viscosity3d"], ["ss", "ss_v", "ri", "epsh", "epsv"], 1e-15),
-    "mixing": (["body_force", "viscosity3d", "mixing"], ["kxt", "kyt", "kzt"], 1e-15),
+    "mixing": (["body_force", "viscosity3d", "mixing"], ["kxt", "kyt", "kzt", "chi_v", "chi_h"], 1e-15),
     "tracers+density": (["body_force", "viscosity3d", "source_terms", "mixing", "work_velocity", "tracer1",
                          "tracer2", "density", "tracer3"], ["t", "s", "c", "rhop", "ud", "vd", "wd"], 1e-11),
     "refresh": (["body_force", "refresh"], ["u", "um", "v", "vm", "w", "wm"], 0.0),
