@@ -293,6 +293,50 @@ module hotcross_capi
       real(c_double), intent(in) :: obs(*)
       real(c_double), intent(out) :: loss, gs(3)
     end function
+    ! the same window under binomial checkpointing with nsnap snapshot slots
+    integer(c_int) function hc_adj_window_binomial_bytes(ctx, nsteps, nsnap, bytes) &
+        bind(C, name="hc_adj_window_binomial_bytes")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps, nsnap
+      integer(c_int64_t), intent(out) :: bytes
+    end function
+    integer(c_int) function hc_adj_window_binomial(ctx, nsteps, nsnap, obs, samples, loss, gq, gs, end_state) &
+        bind(C, name="hc_adj_window_binomial")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx, samples, gq
+      integer(c_int32_t), value :: nsteps, nsnap, end_state
+      real(c_double), intent(in) :: obs(*)
+      real(c_double), intent(out) :: loss, gs(3)
+    end function
+    integer(c_int) function hc_binomial_schedule(nsteps, nsnap, acts, cap, nacts, nfwd) &
+        bind(C, name="hc_binomial_schedule")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      integer(c_int32_t), value :: nsteps, nsnap
+      type(c_ptr), value :: acts
+      integer(c_int64_t), value :: cap
+      integer(c_int64_t), intent(out) :: nacts, nfwd
+    end function
+    ! output diagnostics and the case arrays the context holds
+    integer(c_int) function hc_set_diagnostics(ctx, on) bind(C, name="hc_set_diagnostics")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function hc_get_diagnostic(ctx, which, host) bind(C, name="hc_get_diagnostic")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which
+      real(c_double), intent(out) :: host(*)
+    end function
+    integer(c_int) function hc_get_profiles(ctx, t_bkg, s_bkg, rho_bkg, azzmin) bind(C, name="hc_get_profiles")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, t_bkg, s_bkg, rho_bkg, azzmin
+    end function
+    integer(c_int) function hc_get_sponge(ctx, alphau, alphav, alphae, alphaw) bind(C, name="hc_get_sponge")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, alphau, alphav, alphae, alphaw
+    end function
   end interface
 
 contains
