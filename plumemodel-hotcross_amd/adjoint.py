@@ -43,12 +43,15 @@ class ProbeMisfit(torch.autograd.Function):
 
 class Window:
     """An adjoint window over a context: a saved initial state, the probes,
-    the observations and the window's length and checkpoint interval."""
+    the observations and the window's length and checkpointing (a
+    checkpoint every `seg` steps, or with nsnap binomial checkpointing with
+    that many snapshot slots)."""
 
-    def __init__(self, ctx, probes, every, nsteps, seg, obs, wgt_w=1.0, wgt_t=1.0):
+    def __init__(self, ctx, probes, every, nsteps, seg, obs, wgt_w=1.0, wgt_t=1.0, nsnap=None):
         self.ctx = ctx
         self.probes = np.asarray(probes, dtype=np.int32).reshape(-1, 3)
         self.every, self.nsteps, self.seg = int(every), int(nsteps), int(seg)
+        self.nsnap = None if nsnap is None else int(nsnap)
         self.obs = np.asarray(obs, dtype=np.float64)
         self.wgt = (float(wgt_w), float(wgt_t))
         self.last_samples = None
@@ -58,6 +61,8 @@ class Window:
         of the window from the context's current state, which it keeps."""
         self.ctx.set_controls(qmap, flow_scale, cs)
         self.ctx.set_probes(self.probes, self.every, *self.wgt)
+        if self.nsnap is not None:
+            return self.ctx.adj_window_binomial(self.nsteps, self.nsnap, self.obs, end_state=False)
         return self.ctx.adj_window(self.nsteps, self.seg, self.obs, end_state=False)
 
     def loss(self, qmap, flow_scale, cs):

@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
-"""Recovery of the vent's heat-flux map from probe data: the source-recovery
-configuration of the adjoint (256^3, dx = 0.5 m, 1800 steps, 1 Hz w/T probes
-on the plane 20 m above the vent, checkpoints sized to an HBM budget).
+"""Recovery of the vent's heat-flux map from probe data: the forward + adjoint
+configuration at 256^3 (dx = 0.5 m, FP32, 1800 steps, 1 Hz w/T probes on the
+plane 20 m above the vent, checkpoints under an HBM budget).
 
-A seeded "true" map (a per-cell factor on the vent heat flux, mean 1 so the
-total stays 50 MW) produces synthetic probe data; L-BFGS through the torch
-harness (adjoint.Window) then fits the map from a uniform start.  Each
-evaluation is one adjoint window (forward with checkpoints, reverse sweep).
+A seeded "true" map (a per-cell factor on the vent heat flux over the 8 x 9
+source cells, mean 1 so the total stays 50 MW) produces synthetic probe data;
+the map is then fitted from a uniform start by projected gradient descent
+(the gradient's mean removed and the map kept at mean 1, so every iterate
+keeps the 50 MW total; a safeguarded step length: a trial that lowers the
+loss is taken and the step grows, one that does not is dropped and the step
+halves), one adjoint window (forward with checkpoints, reverse sweep) per
+iteration, or by L-BFGS through the torch harness (adjoint.Window).
 
 usage: python recover_source.py [--workload les256] [--precision fp32]
-           [--steps 1800] [--spinup 300] [--evals 8] [--budget-gb 200]
-Prints one line per evaluation (loss, map error) and a JSON summary.
+           [--steps 1800] [--spinup 300] [--iters 20] [--optimizer gd|lbfgs]
+           [--checkpointing binomial|fixed|auto] [--budget-gb 200] [--log FILE]
+Prints one line per iteration (loss, map error) and a JSON summary; --log
+also writes each iteration as a JSON line as it ends.
 """
 import argparse
 import json
@@ -26,15 +32,42 @@ import adjoint as AD  # noqa: E402
 import hotcross as H  # noqa: E402
 
 
+def plan(ctx, steps, budget, kind):
+    """(seg, nsnap, bytes, description) of the window's checkpointing."""
+    if kind == "auto":
+        kind, k, need = AD.plan_window(ctx, steps, budget)
+    elif kind == "fixed":
+        k, need = AD.plan_segment(ctx, steps, budget)
+    else:
+        lo = 1
+        if ctx.adj_window_binomial_bytes(steps, 1) > budget:
+            raise ValueError("not even one snapshot fits the budget")
+        hi = steps
+        while lo < hi:
+            mid = (lo + hi + 1) // 2
+            if ctx.adj_window_binomial_bytes(steps, mid) <= budget:
+                lo = mid
+            else:
+                hi = mid - 1
+        k, need = lo, ctx.adj_window_binomial_bytes(steps, lo)
+    if kind == "fixed":
+        return k, None, need, "a checkpoint every %d steps" % k
+    nfwd = H.binomial_schedule(steps, k)[1]
+    return 1, k, need, "binomial checkpointing, %d snapshot slots, %d forward steps" % (k, nfwd)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="les256")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
     ap.add_argument("--steps", type=int, default=1800)
     ap.add_argument("--spinup", type=int, default=300)
-    ap.add_argument("--evals", type=int, default=8)
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--optimizer", choices=["gd", "lbfgs"], default="gd")
+    ap.add_argument("--checkpointing", choices=["binomial", "fixed", "auto"], default="binomial")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--log", default=None)
     args = ap.parse_args()
 
     p = H.workload_params(args.workload)
@@ -45,9 +78,9 @@ def main():
     ctx.step(args.spinup)
     ctx.synchronize()
     npatch = ctx.npatch()
-    seg, need = AD.plan_segment(ctx, args.steps, args.budget_gb * 1e9)
-    print("spin-up %d steps %.1f s; %d source cells; checkpoint every %d steps, %.1f GB of checkpoints and tapes"
-          % (args.spinup, time.time() - tic, npatch, seg, need / 1e9), flush=True)
+    seg, nsnap, need, how = plan(ctx, args.steps, args.budget_gb * 1e9, args.checkpointing)
+    print("spin-up %d steps %.1f s; %d source cells; %s, %.1f GB of checkpoints and tapes"
+          % (args.spinup, time.time() - tic, npatch, how, need / 1e9), flush=True)
 
     rng = np.random.default_rng(args.seed)
     truth = 1.0 + 0.4 * (rng.random(npatch) - 0.5)
@@ -56,37 +89,81 @@ def main():
     every = AD.samples_every(p)
     ns = args.steps // every
     # synthetic data: the window's probe samples under the true map
-    ctx.set_controls(truth, 1.0, p.cs_h)
-    ctx.set_probes(probes, every, 1.0, 1.0)
+    win = AD.Window(ctx, probes, every, args.steps, seg, np.zeros((ns, len(probes), 2)), wgt_w=1.0e4,
+                    wgt_t=1.0e4, nsnap=nsnap)
     tic = time.time()
-    _, obs, _, _ = ctx.adj_window(args.steps, seg, np.zeros((ns, len(probes), 2)), end_state=False)
+    _, obs, _, _ = win.evaluate(truth, 1.0, p.cs_h)
     print("synthetic data: one window %.1f s" % (time.time() - tic), flush=True)
-
-    # weights that make w (m/s) and T (K) misfits comparable
-    win = AD.Window(ctx, probes, every, args.steps, seg, obs, wgt_w=1.0e4, wgt_t=1.0e4)
-    q = torch.ones(npatch, dtype=torch.float64, requires_grad=True)
-    a = torch.tensor(1.0, dtype=torch.float64)
-    cs = torch.tensor(p.cs_h, dtype=torch.float64)
-    opt = torch.optim.LBFGS([q], lr=1.0, max_iter=args.evals, max_eval=args.evals, tolerance_grad=0.0,
-                            tolerance_change=0.0, history_size=10, line_search_fn="strong_wolfe")
+    win.obs = obs
     hist = []
+    log = open(args.log, "w") if args.log else None
 
-    def closure():
-        opt.zero_grad()
+    def record(q, loss, g, t0, step=None, **extra):
+        err = float(np.linalg.norm(q - truth) / np.linalg.norm(truth - 1.0))
+        h = {"iteration": len(hist), "loss": float(loss), "map_err": err, "grad_norm": float(np.linalg.norm(g)),
+             "step": step, "seconds": time.time() - t0}
+        h.update(extra)
+        hist.append(h)
+        print("iteration %d: loss %.6e, map error %.4f (relative to the true deviation), |grad| %.3e, %.1f s"
+              % (h["iteration"], h["loss"], err, h["grad_norm"], h["seconds"]), flush=True)
+        if log:
+            log.write(json.dumps(h) + "\n")
+            log.flush()
+
+    if args.optimizer == "gd":
+        # projected gradient descent with a safeguarded step: each iteration
+        # evaluates one trial map q - eta g / max|g| (the largest cell moves
+        # by eta), kept within [0.2, 2] and at mean 1 (total 50 MW); a trial
+        # that lowers the loss is taken and eta grows by 1.5, one that does
+        # not is dropped and eta halves.  Over 1800 steps the plume's
+        # trajectory is chaotic in the map, so unguarded steps (Barzilai-
+        # Borwein) overshoot until the flow blows up.
+        def project(x):
+            x = np.clip(x, 0.2, 2.0)
+            return x - x.mean() + 1.0
+
+        eta = 0.05
+        q_best = np.ones(npatch)
         t0 = time.time()
-        loss = win.loss(q, a, cs)
-        loss.backward()
-        err = float(np.linalg.norm(q.detach().numpy() - truth) / np.linalg.norm(truth - 1.0))
-        hist.append({"loss": float(loss.detach()), "map_err": err, "seconds": time.time() - t0})
-        print("evaluation %d: loss %.6e, map error %.4f (relative to the true deviation), %.1f s"
-              % (len(hist), hist[-1]["loss"], err, hist[-1]["seconds"]), flush=True)
-        return loss
+        loss_best, _, gq, _ = win.evaluate(q_best, 1.0, p.cs_h)
+        g_best = gq - gq.mean()  # the gradient on maps of fixed total
+        record(q_best, loss_best, g_best, t0, eta)
+        while len(hist) <= args.iters:
+            trial = project(q_best - eta * g_best / max(np.abs(g_best).max(), 1e-300))
+            t0 = time.time()
+            loss, _, gq, _ = win.evaluate(trial, 1.0, p.cs_h)
+            g = gq - gq.mean()
+            accepted = bool(np.isfinite(loss) and loss < loss_best)
+            if accepted:
+                q_best, loss_best, g_best = trial, loss, g
+                eta *= 1.5
+            else:
+                eta *= 0.5
+            record(trial, loss, g, t0, eta, accepted=accepted, best_loss=float(loss_best),
+                   best_map_err=float(np.linalg.norm(q_best - truth) / np.linalg.norm(truth - 1.0)))
+    else:
+        q = torch.ones(npatch, dtype=torch.float64, requires_grad=True)
+        a = torch.tensor(1.0, dtype=torch.float64)
+        cs = torch.tensor(p.cs_h, dtype=torch.float64)
+        opt = torch.optim.LBFGS([q], lr=1.0, max_iter=args.iters, max_eval=args.iters, tolerance_grad=0.0,
+                                tolerance_change=0.0, history_size=10, line_search_fn="strong_wolfe")
 
-    opt.step(closure)
-    out = {"workload": args.workload, "precision": args.precision, "steps": args.steps, "seg": seg,
-           "hbm_gb": need / 1e9, "npatch": npatch, "evaluations": hist,
-           "loss_ratio": hist[-1]["loss"] / hist[0]["loss"], "map_err_final": hist[-1]["map_err"]}
+        def closure():
+            opt.zero_grad()
+            t0 = time.time()
+            loss = win.loss(q, a, cs)
+            loss.backward()
+            record(q.detach().numpy().copy(), loss.detach(), q.grad.numpy(), t0)
+            return loss
+
+        opt.step(closure)
+    best = min(hist, key=lambda h: h["loss"] if np.isfinite(h["loss"]) else np.inf)
+    out = {"workload": args.workload, "precision": args.precision, "steps": args.steps, "checkpointing": how,
+           "hbm_gb": need / 1e9, "npatch": npatch, "optimizer": args.optimizer, "iterations": hist,
+           "loss_ratio": best["loss"] / hist[0]["loss"], "map_err_best": best["map_err"]}
     print(json.dumps(out))
+    if log:
+        log.close()
     ctx.close()
 
 
