@@ -45,7 +45,7 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 # algorithmic HBM bytes per cell-step: 130.3 field moves (compulsory reads and
 # writes of every reference subroutine, counted once each) x 8 B in FP64
 FIELD_MOVES = 130.3
-# MPDATA corrective pass, one launch of k_mpdata_pass2: pass 2 reads c and the
+# MPDATA corrective pass, one launch of k_mpdata_pass3: pass 2 reads c and the
 # three Courant numbers and writes the three antidiffusive velocities and the
 # result (8 field moves per cell); pass 3 reads c, the three antidiffusive
 # velocities and the field's entry value and writes the result (6 moves).
@@ -54,9 +54,9 @@ MPDATA_PASS_MOVES = 7.0
 
 
 def profiled_traffic(workload, precision):
-    """HBM bytes per launch of k_mpdata_pass2 from the committed rocprofv3
+    """HBM bytes per launch of k_mpdata_pass3 from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload (None if there is none)."""
-    path = os.path.join(ROOT, "profiles", "k_mpdata_pass2_%s_%s_hbm.md" % (workload, precision))
+    path = os.path.join(ROOT, "profiles", "k_mpdata_pass3_%s_%s_hbm.md" % (workload, precision))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -352,14 +352,14 @@ def main():
     ms_step = sec * 1e3 / args.steps
     esz = 8 if args.precision == "fp64" else 4
     # roofline of the dominant kernel: the fused MPDATA corrective pass
-    # (k_mpdata_pass2), timed per launch with HIP events on the context stream
+    # (k_mpdata_pass3), timed per launch with HIP events on the context stream
     roof = None
     k_ms, k_n = phases.get("mpdata_pass", (0.0, 0))
     if k_n:
         per_launch = k_ms / k_n * 1e-3
         alg = MPDATA_PASS_MOVES * esz * cells
         achieved = alg / per_launch / 1e9
-        roof = {"bound": "hbm", "kernel": "k_mpdata_pass2 (MPDATA corrective pass, mean of passes 2 and 3)",
+        roof = {"bound": "hbm", "kernel": "k_mpdata_pass3 (MPDATA corrective pass, mean of passes 2 and 3)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved / (HBM_PEAK / 1e9), 4),
                 "traffic": profiled_traffic(args.workload, args.precision),

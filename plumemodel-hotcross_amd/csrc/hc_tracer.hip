@@ -27,6 +27,7 @@
 #include "hc_cell.h"
 #include "hc_mpdata.h"
 #include "hc_mpdata2.h"
+#include "hc_mpdata3.h"
 #include "hc_tile.h"
 
 namespace hc {
@@ -233,6 +234,32 @@ __global__ void __launch_bounds__(TI *TJ, 4) k_mpdata_pass2(Layout L, MpPass<T> 
   mp2_block<T, TI, TJ>(x, g, sh, a);
 }
 
+// the pass with each face's donor-cell flux formed once (MpTile3): the
+// fluxes of the flags and the limited fluxes of the finishing step come from
+// products formed right after the face velocities
+template <typename T, int TI, int TJ>
+struct DevExec3 {
+  Mp3State<T, TI, TJ> st;
+  template <class F>
+  __device__ void each(F f) { f((int)threadIdx.x, st); }
+  __device__ void sync() { __syncthreads(); }
+};
+
+template <typename T, int TI, int TJ>
+__global__ void __launch_bounds__(TI *TJ, 4) k_mpdata_pass3(Layout L, MpPass<T> a, int kc, int remap) {
+  __shared__ Mp3Shared<T, TI, TJ> sh;
+  unsigned bx, by, bz;
+  box_block_coords(remap, bx, by, bz);  // XCD x gets the x-th slice of tiles
+  MpGeom g;
+  g.nx = L.nx; g.ny = L.ny; g.nz = L.nz; g.px = L.px; g.py = L.py;
+  g.i0 = 1 + (int)bx * TI;
+  g.j0 = 1 + (int)by * TJ;
+  g.ka = 1 + (int)bz * kc;
+  g.kb = min(L.nz, g.ka + kc - 1);
+  DevExec3<T, TI, TJ> x;
+  mp3_block<T, TI, TJ>(x, g, sh, a);
+}
+
 // ghost cells of a scalar field from their interior images.  One thread per
 // ghost cell: the x faces (with edges), then the y faces, then the z faces.
 // ghost cell number t: the x faces (with edges), then the y faces, then the
@@ -364,15 +391,17 @@ int tile_remap() {
 }
 
 // tile of the corrective-pass kernel (HC_MP_TILE = 0, 1, 2: MpTile at
-// 32x8, 16x16, 32x16; 3, the default, and 4: the fixed-assignment MpTile2 at
-// 32x16 and 16x16).  All give the same values bit for bit; at 256^3 the
-// default took 0.403 ms (FP64) and 0.240 ms (FP32) per launch against
-// 0.461 and 0.304 for MpTile 32x16 (two 512-thread blocks of 79 KiB LDS per
-// CU in FP64).
+// 32x8, 16x16, 32x16; 3 and 4: the fixed-assignment MpTile2 at 32x16 and
+// 16x16; 5, the default, 6 and 7: MpTile3, each face's donor-cell flux formed
+// once, at 32x16, 32x12 and 16x16).  All give the same values bit for bit in
+// FP64 (the FP32 form of MpTile3 forms the antidiffusive velocities with fused
+// multiply-adds).  At 256^3 the default takes 0.379 ms (FP64) and 0.219 ms
+// (FP32) per launch against 0.401 and 0.241 for MpTile2 32x16 and 0.461 and
+// 0.304 for MpTile 32x16.
 static int mp_tile() {
   static int t = [] {
     const char *e = getenv("HC_MP_TILE");
-    return e ? atoi(e) : 3;
+    return e ? atoi(e) : 5;
   }();
   return t;
 }
@@ -391,9 +420,19 @@ static void launch_mpdata_tile2(hipStream_t st, const Layout &L, const MpPass<T>
   hipLaunchKernelGGL((k_mpdata_pass2<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc, tile_remap());
 }
 
+template <typename T, int TI, int TJ>
+static void launch_mpdata_tile3(hipStream_t st, const Layout &L, const MpPass<T> &a) {
+  const int kc = mp_chunk(L, TI, TJ);
+  dim3 grid((L.nx + TI - 1) / TI, (L.ny + TJ - 1) / TJ, (L.nz + kc - 1) / kc);
+  hipLaunchKernelGGL((k_mpdata_pass3<T, TI, TJ>), grid, dim3(TI * TJ), 0, st, L, a, kc, tile_remap());
+}
+
 template <typename T>
 static void launch_mpdata_pass(hipStream_t st, const Layout &L, const MpPass<T> &a) {
   switch (mp_tile()) {
+    case 5: launch_mpdata_tile3<T, 32, 16>(st, L, a); break;
+    case 6: launch_mpdata_tile3<T, 32, 12>(st, L, a); break;
+    case 7: launch_mpdata_tile3<T, 16, 16>(st, L, a); break;
     case 1: launch_mpdata_tile<T, 16, 16>(st, L, a); break;
     case 2: launch_mpdata_tile<T, 32, 16>(st, L, a); break;
     case 3: launch_mpdata_tile2<T, 32, 16>(st, L, a); break;

@@ -1,5 +1,6 @@
 // mpcheck.hip - host build of the MPDATA corrective-pass tiles (hc_mpdata.h:
-// MpTile, the kernel k_mpdata_pass runs; hc_mpdata2.h: MpTile2), for CPU
+// MpTile, the kernel k_mpdata_pass runs; hc_mpdata2.h: MpTile2; hc_mpdata3.h:
+// MpTile3), for CPU
 // tests: every block of a launch runs serially, each phase over all the
 // block's threads before the next (what the barriers guarantee on the
 // device), with every LDS index range-checked and LDS filled with NaN at the
@@ -14,6 +15,7 @@
 #include <vector>
 
 static int g_lim_s = 0, g_lim_c = 0;  // staged / computed LDS plane sizes
+static int g_lim_z = 0;                // halo z-flux entries (MpTile3)
 static long g_lim_n = 0;               // elements of a padded field
 static long g_bad = 0;                 // LDS or field indexes out of range
 static inline int chk(int x, int lim) {
@@ -25,6 +27,7 @@ static inline int chk(int x, int lim) {
 }
 #define HC_MP_CHECK(x) chk((x), g_lim_s)
 #define HC_MP_CHECKC(x) chk((x), g_lim_c)
+#define HC_MP_CHECKZ(x) chk((x), g_lim_z)
 static inline long chkg(long x) {
   if (x < 0 || x >= g_lim_n) {
     ++g_bad;
@@ -34,7 +37,7 @@ static inline long chkg(long x) {
 }
 #define HC_MP_GIDX(x) chkg(x)
 
-#include "../../plumemodel-hotcross_amd/csrc/hc_mpdata2.h"
+#include "../../plumemodel-hotcross_amd/csrc/hc_mpdata3.h"
 
 using namespace hc;
 
@@ -51,8 +54,8 @@ struct HostExec {
   void sync() {}
 };
 
-template <typename T, int TI, int TJ>
-void fill_nan(MpShared<T, TI, TJ> &sh) {
+template <class Sh>
+void fill_nan(Sh &sh) {
   memset(&sh, 0xff, sizeof(sh));  // NaN in every T slot, 255 (no flag) in the flag planes
 }
 
@@ -95,6 +98,10 @@ long run(int variant, int nx, int ny, int nz, int kc, const MpPass<T> &a) {
   g_lim_n = (long)L.n;
   g_bad = 0;
   std::unique_ptr<Sh> sh(new Sh);
+  typedef Mp3Shared<T, TI, TJ> Sh3;
+  static_assert(Sh3::S == Sh::S && Sh3::SC == Sh::SC, "the same plane sizes");
+  std::unique_ptr<Sh3> sh3(new Sh3);
+  g_lim_z = Sh3::NHL;
   const int bxn = (nx + TI - 1) / TI, byn = (ny + TJ - 1) / TJ, bzn = (nz + kc - 1) / kc;
   for (int bz = 0; bz < bzn; ++bz)
     for (int by = 0; by < byn; ++by)
@@ -106,11 +113,17 @@ long run(int variant, int nx, int ny, int nz, int kc, const MpPass<T> &a) {
         g.ka = 1 + bz * kc;
         g.kb = g.ka + kc - 1 < nz ? g.ka + kc - 1 : nz;
         fill_nan(*sh);
+        fill_nan(*sh3);
         if (variant == 1) {
           v1_block<T, TI, TJ>(g, *sh, a, bx + by + bz);
-        } else if constexpr (TJ >= 16) {  // MpTile2 needs one round of halo faces
-          HostExec<Mp2State<T, TI, TJ>> x(TI * TJ);
-          mp2_block<T, TI, TJ>(x, g, *sh, a);
+        } else if constexpr (TJ >= 16) {  // MpTile2 and MpTile3 need one round of halo faces
+          if (variant == 2) {
+            HostExec<Mp2State<T, TI, TJ>> x(TI * TJ);
+            mp2_block<T, TI, TJ>(x, g, *sh, a);
+          } else {
+            HostExec<Mp3State<T, TI, TJ>> x(TI * TJ);
+            mp3_block<T, TI, TJ>(x, g, *sh3, a);
+          }
         }
       }
   return g_bad;
@@ -123,7 +136,7 @@ extern "C" {
 int64_t mpc_layout(int nx, int ny, int nz) { return (int64_t)make_layout(nx, ny, nz).n; }
 
 // One corrective pass over padded FP64 fields (the library's layout).
-// variant 1: the MpTile loop of k_mpdata_pass; 2: mp2_block.  tile 0: 32x16,
+// variant 1: the MpTile loop of k_mpdata_pass; 2: mp2_block; 3: mp3_block.  tile 0: 32x16,
 // 1: 16x16, 2: 32x8 (variant 1 only).  ud/vd/wd/cpre may be null.  dbl
 // holds cmin, cmax, tau3, dxdydz; ints holds last, map_valid, nxs, nxe, nys,
 // nye.  Returns the number of LDS indexes out of range.

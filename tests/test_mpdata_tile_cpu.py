@@ -1,5 +1,6 @@
 """The tiled MPDATA corrective pass (the phases of k_mpdata_pass, hc_mpdata.h,
-and of the fixed-assignment tile, hc_mpdata2.h), built for the host
+of the fixed-assignment tile, hc_mpdata2.h, and of the tile that forms each
+face's donor-cell flux once, hc_mpdata3.h), built for the host
 (tests/mpcheck) and run block by block, against one corrective pass of the
 oracle's TRACER31 (mpdata-hotcross80.f:196-381), bit for bit in FP64.
 
@@ -28,7 +29,7 @@ NX, NY, NZ = 72, 36, 40
 
 
 def build():
-    deps = [SRC] + [os.path.join(CSRC, f) for f in ("hc_mpdata.h", "hc_mpdata2.h", "hc_common.h")]
+    deps = [SRC] + [os.path.join(CSRC, f) for f in ("hc_mpdata.h", "hc_mpdata2.h", "hc_mpdata3.h", "hc_common.h")]
     if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(f) for f in deps):
         subprocess.run(["hipcc", "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared",
                         "-ffp-contract=off", "-Wno-unused-result", "-o", LIB, SRC], check=True,
@@ -109,8 +110,10 @@ def host_pass(L, variant, tile, kc, c, vel, cmin, cmax, last=0, entry=None, tau3
             "ud": unpad(ud, "uc", NX, NY, NZ), "vd": unpad(vd, "vc", NX, NY, NZ), "wd": unpad(wd, "wc", NX, NY, NZ)}
 
 
-VARIANTS = [(1, 0, 4), (1, 0, 8), (1, 2, 40), (1, 1, 8), (2, 0, 4), (2, 0, 8), (2, 0, 40), (2, 1, 4)]
-IDS = ["%s_%s_kc%d" % ("mptile" if v == 1 else "mptile2", ("32x16", "16x16", "32x8")[t], k) for v, t, k in VARIANTS]
+VARIANTS = [(1, 0, 4), (1, 0, 8), (1, 2, 40), (1, 1, 8), (2, 0, 4), (2, 0, 8), (2, 0, 40), (2, 1, 4),
+            (3, 0, 4), (3, 0, 8), (3, 0, 40), (3, 1, 4)]
+IDS = ["%s_%s_kc%d" % (("mptile", "mptile2", "mptile3")[v - 1], ("32x16", "16x16", "32x8")[t], k)
+       for v, t, k in VARIANTS]
 
 
 @pytest.mark.parametrize("variant,tile,kc", VARIANTS, ids=IDS)
@@ -136,8 +139,9 @@ def test_corrective_pass_matches_oracle(lib, variant, tile, kc):
     assert np.array_equal(got["wd"][1:NX + 1, 1:NY + 1, 1:NZ + 2], ref_wd[1:NX + 1, 1:NY + 1, 1:NZ + 2])
 
 
-@pytest.mark.parametrize("variant,tile,kc", [(1, 0, 8), (2, 0, 4), (2, 1, 8)],
-                         ids=["mptile_32x16_kc8", "mptile2_32x16_kc4", "mptile2_16x16_kc8"])
+@pytest.mark.parametrize("variant,tile,kc", [(1, 0, 8), (2, 0, 4), (2, 1, 8), (3, 0, 4), (3, 1, 8)],
+                         ids=["mptile_32x16_kc8", "mptile2_32x16_kc4", "mptile2_16x16_kc8", "mptile3_32x16_kc4",
+                              "mptile3_16x16_kc8"])
 def test_last_corrective_pass_with_sponge_and_source(lib, variant, tile, kc):
     """The last pass (iord = 3) with the sponge relative to the entry value
     and the vent source on its patch at k = nz (mpdata-hotcross80.f:356-381);
