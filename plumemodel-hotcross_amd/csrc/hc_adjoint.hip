@@ -22,6 +22,7 @@
 // backwards: each segment is recomputed from its checkpoint into a tape of
 // its states in HBM, and its steps are reversed one by one (the step's
 // intermediates are recomputed from its taped state).
+#include <stdlib.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -188,6 +189,15 @@ inline dim3 adj_tile_grid(const Layout &L, int kc) {
   return dim3((unsigned)((L.nx + TI - 1) / TI), (unsigned)((L.ny + TJ - 1) / TJ), (unsigned)((L.nz + kc - 1) / kc));
 }
 constexpr int ATI = 32, ATJ = 8, AKC = 32;
+// planes per block of the adjoint tiles (HC_ADJ_MINB overrides the least
+// number of blocks a launch should have)
+static int adj_chunk(const Layout &L) {
+  static long minb = [] {
+    const char *e = getenv("HC_ADJ_MINB");
+    return e ? atol(e) : 1024L;
+  }();
+  return tile_chunk(L.nz, (long)((L.nx + ATI - 1) / ATI) * ((L.ny + ATJ - 1) / ATJ), AKC, minb);
+}
 
 // ---------------------------------------------------------------------------
 // FORCES
@@ -301,12 +311,17 @@ __global__ void __launch_bounds__(256) k_adj_mom_lambda(Layout L, Box3 b, int co
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_mom_p(Layout L, Box3 b, int comp, T c, const T *__restrict__ lamb,
-                                                   T *__restrict__ lp) {
+__global__ void __launch_bounds__(256) k_adj_mom_p(Layout L, Box3 b, Box3 fb, int comp, T c,
+                                                   const T *__restrict__ lamb, T *__restrict__ lp) {
   ADJ_IJK(b);
   if (!in) return;
-  const T prev = comp == 0 ? F3(lamb, i - 1, j, k) : (comp == 1 ? F3(lamb, i, j - 1, k) : F3(lamb, i, j, k - 1));
-  F3(lp, i, j, k) += c * (F3(lamb, i, j, k) - prev);
+  // Lambda exists on the formula's box fb (k_adj_mom_lambda writes it
+  // there); it is zero elsewhere
+  auto la = [&](int a, int d, int e) -> T {
+    return (a >= fb.i0 && a <= fb.i1 && d >= fb.j0 && d <= fb.j1 && e >= fb.k0 && e <= fb.k1) ? F3(lamb, a, d, e) : (T)0;
+  };
+  const T prev = comp == 0 ? la(i - 1, j, k) : (comp == 1 ? la(i, j - 1, k) : la(i, j, k - 1));
+  F3(lp, i, j, k) += c * (la(i, j, k) - prev);
 }
 
 // ---------------------------------------------------------------------------
@@ -525,152 +540,46 @@ __global__ void __launch_bounds__(256) k_adj_anti_fwd(Layout L, Box3 b, const T 
   if (i >= 1 && j >= 1) F3(wd, i, j, k) = mp_anti_w<T>(C, Uf, Vf, Wf);
 }
 
-// limiter flags of the interior cells (mpdata-hotcross80.f:660-690)
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_flags(Layout L, Box3 b, const T *__restrict__ cc,
-                                                   const T *__restrict__ ud, const T *__restrict__ vd,
-                                                   const T *__restrict__ wd, double cmin, double cmax,
-                                                   uint8_t *__restrict__ fl) {
-  ADJ_IJK(b);
-  if (!in) return;
-  const T c = F3(cc, i, j, k);
-  const T x = c + mp_donor<T>(c, F3(cc, i - 1, j, k), F3(cc, i + 1, j, k), F3(cc, i, j - 1, k), F3(cc, i, j + 1, k),
-                              F3(cc, i, j, k - 1), F3(cc, i, j, k + 1), F3(ud, i - 1, j, k), F3(ud, i, j, k),
-                              F3(vd, i, j - 1, k), F3(vd, i, j, k), F3(wd, i, j, k - 1), F3(wd, i, j, k), false, 0, 0,
-                              0, 0, 0, 0, 0);
-  uint8_t f = 0;
-  if (x <= (T)cmin) f = 1;
-  else if (x >= (T)cmax) f = 2;
-  F3(fl, i, j, k) = f;
-}
-
 struct DonorLast {
   int last;
+  // k_adj_donor_gather: the outputs it is the first to write, which it
+  // assigns (over its whole box) instead of adding to: 1 the entry
+  // concentration's adjoint of the pass (lcc), 2 the face velocities' (lud,
+  // lvd, lwd), 4 the tracer's entry value (lentry, sponge term)
+  int assign;
   double tau3, dxdydz;
   const double *alphae0, *alphaw, *qbase;
   double *gq;
   int map_valid, nxs, nxe, nys, nye;
 };
 
-// the limited donor-cell step of a pass, reversed (interior cells); on the
-// last pass also the sponge and source
+// the antidiffusive velocities reversed on the domain's first faces, in one
+// launch: ud at i = 0 (j, k >= 1), vd at j = 0 (i, k >= 1), wd at k = 0 (i,
+// j >= 1), one thread per face
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_donor(Layout L, Box3 b, DonorLast dl, const T *__restrict__ lout,
-                                                   const T *__restrict__ lpre, const T *__restrict__ cc,
-                                                   const T *__restrict__ ud, const T *__restrict__ vd,
-                                                   const T *__restrict__ wd, const uint8_t *__restrict__ fl,
-                                                   T *__restrict__ lcc, T *__restrict__ lud, T *__restrict__ lvd,
-                                                   T *__restrict__ lwd, T *__restrict__ lentry) {
-  ADJ_IJK(b);
-  if (!in) return;
-  const T lo = F3(lout, i, j, k);
-  const T ln = lo + (lpre ? F3(lpre, i, j, k) : (T)0);
-  if (dl.last && lo != (T)0) {
-    const double ae = fmax(dl.alphae0[i + (L.nx + 2) * j], (dl.alphaw[k + 1] + dl.alphaw[k]) / 2.0);
-    F3(lentry, i, j, k) += -(T)(0.5 * ae * dl.tau3) * lo;
-    if (k == L.nz && dl.map_valid && i >= dl.nxs && i <= dl.nxe && j >= dl.nys && j <= dl.nye) {
-      const int n = (i - dl.nxs) + (dl.nxe - dl.nxs + 1) * (j - dl.nys);
-      dl.gq[n] += (double)lo * dl.tau3 / dl.dxdydz * dl.qbase[n];
-    }
+__global__ void __launch_bounds__(256) k_adj_anti_faces(Layout L, const T *__restrict__ cc, const T *__restrict__ U,
+                                                        const T *__restrict__ V, const T *__restrict__ W,
+                                                        const T *__restrict__ lud, const T *__restrict__ lvd,
+                                                        const T *__restrict__ lwd, T *__restrict__ lcc,
+                                                        T *__restrict__ lU, T *__restrict__ lV, T *__restrict__ lW) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nu = (long)L.ny * L.nz, nv = (long)L.nx * L.nz, nw = (long)L.nx * L.ny;
+  int i, j, k;
+  if (t < nu) {
+    i = 0; j = 1 + (int)(t % L.ny); k = 1 + (int)(t / L.ny);
+  } else if ((t -= nu) < nv) {
+    i = 1 + (int)(t % L.nx); j = 0; k = 1 + (int)(t / L.nx);
+  } else if ((t -= nv) < nw) {
+    i = 1 + (int)(t % L.nx); j = 1 + (int)(t / L.nx); k = 0;
+  } else {
+    return;
   }
-  if (ln == (T)0) return;
-  T lc[7], lv[6];
-  adj_mp_donor<T>(F3(cc, i, j, k), F3(cc, i - 1, j, k), F3(cc, i + 1, j, k), F3(cc, i, j - 1, k), F3(cc, i, j + 1, k),
-                  F3(cc, i, j, k - 1), F3(cc, i, j, k + 1), F3(ud, i - 1, j, k), F3(ud, i, j, k),
-                  F3(vd, i, j - 1, k), F3(vd, i, j, k), F3(wd, i, j, k - 1), F3(wd, i, j, k), true,
-                  F3(fl, i, j, k), F3(fl, i - 1, j, k), F3(fl, i + 1, j, k), F3(fl, i, j - 1, k),
-                  F3(fl, i, j + 1, k), F3(fl, i, j, k - 1), F3(fl, i, j, k + 1), ln, lc, lv);
-  adj_add(&F3(lcc, i, j, k), ln + lc[0]);
-  adj_add(&F3(lcc, i - 1, j, k), lc[1]);
-  adj_add(&F3(lcc, i + 1, j, k), lc[2]);
-  adj_add(&F3(lcc, i, j - 1, k), lc[3]);
-  adj_add(&F3(lcc, i, j + 1, k), lc[4]);
-  adj_add(&F3(lcc, i, j, k - 1), lc[5]);
-  adj_add(&F3(lcc, i, j, k + 1), lc[6]);
-  adj_add(&F3(lud, i - 1, j, k), lv[0]);
-  adj_add(&F3(lud, i, j, k), lv[1]);
-  adj_add(&F3(lvd, i, j - 1, k), lv[2]);
-  adj_add(&F3(lvd, i, j, k), lv[3]);
-  adj_add(&F3(lwd, i, j, k - 1), lv[4]);
-  adj_add(&F3(lwd, i, j, k), lv[5]);
-}
-
-// the antidiffusive velocities' formulas reversed (one thread per face
-// position, the three face kinds of k_adj_anti_fwd)
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_anti(Layout L, Box3 b, const T *__restrict__ cc, const T *__restrict__ U,
-                                                  const T *__restrict__ V, const T *__restrict__ W,
-                                                  const T *__restrict__ lud, const T *__restrict__ lvd,
-                                                  const T *__restrict__ lwd, T *__restrict__ lcc, T *__restrict__ lU,
-                                                  T *__restrict__ lV, T *__restrict__ lW) {
-  ADJ_IJK(b);
-  if (!in) return;
   const RelR<T> C{cc, L, i, j, k}, Uf{U, L, i, j, k}, Vf{V, L, i, j, k}, Wf{W, L, i, j, k};
   const RelA<T> ac{lcc, L, i, j, k}, aU{lU, L, i, j, k}, aV{lV, L, i, j, k}, aW{lW, L, i, j, k};
-  if (j >= 1 && k >= 1) adj_anti_u<T>(C, Uf, Vf, Wf, F3(lud, i, j, k), ac, aU, aV, aW);
-  if (i >= 1 && k >= 1) adj_anti_v<T>(C, Uf, Vf, Wf, F3(lvd, i, j, k), ac, aU, aV, aW);
-  if (i >= 1 && j >= 1) adj_anti_w<T>(C, Uf, Vf, Wf, F3(lwd, i, j, k), ac, aU, aV, aW);
+  if (i == 0) adj_anti_u<T>(C, Uf, Vf, Wf, F3(lud, i, j, k), ac, aU, aV, aW);
+  else if (j == 0) adj_anti_v<T>(C, Uf, Vf, Wf, F3(lvd, i, j, k), ac, aU, aV, aW);
+  else adj_anti_w<T>(C, Uf, Vf, Wf, F3(lwd, i, j, k), ac, aU, aV, aW);
 }
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_pass1(Layout L, Box3 b, StepArgs s, Phys ph, FieldPtrs<T> f,
-                                                   const T *__restrict__ c, const T *__restrict__ uc,
-                                                   const T *__restrict__ vc, const T *__restrict__ wc,
-                                                   const double *__restrict__ azzmin, const double *__restrict__ cbkg,
-                                                   const T *__restrict__ lc1, T *__restrict__ lentry,
-                                                   T *__restrict__ luc, T *__restrict__ lvc, T *__restrict__ lwc,
-                                                   T *__restrict__ lss, T *__restrict__ lssv) {
-  ADJ_IJK(b);
-  if (!in) return;
-  const T l = F3(lc1, i, j, k);
-  if (l == (T)0) return;
-  const RView<T> C{c, L}, UC{uc, L}, VC{vc, L}, WC{wc, L}, SS{f.ss, L}, SV{f.ssv, L};
-  const AAcc<T> ac{lentry, L}, auc{luc, L}, avc{lvc, L}, awc{lwc, L}, ass{lss, L}, assv{lssv, L};
-  adj_pass1<T>(L, s, ph, C, UC, VC, WC, SS, SV, azzmin, cbkg, i, j, k, l, ac, auc, avc, awc, ass, assv);
-}
-
-
-// the limited donor step reversed, as a tile (interior cells)
-template <typename T>
-__global__ void __launch_bounds__(ATI *ATJ) k_adj_donor_tile(Layout L, DonorLast dl, const T *__restrict__ lout,
-                                                             const T *__restrict__ lpre, const T *__restrict__ cc,
-                                                             const T *__restrict__ ud, const T *__restrict__ vd,
-                                                             const T *__restrict__ wd, const uint8_t *__restrict__ fl,
-                                                             T *lcc, T *lud, T *lvd, T *lwd, T *lentry, int kc) {
-  typedef AdjRing<T, ATI, ATJ, 4> R;
-  __shared__ R sh;
-  const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
-  const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
-  T *const dst[4] = {lcc, lud, lvd, lwd};
-  typedef LAcc<T, R::SI, R::S> A;
-  const A ac{sh.a[0][0], i0 - 1, j0 - 1}, au{sh.a[1][0], i0 - 1, j0 - 1}, av{sh.a[2][0], i0 - 1, j0 - 1},
-      aw{sh.a[3][0], i0 - 1, j0 - 1};
-  adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
-    const T lo = F3(lout, i, j, k);
-    const T ln = lo + (lpre ? F3(lpre, i, j, k) : (T)0);
-    if (dl.last && lo != (T)0) {
-      const double ae = fmax(dl.alphae0[i + (L.nx + 2) * j], (dl.alphaw[k + 1] + dl.alphaw[k]) / 2.0);
-      F3(lentry, i, j, k) += -(T)(0.5 * ae * dl.tau3) * lo;
-      if (k == L.nz && dl.map_valid && i >= dl.nxs && i <= dl.nxe && j >= dl.nys && j <= dl.nye) {
-        const int n = (i - dl.nxs) + (dl.nxe - dl.nxs + 1) * (j - dl.nys);
-        dl.gq[n] += (double)lo * dl.tau3 / dl.dxdydz * dl.qbase[n];
-      }
-    }
-    if (ln == (T)0) return;
-    T lc[7], lv[6];
-    adj_mp_donor<T>(F3(cc, i, j, k), F3(cc, i - 1, j, k), F3(cc, i + 1, j, k), F3(cc, i, j - 1, k),
-                    F3(cc, i, j + 1, k), F3(cc, i, j, k - 1), F3(cc, i, j, k + 1), F3(ud, i - 1, j, k), F3(ud, i, j, k),
-                    F3(vd, i, j - 1, k), F3(vd, i, j, k), F3(wd, i, j, k - 1), F3(wd, i, j, k), true,
-                    F3(fl, i, j, k), F3(fl, i - 1, j, k), F3(fl, i + 1, j, k), F3(fl, i, j - 1, k),
-                    F3(fl, i, j + 1, k), F3(fl, i, j, k - 1), F3(fl, i, j, k + 1), ln, lc, lv);
-    ac(i, j, k, ln + lc[0]);
-    ac(i - 1, j, k, lc[1]); ac(i + 1, j, k, lc[2]); ac(i, j - 1, k, lc[3]); ac(i, j + 1, k, lc[4]);
-    ac(i, j, k - 1, lc[5]); ac(i, j, k + 1, lc[6]);
-    au(i - 1, j, k, lv[0]); au(i, j, k, lv[1]); av(i, j - 1, k, lv[2]); av(i, j, k, lv[3]);
-    aw(i, j, k - 1, lv[4]); aw(i, j, k, lv[5]);
-  });
-}
-
 
 // The limited donor-cell step of a pass reversed as a gather over the box
 // 0..n+1 (no atomics).  In flux form the step is cn = cc + sum of signed
@@ -713,10 +622,14 @@ __global__ void __launch_bounds__(256) k_adj_donor_gather(Layout L, Box3 b, Dono
   const bool cin = inner(i, j, k);
   const T lc = lnv(i, j, k);
   T acc = lc;
+  const bool set_c = dl.assign & 1, set_u = dl.assign & 2;
+  if ((dl.assign & 4) && !(dl.last && cin)) F3(lentry, i, j, k) = (T)0;
   if (dl.last && cin) {
     const T lo = F3(lout, i, j, k);
     const double ae = fmax(dl.alphae0[i + (nx + 2) * j], (dl.alphaw[k + 1] + dl.alphaw[k]) / 2.0);
-    F3(lentry, i, j, k) += -(T)(0.5 * ae * dl.tau3) * lo;
+    const T le = -(T)(0.5 * ae * dl.tau3) * lo;
+    if (dl.assign & 4) F3(lentry, i, j, k) = le;
+    else F3(lentry, i, j, k) += le;
     if (k == nz && dl.map_valid && i >= dl.nxs && i <= dl.nxe && j >= dl.nys && j <= dl.nye) {
       const int n = (i - dl.nxs) + (dl.nxe - dl.nxs + 1) * (j - dl.nys);
       dl.gq[n] += (double)lo * dl.tau3 / dl.dxdydz * dl.qbase[n];
@@ -733,7 +646,9 @@ __global__ void __launch_bounds__(256) k_adj_donor_gather(Layout L, Box3 b, Dono
       neg(u, mB(f0, fR), v2, dv2);
       const T lf = lnv(i + 1, j, k) - lc;
       acc += lf * v;
-      F3(lud, i, j, k) += lf * (dv * c0 + dv2 * F3(cc, i + 1, j, k));
+      const T g = lf * (dv * c0 + dv2 * F3(cc, i + 1, j, k));
+      if (set_u) F3(lud, i, j, k) = g;
+      else F3(lud, i, j, k) += g;
     }
     if (i >= 1) {  // x face i-1: L = (i-1,j,k), R = (i,j,k)
       const T u = F3(ud, i - 1, j, k);
@@ -749,7 +664,9 @@ __global__ void __launch_bounds__(256) k_adj_donor_gather(Layout L, Box3 b, Dono
       neg(u, mB(f0, fR), v2, dv2);
       const T lf = lnv(i, j + 1, k) - lc;
       acc += lf * v;
-      F3(lvd, i, j, k) += lf * (dv * c0 + dv2 * F3(cc, i, j + 1, k));
+      const T g = lf * (dv * c0 + dv2 * F3(cc, i, j + 1, k));
+      if (set_u) F3(lvd, i, j, k) = g;
+      else F3(lvd, i, j, k) += g;
     }
     if (j >= 1) {  // y face j-1
       const T u = F3(vd, i, j - 1, k);
@@ -765,7 +682,9 @@ __global__ void __launch_bounds__(256) k_adj_donor_gather(Layout L, Box3 b, Dono
       pos(w, mB(f0, fR), v, dv);    // wp
       const T lg = lc - lnv(i, j, k + 1);
       acc += lg * v2;
-      F3(lwd, i, j, k) += lg * (dv2 * c0 + dv * F3(cc, i, j, k + 1));
+      const T g = lg * (dv2 * c0 + dv * F3(cc, i, j, k + 1));
+      if (set_u) F3(lwd, i, j, k) = g;
+      else F3(lwd, i, j, k) += g;
     }
     if (k >= 1) {  // z face k-1: L = k-1, R = k
       const T w = F3(wd, i, j, k - 1);
@@ -773,7 +692,8 @@ __global__ void __launch_bounds__(256) k_adj_donor_gather(Layout L, Box3 b, Dono
       acc += (lnv(i, j, k - 1) - lc) * v;
     }
   }
-  F3(lcc, i, j, k) += acc;
+  if (set_c) F3(lcc, i, j, k) = acc;
+  else F3(lcc, i, j, k) += acc;
 }
 
 // the antidiffusive velocities reversed on the faces of the interior cells
@@ -862,8 +782,15 @@ static const int TAPE_FIELDS[NTAPE] = {HC_U,    HC_V,   HC_W, HC_UM, HC_VM, HC_W
 // a step's taped tracer intermediates: the results of passes 1 and 2 and
 // the antidiffusive velocities of passes 2 and 3
 static const int TTAPE = 8;
+// bytes of one tracer's taped step: TTAPE fields and the limiter flags of
+// passes 2 and 3 (one byte per cell each, 256-byte aligned)
+static size_t ttape_block(const hc_ctx *ctx) {
+  const size_t fl = (ctx->L.n + 255) / 256 * 256;
+  return (size_t)TTAPE * ctx->L.n * ctx->esz + 2 * fl;
+}
 struct TracerSlot {
   void *c1, *c2, *ud2[3], *ud3[3];
+  uint8_t *fl2, *fl3;  // limiter flags of passes 2 and 3
 };
 
 struct HostState {
@@ -984,7 +911,7 @@ static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
     hipLaunchKernelGGL(k_adj_forces<T>, grid3(b), dim3(64, 4), 0, ctx->st, L, b, s, ctx->P, fs, ptrs<T>(ctx), ss0,
                        ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav, ctx->aux.alphae0,
                        ctx->aux.alphaw, A->d_sc);
-  const int kc = tile_chunk(L.nz, (long)((L.nx + ATI - 1) / ATI) * ((L.ny + ATJ - 1) / ATJ), AKC, 512);
+  const int kc = adj_chunk(L);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
   hipLaunchKernelGGL(k_adj_forces_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
@@ -1005,11 +932,10 @@ static void adj_momentum_t(hc_ctx *ctx, int comp) {
   T *lf = comp == 0 ? lam.fx : comp == 1 ? lam.fy : lam.fz;
   Box3 fb = comp == 0 ? Box3{0, nx, 1, ny, 1, nz} : comp == 1 ? Box3{1, nx, 0, ny, 1, nz} : Box3{1, nx, 1, ny, 1, nz - 1};
   Box3 pb = comp == 0 ? Box3{0, nx + 1, 1, ny, 1, nz} : comp == 1 ? Box3{1, nx, 0, ny + 1, 1, nz} : Box3{1, nx, 1, ny, 1, nz};
-  zero(ctx, A->lamb);
   hipLaunchKernelGGL(k_adj_mom_lambda<T>, grid3(fb), dim3(64, 4), 0, ctx->st, L, fb, comp, s, ctx->P, (const T *)lup,
                      (T *)A->lamb, lu, lum, lf);
   const double d = comp == 0 ? s.dx : comp == 1 ? s.dy : s.dz;
-  hipLaunchKernelGGL(k_adj_mom_p<T>, grid3(pb), dim3(64, 4), 0, ctx->st, L, pb, comp, (T)(s.dt * s.leap / d),
+  hipLaunchKernelGGL(k_adj_mom_p<T>, grid3(pb), dim3(64, 4), 0, ctx->st, L, pb, fb, comp, (T)(s.dt * s.leap / d),
                      (const T *)A->lamb, lam.p);
   zero(ctx, lup);
 }
@@ -1032,7 +958,7 @@ static void adj_viscosity_t(hc_ctx *ctx, const T *ss1, const T *ssv1) {
                      (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
                      (const double *)ctx->aux.rho_bkg, ss1, ssv1, (const T *)lam.ss, (const T *)lam.ssv, lam.um,
                      lam.vm, lam.wm, lam.rhop, A->d_sc + ADJ_SLOTS);
-  const int kc = tile_chunk(L.nz, (long)((L.nx + ATI - 1) / ATI) * ((L.ny + ATJ - 1) / ATJ), AKC, 512);
+  const int kc = adj_chunk(L);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
   hipLaunchKernelGGL(k_adj_visc_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
@@ -1091,7 +1017,7 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   VentArgs va = vent_args(ctx);
   FieldPtrs<T> f = ptrs<T>(ctx);
   const T *c = (const T *)ctx->f[field];
-  TracerSlot is{A->c1, A->c2, {A->ud2[0], A->ud2[1], A->ud2[2]}, {A->ud3[0], A->ud3[1], A->ud3[2]}};
+  TracerSlot is{A->c1, A->c2, {A->ud2[0], A->ud2[1], A->ud2[2]}, {A->ud3[0], A->ud3[1], A->ud3[2]}, A->fl2, A->fl3};
   if (taped) {
     is = *taped;
   } else {
@@ -1101,18 +1027,21 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
     tw.ud2 = is.ud2[0]; tw.vd2 = is.ud2[1]; tw.wd2 = is.ud2[2];
     tw.c1 = is.c1; tw.c2 = is.c2; tw.cpre = A->cpre;
     tw.ud3 = is.ud3[0]; tw.vd3 = is.ud3[1]; tw.wd3 = is.ud3[2];
+    tw.fl2 = is.fl2; tw.fl3 = is.fl3;
     tw.budget_part = ctx->tw.budget_part; tw.budget = ctx->tw.budget;
     TracerArgs ta{cmin, cmax, tau3, 0, 0.0, 0};
     launch_tracer<T>(st, L, s, ctx->P, va, f, ctx->aux, tw, (T *)A->cw, d_cbkg, qpatch, ta);
   }
   const T *c1 = (const T *)is.c1, *c2 = (const T *)is.c2;
-  const Box3 ib = interior(L);
   const T *ud2 = (const T *)is.ud2[0], *vd2 = (const T *)is.ud2[1], *wd2 = (const T *)is.ud2[2];
   const T *ud3 = (const T *)is.ud3[0], *vd3 = (const T *)is.ud3[1], *wd3 = (const T *)is.ud3[2];
-  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, c1, ud2, vd2, wd2, cmin, cmax, A->fl2);
-  hipLaunchKernelGGL(k_adj_flags<T>, grid3(ib), dim3(64, 4), 0, st, L, ib, c2, ud3, vd3, wd3, cmin, cmax, A->fl3);
-  // adjoint buffers
-  void *zs[] = {A->lcnew, A->lpre, A->lc1, A->lc2, A->lud2[0], A->lud2[1], A->lud2[2], A->lud3[0], A->lud3[1], A->lud3[2]};
+  // the limiter flags of both passes, as the forward passes formed them
+  // (taped, or written by the recompute above)
+  const uint8_t *fl2 = is.fl2, *fl3 = is.fl3;
+  // adjoint buffers: lpre and the pass-2 face velocities' adjoints are
+  // first added to by atomics; the donor gathers assign the others on their
+  // first write (lc2, lud3 and lcnew in pass 3, lc1 in pass 2)
+  void *zs[] = {A->lpre, A->lud2[0], A->lud2[1], A->lud2[2]};
   for (void *p : zs) zero(ctx, p);
   T *lout = (T *)A->lam[field];
   const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
@@ -1121,33 +1050,35 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 1, vent, lout, (T *)A->lpre);
   // pass 3
   DonorLast dl{};
-  dl.last = 1; dl.tau3 = tau3; dl.dxdydz = ctx->P.dx * ctx->P.dy * ctx->P.dz;
+  dl.last = 1; dl.tau3 = tau3;
+  dl.assign = 1 | 2 | 4; dl.dxdydz = ctx->P.dx * ctx->P.dy * ctx->P.dz;
   dl.alphae0 = ctx->aux.alphae0; dl.alphaw = ctx->aux.alphaw; dl.qbase = ctx->d_qbase;
   dl.gq = (field == HC_T) ? A->d_gq : nullptr;
   dl.map_valid = (field == HC_T && ctx->d_qbase) ? va.map_valid : 0;
   dl.nxs = va.nxs; dl.nxe = va.nxe; dl.nys = va.nys; dl.nye = va.nye;
-  const int kc = tile_chunk(L.nz, (long)((L.nx + ATI - 1) / ATI) * ((L.ny + ATJ - 1) / ATJ), AKC, 512);
+  const int kc = adj_chunk(L);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc), tb(ATI * ATJ);
-  // the antidiffusive velocities' adjoint: boundary faces cell by cell,
-  // the interior faces as tiles
-  const Box3 afaces[3] = {{0, 0, 1, L.ny, 1, L.nz}, {1, L.nx, 0, 0, 1, L.nz}, {1, L.nx, 1, L.ny, 0, 0}};
+  // the antidiffusive velocities' adjoint: the domain's first faces in one
+  // launch, the interior faces as tiles
+  const long nfaces = (long)L.ny * L.nz + (long)L.nx * L.nz + (long)L.nx * L.ny;
   auto anti = [&](const T *cc, const T *U, const T *V, const T *W, const T *lu, const T *lv, const T *lw, T *lcc,
                   T *lU, T *lV, T *lW) {
-    for (const Box3 &b : afaces)
-      hipLaunchKernelGGL(k_adj_anti<T>, grid3(b), dim3(64, 4), 0, st, L, b, cc, U, V, W, lu, lv, lw, lcc, lU, lV, lW);
+    hipLaunchKernelGGL(k_adj_anti_faces<T>, dim3((unsigned)((nfaces + 255) / 256)), dim3(256), 0, st, L, cc, U, V, W,
+                       lu, lv, lw, lcc, lU, lV, lW);
     hipLaunchKernelGGL(k_adj_anti_tile<T>, tg, tb, 0, st, L, cc, U, V, W, lu, lv, lw, lcc, lU, lV, lW, kc);
   };
   const Box3 full{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
   hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, dl, (const T *)lout,
-                     (const T *)A->lpre, c2, ud3, vd3, wd3, (const uint8_t *)A->fl3, (T *)A->lc2, (T *)A->lud3[0],
+                     (const T *)A->lpre, c2, ud3, vd3, wd3, fl3, (T *)A->lc2, (T *)A->lud3[0],
                      (T *)A->lud3[1], (T *)A->lud3[2], (T *)A->lcnew);
   anti(c2, ud2, vd2, wd2, (const T *)A->lud3[0], (const T *)A->lud3[1], (const T *)A->lud3[2], (T *)A->lc2,
        (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2]);
   hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc2, (T *)A->lc2);
   // pass 2
   DonorLast d2{};
+  d2.assign = 1;
   hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, d2, (const T *)A->lc2,
-                     (const T *)nullptr, c1, ud2, vd2, wd2, (const uint8_t *)A->fl2, (T *)A->lc1,
+                     (const T *)nullptr, c1, ud2, vd2, wd2, fl2, (T *)A->lc1,
                      (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2], (T *)nullptr);
   anti(c1, (const T *)ctx->tw.uc, (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)A->lud2[0],
        (const T *)A->lud2[1], (const T *)A->lud2[2], (T *)A->lc1, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2]);
@@ -1464,7 +1395,7 @@ static void adj_head(hc_ctx *ctx, int m, size_t s0, size_t s1) {
 // index `q`
 static TracerSlot tracer_slot(hc_ctx *ctx, int q, int tr) {
   const size_t fb = ctx->L.n * ctx->esz;
-  char *b = (char *)ctx->adj->ttape + ((size_t)q * 2 + tr) * TTAPE * fb;
+  char *b = (char *)ctx->adj->ttape + ((size_t)q * 2 + tr) * ttape_block(ctx);
   TracerSlot t;
   t.c1 = b;
   t.c2 = b + fb;
@@ -1472,6 +1403,8 @@ static TracerSlot tracer_slot(hc_ctx *ctx, int q, int tr) {
     t.ud2[d] = b + (2 + d) * fb;
     t.ud3[d] = b + (5 + d) * fb;
   }
+  t.fl2 = (uint8_t *)(b + TTAPE * fb);
+  t.fl3 = t.fl2 + (ctx->L.n + 255) / 256 * 256;
   return t;
 }
 
@@ -1483,6 +1416,7 @@ static void taped_tracer(hc_ctx *ctx, int field, const TracerSlot &t, bool pass1
   ctx->tw.c1 = t.c1; ctx->tw.c2 = t.c2; ctx->tw.cpre = ctx->adj->cpre;
   ctx->tw.ud2 = t.ud2[0]; ctx->tw.vd2 = t.ud2[1]; ctx->tw.wd2 = t.ud2[2];
   ctx->tw.ud3 = t.ud3[0]; ctx->tw.vd3 = t.ud3[1]; ctx->tw.wd3 = t.ud3[2];
+  ctx->tw.fl2 = t.fl2; ctx->tw.fl3 = t.fl3;
   native_tracer(ctx, field, pass1_done ? t.c1 : nullptr);
   ctx->tw = keep;
 }
@@ -1626,7 +1560,7 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
     // zeroed once: the passes leave the ghost faces of the antidiffusive
     // velocities unwritten, and the adjoint kernels read them (with zero
     // weight), so they must hold finite values
-    const size_t tb = tsteps * 2 * TTAPE * ctx->L.n * ctx->esz;
+    const size_t tb = tsteps * 2 * ttape_block(ctx);
     if (hipMalloc(&A->ttape, tb) == hipSuccess) {
       A->ttape_steps = tsteps;
       HIPCHK(hipMemsetAsync(A->ttape, 0, tb, ctx->st));
@@ -1861,7 +1795,7 @@ static int window_binomial_t(hc_ctx *ctx, int nsteps, int nsnap, const double *o
   if (ns) HIPCHK(hipMemcpyAsync(A->d_obs, obs, ns * 8, hipMemcpyHostToDevice, ctx->st));
   if (A->ttape_steps < 1) {  // one step of the tracers' intermediates, for the turns
     if (A->ttape) { hipStreamSynchronize(ctx->st); hipFree(A->ttape); A->ttape = nullptr; A->ttape_steps = 0; }
-    const size_t tb = 2 * TTAPE * ctx->L.n * ctx->esz;
+    const size_t tb = 2 * ttape_block(ctx);
     if (hipMalloc(&A->ttape, tb) == hipSuccess) {
       A->ttape_steps = 1;
       HIPCHK(hipMemsetAsync(A->ttape, 0, tb, ctx->st));  // finite ghost faces, as in the fixed-interval window
@@ -1960,22 +1894,23 @@ static int window_binomial_t(hc_ctx *ctx, int nsteps, int nsnap, const double *o
 
 // HBM the window's checkpoints and tapes take (beyond the context's own
 // state and the adjoint fields): (nck + 1 + seg) states of NTAPE fields and
-// (seg - 1) steps of TTAPE fields for each of the two tracers
+// (seg - 1) steps of TTAPE fields and two flag planes for each of the two
+// tracers
 extern "C" int hc_adj_window_bytes(hc_ctx *ctx, int32_t nsteps, int32_t seg, uint64_t *bytes) {
   if (!ctx || !bytes || nsteps < 1 || seg < 1) return fail(ctx, HC_EINVAL, "nsteps and seg must be positive");
   const uint64_t fb = (uint64_t)ctx->L.n * ctx->esz;
   const uint64_t nck = (uint64_t)((nsteps + seg - 1) / seg);
-  *bytes = (nck + 1 + (uint64_t)seg) * NTAPE * fb + (uint64_t)(seg - 1) * 2 * TTAPE * fb;
+  *bytes = (nck + 1 + (uint64_t)seg) * NTAPE * fb + (uint64_t)(seg - 1) * 2 * ttape_block(ctx);
   return HC_OK;
 }
 
 // HBM the binomial window takes: nsnap + 2 states of NTAPE fields (the
 // snapshots, the turns' temporary, the end state) and one step of TTAPE
-// fields for each of the two tracers
+// fields and two flag planes for each of the two tracers
 extern "C" int hc_adj_window_binomial_bytes(hc_ctx *ctx, int32_t nsteps, int32_t nsnap, uint64_t *bytes) {
   if (!ctx || !bytes || nsteps < 1 || nsnap < 1) return fail(ctx, HC_EINVAL, "nsteps and nsnap must be positive");
   const uint64_t fb = (uint64_t)ctx->L.n * ctx->esz;
-  *bytes = ((uint64_t)nsnap + 2) * NTAPE * fb + 2 * TTAPE * fb;
+  *bytes = ((uint64_t)nsnap + 2) * NTAPE * fb + 2 * ttape_block(ctx);
   return HC_OK;
 }
 

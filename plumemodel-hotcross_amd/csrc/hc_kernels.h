@@ -79,6 +79,7 @@ struct TracerWork {
   void *c1, *c2;           // pass results (c1: pass 1, then the pre-sponge boundary values)
   void *cpre;              // if set, receives the pre-sponge boundary values instead of c1
   void *ud3, *vd3, *wd3;   // if set, receive pass 3's antidiffusive velocities
+  uint8_t *fl2, *fl3;      // if set, receive the limiter flags of passes 2 and 3 (interior cells)
   double *budget_part;     // per-block partial sums (FP64)
   double *budget;          // 3 sums
 };

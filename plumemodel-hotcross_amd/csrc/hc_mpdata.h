@@ -81,6 +81,7 @@ struct MpPass {
   T *ud, *vd, *wd;          // antidiffusive velocities written for the next pass (or null)
   T *cout;                  // result; on the last pass also read as the entry value
   T *cpre;                  // last pass: pre-sponge value for the boundary images
+  unsigned char *fl;        // limiter flags of the interior cells (the padded index), or null
   double cmin, cmax;
   int last;
   // last pass only

@@ -222,6 +222,7 @@ struct MpTile2 {
     unsigned char *fl = sh.fl[q & 1];
     const int f = flag_at(sh, a, pl.po, q);  // formed everywhere, kept where the cell exists
     fl[HC_MP_CHECKC(pl.po)] = (unsigned char)((qin && pl.go >= 0) ? f : 0);
+    if (a.fl && q >= g.ka && q <= g.kb && pl.go >= 0) (a.fl + plane(g, q))[HC_MP_GIDX(pl.go)] = (unsigned char)f;
     if (pl.lp >= 0) {
       const int p = pl.lp & (INTERIOR - 1);
       fl[HC_MP_CHECKC(p)] = (unsigned char)((qin && (pl.lp & INTERIOR)) ? flag_at(sh, a, p, q) : 0);

@@ -337,6 +337,9 @@ struct MpTile3 {
       const int f = x <= (T)a.cmin ? 1 : (x >= (T)a.cmax ? 2 : 0);  // formed everywhere, kept where the cell exists
       r.fC = (qin && pl.go >= 0) ? f : 0;
       fl[HC_MP_CHECKC(p)] = (unsigned char)r.fC;
+      // the flags of the planes this block finishes, for a caller that keeps
+      // them (the adjoint's tape)
+      if (a.fl && q >= g.ka && q <= g.kb && pl.go >= 0) (a.fl + T2::plane(g, q))[HC_MP_GIDX(pl.go)] = (unsigned char)f;
     }
     if (pl.lp >= 0) {
       const int p = pl.lp & (INTERIOR - 1);

@@ -429,7 +429,10 @@ static void launch_mpdata_tile3(hipStream_t st, const Layout &L, const MpPass<T>
 
 template <typename T>
 static void launch_mpdata_pass(hipStream_t st, const Layout &L, const MpPass<T> &a) {
-  switch (mp_tile()) {
+  // the first form (MpTile) does not write the flags: a pass that keeps them
+  // runs MpTile2
+  const int t = (a.fl && mp_tile() < 3) ? 3 : mp_tile();
+  switch (t) {
     case 5: launch_mpdata_tile3<T, 32, 16>(st, L, a); break;
     case 6: launch_mpdata_tile3<T, 32, 12>(st, L, a); break;
     case 7: launch_mpdata_tile3<T, 16, 16>(st, L, a); break;
@@ -481,6 +484,7 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   a.cc = c1; a.U = (T *)tw.uc; a.V = (T *)tw.vc; a.W = (T *)tw.wc;
   a.ud = (T *)tw.ud2; a.vd = (T *)tw.vd2; a.wd = (T *)tw.wd2;
   a.cout = c2; a.cpre = nullptr; a.last = 0;
+  a.fl = tw.fl2;
   if (mark) mark(mark_arg, 0);
   launch_mpdata_pass<T>(st, L, a);
   if (mark) mark(mark_arg, 1);
@@ -491,6 +495,7 @@ void launch_tracer(hipStream_t st, const Layout &L, const StepArgs &s, const Phy
   a.ud = (T *)tw.ud3; a.vd = (T *)tw.vd3; a.wd = (T *)tw.wd3;
   T *cpre = tw.cpre ? (T *)tw.cpre : c1;
   a.cout = c; a.cpre = cpre; a.last = 1;
+  a.fl = tw.fl3;
   if (mark) mark(mark_arg, 0);
   launch_mpdata_pass<T>(st, L, a);
   if (mark) mark(mark_arg, 1);
