@@ -217,20 +217,25 @@ __global__ void __launch_bounds__(256) k_adj_forces(Layout L, Box3 b, StepArgs s
         RH{f.rhop, L};
     const AAcc<T> au{lam.u, L}, av{lam.v, L}, aw{lam.w, L}, aum{lam.um, L}, avm{lam.vm, L}, awm{lam.wm, L},
         ass{lam.ss, L}, assv{lam.ssv, L}, arh{lam.rhop, L};
+    // each cell's output adjoints are consumed here (read, then cleared:
+    // FORCES overwrites fx, fy, fz)
     if (j >= 1 && k >= 1) {
       const T l = F3(lam.fx, i, j, k);
+      F3(lam.fx, i, j, k) = (T)0;
       if (l != (T)0)
         adj_forces_x<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphau, l, au, av, aw, aum, avm,
                         awm, ass, assv, gf);
     }
     if (i >= 1 && k >= 1) {
       const T l = F3(lam.fy, i, j, k);
+      F3(lam.fy, i, j, k) = (T)0;
       if (l != (T)0)
         adj_forces_y<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphav, l, au, av, aw, aum, avm,
                         awm, ass, assv, gf);
     }
     if (i >= 1 && j >= 1) {
       const T l = F3(lam.fz, i, j, k);
+      F3(lam.fz, i, j, k) = (T)0;
       if (l != (T)0)
         adj_forces_z<T>(L, i, j, k, s, ph, U, V, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, l, au, av, aw,
                         aum, avm, awm, ass, assv, arh);
@@ -264,6 +269,9 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs
   double gf = 0.0;
   adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
     const T lx = F3(lam.fx, i, j, k), ly = F3(lam.fy, i, j, k), lz = F3(lam.fz, i, j, k);
+    F3(lam.fx, i, j, k) = (T)0;  // consumed (FORCES overwrites them)
+    F3(lam.fy, i, j, k) = (T)0;
+    F3(lam.fz, i, j, k) = (T)0;
     if (lx != (T)0)
       adj_forces_x<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphau, lx, au, av, aw, aum, avm, awm,
                       ass, assv, gf);
@@ -326,32 +334,6 @@ __global__ void __launch_bounds__(256) k_adj_mom_p(Layout L, Box3 b, Box3 fb, in
 
 // ---------------------------------------------------------------------------
 // VISCOSITY3D
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_visc(Layout L, Box3 b, StepArgs s, Phys ph, const T *__restrict__ um,
-                                                  const T *__restrict__ vm, const T *__restrict__ wm,
-                                                  const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
-                                                  const T *__restrict__ ss1, const T *__restrict__ ssv1,
-                                                  const T *__restrict__ lss, const T *__restrict__ lssv,
-                                                  T *__restrict__ lum, T *__restrict__ lvm, T *__restrict__ lwm,
-                                                  T *__restrict__ lrhop, double *gcs) {
-  ADJ_IJK(b);
-  double sh = 0.0, sv = 0.0;
-  if (in) {
-    const T a = visc_ss_written(L, i, j, k) ? F3(lss, i, j, k) : (T)0;
-    const T c = visc_ssv_written(L, i, j, k) ? F3(lssv, i, j, k) : (T)0;
-    if (a != (T)0 || c != (T)0) {
-      const RView<T> UM{um, L}, VM{vm, L}, WM{wm, L}, RH{rhop, L};
-      const AAcc<T> aum{lum, L}, avm{lvm, L}, awm{lwm, L}, arh{lrhop, L};
-      adj_visc_at<T>(L, s, ph, UM, VM, WM, RH, rho_bkg, a, c, i, j, k, aum, avm, awm, arh);
-      sh = (double)a * (double)F3(ss1, i, j, k);
-      sv = (double)c * (double)F3(ssv1, i, j, k);
-    }
-  }
-  block_add(sh, gcs);
-  block_add(sv, gcs + ADJ_SLOTS);
-}
-
-
 // VISCOSITY3D: interior cells as a tile, the ghost shell cell by cell
 template <typename T>
 __global__ void __launch_bounds__(ATI *ATJ) k_adj_visc_tile(Layout L, StepArgs s, Phys ph, const T *__restrict__ um,
@@ -359,7 +341,7 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_visc_tile(Layout L, StepArgs s
                                                             const T *__restrict__ rhop,
                                                             const double *__restrict__ rho_bkg,
                                                             const T *__restrict__ ss1, const T *__restrict__ ssv1,
-                                                            const T *__restrict__ lss, const T *__restrict__ lssv,
+                                                            T *__restrict__ lss, T *__restrict__ lssv,
                                                             T *lum, T *lvm, T *lwm, T *lrhop, double *gcs, int kc) {
   typedef AdjRing<T, ATI, ATJ, 4> R;
   __shared__ R sh;
@@ -373,6 +355,8 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_visc_tile(Layout L, StepArgs s
   double sh_ = 0.0, sv_ = 0.0;
   adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
     const T a = F3(lss, i, j, k), c = F3(lssv, i, j, k);
+    F3(lss, i, j, k) = (T)0;  // consumed: VISCOSITY3D overwrites both
+    F3(lssv, i, j, k) = (T)0;
     adj_ss_val<T>(L, ph, s, UM, VM, WM, i, j, k, a, aum, avm, awm);
     adj_ssv_val<T>(L, ph, s, UM, VM, WM, RH, rho_bkg, i, j, k, c, aum, avm, awm, arh);
     sh_ += (double)a * (double)F3(ss1, i, j, k);
@@ -387,7 +371,7 @@ __global__ void __launch_bounds__(256) k_adj_visc_shell(Layout L, StepArgs s, Ph
                                                         const T *__restrict__ vm, const T *__restrict__ wm,
                                                         const T *__restrict__ rhop, const double *__restrict__ rho_bkg,
                                                         const T *__restrict__ ss1, const T *__restrict__ ssv1,
-                                                        const T *__restrict__ lss, const T *__restrict__ lssv,
+                                                        T *__restrict__ lss, T *__restrict__ lssv,
                                                         T *__restrict__ lum, T *__restrict__ lvm, T *__restrict__ lwm,
                                                         T *__restrict__ lrhop, double *gcs) {
   const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -409,8 +393,13 @@ __global__ void __launch_bounds__(256) k_adj_visc_shell(Layout L, StepArgs s, Ph
   }
   double sh = 0.0, sv = 0.0;
   if (in) {
+    const bool wc = visc_ssv_written(L, i, j, k);
     const T a = visc_ss_written(L, i, j, k) ? F3(lss, i, j, k) : (T)0;
-    const T c = visc_ssv_written(L, i, j, k) ? F3(lssv, i, j, k) : (T)0;
+    const T c = wc ? F3(lssv, i, j, k) : (T)0;
+    // consumed: VISCOSITY3D overwrites ss on the whole shell (constants on
+    // the edges and the k ghosts) and ss_v on the interior columns
+    F3(lss, i, j, k) = (T)0;
+    if (wc) F3(lssv, i, j, k) = (T)0;
     if (a != (T)0 || c != (T)0) {
       const RView<T> UM{um, L}, VM{vm, L}, WM{wm, L}, RH{rhop, L};
       const AAcc<T> aum{lum, L}, avm{lvm, L}, awm{lwm, L}, arh{lrhop, L};
@@ -421,14 +410,6 @@ __global__ void __launch_bounds__(256) k_adj_visc_shell(Layout L, StepArgs s, Ph
   }
   block_add(sh, gcs);
   block_add(sv, gcs + ADJ_SLOTS);
-}
-
-// zero the adjoint of a field over a box of cells (those the forward
-// overwrote); the adjoint of the cells it left alone passes through
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_zero_box(Layout L, Box3 b, T *__restrict__ p) {
-  ADJ_IJK(b);
-  if (in) F3(p, i, j, k) = (T)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -555,7 +536,8 @@ struct DonorLast {
 
 // the antidiffusive velocities reversed on the domain's first faces, in one
 // launch: ud at i = 0 (j, k >= 1), vd at j = 0 (i, k >= 1), wd at k = 0 (i,
-// j >= 1), one thread per face
+// j >= 1), one thread per face (inside the tile kernel, these faces raised
+// its registers and took longer than this launch)
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_anti_faces(Layout L, const T *__restrict__ cc, const T *__restrict__ U,
                                                         const T *__restrict__ V, const T *__restrict__ W,
@@ -916,7 +898,9 @@ static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
   hipLaunchKernelGGL(k_adj_forces_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
                      ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc, kc);
-  zero(ctx, lam.fx); zero(ctx, lam.fy); zero(ctx, lam.fz);
+  // lam.fx, fy, fz are nonzero only where FORCES writes (MOMENTUM's and
+  // PRESSURE's adjoints add to them there), and the kernels above cleared
+  // every such cell after reading it
 }
 
 template <typename T>
@@ -956,19 +940,18 @@ static void adj_viscosity_t(hc_ctx *ctx, const T *ss1, const T *ssv1) {
   const long nshell = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
   hipLaunchKernelGGL(k_adj_visc_shell<T>, dim3((unsigned)((nshell + 255) / 256)), dim3(256), 0, ctx->st, L, s, ctx->P,
                      (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
-                     (const double *)ctx->aux.rho_bkg, ss1, ssv1, (const T *)lam.ss, (const T *)lam.ssv, lam.um,
+                     (const double *)ctx->aux.rho_bkg, ss1, ssv1, lam.ss, lam.ssv, lam.um,
                      lam.vm, lam.wm, lam.rhop, A->d_sc + ADJ_SLOTS);
   const int kc = adj_chunk(L);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
   hipLaunchKernelGGL(k_adj_visc_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      (const T *)f.um, (const T *)f.vm, (const T *)f.wm, (const T *)f.rhop,
-                     (const double *)ctx->aux.rho_bkg, ss1, ssv1, (const T *)lam.ss, (const T *)lam.ssv, lam.um,
+                     (const double *)ctx->aux.rho_bkg, ss1, ssv1, lam.ss, lam.ssv, lam.um,
                      lam.vm, lam.wm, lam.rhop, A->d_sc + ADJ_SLOTS, kc);
   // VISCOSITY3D writes every cell of ss (constants on the edges and the k
-  // ghosts) but leaves ss_v on the i and j ghost columns as it was
-  zero(ctx, lam.ss);
-  const Box3 vb{1, L.nx, 1, L.ny, 0, L.nz + 1};
-  hipLaunchKernelGGL(k_adj_zero_box<T>, grid3(vb), dim3(64, 4), 0, ctx->st, L, vb, lam.ssv);
+  // ghosts) but leaves ss_v on the i and j ghost columns as it was: the two
+  // kernels clear lam.ss everywhere and lam.ssv on the interior columns
+  // after reading them
 }
 
 template <typename T>
