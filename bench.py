@@ -161,6 +161,19 @@ def adjoint_leg(workload="les128", nsteps=400, seg=20, precision="fp64", budget=
     return out
 
 
+def step_only_leg(workload, nsteps, precision):
+    """The adjoint of the timestep alone: a window of nsteps steps whose one
+    segment the forward sweep tapes as it runs, so that the reverse sweep
+    recomputes nothing and runs only the adjoint of each step.  (window -
+    forward) / forward is then the adjoint of one step (plus the taping in
+    the forward sweep) per forward step."""
+    r = adjoint_leg(workload, nsteps, nsteps, precision)
+    return {"workload": "%s %s, %d steps, one segment taped by the forward sweep (no recompute)"
+                        % (workload, precision, nsteps),
+            "forward_ms_per_step": r["forward_ms_per_step"], "window_ms_per_step": r["window_ms_per_step"],
+            "adjoint_step_over_forward": r["ratio"], "finite": r["finite"]}
+
+
 def fp32_error_leg(workload, nsteps=50):
     """The FP32 form against the FP64 form after nsteps steps of the native
     loop from the same initial state: per field, the largest difference
@@ -377,9 +390,11 @@ def main():
     adj = adj256 = None
     if not args.no_adjoint:
         adj = adjoint_leg(args.adjoint_workload, args.adjoint_steps, args.adjoint_seg, args.adjoint_precision)
+        adj["step_only"] = step_only_leg(args.adjoint_workload, 20, args.adjoint_precision)
         # the headline size's forward + adjoint configuration: 256^3 in the
         # FP32 form, 1800 steps, checkpoints under a 200 GB HBM budget
         adj256 = adjoint_leg("les256", args.c5_steps, None, "fp32", budget=args.c5_budget_gb * 1e9)
+        adj256["step_only"] = step_only_leg("les256", 10, "fp32")
     # the CPU baseline last: run right after it, the launch-bound 128^3
     # adjoint leg once measured 1.5x slower
     cpu = None
