@@ -113,12 +113,15 @@ inline dim3 box_block() { return dim3(64, 4, 1); }
   const int k = (b).k0 + (int)bz_;                                    \
   if (j > (b).j1) return;
 
-// Planes per block of a k-marching tile kernel: kmax, halved (not below 4)
+// Planes per block of a k-marching tile kernel: kmax, halved (not below 2)
 // until the launch has at least min_blocks blocks, so that a small grid
-// still fills the chip's 256 CUs.
+// still fills the chip's 256 CUs (at 64^3 the corrective pass has 8 tiles a
+// plane: 2 planes per block gave it 256 blocks and took a launch from 30 to
+// 23 us, the step from 0.35 to 0.31 ms, though each block stages two
+// planes beyond its own).
 inline int tile_chunk(int nz, long tiles, int kmax, long min_blocks) {
   int kc = kmax;
-  while (kc > 4 && tiles * ((nz + kc - 1) / kc) < min_blocks) kc /= 2;
+  while (kc > 2 && tiles * ((nz + kc - 1) / kc) < min_blocks) kc /= 2;
   return kc < nz ? kc : nz;
 }
 

@@ -5,7 +5,7 @@ face's donor-cell flux once, hc_mpdata3.h), built for the host
 oracle's TRACER31 (mpdata-hotcross80.f:196-381), bit for bit in FP64.
 
 The grid (72 x 36 x 40) is not a multiple of any tile (32 or 16 wide, 16 or
-8 rows), and the planes split into chunks of 4, 8 and 40, so every x, y and
+8 rows), and the planes split into chunks of 2, 4, 8 and 40, so every x, y and
 z seam of the tiles, the partial tiles at the far x and y ends and the
 domain's first faces (ud at i = 0, vd at j = 0, wd at k = 0) meet the
 oracle.  LDS starts each block filled with NaN and every LDS index is
@@ -111,7 +111,7 @@ def host_pass(L, variant, tile, kc, c, vel, cmin, cmax, last=0, entry=None, tau3
 
 
 VARIANTS = [(1, 0, 4), (1, 0, 8), (1, 2, 40), (1, 1, 8), (2, 0, 4), (2, 0, 8), (2, 0, 40), (2, 1, 4),
-            (3, 0, 4), (3, 0, 8), (3, 0, 40), (3, 1, 4)]
+            (3, 0, 2), (3, 0, 4), (3, 0, 8), (3, 0, 40), (3, 1, 4)]
 IDS = ["%s_%s_kc%d" % (("mptile", "mptile2", "mptile3")[v - 1], ("32x16", "16x16", "32x8")[t], k)
        for v, t, k in VARIANTS]
 
