@@ -28,7 +28,17 @@ adjoint 1.03600501e5 against 1.0475e5 (h = 1e-4), 1.0336e5 (1e-5),
 carry a cell across the Richardson cutoff (the square root of
 smagorinsky_split_mixing-revised2014.f:112, whose slope is unbounded there)
 are off by up to 12%, the smallest agree to 3e-6 and 1.5e-6.  For the
-heat-flux cell: 4.4e-5 at h = 1e-4, 2.6e-5 at 1e-6."""
+heat-flux cell: 4.4e-5 at h = 1e-4, 2.6e-5 at 1e-6.
+
+C5, the FP32 forward + adjoint at 256^3 (dx = 0.5 m, dt = 0.2 s, an 8 x 9
+source map seeded +-20%): the checkpointing plan_window picks for the
+configuration's 1800 steps under 200 GB (a fixed interval of 30, which fits)
+and under 60 GB (binomial snapshots), and a 30-step window at the full grid
+under binomial checkpointing (3 snapshot slots) against the fixed-interval
+window (a checkpoint every 10): the same forward, so the loss and the probe
+samples agree bit for bit; the gradients agree to the rounding of the
+adjoint's atomic sums in FP32 (measured 3e-5 between two evaluations of the
+same window at this size, `tests/test_gpu_adjoint_determinism.py`)."""
 import numpy as np
 import pytest
 
@@ -129,5 +139,42 @@ def test_c4_adjoint_gradients_match_finite_differences():
             fd = (lp - lm) / (2 * h)
             print("C4 %s: adjoint %.10e fd %.10e rel %.2e" % (name, ad, fd, abs(ad - fd) / abs(fd)), flush=True)
             assert abs(ad - fd) <= 1e-4 * abs(fd), (name, ad, fd)
+    finally:
+        ctx.close()
+
+
+def test_c5_checkpointing_plan_and_binomial_window_at_full_size():
+    import adjoint as AD
+    p = H.workload_params("les256")
+    p.toggle_c = 0
+    z, t, s = H.linear_stratification(p)
+    ctx = H.Context(p, H.FP32)
+    try:
+        kind, seg, need = AD.plan_window(ctx, 1800, 200e9)
+        print("C5 plan under 200 GB:", kind, seg, need / 1e9, "GB", flush=True)
+        assert kind == "fixed" and need <= 200e9
+        kind2, nsnap, need2 = AD.plan_window(ctx, 1800, 60e9)
+        print("C5 plan under 60 GB:", kind2, nsnap, need2 / 1e9, "GB", flush=True)
+        assert kind2 == "binomial" and need2 <= 60e9
+        ctx.init_case(z, t, s, t_noise=1.0e-4, seed=5)
+        ctx.step(4)
+        npatch = ctx.npatch()
+        rng = np.random.default_rng(5)
+        qmap = 1.0 + 0.2 * (2.0 * rng.random(npatch) - 1.0)
+        qmap /= qmap.mean()
+        ctx.set_controls(qmap, 1.0, 0.2)
+        probes = AD.plane_probes(p)
+        every = AD.samples_every(p)
+        ctx.set_probes(probes, every, 1.0e4, 1.0)
+        nsteps = 30
+        obs = np.full((nsteps // every, len(probes), 2), 0.01)
+        lf, sf, gqf, gsf = ctx.adj_window(nsteps, 10, obs, end_state=False)
+        lb, sb, gqb, gsb = ctx.adj_window_binomial(nsteps, 3, obs, end_state=False)
+        rel = float(np.max(np.abs(gqb - gqf)) / np.max(np.abs(gqf)))
+        print("C5 window: loss", lf, lb, "map rel", rel, "cs", gsf[1] + gsf[2], gsb[1] + gsb[2], flush=True)
+        assert np.isfinite(lf) and lf == lb and np.array_equal(sf, sb)
+        assert np.all(np.isfinite(gqf)) and np.all(np.isfinite(gqb))
+        assert rel <= 5e-4
+        assert abs((gsb[1] + gsb[2]) - (gsf[1] + gsf[2])) <= 5e-4 * abs(gsf[1] + gsf[2])
     finally:
         ctx.close()
