@@ -134,6 +134,10 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
     *out = ctx;
     return fail(ctx, HC_EINVAL, "grid too large: the padded box must hold fewer than 2^31 cells");
   }
+  {
+    const char *e = getenv("HC_SERIAL_TRACERS");
+    ctx->par_tracers = !(e && e[0] == '1');
+  }
   *out = ctx;
   HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
   int rc = 0;
@@ -254,6 +258,18 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
     if (p) hipFree(p);
   for (void *p : ctx->c1x)
     if (p) hipFree(p);
+  for (int q = 0; q < 2; ++q) {
+    if (ctx->tst[q]) {
+      hipStreamSynchronize(ctx->tst[q]);
+      hipStreamDestroy(ctx->tst[q]);
+    }
+    void *pq[] = {ctx->tw_par[q].ud2, ctx->tw_par[q].vd2, ctx->tw_par[q].wd2, ctx->tw_par[q].c2,
+                  ctx->tw_par[q].budget_part, ctx->tw_par[q].budget};
+    for (void *p : pq)
+      if (p) hipFree(p);
+  }
+  for (hipEvent_t e : ctx->tev)
+    if (e) hipEventDestroy(e);
   for (void *p : ctx->d_diag)
     if (p) hipFree(p);
   if (ctx->pw.pivt && ctx->pw.pivt != (void *)ctx->pw.piv) hipFree(ctx->pw.pivt);
@@ -699,22 +715,49 @@ static void mark_pass(void *arg, int end) {
 }
 
 // c1: if set, holds the tracer's pass-1 result already (launch_pass1_multi)
-static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3, int vent,
-                  double vent_value, int want_budget, void *c1 = nullptr) {
+// TRACER31 of `field` on stream st with the corrective-pass scratch of
+// `work` (the context's own when null)
+static int tracer_on(hc_ctx *ctx, hipStream_t st, const TracerWork *work, int field, const double *d_cbkg,
+                     double cmin, double cmax, double tau3, int vent, double vent_value, int want_budget,
+                     void *c1) {
   PhaseScope ps(ctx, "tracer31");
   int which = field == HC_T ? 0 : field == HC_S ? 1 : 2;
   StepArgs s = step_args(ctx);
   VentArgs va = vent_args(ctx);
   TracerArgs ta{cmin, cmax, tau3, vent, vent_value, want_budget, c1 != nullptr};
-  TracerWork tw = ctx->tw;
+  TracerWork tw = work ? *work : ctx->tw;
   if (c1) tw.c1 = c1;
   MarkFn mk = ctx->timing ? mark_pass : nullptr;
   if (ctx->prec == HC_FP64)
-    launch_tracer<double>(ctx->st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, tw,
+    launch_tracer<double>(st, ctx->L, s, ctx->P, va, ptrs<double>(ctx), ctx->aux, tw,
                           (double *)ctx->f[field], d_cbkg, ctx->d_q[which], ta, mk, ctx);
   else
-    launch_tracer<float>(ctx->st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, tw,
+    launch_tracer<float>(st, ctx->L, s, ctx->P, va, ptrs<float>(ctx), ctx->aux, tw,
                          (float *)ctx->f[field], d_cbkg, ctx->d_q[which], ta, mk, ctx);
+  return HC_OK;
+}
+
+static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3, int vent,
+                  double vent_value, int want_budget, void *c1 = nullptr) {
+  return tracer_on(ctx, ctx->st, nullptr, field, d_cbkg, cmin, cmax, tau3, vent, vent_value, want_budget, c1);
+}
+
+// the streams, events and scratch of the concurrent S and C tracers
+static int par_tracers_init(hc_ctx *ctx) {
+  if (ctx->tst[0]) return HC_OK;
+  int rc = 0;
+  const size_t fb = ctx->L.n * ctx->esz;
+  for (int q = 0; q < 2; ++q) {
+    TracerWork &w = ctx->tw_par[q];
+    w.uc = ctx->tw.uc; w.vc = ctx->tw.vc; w.wc = ctx->tw.wc;  // read only
+    void **ps[] = {&w.ud2, &w.vd2, &w.wd2, &w.c2};
+    for (auto p : ps)
+      if (!(*p = dalloc(ctx, fb, &rc))) return rc;
+    if (!(w.budget_part = (double *)dalloc(ctx, (size_t)3 * ctx->P.ny * ctx->P.nz * 8, &rc))) return rc;
+    if (!(w.budget = (double *)dalloc(ctx, 64, &rc))) return rc;
+  }
+  for (int q = 0; q < 2; ++q) HIPCHK(hipStreamCreateWithFlags(&ctx->tst[q], hipStreamNonBlocking));
+  for (hipEvent_t &e : ctx->tev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return HC_OK;
 }
 
@@ -933,6 +976,30 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
       int r = tracers_pass1(ctx, c1);
       if (r) return r;
     }
+    if (fuse && ctx->par_tracers && !ctx->timing && (P.toggle_s == 1 || P.toggle_c == 1)) {
+      // S and C on streams of their own, T on the context stream; DENSITY
+      // waits for S, the step's end for C.  Pass 1 of all three ran above,
+      // and the passes read nothing another tracer writes.
+      int r = par_tracers_init(ctx);
+      if (r) return r;
+      HIPCHK(hipEventRecord(ctx->tev[0], ctx->st));
+      if (P.toggle_s == 1) {
+        HIPCHK(hipStreamWaitEvent(ctx->tst[0], ctx->tev[0], 0));
+        tracer_on(ctx, ctx->tst[0], &ctx->tw_par[0], HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1,
+                  P.ssource + ctx->h_sbkg[nz], budgets, c1[1]);
+        HIPCHK(hipEventRecord(ctx->tev[1], ctx->tst[0]));
+      }
+      if (P.toggle_c == 1) {
+        HIPCHK(hipStreamWaitEvent(ctx->tst[1], ctx->tev[0], 0));
+        tracer_on(ctx, ctx->tst[1], &ctx->tw_par[1], HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, budgets, c1[2]);
+        HIPCHK(hipEventRecord(ctx->tev[2], ctx->tst[1]));
+      }
+      if (P.toggle_t == 1)
+        tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], budgets, c1[0]);
+      if (P.toggle_s == 1) HIPCHK(hipStreamWaitEvent(ctx->st, ctx->tev[1], 0));
+      if (P.bit3 == 1) hc_density(ctx);
+      if (P.toggle_c == 1) HIPCHK(hipStreamWaitEvent(ctx->st, ctx->tev[2], 0));
+    } else {
     if (P.toggle_t == 1) {
       if (!fuse) hc_work_velocity(ctx, P.dt);
       tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], budgets, c1[0]);
@@ -941,6 +1008,7 @@ extern "C" int hc_step(hc_ctx *ctx, int32_t nsteps) {
       tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], budgets, c1[1]);
     if (P.bit3 == 1) hc_density(ctx);
     if (P.toggle_c == 1) tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, budgets, c1[2]);
+    }
     if (fuse) {
       PhaseScope ps(ctx, "refresh");
       refresh_rotate(ctx);

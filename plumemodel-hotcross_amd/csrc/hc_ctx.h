@@ -77,6 +77,14 @@ struct hc_ctx {
   void *d_diag[5] = {};
   void *spare[3] = {nullptr, nullptr, nullptr};
   void *c1x[2] = {nullptr, nullptr};  // pass-1 results of S and C in the native loop
+  // the native loop's S and C tracers run on streams of their own beside T
+  // (each with its own corrective-pass scratch), so that the blocks of one
+  // tracer's passes fill the tail of the other's; HC_SERIAL_TRACERS=1 at
+  // creation keeps them on the context stream.  Created on first use.
+  bool par_tracers = true;
+  hipStream_t tst[2] = {nullptr, nullptr};
+  hipEvent_t tev[3] = {nullptr, nullptr, nullptr};
+  TracerWork tw_par[2] = {};
   // controls that the adjoint differentiates: a per-cell multiplier of the
   // vent heat flux (null: 1 everywhere), the crossflow amplitude (a factor on
   // the series of site_forcing.inc) and the Smagorinsky constants in P
