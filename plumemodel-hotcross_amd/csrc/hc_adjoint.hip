@@ -783,15 +783,24 @@ struct HostState {
   bool up_alias[3];
 };
 
+// scratch of one TRACER31 adjoint: the recompute's intermediates (when no
+// tape holds them) and the adjoints of the passes' intermediates
+struct AdjTracerScratch {
+  void *lcnew = nullptr;
+  void *cw = nullptr, *c1 = nullptr, *c2 = nullptr, *cpre = nullptr, *ud2[3] = {}, *ud3[3] = {};
+  void *lc1 = nullptr, *lc2 = nullptr, *lpre = nullptr, *lud2[3] = {}, *lud3[3] = {};
+  uint8_t *fl2 = nullptr, *fl3 = nullptr;
+};
+
 struct AdjCtx {
   void *lam[HC_NFIELDS] = {};
   void *luc[3] = {};
   void *lamb = nullptr;  // MOMENTUM's Lambda
-  void *lcnew = nullptr;
-  // tracer recompute and its adjoints
-  void *cw = nullptr, *c1 = nullptr, *c2 = nullptr, *cpre = nullptr, *ud2[3] = {}, *ud3[3] = {};
-  void *lc1 = nullptr, *lc2 = nullptr, *lpre = nullptr, *lud2[3] = {}, *lud3[3] = {};
-  uint8_t *fl2 = nullptr, *fl3 = nullptr;
+  // the T adjoint's scratch (and that of the per-subroutine entry), and the
+  // S adjoint's when a window's step runs S on a stream of its own beside T
+  AdjTracerScratch sc[2];
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
   void *g = nullptr;          // compact pressure work
   // ADJ_SLOTS partial sums each of d/d flow_scale, sum lam_ss*ss and
   // sum lam_ssv*ssv, then a device 0.0
@@ -818,16 +827,50 @@ void adj_free(hc_ctx *ctx) {
   if (!A) return;
   hipStreamSynchronize(ctx->st);
   std::vector<void *> ps;
+  if (A->st2) {
+    hipStreamSynchronize(A->st2);
+    hipStreamDestroy(A->st2);
+  }
+  for (hipEvent_t e : A->ev)
+    if (e) hipEventDestroy(e);
   for (void *p : A->lam) ps.push_back(p);
   for (void *p : A->luc) ps.push_back(p);
-  for (int q = 0; q < 3; ++q) { ps.push_back(A->ud2[q]); ps.push_back(A->ud3[q]); ps.push_back(A->lud2[q]); ps.push_back(A->lud3[q]); }
-  void *rest[] = {A->lamb, A->lcnew, A->cw, A->c1, A->c2, A->cpre, A->lc1, A->lc2, A->lpre, A->fl2, A->fl3, A->g,
-                  A->d_sc, A->d_part, A->d_gq, A->tape, A->ttape, A->d_iw, A->d_it, A->d_samp, A->d_obs};
+  for (AdjTracerScratch &X : A->sc) {
+    for (int q = 0; q < 3; ++q) { ps.push_back(X.ud2[q]); ps.push_back(X.ud3[q]); ps.push_back(X.lud2[q]); ps.push_back(X.lud3[q]); }
+    void *xs[] = {X.lcnew, X.cw, X.c1, X.c2, X.cpre, X.lc1, X.lc2, X.lpre, X.fl2, X.fl3};
+    for (void *p : xs) ps.push_back(p);
+  }
+  void *rest[] = {A->lamb, A->g, A->d_sc, A->d_part, A->d_gq, A->tape, A->ttape, A->d_iw, A->d_it, A->d_samp,
+                  A->d_obs};
   for (void *p : rest) ps.push_back(p);
   for (void *p : ps)
     if (p) hipFree(p);
   delete A;
   ctx->adj = nullptr;
+}
+
+static int scratch_alloc(hc_ctx *ctx, AdjTracerScratch &X) {
+  const size_t fb = ctx->L.n * ctx->esz;
+  int rc = 0;
+  void **fields[] = {&X.lcnew, &X.cw, &X.c1, &X.c2, &X.cpre, &X.ud2[0], &X.ud2[1], &X.ud2[2], &X.ud3[0], &X.ud3[1],
+                     &X.ud3[2], &X.lc1, &X.lc2, &X.lpre, &X.lud2[0], &X.lud2[1], &X.lud2[2], &X.lud3[0], &X.lud3[1],
+                     &X.lud3[2]};
+  for (auto p : fields)
+    if (!(*p = dalloc(ctx, fb, &rc))) return rc;
+  if (!(X.fl2 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
+  if (!(X.fl3 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
+  return HC_OK;
+}
+
+// the S adjoint's stream and scratch, for running it beside T's
+static int adj_par_init(hc_ctx *ctx) {
+  AdjCtx *A = ctx->adj;
+  if (A->st2) return HC_OK;
+  int r = scratch_alloc(ctx, A->sc[1]);
+  if (r) return r;
+  HIPCHK(hipStreamCreateWithFlags(&A->st2, hipStreamNonBlocking));
+  for (hipEvent_t &e : A->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return HC_OK;
 }
 
 static int adj_alloc(hc_ctx *ctx) {
@@ -838,13 +881,10 @@ static int adj_alloc(hc_ctx *ctx) {
   int rc = 0;
   for (int q = 0; q < HC_NFIELDS; ++q)
     if (!(A->lam[q] = dalloc(ctx, fb, &rc))) return rc;
-  void **fields[] = {&A->luc[0], &A->luc[1], &A->luc[2], &A->lamb, &A->lcnew, &A->cw, &A->c1, &A->c2, &A->cpre,
-                     &A->ud2[0], &A->ud2[1], &A->ud2[2], &A->ud3[0], &A->ud3[1], &A->ud3[2], &A->lc1, &A->lc2,
-                     &A->lpre, &A->lud2[0], &A->lud2[1], &A->lud2[2], &A->lud3[0], &A->lud3[1], &A->lud3[2]};
+  void **fields[] = {&A->luc[0], &A->luc[1], &A->luc[2], &A->lamb};
   for (auto p : fields)
     if (!(*p = dalloc(ctx, fb, &rc))) return rc;
-  if (!(A->fl2 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
-  if (!(A->fl3 = (uint8_t *)dalloc(ctx, ctx->L.n, &rc))) return rc;
+  if ((rc = scratch_alloc(ctx, A->sc[0]))) return rc;
   if (!(A->g = dalloc(ctx, (size_t)ctx->P.nx * ctx->P.ny * ctx->P.nz * ctx->esz, &rc))) return rc;
   if (!(A->d_sc = (double *)dalloc(ctx, (3 * ADJ_SLOTS + 1) * 8, &rc))) return rc;
   if (!(A->d_part = (double *)dalloc(ctx, (size_t)(ctx->P.nz + 2) * 8, &rc))) return rc;
@@ -990,30 +1030,31 @@ static void adj_courant_t(hc_ctx *ctx, double tau3) {
 // are recomputed here on a copy of the entry value.
 template <typename T>
 static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, double cmax, double tau3,
-                         int vent, const double *qpatch, const TracerSlot *taped = nullptr) {
+                         int vent, const double *qpatch, const TracerSlot *taped = nullptr, int which = 0) {
   AdjCtx *A = ctx->adj;
+  AdjTracerScratch &X = A->sc[which];
   const Layout &L = ctx->L;
-  hipStream_t st = ctx->st;
+  hipStream_t st = which ? A->st2 : ctx->st;
   const size_t fb = L.n * sizeof(T);
   StepArgs s = step_args(ctx);
   s.tau3 = tau3;
   VentArgs va = vent_args(ctx);
   FieldPtrs<T> f = ptrs<T>(ctx);
   const T *c = (const T *)ctx->f[field];
-  TracerSlot is{A->c1, A->c2, {A->ud2[0], A->ud2[1], A->ud2[2]}, {A->ud3[0], A->ud3[1], A->ud3[2]}, A->fl2, A->fl3};
+  TracerSlot is{X.c1, X.c2, {X.ud2[0], X.ud2[1], X.ud2[2]}, {X.ud3[0], X.ud3[1], X.ud3[2]}, X.fl2, X.fl3};
   if (taped) {
     is = *taped;
   } else {
-    hipMemcpyAsync(A->cw, c, fb, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(X.cw, c, fb, hipMemcpyDeviceToDevice, st);
     TracerWork tw{};
     tw.uc = ctx->tw.uc; tw.vc = ctx->tw.vc; tw.wc = ctx->tw.wc;
     tw.ud2 = is.ud2[0]; tw.vd2 = is.ud2[1]; tw.wd2 = is.ud2[2];
-    tw.c1 = is.c1; tw.c2 = is.c2; tw.cpre = A->cpre;
+    tw.c1 = is.c1; tw.c2 = is.c2; tw.cpre = X.cpre;
     tw.ud3 = is.ud3[0]; tw.vd3 = is.ud3[1]; tw.wd3 = is.ud3[2];
     tw.fl2 = is.fl2; tw.fl3 = is.fl3;
     tw.budget_part = ctx->tw.budget_part; tw.budget = ctx->tw.budget;
     TracerArgs ta{cmin, cmax, tau3, 0, 0.0, 0};
-    launch_tracer<T>(st, L, s, ctx->P, va, f, ctx->aux, tw, (T *)A->cw, d_cbkg, qpatch, ta);
+    launch_tracer<T>(st, L, s, ctx->P, va, f, ctx->aux, tw, (T *)X.cw, d_cbkg, qpatch, ta);
   }
   const T *c1 = (const T *)is.c1, *c2 = (const T *)is.c2;
   const T *ud2 = (const T *)is.ud2[0], *vd2 = (const T *)is.ud2[1], *wd2 = (const T *)is.ud2[2];
@@ -1024,13 +1065,13 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   // adjoint buffers: lpre and the pass-2 face velocities' adjoints are
   // first added to by atomics; the donor gathers assign the others on their
   // first write (lc2, lud3 and lcnew in pass 3, lc1 in pass 2)
-  void *zs[] = {A->lpre, A->lud2[0], A->lud2[1], A->lud2[2]};
-  for (void *p : zs) zero(ctx, p);
+  void *zs[] = {X.lpre, X.lud2[0], X.lud2[1], X.lud2[2]};
+  for (void *p : zs) hipMemsetAsync(p, 0, fb, st);
   T *lout = (T *)A->lam[field];
   const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
   const unsigned nbc = (unsigned)((nghost + 255) / 256);
   // final BOUNDARY_CONDITIONS from the pre-sponge values
-  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 1, vent, lout, (T *)A->lpre);
+  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 1, vent, lout, (T *)X.lpre);
   // pass 3
   DonorLast dl{};
   dl.last = 1; dl.tau3 = tau3;
@@ -1052,30 +1093,30 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   };
   const Box3 full{0, L.nx + 1, 0, L.ny + 1, 0, L.nz + 1};
   hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, dl, (const T *)lout,
-                     (const T *)A->lpre, c2, ud3, vd3, wd3, fl3, (T *)A->lc2, (T *)A->lud3[0],
-                     (T *)A->lud3[1], (T *)A->lud3[2], (T *)A->lcnew);
-  anti(c2, ud2, vd2, wd2, (const T *)A->lud3[0], (const T *)A->lud3[1], (const T *)A->lud3[2], (T *)A->lc2,
-       (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2]);
-  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc2, (T *)A->lc2);
+                     (const T *)X.lpre, c2, ud3, vd3, wd3, fl3, (T *)X.lc2, (T *)X.lud3[0],
+                     (T *)X.lud3[1], (T *)X.lud3[2], (T *)X.lcnew);
+  anti(c2, ud2, vd2, wd2, (const T *)X.lud3[0], (const T *)X.lud3[1], (const T *)X.lud3[2], (T *)X.lc2,
+       (T *)X.lud2[0], (T *)X.lud2[1], (T *)X.lud2[2]);
+  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)X.lc2, (T *)X.lc2);
   // pass 2
   DonorLast d2{};
   d2.assign = 1;
-  hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, d2, (const T *)A->lc2,
-                     (const T *)nullptr, c1, ud2, vd2, wd2, fl2, (T *)A->lc1,
-                     (T *)A->lud2[0], (T *)A->lud2[1], (T *)A->lud2[2], (T *)nullptr);
-  anti(c1, (const T *)ctx->tw.uc, (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)A->lud2[0],
-       (const T *)A->lud2[1], (const T *)A->lud2[2], (T *)A->lc1, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2]);
-  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)A->lc1, (T *)A->lc1);
+  hipLaunchKernelGGL(k_adj_donor_gather<T>, grid3(full), dim3(64, 4), 0, st, L, full, d2, (const T *)X.lc2,
+                     (const T *)nullptr, c1, ud2, vd2, wd2, fl2, (T *)X.lc1,
+                     (T *)X.lud2[0], (T *)X.lud2[1], (T *)X.lud2[2], (T *)nullptr);
+  anti(c1, (const T *)ctx->tw.uc, (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const T *)X.lud2[0],
+       (const T *)X.lud2[1], (const T *)X.lud2[2], (T *)X.lc1, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2]);
+  hipLaunchKernelGGL(k_adj_bc<T>, dim3(nbc), dim3(256), 0, st, L, ctx->P, va, 0, 0, (T *)X.lc1, (T *)X.lc1);
   // pass 1
   FieldPtrs<T> lam = lptrs<T>(ctx);
   hipLaunchKernelGGL(k_adj_pass1_tile<T>, tg, tb, 0, st, L, s, ctx->P, f, c, (const T *)ctx->tw.uc,
                      (const T *)ctx->tw.vc, (const T *)ctx->tw.wc, (const double *)ctx->aux.azzmin, d_cbkg,
-                     (const T *)A->lc1, (T *)A->lcnew, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2], lam.ss,
+                     (const T *)X.lc1, (T *)X.lcnew, (T *)A->luc[0], (T *)A->luc[1], (T *)A->luc[2], lam.ss,
                      lam.ssv, kc);
   // the entry value's adjoint becomes the field's
   void *tmp = A->lam[field];
-  A->lam[field] = A->lcnew;
-  A->lcnew = tmp;
+  A->lam[field] = X.lcnew;
+  X.lcnew = tmp;
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,7 +1437,7 @@ static TracerSlot tracer_slot(hc_ctx *ctx, int q, int tr) {
 // pass1_done: t.c1 holds pass 1's result already
 static void taped_tracer(hc_ctx *ctx, int field, const TracerSlot &t, bool pass1_done = false) {
   const TracerWork keep = ctx->tw;
-  ctx->tw.c1 = t.c1; ctx->tw.c2 = t.c2; ctx->tw.cpre = ctx->adj->cpre;
+  ctx->tw.c1 = t.c1; ctx->tw.c2 = t.c2; ctx->tw.cpre = ctx->adj->sc[0].cpre;
   ctx->tw.ud2 = t.ud2[0]; ctx->tw.vd2 = t.ud2[1]; ctx->tw.wd2 = t.ud2[2];
   ctx->tw.ud3 = t.ud3[0]; ctx->tw.vd3 = t.ud3[1]; ctx->tw.wd3 = t.ud3[2];
   ctx->tw.fl2 = t.fl2; ctx->tw.fl3 = t.fl3;
@@ -1455,10 +1496,43 @@ static int fwd_step(hc_ctx *ctx, int m, int tq = -1) {
     dst[n++] = c1[q];
   }
   native_pass1(ctx, n, fields, dst);
-  for (int q = 0; q < 2; ++q) {
-    if (!on[q]) continue;
-    if (tape) taped_tracer(ctx, fid[q], ts[q], c1[q] != nullptr);
-    else native_tracer(ctx, fid[q], c1[q]);
+  // S on the adjoint's second stream beside T (its own corrective-pass
+  // scratch, and the taped slot's buffers when taping), DENSITY after both
+  bool par = ctx->adj && ctx->par_tracers && !ctx->timing && on[0] && on[1];
+  if (par && adj_par_init(ctx) != HC_OK) {
+    par = false;
+    (void)hipGetLastError();
+  }
+  if (par) {
+    AdjCtx *A = ctx->adj;
+    AdjTracerScratch &X = A->sc[1];
+    TracerWork tw{};
+    tw.uc = ctx->tw.uc; tw.vc = ctx->tw.vc; tw.wc = ctx->tw.wc;
+    tw.budget_part = ctx->tw.budget_part; tw.budget = ctx->tw.budget;  // no budgets in a window's step
+    tw.cpre = X.cpre;
+    if (tape) {
+      const TracerSlot &t = ts[1];
+      tw.c2 = t.c2;
+      tw.ud2 = t.ud2[0]; tw.vd2 = t.ud2[1]; tw.wd2 = t.ud2[2];
+      tw.ud3 = t.ud3[0]; tw.vd3 = t.ud3[1]; tw.wd3 = t.ud3[2];
+      tw.fl2 = t.fl2; tw.fl3 = t.fl3;
+    } else {
+      tw.c2 = X.c2;
+      tw.ud2 = X.ud2[0]; tw.vd2 = X.ud2[1]; tw.wd2 = X.ud2[2];
+    }
+    hipEventRecord(A->ev[0], ctx->st);
+    hipStreamWaitEvent(A->st2, A->ev[0], 0);
+    native_tracer_on(ctx, HC_S, c1[1], A->st2, &tw);
+    hipEventRecord(A->ev[1], A->st2);
+    if (tape) taped_tracer(ctx, HC_T, ts[0], true);
+    else native_tracer(ctx, HC_T, c1[0]);
+    hipStreamWaitEvent(ctx->st, A->ev[1], 0);
+  } else {
+    for (int q = 0; q < 2; ++q) {
+      if (!on[q]) continue;
+      if (tape) taped_tracer(ctx, fid[q], ts[q], c1[q] != nullptr);
+      else native_tracer(ctx, fid[q], c1[q]);
+    }
   }
   if (P.bit3 == 1) hc_density(ctx);
   refresh_rotate(ctx);
@@ -1482,10 +1556,25 @@ static void adj_step(hc_ctx *ctx, int m, const T *t1, const T *s1, const T *ss0,
   TracerSlot ts[2];
   if (taped)
     for (int tr = 0; tr < 2; ++tr) ts[tr] = tracer_slot(ctx, tq, tr);
+  // the two tracers' adjoints read and consume disjoint fields and add to
+  // the shared adjoints (Courant numbers, ss, ss_v) by atomics only, so S
+  // runs on a stream of its own beside T (with its own scratch)
+  bool par = ctx->par_tracers && P.toggle_s == 1 && P.toggle_t == 1;
+  if (par && adj_par_init(ctx) != HC_OK) {
+    par = false;  // no memory for the second scratch: one stream
+    (void)hipGetLastError();
+  }
+  AdjCtx *A = ctx->adj;
+  if (par) {
+    hipEventRecord(A->ev[0], ctx->st);
+    hipStreamWaitEvent(A->st2, A->ev[0], 0);
+  }
   if (P.toggle_s == 1)
-    adj_tracer_t<T>(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, ctx->d_q[1], taped ? &ts[1] : nullptr);
+    adj_tracer_t<T>(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, ctx->d_q[1], taped ? &ts[1] : nullptr, par ? 1 : 0);
+  if (par) hipEventRecord(A->ev[1], A->st2);
   if (P.toggle_t == 1)
     adj_tracer_t<T>(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, ctx->d_q[0], taped ? &ts[0] : nullptr);
+  if (par) hipStreamWaitEvent(ctx->st, A->ev[1], 0);
   (void)nz;
   adj_courant_t<T>(ctx, P.dt);
   adj_viscosity_t<T>(ctx, (const T *)ctx->f[HC_SS], (const T *)ctx->f[HC_SSV]);

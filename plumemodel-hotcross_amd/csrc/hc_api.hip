@@ -826,16 +826,19 @@ void native_bounds(hc_ctx *ctx, double *tmin, double *smax) {
 // TRACER31 for T, S or C as the native loop calls it (bounds, background,
 // vent ghost cells of T and S), without budgets; c1, if set, holds the
 // call's pass-1 result already (native_pass1)
-int native_tracer(hc_ctx *ctx, int field, void *c1) {
+int native_tracer_on(hc_ctx *ctx, int field, void *c1, hipStream_t st, const TracerWork *tw) {
   const hc_params &P = ctx->P;
   double tmin, smax;
   native_bounds(ctx, &tmin, &smax);
   const int nz = P.nz;
   if (field == HC_T)
-    return tracer(ctx, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], 0, c1);
-  if (field == HC_S) return tracer(ctx, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], 0, c1);
-  return tracer(ctx, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, 0, c1);
+    return tracer_on(ctx, st, tw, HC_T, ctx->d_tbkg, tmin, 1.0e+10, P.dt, 1, P.tsource + ctx->h_tbkg[nz], 0, c1);
+  if (field == HC_S)
+    return tracer_on(ctx, st, tw, HC_S, ctx->d_sbkg, 0.0, smax, P.dt, 1, P.ssource + ctx->h_sbkg[nz], 0, c1);
+  return tracer_on(ctx, st, tw, HC_C, ctx->d_zero, 0.0, 1.0e+10, P.dt, 0, 0.0, 0, c1);
 }
+
+int native_tracer(hc_ctx *ctx, int field, void *c1) { return native_tracer_on(ctx, field, c1, ctx->st, nullptr); }
 
 extern "C" int hc_tracer31(hc_ctx *ctx, int field, const double *c_bkg, double cmin, double cmax, double tau3,
                            double *budgets) {

@@ -110,6 +110,9 @@ int box_copy(hc_ctx *ctx, void *dev, int bounds_field, double *host, const int32
 void update_sums(hc_ctx *ctx);
 void native_bounds(hc_ctx *ctx, double *tmin, double *smax);
 int native_tracer(hc_ctx *ctx, int field, void *c1 = nullptr);
+// the same on stream st with the corrective-pass scratch of tw (the
+// context's own when null)
+int native_tracer_on(hc_ctx *ctx, int field, void *c1, hipStream_t st, const TracerWork *tw);
 int native_pass1(hc_ctx *ctx, int n, const int *fields, void *const *dst);
 void *native_c1(hc_ctx *ctx, int q, int *rc);
 // MOMENTUM with the work velocity and REFRESH's filter fused in, and the

@@ -6,8 +6,11 @@ The forward is deterministic, so the loss agrees bit for bit.  The reverse
 sweep adds contributions with atomics (LDS rings flushed to HBM by global
 atomic adds, block partial sums of the scalar sensitivities), whose order
 varies from run to run; the gradients agree to the rounding of those sums:
-- FP64: the heat-flux map, the crossflow amplitude and Cs gradients to 1e-9
-  relative (measured 1e-14 for the map, 1e-9 for the crossflow amplitude);
+- FP64: the heat-flux map and Cs gradients to 1e-9 relative (measured
+  1e-14 for both); the crossflow-amplitude gradient, a domain sum of the
+  velocity adjoints that largely cancels, to 1e-8
+  (measured 1e-9 and 2e-9 with the T and S adjoints on two streams, whose
+  atomic adds into the shared velocity adjoints interleave freely);
 - FP32: the heat-flux map and Cs gradients to 1e-4 relative (measured 4e-6
   and 4e-6).  The FP32 crossflow-amplitude gradient is a domain sum of the
   velocity adjoints that nearly cancels; the FP32 rounding of the adjoint
@@ -66,4 +69,4 @@ def test_identical_windows_agree(prec, tol):
     assert rel(q2, q1) <= tol
     assert abs(cs2 - cs1) <= tol * abs(cs1)
     if prec == H.FP64:
-        assert abs(g2[0] - g1[0]) <= tol * abs(g1[0])
+        assert abs(g2[0] - g1[0]) <= 1e-8 * abs(g1[0])
