@@ -62,7 +62,11 @@ HC_HD inline AdjConsts adj_consts(const StepArgs &s, const Phys &ph) {
   return c;
 }
 
-template <typename T, class R, class A>
+// PART selects the terms of a component: 1 advection and the Coriolis and
+// crossflow terms (the adjoints of u, v, w and d/d flow through fbx, fby),
+// 2 diffusion, buoyancy and sponge (the adjoints of um, vm, wm, ss, ssv,
+// rhop and d/d flow through the sponge's background), 3 both
+template <typename T, int PART = 3, class R, class A>
 HC_HD inline void adj_forces_x(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph,
                                const FlowSens &fs, const R &u, const R &v, const R &w, const R &um, const R &vm,
                                const R &wm, const R &ss, const R &ssv, const double *azzmin, const double *alphau,
@@ -70,164 +74,172 @@ HC_HD inline void adj_forces_x(const Layout &L, int i, int j, int k, const StepA
                                const A &awm, const A &ass, const A &assv, double &gflow) {
   (void)ssv; (void)azzmin;
   const AdjConsts c = adj_consts(s, ph);
-  const T axu_p = ss(i + 1, j, k) + (T)c.axmin, axu_c = ss(i, j, k) + (T)c.axmin;
-  const T ayu_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + (T)c.aymin;
-  const T ayu_m = (T).25 * (ss(i, j - 1, k) + ss(i + 1, j - 1, k) + ss(i, j, k) + ss(i + 1, j, k)) + (T)c.aymin;
-  const T azu_c = azu_at<T>(L, ssv, azzmin, s, i, j, k), azu_m = azu_at<T>(L, ssv, azzmin, s, i, j, k - 1);
-  const T umc = um(i, j, k);
-  T lumc = 0;
-  // 2 bit5 (axu_p (um(i+1)-umc) - axu_c (umc-um(i-1))) / dx^2
-  const T g1 = lam * (T)(2.0 * c.bit5 * c.idx * c.idx);
-  ass(i + 1, j, k, g1 * (um(i + 1, j, k) - umc));
-  ass(i, j, k, -g1 * (umc - um(i - 1, j, k)));
-  aum(i + 1, j, k, g1 * axu_p); aum(i - 1, j, k, g1 * axu_c);
-  lumc -= g1 * (axu_p + axu_c);
-  // bit5 (ayu_c (um(j+1)-umc) - ayu_m (umc-um(j-1))) / dy^2
-  const T g2 = lam * (T)(c.bit5 * c.idy * c.idy);
-  T lay_c = g2 * (um(i, j + 1, k) - umc), lay_m = -g2 * (umc - um(i, j - 1, k));
-  aum(i, j + 1, k, g2 * ayu_c); aum(i, j - 1, k, g2 * ayu_m);
-  lumc -= g2 * (ayu_c + ayu_m);
-  // bit5 (ayu_c (vm(i+1,j)-vm(i,j)) - ayu_m (vm(i+1,j-1)-vm(i,j-1))) / (dx dy)
-  const T g3 = lam * (T)(c.bit5 * c.idx * c.idy);
-  lay_c += g3 * (vm(i + 1, j, k) - vm(i, j, k));
-  lay_m -= g3 * (vm(i + 1, j - 1, k) - vm(i, j - 1, k));
-  avm(i + 1, j, k, g3 * ayu_c); avm(i, j, k, -g3 * ayu_c);
-  avm(i + 1, j - 1, k, -g3 * ayu_m); avm(i, j - 1, k, g3 * ayu_m);
-  {
-    const T qc = (T).25 * lay_c, qm = (T).25 * lay_m;
-    ass(i, j, k, qc + qm); ass(i + 1, j, k, qc + qm);
-    ass(i, j + 1, k, qc); ass(i + 1, j + 1, k, qc);
-    ass(i, j - 1, k, qm); ass(i + 1, j - 1, k, qm);
+  if constexpr ((PART & 2) != 0) {
+    const T axu_p = ss(i + 1, j, k) + (T)c.axmin, axu_c = ss(i, j, k) + (T)c.axmin;
+    const T ayu_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + (T)c.aymin;
+    const T ayu_m = (T).25 * (ss(i, j - 1, k) + ss(i + 1, j - 1, k) + ss(i, j, k) + ss(i + 1, j, k)) + (T)c.aymin;
+    const T azu_c = azu_at<T>(L, ssv, azzmin, s, i, j, k), azu_m = azu_at<T>(L, ssv, azzmin, s, i, j, k - 1);
+    const T umc = um(i, j, k);
+    T lumc = 0;
+    // 2 bit5 (axu_p (um(i+1)-umc) - axu_c (umc-um(i-1))) / dx^2
+    const T g1 = lam * (T)(2.0 * c.bit5 * c.idx * c.idx);
+    ass(i + 1, j, k, g1 * (um(i + 1, j, k) - umc));
+    ass(i, j, k, -g1 * (umc - um(i - 1, j, k)));
+    aum(i + 1, j, k, g1 * axu_p); aum(i - 1, j, k, g1 * axu_c);
+    lumc -= g1 * (axu_p + axu_c);
+    // bit5 (ayu_c (um(j+1)-umc) - ayu_m (umc-um(j-1))) / dy^2
+    const T g2 = lam * (T)(c.bit5 * c.idy * c.idy);
+    T lay_c = g2 * (um(i, j + 1, k) - umc), lay_m = -g2 * (umc - um(i, j - 1, k));
+    aum(i, j + 1, k, g2 * ayu_c); aum(i, j - 1, k, g2 * ayu_m);
+    lumc -= g2 * (ayu_c + ayu_m);
+    // bit5 (ayu_c (vm(i+1,j)-vm(i,j)) - ayu_m (vm(i+1,j-1)-vm(i,j-1))) / (dx dy)
+    const T g3 = lam * (T)(c.bit5 * c.idx * c.idy);
+    lay_c += g3 * (vm(i + 1, j, k) - vm(i, j, k));
+    lay_m -= g3 * (vm(i + 1, j - 1, k) - vm(i, j - 1, k));
+    avm(i + 1, j, k, g3 * ayu_c); avm(i, j, k, -g3 * ayu_c);
+    avm(i + 1, j - 1, k, -g3 * ayu_m); avm(i, j - 1, k, g3 * ayu_m);
+    {
+      const T qc = (T).25 * lay_c, qm = (T).25 * lay_m;
+      ass(i, j, k, qc + qm); ass(i + 1, j, k, qc + qm);
+      ass(i, j + 1, k, qc); ass(i + 1, j + 1, k, qc);
+      ass(i, j - 1, k, qm); ass(i + 1, j - 1, k, qm);
+    }
+    // bit2 (azu_c (um(k+1)-umc) - azu_m (umc-um(k-1))) / dz^2
+    const T g4 = lam * (T)(c.bit2 * c.idz * c.idz);
+    T laz_c = g4 * (um(i, j, k + 1) - umc), laz_m = -g4 * (umc - um(i, j, k - 1));
+    aum(i, j, k + 1, g4 * azu_c); aum(i, j, k - 1, g4 * azu_m);
+    lumc -= g4 * (azu_c + azu_m);
+    // bit2 (azu_c (wm(i+1,k)-wm(i,k)) - azu_m (wm(i+1,k-1)-wm(i,k-1))) / (dx dz)
+    const T g5 = lam * (T)(c.bit2 * c.idx * c.idz);
+    laz_c += g5 * (wm(i + 1, j, k) - wm(i, j, k));
+    laz_m -= g5 * (wm(i + 1, j, k - 1) - wm(i, j, k - 1));
+    awm(i + 1, j, k, g5 * azu_c); awm(i, j, k, -g5 * azu_c);
+    awm(i + 1, j, k - 1, -g5 * azu_m); awm(i, j, k - 1, g5 * azu_m);
+    adj_azu(L, i, j, k, laz_c, assv);
+    adj_azu(L, i, j, k - 1, laz_m, assv);
+    // sponge -0.5 alphau (um - ubkg_m)
+    const double au_ = alphau[(i + 1) + (L.nx + 3) * j];
+    lumc -= (T)(0.5 * au_) * lam;
+    gflow += 0.5 * au_ * (double)lam * fs.dubkg_m;
+    aum(i, j, k, lumc);
   }
-  // bit2 (azu_c (um(k+1)-umc) - azu_m (umc-um(k-1))) / dz^2
-  const T g4 = lam * (T)(c.bit2 * c.idz * c.idz);
-  T laz_c = g4 * (um(i, j, k + 1) - umc), laz_m = -g4 * (umc - um(i, j, k - 1));
-  aum(i, j, k + 1, g4 * azu_c); aum(i, j, k - 1, g4 * azu_m);
-  lumc -= g4 * (azu_c + azu_m);
-  // bit2 (azu_c (wm(i+1,k)-wm(i,k)) - azu_m (wm(i+1,k-1)-wm(i,k-1))) / (dx dz)
-  const T g5 = lam * (T)(c.bit2 * c.idx * c.idz);
-  laz_c += g5 * (wm(i + 1, j, k) - wm(i, j, k));
-  laz_m -= g5 * (wm(i + 1, j, k - 1) - wm(i, j, k - 1));
-  awm(i + 1, j, k, g5 * azu_c); awm(i, j, k, -g5 * azu_c);
-  awm(i + 1, j, k - 1, -g5 * azu_m); awm(i, j, k - 1, g5 * azu_m);
-  adj_azu(L, i, j, k, laz_c, assv);
-  adj_azu(L, i, j, k - 1, laz_m, assv);
-  // sponge -0.5 alphau (um - ubkg_m)
-  const double au_ = alphau[(i + 1) + (L.nx + 3) * j];
-  lumc -= (T)(0.5 * au_) * lam;
-  gflow += 0.5 * au_ * (double)lam * fs.dubkg_m;
-  aum(i, j, k, lumc);
-  // advection
-  const T uc = u(i, j, k);
-  const T a1 = u(i + 1, j, k) + uc, a2 = uc + u(i - 1, j, k);
-  const T la1 = -(T)c.coe_4x * (T)2 * a1 * lam, la2 = (T)c.coe_4x * (T)2 * a2 * lam;
-  T luc = la1 + la2;
-  au(i + 1, j, k, la1); au(i - 1, j, k, la2);
-  {  // -coe_4z ((w(i,k)+w(i+1,k)) (u(k+1)+uc) - (w(i,k-1)+w(i+1,k-1)) (uc+u(k-1)))
-    const T P1 = w(i, j, k) + w(i + 1, j, k), Q1 = u(i, j, k + 1) + uc;
-    const T P2 = w(i, j, k - 1) + w(i + 1, j, k - 1), Q2 = uc + u(i, j, k - 1);
-    const T z = (T)c.coe_4z * lam, c9 = (T)c.coe_9 * lam;
-    aw(i, j, k, -z * Q1 - c9); aw(i + 1, j, k, -z * Q1 - c9);
-    aw(i, j, k - 1, z * Q2 - c9); aw(i + 1, j, k - 1, z * Q2 - c9);
-    au(i, j, k + 1, -z * P1); au(i, j, k - 1, z * P2);
-    luc += -z * P1 + z * P2;
+  if constexpr ((PART & 1) != 0) {
+    // advection
+    const T uc = u(i, j, k);
+    const T a1 = u(i + 1, j, k) + uc, a2 = uc + u(i - 1, j, k);
+    const T la1 = -(T)c.coe_4x * (T)2 * a1 * lam, la2 = (T)c.coe_4x * (T)2 * a2 * lam;
+    T luc = la1 + la2;
+    au(i + 1, j, k, la1); au(i - 1, j, k, la2);
+    {  // -coe_4z ((w(i,k)+w(i+1,k)) (u(k+1)+uc) - (w(i,k-1)+w(i+1,k-1)) (uc+u(k-1)))
+      const T P1 = w(i, j, k) + w(i + 1, j, k), Q1 = u(i, j, k + 1) + uc;
+      const T P2 = w(i, j, k - 1) + w(i + 1, j, k - 1), Q2 = uc + u(i, j, k - 1);
+      const T z = (T)c.coe_4z * lam, c9 = (T)c.coe_9 * lam;
+      aw(i, j, k, -z * Q1 - c9); aw(i + 1, j, k, -z * Q1 - c9);
+      aw(i, j, k - 1, z * Q2 - c9); aw(i + 1, j, k - 1, z * Q2 - c9);
+      au(i, j, k + 1, -z * P1); au(i, j, k - 1, z * P2);
+      luc += -z * P1 + z * P2;
+    }
+    {  // -coe_4y ((v(i,j)+v(i+1,j)) (u(j+1)+uc) - (v(i,j-1)+v(i+1,j-1)) (uc+u(j-1))) + coe_7 (...)
+      const T P1 = v(i, j, k) + v(i + 1, j, k), Q1 = u(i, j + 1, k) + uc;
+      const T P2 = v(i, j - 1, k) + v(i + 1, j - 1, k), Q2 = uc + u(i, j - 1, k);
+      const T y = (T)c.coe_4y * lam, c7 = (T)c.coe_7 * lam;
+      av(i, j, k, -y * Q1 + c7); av(i + 1, j, k, -y * Q1 + c7);
+      av(i, j - 1, k, y * Q2 + c7); av(i + 1, j - 1, k, y * Q2 + c7);
+      au(i, j + 1, k, -y * P1); au(i, j - 1, k, y * P2);
+      luc += -y * P1 + y * P2;
+    }
+    au(i, j, k, luc);
+    gflow += c.coe_10 * (double)lam * fs.dfbx;
   }
-  {  // -coe_4y ((v(i,j)+v(i+1,j)) (u(j+1)+uc) - (v(i,j-1)+v(i+1,j-1)) (uc+u(j-1))) + coe_7 (...)
-    const T P1 = v(i, j, k) + v(i + 1, j, k), Q1 = u(i, j + 1, k) + uc;
-    const T P2 = v(i, j - 1, k) + v(i + 1, j - 1, k), Q2 = uc + u(i, j - 1, k);
-    const T y = (T)c.coe_4y * lam, c7 = (T)c.coe_7 * lam;
-    av(i, j, k, -y * Q1 + c7); av(i + 1, j, k, -y * Q1 + c7);
-    av(i, j - 1, k, y * Q2 + c7); av(i + 1, j - 1, k, y * Q2 + c7);
-    au(i, j + 1, k, -y * P1); au(i, j - 1, k, y * P2);
-    luc += -y * P1 + y * P2;
-  }
-  au(i, j, k, luc);
-  gflow += c.coe_10 * (double)lam * fs.dfbx;
 }
 
-template <typename T, class R, class A>
+template <typename T, int PART = 3, class R, class A>
 HC_HD inline void adj_forces_y(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph,
                                const FlowSens &fs, const R &u, const R &v, const R &w, const R &um, const R &vm,
                                const R &wm, const R &ss, const R &ssv, const double *azzmin, const double *alphav,
                                T lam, const A &au, const A &av, const A &aw, const A &aum, const A &avm,
                                const A &awm, const A &ass, const A &assv, double &gflow) {
   const AdjConsts c = adj_consts(s, ph);
-  const T axv_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + (T)c.axmin;
-  const T axv_m = (T).25 * (ss(i - 1, j, k) + ss(i, j, k) + ss(i - 1, j + 1, k) + ss(i, j + 1, k)) + (T)c.axmin;
-  const T ayv_p = ss(i, j + 1, k) + (T)c.aymin, ayv_c = ss(i, j, k) + (T)c.aymin;
-  const T azv_c = azv_at<T>(L, ssv, azzmin, s, i, j, k), azv_m = azv_at<T>(L, ssv, azzmin, s, i, j, k - 1);
-  const T vmc = vm(i, j, k);
-  T lvmc = 0;
-  // bit5 (axv_c (vm(i+1)-vmc) - axv_m (vmc-vm(i-1))) / dx^2
-  const T g1 = lam * (T)(c.bit5 * c.idx * c.idx);
-  T lax_c = g1 * (vm(i + 1, j, k) - vmc), lax_m = -g1 * (vmc - vm(i - 1, j, k));
-  avm(i + 1, j, k, g1 * axv_c); avm(i - 1, j, k, g1 * axv_m);
-  lvmc -= g1 * (axv_c + axv_m);
-  // bit5 (axv_c (um(i,j+1)-um(i,j)) - axv_m (um(i-1,j+1)-um(i-1,j))) / (dx dy)
-  const T g2 = lam * (T)(c.bit5 * c.idx * c.idy);
-  lax_c += g2 * (um(i, j + 1, k) - um(i, j, k));
-  lax_m -= g2 * (um(i - 1, j + 1, k) - um(i - 1, j, k));
-  aum(i, j + 1, k, g2 * axv_c); aum(i, j, k, -g2 * axv_c);
-  aum(i - 1, j + 1, k, -g2 * axv_m); aum(i - 1, j, k, g2 * axv_m);
-  {
-    const T qc = (T).25 * lax_c, qm = (T).25 * lax_m;
-    ass(i, j, k, qc + qm); ass(i, j + 1, k, qc + qm);
-    ass(i + 1, j, k, qc); ass(i + 1, j + 1, k, qc);
-    ass(i - 1, j, k, qm); ass(i - 1, j + 1, k, qm);
+  if constexpr ((PART & 2) != 0) {
+    const T axv_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + (T)c.axmin;
+    const T axv_m = (T).25 * (ss(i - 1, j, k) + ss(i, j, k) + ss(i - 1, j + 1, k) + ss(i, j + 1, k)) + (T)c.axmin;
+    const T ayv_p = ss(i, j + 1, k) + (T)c.aymin, ayv_c = ss(i, j, k) + (T)c.aymin;
+    const T azv_c = azv_at<T>(L, ssv, azzmin, s, i, j, k), azv_m = azv_at<T>(L, ssv, azzmin, s, i, j, k - 1);
+    const T vmc = vm(i, j, k);
+    T lvmc = 0;
+    // bit5 (axv_c (vm(i+1)-vmc) - axv_m (vmc-vm(i-1))) / dx^2
+    const T g1 = lam * (T)(c.bit5 * c.idx * c.idx);
+    T lax_c = g1 * (vm(i + 1, j, k) - vmc), lax_m = -g1 * (vmc - vm(i - 1, j, k));
+    avm(i + 1, j, k, g1 * axv_c); avm(i - 1, j, k, g1 * axv_m);
+    lvmc -= g1 * (axv_c + axv_m);
+    // bit5 (axv_c (um(i,j+1)-um(i,j)) - axv_m (um(i-1,j+1)-um(i-1,j))) / (dx dy)
+    const T g2 = lam * (T)(c.bit5 * c.idx * c.idy);
+    lax_c += g2 * (um(i, j + 1, k) - um(i, j, k));
+    lax_m -= g2 * (um(i - 1, j + 1, k) - um(i - 1, j, k));
+    aum(i, j + 1, k, g2 * axv_c); aum(i, j, k, -g2 * axv_c);
+    aum(i - 1, j + 1, k, -g2 * axv_m); aum(i - 1, j, k, g2 * axv_m);
+    {
+      const T qc = (T).25 * lax_c, qm = (T).25 * lax_m;
+      ass(i, j, k, qc + qm); ass(i, j + 1, k, qc + qm);
+      ass(i + 1, j, k, qc); ass(i + 1, j + 1, k, qc);
+      ass(i - 1, j, k, qm); ass(i - 1, j + 1, k, qm);
+    }
+    // 2 bit5 (ayv_p (vm(j+1)-vmc) - ayv_c (vmc-vm(j-1))) / dy^2
+    const T g3 = lam * (T)(2.0 * c.bit5 * c.idy * c.idy);
+    ass(i, j + 1, k, g3 * (vm(i, j + 1, k) - vmc));
+    ass(i, j, k, -g3 * (vmc - vm(i, j - 1, k)));
+    avm(i, j + 1, k, g3 * ayv_p); avm(i, j - 1, k, g3 * ayv_c);
+    lvmc -= g3 * (ayv_p + ayv_c);
+    // bit2 (azv_c (vm(k+1)-vmc) - azv_m (vmc-vm(k-1))) / dz^2
+    const T g4 = lam * (T)(c.bit2 * c.idz * c.idz);
+    T laz_c = g4 * (vm(i, j, k + 1) - vmc), laz_m = -g4 * (vmc - vm(i, j, k - 1));
+    avm(i, j, k + 1, g4 * azv_c); avm(i, j, k - 1, g4 * azv_m);
+    lvmc -= g4 * (azv_c + azv_m);
+    // bit2 (azv_c (wm(j+1,k)-wm(j,k)) - azv_m (wm(j+1,k-1)-wm(j,k-1))) / (dy dz)
+    const T g5 = lam * (T)(c.bit2 * c.idy * c.idz);
+    laz_c += g5 * (wm(i, j + 1, k) - wm(i, j, k));
+    laz_m -= g5 * (wm(i, j + 1, k - 1) - wm(i, j, k - 1));
+    awm(i, j + 1, k, g5 * azv_c); awm(i, j, k, -g5 * azv_c);
+    awm(i, j + 1, k - 1, -g5 * azv_m); awm(i, j, k - 1, g5 * azv_m);
+    adj_azv(L, i, j, k, laz_c, assv);
+    adj_azv(L, i, j, k - 1, laz_m, assv);
+    // sponge -0.5 alphav (vm - vbkg_m)
+    const double av_ = alphav[i + (L.nx + 2) * (j + 1)];
+    lvmc -= (T)(0.5 * av_) * lam;
+    gflow += 0.5 * av_ * (double)lam * fs.dvbkg_m;
+    avm(i, j, k, lvmc);
   }
-  // 2 bit5 (ayv_p (vm(j+1)-vmc) - ayv_c (vmc-vm(j-1))) / dy^2
-  const T g3 = lam * (T)(2.0 * c.bit5 * c.idy * c.idy);
-  ass(i, j + 1, k, g3 * (vm(i, j + 1, k) - vmc));
-  ass(i, j, k, -g3 * (vmc - vm(i, j - 1, k)));
-  avm(i, j + 1, k, g3 * ayv_p); avm(i, j - 1, k, g3 * ayv_c);
-  lvmc -= g3 * (ayv_p + ayv_c);
-  // bit2 (azv_c (vm(k+1)-vmc) - azv_m (vmc-vm(k-1))) / dz^2
-  const T g4 = lam * (T)(c.bit2 * c.idz * c.idz);
-  T laz_c = g4 * (vm(i, j, k + 1) - vmc), laz_m = -g4 * (vmc - vm(i, j, k - 1));
-  avm(i, j, k + 1, g4 * azv_c); avm(i, j, k - 1, g4 * azv_m);
-  lvmc -= g4 * (azv_c + azv_m);
-  // bit2 (azv_c (wm(j+1,k)-wm(j,k)) - azv_m (wm(j+1,k-1)-wm(j,k-1))) / (dy dz)
-  const T g5 = lam * (T)(c.bit2 * c.idy * c.idz);
-  laz_c += g5 * (wm(i, j + 1, k) - wm(i, j, k));
-  laz_m -= g5 * (wm(i, j + 1, k - 1) - wm(i, j, k - 1));
-  awm(i, j + 1, k, g5 * azv_c); awm(i, j, k, -g5 * azv_c);
-  awm(i, j + 1, k - 1, -g5 * azv_m); awm(i, j, k - 1, g5 * azv_m);
-  adj_azv(L, i, j, k, laz_c, assv);
-  adj_azv(L, i, j, k - 1, laz_m, assv);
-  // sponge -0.5 alphav (vm - vbkg_m)
-  const double av_ = alphav[i + (L.nx + 2) * (j + 1)];
-  lvmc -= (T)(0.5 * av_) * lam;
-  gflow += 0.5 * av_ * (double)lam * fs.dvbkg_m;
-  avm(i, j, k, lvmc);
-  // advection
-  const T vc = v(i, j, k);
-  const T b1 = v(i, j + 1, k) + vc, b2 = vc + v(i, j - 1, k);
-  const T lb1 = -(T)c.coe_4y * (T)2 * b1 * lam, lb2 = (T)c.coe_4y * (T)2 * b2 * lam;
-  T lvc = lb1 + lb2;
-  av(i, j + 1, k, lb1); av(i, j - 1, k, lb2);
-  {  // -coe_4z ((v(k+1)+vc) (w(j+1,k)+w(j,k)) - (vc+v(k-1)) (w(j+1,k-1)+w(j,k-1)))
-    const T P1 = v(i, j, k + 1) + vc, Q1 = w(i, j + 1, k) + w(i, j, k);
-    const T P2 = vc + v(i, j, k - 1), Q2 = w(i, j + 1, k - 1) + w(i, j, k - 1);
-    const T z = (T)c.coe_4z * lam;
-    av(i, j, k + 1, -z * Q1); av(i, j, k - 1, z * Q2);
-    lvc += -z * Q1 + z * Q2;
-    aw(i, j + 1, k, -z * P1); aw(i, j, k, -z * P1);
-    aw(i, j + 1, k - 1, z * P2); aw(i, j, k - 1, z * P2);
+  if constexpr ((PART & 1) != 0) {
+    // advection
+    const T vc = v(i, j, k);
+    const T b1 = v(i, j + 1, k) + vc, b2 = vc + v(i, j - 1, k);
+    const T lb1 = -(T)c.coe_4y * (T)2 * b1 * lam, lb2 = (T)c.coe_4y * (T)2 * b2 * lam;
+    T lvc = lb1 + lb2;
+    av(i, j + 1, k, lb1); av(i, j - 1, k, lb2);
+    {  // -coe_4z ((v(k+1)+vc) (w(j+1,k)+w(j,k)) - (vc+v(k-1)) (w(j+1,k-1)+w(j,k-1)))
+      const T P1 = v(i, j, k + 1) + vc, Q1 = w(i, j + 1, k) + w(i, j, k);
+      const T P2 = vc + v(i, j, k - 1), Q2 = w(i, j + 1, k - 1) + w(i, j, k - 1);
+      const T z = (T)c.coe_4z * lam;
+      av(i, j, k + 1, -z * Q1); av(i, j, k - 1, z * Q2);
+      lvc += -z * Q1 + z * Q2;
+      aw(i, j + 1, k, -z * P1); aw(i, j, k, -z * P1);
+      aw(i, j + 1, k - 1, z * P2); aw(i, j, k - 1, z * P2);
+    }
+    {  // -coe_4x ((v(i+1)+vc) (u(i,j)+u(i,j+1)) - (vc+v(i-1)) (u(i-1,j)+u(i-1,j+1))) - coe_7 (u ...)
+      const T P1 = v(i + 1, j, k) + vc, Q1 = u(i, j, k) + u(i, j + 1, k);
+      const T P2 = vc + v(i - 1, j, k), Q2 = u(i - 1, j, k) + u(i - 1, j + 1, k);
+      const T x = (T)c.coe_4x * lam, c7 = (T)c.coe_7 * lam;
+      av(i + 1, j, k, -x * Q1); av(i - 1, j, k, x * Q2);
+      lvc += -x * Q1 + x * Q2;
+      au(i, j, k, -x * P1 - c7); au(i, j + 1, k, -x * P1 - c7);
+      au(i - 1, j, k, x * P2 - c7); au(i - 1, j + 1, k, x * P2 - c7);
+    }
+    av(i, j, k, lvc);
+    gflow += c.coe_10 * (double)lam * fs.dfby;
   }
-  {  // -coe_4x ((v(i+1)+vc) (u(i,j)+u(i,j+1)) - (vc+v(i-1)) (u(i-1,j)+u(i-1,j+1))) - coe_7 (u ...)
-    const T P1 = v(i + 1, j, k) + vc, Q1 = u(i, j, k) + u(i, j + 1, k);
-    const T P2 = vc + v(i - 1, j, k), Q2 = u(i - 1, j, k) + u(i - 1, j + 1, k);
-    const T x = (T)c.coe_4x * lam, c7 = (T)c.coe_7 * lam;
-    av(i + 1, j, k, -x * Q1); av(i - 1, j, k, x * Q2);
-    lvc += -x * Q1 + x * Q2;
-    au(i, j, k, -x * P1 - c7); au(i, j + 1, k, -x * P1 - c7);
-    au(i - 1, j, k, x * P2 - c7); au(i - 1, j + 1, k, x * P2 - c7);
-  }
-  av(i, j, k, lvc);
-  gflow += c.coe_10 * (double)lam * fs.dfby;
 }
 
-template <typename T, class R, class RR, class A>
+template <typename T, int PART = 3, class R, class RR, class A>
 HC_HD inline void adj_forces_z(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph,
                                const R &u, const R &v, const R &w, const R &um, const R &vm, const R &wm,
                                const R &ss, const R &ssv, const RR &rhop, const double *azzmin,
@@ -236,87 +248,91 @@ HC_HD inline void adj_forces_z(const Layout &L, int i, int j, int k, const StepA
                                const A &arhop) {
   (void)rhop;
   const AdjConsts c = adj_consts(s, ph);
-  const T ahp = ss(i, j, k + 1) + (T)c.axmin, ahm = ss(i, j, k) + (T)c.axmin;
-  const T zk = (T)azzmin[k];
-  const T s0 = ssv(i, j, k + 1), s1 = ssv(i, j, k);
-  const T avxp = (T)0.25 * (s0 + s1 + ssv(i + 1, j, k + 1) + ssv(i + 1, j, k)) + zk;
-  const T avxm = (T)0.25 * (s0 + s1 + ssv(i - 1, j, k + 1) + ssv(i - 1, j, k)) + zk;
-  const T avyp = (T)0.25 * (s0 + s1 + ssv(i, j + 1, k + 1) + ssv(i, j + 1, k)) + zk;
-  const T avym = (T)0.25 * (s0 + s1 + ssv(i, j - 1, k + 1) + ssv(i, j - 1, k)) + zk;
-  const T wmc = wm(i, j, k);
-  T lwmc = 0;
-  // 2 bit5 (ahp (wm(k+1)-wmc) - ahm (wmc-wm(k-1))) / dz^2
-  const T g1 = lam * (T)(2.0 * c.bit5 * c.idz * c.idz);
-  ass(i, j, k + 1, g1 * (wm(i, j, k + 1) - wmc));
-  ass(i, j, k, -g1 * (wmc - wm(i, j, k - 1)));
-  awm(i, j, k + 1, g1 * ahp); awm(i, j, k - 1, g1 * ahm);
-  lwmc -= g1 * (ahp + ahm);
-  // bit2 (avxp (wm(i+1)-wmc) - avxm (wmc-wm(i-1))) / dx^2
-  const T g2 = lam * (T)(c.bit2 * c.idx * c.idx);
-  T lxp = g2 * (wm(i + 1, j, k) - wmc), lxm = -g2 * (wmc - wm(i - 1, j, k));
-  awm(i + 1, j, k, g2 * avxp); awm(i - 1, j, k, g2 * avxm);
-  lwmc -= g2 * (avxp + avxm);
-  // bit2 (avxp (um(i,k)-um(i,k+1)) - avxm (um(i-1,k)-um(i-1,k+1))) / (dz dx)
-  const T g3 = lam * (T)(c.bit2 * c.idz * c.idx);
-  lxp += g3 * (um(i, j, k) - um(i, j, k + 1));
-  lxm -= g3 * (um(i - 1, j, k) - um(i - 1, j, k + 1));
-  aum(i, j, k, g3 * avxp); aum(i, j, k + 1, -g3 * avxp);
-  aum(i - 1, j, k, -g3 * avxm); aum(i - 1, j, k + 1, g3 * avxm);
-  // bit2 (avyp (wm(j+1)-wmc) - avym (wmc-wm(j-1))) / dy^2
-  const T g4 = lam * (T)(c.bit2 * c.idy * c.idy);
-  T lyp = g4 * (wm(i, j + 1, k) - wmc), lym = -g4 * (wmc - wm(i, j - 1, k));
-  awm(i, j + 1, k, g4 * avyp); awm(i, j - 1, k, g4 * avym);
-  lwmc -= g4 * (avyp + avym);
-  // bit2 (avyp (vm(j,k)-vm(j,k+1)) - avym (vm(j-1,k)-vm(j-1,k+1))) / (dz dy)
-  const T g5 = lam * (T)(c.bit2 * c.idz * c.idy);
-  lyp += g5 * (vm(i, j, k) - vm(i, j, k + 1));
-  lym -= g5 * (vm(i, j - 1, k) - vm(i, j - 1, k + 1));
-  avm(i, j, k, g5 * avyp); avm(i, j, k + 1, -g5 * avyp);
-  avm(i, j - 1, k, -g5 * avym); avm(i, j - 1, k + 1, g5 * avym);
-  {
-    // the four averaged vertical viscosities share s0, s1
-    const T qxp = (T).25 * lxp, qxm = (T).25 * lxm, qyp = (T).25 * lyp, qym = (T).25 * lym;
-    const T q01 = qxp + qxm + qyp + qym;
-    assv(i, j, k + 1, q01); assv(i, j, k, q01);
-    assv(i + 1, j, k + 1, qxp); assv(i + 1, j, k, qxp);
-    assv(i - 1, j, k + 1, qxm); assv(i - 1, j, k, qxm);
-    assv(i, j + 1, k + 1, qyp); assv(i, j + 1, k, qyp);
-    assv(i, j - 1, k + 1, qym); assv(i, j - 1, k, qym);
+  if constexpr ((PART & 2) != 0) {
+    const T ahp = ss(i, j, k + 1) + (T)c.axmin, ahm = ss(i, j, k) + (T)c.axmin;
+    const T zk = (T)azzmin[k];
+    const T s0 = ssv(i, j, k + 1), s1 = ssv(i, j, k);
+    const T avxp = (T)0.25 * (s0 + s1 + ssv(i + 1, j, k + 1) + ssv(i + 1, j, k)) + zk;
+    const T avxm = (T)0.25 * (s0 + s1 + ssv(i - 1, j, k + 1) + ssv(i - 1, j, k)) + zk;
+    const T avyp = (T)0.25 * (s0 + s1 + ssv(i, j + 1, k + 1) + ssv(i, j + 1, k)) + zk;
+    const T avym = (T)0.25 * (s0 + s1 + ssv(i, j - 1, k + 1) + ssv(i, j - 1, k)) + zk;
+    const T wmc = wm(i, j, k);
+    T lwmc = 0;
+    // 2 bit5 (ahp (wm(k+1)-wmc) - ahm (wmc-wm(k-1))) / dz^2
+    const T g1 = lam * (T)(2.0 * c.bit5 * c.idz * c.idz);
+    ass(i, j, k + 1, g1 * (wm(i, j, k + 1) - wmc));
+    ass(i, j, k, -g1 * (wmc - wm(i, j, k - 1)));
+    awm(i, j, k + 1, g1 * ahp); awm(i, j, k - 1, g1 * ahm);
+    lwmc -= g1 * (ahp + ahm);
+    // bit2 (avxp (wm(i+1)-wmc) - avxm (wmc-wm(i-1))) / dx^2
+    const T g2 = lam * (T)(c.bit2 * c.idx * c.idx);
+    T lxp = g2 * (wm(i + 1, j, k) - wmc), lxm = -g2 * (wmc - wm(i - 1, j, k));
+    awm(i + 1, j, k, g2 * avxp); awm(i - 1, j, k, g2 * avxm);
+    lwmc -= g2 * (avxp + avxm);
+    // bit2 (avxp (um(i,k)-um(i,k+1)) - avxm (um(i-1,k)-um(i-1,k+1))) / (dz dx)
+    const T g3 = lam * (T)(c.bit2 * c.idz * c.idx);
+    lxp += g3 * (um(i, j, k) - um(i, j, k + 1));
+    lxm -= g3 * (um(i - 1, j, k) - um(i - 1, j, k + 1));
+    aum(i, j, k, g3 * avxp); aum(i, j, k + 1, -g3 * avxp);
+    aum(i - 1, j, k, -g3 * avxm); aum(i - 1, j, k + 1, g3 * avxm);
+    // bit2 (avyp (wm(j+1)-wmc) - avym (wmc-wm(j-1))) / dy^2
+    const T g4 = lam * (T)(c.bit2 * c.idy * c.idy);
+    T lyp = g4 * (wm(i, j + 1, k) - wmc), lym = -g4 * (wmc - wm(i, j - 1, k));
+    awm(i, j + 1, k, g4 * avyp); awm(i, j - 1, k, g4 * avym);
+    lwmc -= g4 * (avyp + avym);
+    // bit2 (avyp (vm(j,k)-vm(j,k+1)) - avym (vm(j-1,k)-vm(j-1,k+1))) / (dz dy)
+    const T g5 = lam * (T)(c.bit2 * c.idz * c.idy);
+    lyp += g5 * (vm(i, j, k) - vm(i, j, k + 1));
+    lym -= g5 * (vm(i, j - 1, k) - vm(i, j - 1, k + 1));
+    avm(i, j, k, g5 * avyp); avm(i, j, k + 1, -g5 * avyp);
+    avm(i, j - 1, k, -g5 * avym); avm(i, j - 1, k + 1, g5 * avym);
+    {
+      // the four averaged vertical viscosities share s0, s1
+      const T qxp = (T).25 * lxp, qxm = (T).25 * lxm, qyp = (T).25 * lyp, qym = (T).25 * lym;
+      const T q01 = qxp + qxm + qyp + qym;
+      assv(i, j, k + 1, q01); assv(i, j, k, q01);
+      assv(i + 1, j, k + 1, qxp); assv(i + 1, j, k, qxp);
+      assv(i - 1, j, k + 1, qxm); assv(i - 1, j, k, qxm);
+      assv(i, j + 1, k + 1, qyp); assv(i, j + 1, k, qyp);
+      assv(i, j - 1, k + 1, qym); assv(i, j - 1, k, qym);
+    }
+    // buoyancy bit3 g (rhop(k) + rhop(k+1)) / rho0 / 2
+    {
+      const T b = (T)((double)ph.bit3 * ph.g / ph.rho0 / 2.0) * lam;
+      arhop(i, j, k, b); arhop(i, j, k + 1, b);
+    }
+    // sponge -0.5 alphaee wm
+    const double ae = fmax(alphae[i + (L.nx + 2) * j], (alphaw[k + 1] + alphaw[k]) / 2.0);
+    lwmc -= (T)(0.5 * ae) * lam;
+    awm(i, j, k, lwmc);
   }
-  // buoyancy bit3 g (rhop(k) + rhop(k+1)) / rho0 / 2
-  {
-    const T b = (T)((double)ph.bit3 * ph.g / ph.rho0 / 2.0) * lam;
-    arhop(i, j, k, b); arhop(i, j, k + 1, b);
+  if constexpr ((PART & 1) != 0) {
+    // advection
+    const T wc = w(i, j, k);
+    const T c1 = w(i, j, k + 1) + wc, c2 = wc + w(i, j, k - 1);
+    const T lc1 = -(T)c.coe_4z * (T)2 * c1 * lam, lc2 = (T)c.coe_4z * (T)2 * c2 * lam;
+    T lwc = lc1 + lc2;
+    aw(i, j, k + 1, lc1); aw(i, j, k - 1, lc2);
+    {  // -coe_4x ((w(i+1)+wc) (u(i,k)+u(i,k+1)) - (wc+w(i-1)) (u(i-1,k)+u(i-1,k+1))) + coe_9 (u ...)
+      const T P1 = w(i + 1, j, k) + wc, Q1 = u(i, j, k) + u(i, j, k + 1);
+      const T P2 = wc + w(i - 1, j, k), Q2 = u(i - 1, j, k) + u(i - 1, j, k + 1);
+      const T x = (T)c.coe_4x * lam, c9 = (T)c.coe_9 * lam;
+      aw(i + 1, j, k, -x * Q1); aw(i - 1, j, k, x * Q2);
+      lwc += -x * Q1 + x * Q2;
+      au(i, j, k, -x * P1 + c9); au(i, j, k + 1, -x * P1 + c9);
+      au(i - 1, j, k, x * P2 + c9); au(i - 1, j, k + 1, x * P2 + c9);
+    }
+    {  // -coe_4y ((w(j+1)+wc) (v(j,k)+v(j,k+1)) - (wc+w(j-1)) (v(j-1,k)+v(j-1,k+1)))
+      const T P1 = w(i, j + 1, k) + wc, Q1 = v(i, j, k) + v(i, j, k + 1);
+      const T P2 = wc + w(i, j - 1, k), Q2 = v(i, j - 1, k) + v(i, j - 1, k + 1);
+      const T y = (T)c.coe_4y * lam;
+      aw(i, j + 1, k, -y * Q1); aw(i, j - 1, k, y * Q2);
+      lwc += -y * Q1 + y * Q2;
+      av(i, j, k, -y * P1); av(i, j, k + 1, -y * P1);
+      av(i, j - 1, k, y * P2); av(i, j - 1, k + 1, y * P2);
+    }
+    aw(i, j, k, lwc);
   }
-  // sponge -0.5 alphaee wm
-  const double ae = fmax(alphae[i + (L.nx + 2) * j], (alphaw[k + 1] + alphaw[k]) / 2.0);
-  lwmc -= (T)(0.5 * ae) * lam;
-  awm(i, j, k, lwmc);
-  // advection
-  const T wc = w(i, j, k);
-  const T c1 = w(i, j, k + 1) + wc, c2 = wc + w(i, j, k - 1);
-  const T lc1 = -(T)c.coe_4z * (T)2 * c1 * lam, lc2 = (T)c.coe_4z * (T)2 * c2 * lam;
-  T lwc = lc1 + lc2;
-  aw(i, j, k + 1, lc1); aw(i, j, k - 1, lc2);
-  {  // -coe_4x ((w(i+1)+wc) (u(i,k)+u(i,k+1)) - (wc+w(i-1)) (u(i-1,k)+u(i-1,k+1))) + coe_9 (u ...)
-    const T P1 = w(i + 1, j, k) + wc, Q1 = u(i, j, k) + u(i, j, k + 1);
-    const T P2 = wc + w(i - 1, j, k), Q2 = u(i - 1, j, k) + u(i - 1, j, k + 1);
-    const T x = (T)c.coe_4x * lam, c9 = (T)c.coe_9 * lam;
-    aw(i + 1, j, k, -x * Q1); aw(i - 1, j, k, x * Q2);
-    lwc += -x * Q1 + x * Q2;
-    au(i, j, k, -x * P1 + c9); au(i, j, k + 1, -x * P1 + c9);
-    au(i - 1, j, k, x * P2 + c9); au(i - 1, j, k + 1, x * P2 + c9);
-  }
-  {  // -coe_4y ((w(j+1)+wc) (v(j,k)+v(j,k+1)) - (wc+w(j-1)) (v(j-1,k)+v(j-1,k+1)))
-    const T P1 = w(i, j + 1, k) + wc, Q1 = v(i, j, k) + v(i, j, k + 1);
-    const T P2 = wc + w(i, j - 1, k), Q2 = v(i, j - 1, k) + v(i, j - 1, k + 1);
-    const T y = (T)c.coe_4y * lam;
-    aw(i, j + 1, k, -y * Q1); aw(i, j - 1, k, y * Q2);
-    lwc += -y * Q1 + y * Q2;
-    av(i, j, k, -y * P1); av(i, j, k + 1, -y * P1);
-    av(i, j - 1, k, y * P2); av(i, j - 1, k + 1, y * P2);
-  }
-  aw(i, j, k, lwc);
 }
 
 // ---------------------------------------------------------------------------

@@ -245,8 +245,13 @@ __global__ void __launch_bounds__(256) k_adj_forces(Layout L, Box3 b, StepArgs s
 }
 
 
-// FORCES, interior cells i, j, k >= 1 of all three components as a tile
-template <typename T>
+// FORCES, interior cells i, j, k >= 1 of all three components as a tile.
+// PART 1 takes the advection terms (rings of u, v, w), PART 2 the
+// diffusion, buoyancy and sponge terms (rings of um, vm, wm, ss, ssv, rhop)
+// and then clears the output adjoints both read; two launches with a third
+// and two thirds of the LDS of one that takes all terms keep more waves in
+// flight on each SIMD.
+template <typename T, int PART>
 __global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs s, Phys ph, FlowSens fs,
                                                               FieldPtrs<T> f, const T *__restrict__ ss0,
                                                               const T *__restrict__ ssv0, FieldPtrs<T> lam,
@@ -255,32 +260,40 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs
                                                               const double *__restrict__ alphav,
                                                               const double *__restrict__ alphae0,
                                                               const double *__restrict__ alphaw, double *gflow, int kc) {
-  typedef AdjRing<T, ATI, ATJ, 9> R;
+  static_assert(PART == 1 || PART == 2, "one part per launch");
+  constexpr int NF = PART == 1 ? 3 : 6;
+  typedef AdjRing<T, ATI, ATJ, NF> R;
   __shared__ R sh;
   const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
   const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
-  T *const dst[9] = {lam.u, lam.v, lam.w, lam.um, lam.vm, lam.wm, lam.ss, lam.ssv, lam.rhop};
+  T *const dst1[3] = {lam.u, lam.v, lam.w};
+  T *const dst2[6] = {lam.um, lam.vm, lam.wm, lam.ss, lam.ssv, lam.rhop};
+  T *const *dst = PART == 1 ? dst1 : dst2;
   const RView<T> U{f.u, L}, V{f.v, L}, W{f.w, L}, UM{f.um, L}, VM{f.vm, L}, WM{f.wm, L}, SS{ss0, L}, SV{ssv0, L},
       RH{f.rhop, L};
   typedef LAcc<T, R::SI, R::S> A;
-  const A au{sh.a[0][0], i0 - 1, j0 - 1}, av{sh.a[1][0], i0 - 1, j0 - 1}, aw{sh.a[2][0], i0 - 1, j0 - 1},
-      aum{sh.a[3][0], i0 - 1, j0 - 1}, avm{sh.a[4][0], i0 - 1, j0 - 1}, awm{sh.a[5][0], i0 - 1, j0 - 1},
-      ass{sh.a[6][0], i0 - 1, j0 - 1}, assv{sh.a[7][0], i0 - 1, j0 - 1}, arh{sh.a[8][0], i0 - 1, j0 - 1};
+  // the ring of each accumulated field; the other part's accumulators are
+  // never called (their terms are compiled out) and point at ring 0
+  auto ring = [&](int q1, int q2) { return A{sh.a[PART == 1 ? q1 : q2][0], i0 - 1, j0 - 1}; };
+  const A au = ring(0, 0), av = ring(1, 0), aw = ring(2, 0), aum = ring(0, 0), avm = ring(0, 1), awm = ring(0, 2),
+          ass = ring(0, 3), assv = ring(0, 4), arh = ring(0, 5);
   double gf = 0.0;
   adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
     const T lx = F3(lam.fx, i, j, k), ly = F3(lam.fy, i, j, k), lz = F3(lam.fz, i, j, k);
-    F3(lam.fx, i, j, k) = (T)0;  // consumed (FORCES overwrites them)
-    F3(lam.fy, i, j, k) = (T)0;
-    F3(lam.fz, i, j, k) = (T)0;
+    if (PART == 2) {
+      F3(lam.fx, i, j, k) = (T)0;  // consumed (FORCES overwrites them)
+      F3(lam.fy, i, j, k) = (T)0;
+      F3(lam.fz, i, j, k) = (T)0;
+    }
     if (lx != (T)0)
-      adj_forces_x<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphau, lx, au, av, aw, aum, avm, awm,
-                      ass, assv, gf);
+      adj_forces_x<T, PART>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphau, lx, au, av, aw, aum,
+                            avm, awm, ass, assv, gf);
     if (ly != (T)0)
-      adj_forces_y<T>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphav, ly, au, av, aw, aum, avm, awm,
-                      ass, assv, gf);
+      adj_forces_y<T, PART>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphav, ly, au, av, aw, aum,
+                            avm, awm, ass, assv, gf);
     if (lz != (T)0)
-      adj_forces_z<T>(L, i, j, k, s, ph, U, V, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, lz, au, av, aw, aum,
-                      avm, awm, ass, assv, arh);
+      adj_forces_z<T, PART>(L, i, j, k, s, ph, U, V, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, lz, au, av, aw,
+                            aum, avm, awm, ass, assv, arh);
   });
   block_add(gf, gflow);
 }
@@ -935,7 +948,10 @@ static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
                        ctx->aux.alphaw, A->d_sc);
   const int kc = adj_chunk(L);
   const dim3 tg = adj_tile_grid<ATI, ATJ>(L, kc);
-  hipLaunchKernelGGL(k_adj_forces_tile<T>, tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
+  hipLaunchKernelGGL((k_adj_forces_tile<T, 1>), tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
+                     fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
+                     ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc, kc);
+  hipLaunchKernelGGL((k_adj_forces_tile<T, 2>), tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
                      ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc, kc);
   // lam.fx, fy, fz are nonzero only where FORCES writes (MOMENTUM's and
@@ -1359,8 +1375,12 @@ static void set_host_state(hc_ctx *ctx, const HostState &h) {
 static void *slot_ptr(hc_ctx *ctx, size_t slot, int f) {
   return (char *)ctx->adj->tape + (slot * NTAPE + f) * ctx->L.n * ctx->esz;
 }
-static void save_slot(hc_ctx *ctx, size_t slot) {
-  for (int f = 0; f < NTAPE; ++f)
+// a reverse-sweep tape slot holds the 11 fields the step reads: its state
+// is read by the adjoint (adj_head, adj_step) and never restored to run
+// forward from, so the up, vp, wp buffers are not kept
+static const int NTAPE_REV = 11;
+static void save_slot(hc_ctx *ctx, size_t slot, int nf = NTAPE) {
+  for (int f = 0; f < nf; ++f)
     hipMemcpyAsync(slot_ptr(ctx, slot, f), ctx->f[TAPE_FIELDS[f]], ctx->L.n * ctx->esz, hipMemcpyDeviceToDevice, ctx->st);
 }
 static void load_slot(hc_ctx *ctx, size_t slot) {
@@ -1655,7 +1675,7 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
     int rs;
     if (n >= alast && n < nsteps - 1) {
       rs = fwd_step(ctx, m0 + n, n - alast);
-      save_slot(ctx, tape0 + (n + 1 - alast));
+      save_slot(ctx, tape0 + (n + 1 - alast), NTAPE_REV);
       tp[n + 1 - alast] = host_state(ctx);
     } else {
       rs = fwd_step(ctx, m0 + n);
@@ -1696,7 +1716,7 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
       for (int n = a; n < b - 1; ++n) {
         const int rs = fwd_step(ctx, m0 + n, n - a);
         if (rs) return rs;
-        save_slot(ctx, tape0 + (n + 1 - a));
+        save_slot(ctx, tape0 + (n + 1 - a), NTAPE_REV);
         tp[n + 1 - a] = host_state(ctx);
       }
     }
