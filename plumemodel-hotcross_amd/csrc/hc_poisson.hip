@@ -1070,59 +1070,79 @@ __global__ void __launch_bounds__(256) k_padj_plane(Layout L, const T *__restric
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// transpose of k_rhs: one thread per face (i,j,k), i,j,k = 0..n
-template <typename T>
-__global__ void __launch_bounds__(256) k_padj_rhs(Layout L, Box b, StepArgs s, FlowSens fs,
-                                                  const T *__restrict__ pad, const double *__restrict__ dsum,
-                                                  T *__restrict__ lum, T *__restrict__ lvm, T *__restrict__ lwm,
-                                                  T *__restrict__ lw, T *__restrict__ lfx, T *__restrict__ lfy,
-                                                  T *__restrict__ lfz, double *gflow) {
+// the quotients k_padj_rhs uses, formed once on the host (the same
+// correctly rounded FP64 values each thread would form)
+struct PadjConsts {
+  double rdx, rdy, rdz, rdt;  // 1/dx, 1/dy, 1/dz, 1/dt
+  double dlx, dly, dlz;       // 1 / ((n d - 0) / n) per direction
+  double cm;                  // 1 / dt / leap
+  double dbdu, dbdv;          // -(d ubkg - d ubkg_m) / dt, the same for v
+};
+static PadjConsts padj_consts(const Layout &L, const StepArgs &s, const FlowSens &fs) {
+  PadjConsts c;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y),
-            k = (int)blockIdx.z;
-  (void)b;
+  const double dx = s.dx, dy = s.dy, dz = s.dz, dt = s.dt;
+  c.rdx = 1.0 / dx; c.rdy = 1.0 / dy; c.rdz = 1.0 / dz; c.rdt = 1.0 / dt;
+  c.dlx = 1.0 / ((nx * dx - 0.0) / nx); c.dly = 1.0 / ((ny * dy - 0.0) / ny); c.dlz = 1.0 / ((nz * dz - 0.0) / nz);
+  c.cm = 1.0 / dt / s.leap;
+  c.dbdu = -(fs.dubkg - fs.dubkg_m) / dt;
+  c.dbdv = -(fs.dvbkg - fs.dvbkg_m) / dt;
+  return c;
+}
+
+// transpose of k_rhs: one thread per face (i,j,k), i,j,k = 0..n; the 256
+// threads of a block take 256 consecutive (i,j) of a plane (i fastest, rows
+// of nx+1 wrapping), blockIdx.z = k
+template <typename T>
+__global__ void __launch_bounds__(256) k_padj_rhs(Layout L, PadjConsts c, const T *__restrict__ pad,
+                                                  const double *__restrict__ dsum, T *__restrict__ lum,
+                                                  T *__restrict__ lvm, T *__restrict__ lwm, T *__restrict__ lw,
+                                                  T *__restrict__ lfx, T *__restrict__ lfy, T *__restrict__ lfz,
+                                                  double *gflow) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const int t = (int)(blockIdx.x * 256u + threadIdx.y * 64u + threadIdx.x);
+  const int j = t / (nx + 1), i = t - j * (nx + 1), k = (int)blockIdx.z;
   double gf = 0.0;
-  if (i <= nx && j <= ny) {
+  if (j <= ny) {
     const T corr = (T)(dsum[0] / ((double)ny * nz));
     auto LR = [&](int ii, int jj, int kk) -> T {
       return (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny && kk >= 1 && kk <= nz) ? F3(pad, ii, jj, kk) - corr : (T)0;
     };
-    const double dx = s.dx, dy = s.dy, dz = s.dz, dt = s.dt;
-    const double dlx = 1.0 / ((nx * dx - 0.0) / nx), dly = 1.0 / ((ny * dy - 0.0) / ny), dlz = 1.0 / ((nz * dz - 0.0) / nz);
-    const T cm = (T)(1.0 / dt / s.leap);
+    const double dlx = c.dlx, dly = c.dly, dlz = c.dlz;
+    const T cm = (T)c.cm;
     if (j >= 1 && k >= 1) {
       const T d = LR(i, j, k) - LR(i + 1, j, k);
-      F3(lum, i, j, k) += d * (T)(1.0 / dx) * cm;
-      T f = d * (T)(1.0 / dx);
-      const double dbd = -(fs.dubkg - fs.dubkg_m) / dt;
+      F3(lum, i, j, k) += d * (T)c.rdx * cm;
+      T f = d * (T)c.rdx;
+      const double dbd = c.dbdu;
       if (i == 0) { f += LR(1, j, k) * (T)dlx; gf += (double)LR(1, j, k) * dlx * dbd; }
       if (i == nx) { f -= LR(nx, j, k) * (T)dlx; gf -= (double)LR(nx, j, k) * dlx * dbd; }
       F3(lfx, i, j, k) += f;
     }
     if (i >= 1 && k >= 1) {
       const T d = LR(i, j, k) - LR(i, j + 1, k);
-      F3(lvm, i, j, k) += d * (T)(1.0 / dy) * cm;
-      T f = d * (T)(1.0 / dy);
-      const double dbd = -(fs.dvbkg - fs.dvbkg_m) / dt;
+      F3(lvm, i, j, k) += d * (T)c.rdy * cm;
+      T f = d * (T)c.rdy;
+      const double dbd = c.dbdv;
       if (j == 0) { f += LR(i, 1, k) * (T)dly; gf += (double)LR(i, 1, k) * dly * dbd; }
       if (j == ny) { f -= LR(i, ny, k) * (T)dly; gf -= (double)LR(i, ny, k) * dly * dbd; }
       F3(lfy, i, j, k) += f;
     }
     if (i >= 1 && j >= 1) {
       const T d = LR(i, j, k) - LR(i, j, k + 1);
-      T wmv = d * (T)(1.0 / dz) * cm;
-      T f = d * (T)(1.0 / dz);
+      T wmv = d * (T)c.rdz * cm;
+      T f = d * (T)c.rdz;
       if (k == 0) {
         const T a = LR(i, j, 1) * (T)dlz;
         f += a;
-        F3(lw, i, j, 0) -= a * (T)(1.0 / dt);
-        wmv += a * (T)(1.0 / dt);
+        F3(lw, i, j, 0) -= a * (T)c.rdt;
+        wmv += a * (T)c.rdt;
       }
       if (k == nz) {
         const T a = LR(i, j, nz) * (T)dlz;
         f -= a;
-        F3(lw, i, j, nz) += a * (T)(1.0 / dt);
-        wmv -= a * (T)(1.0 / dt);
+        F3(lw, i, j, nz) += a * (T)c.rdt;
+        wmv -= a * (T)c.rdt;
       }
       F3(lwm, i, j, k) += wmv;
       F3(lfz, i, j, k) += f;
@@ -1177,8 +1197,8 @@ void launch_pressure_adj(hipStream_t st, const Layout &L, const StepArgs &s, con
                        zero, s, ph, lam.p, f.fx, f.w, f.wm, f.fz, aux.pgradx);
   hipLaunchKernelGGL(k_padj_plane<T>, dim3(nz), dim3(256), 0, st, L, (const T *)lam.p, part);
   hipLaunchKernelGGL(k_sum1, dim3(1), dim3(SUM1_THREADS), 0, st, nz, part, part + nz);
-  Box b{0, nx, 0, ny, 0, nz};
-  hipLaunchKernelGGL(k_padj_rhs<T>, dim3((nx + 1 + 63) / 64, (ny + 1 + 3) / 4, nz + 1), dim3(64, 4), 0, st, L, b, s, fs,
+  const unsigned nf = (unsigned)(((long)(nx + 1) * (ny + 1) + 255) / 256);
+  hipLaunchKernelGGL(k_padj_rhs<T>, dim3(nf, 1, nz + 1), dim3(64, 4), 0, st, L, padj_consts(L, s, fs),
                      (const T *)lam.p, (const double *)(part + nz), lam.um, lam.vm, lam.wm, lam.w, lam.fx, lam.fy,
                      lam.fz, gflow);
   hipMemsetAsync(lam.p, 0, L.n * sizeof(T), st);
