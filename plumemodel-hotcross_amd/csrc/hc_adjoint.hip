@@ -160,6 +160,25 @@ struct LRel {
   }
 };
 
+// values of a field at offsets (-1..1)^3 around a cell (i,j,k), loaded
+// into registers up front and read by absolute indices; the formulas call
+// it with the cell's indices plus constants, which fold after inlining
+// (entries never loaded are never read)
+template <typename T>
+struct NbAt {
+  T v[27];
+  int i, j, k;
+  __device__ NbAt(int i_, int j_, int k_) : i(i_), j(j_), k(k_) {}
+  __device__ T operator()(int a, int b, int c) const { return v[(a - i + 1) + 3 * (b - j + 1) + 9 * (c - k + 1)]; }
+  template <int N>
+  __device__ void load(const T *p, const Layout &L, const int (&o)[N][3]) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[(o[q][0] + 1) + 3 * (o[q][1] + 1) + 9 * (o[q][2] + 1)] = p[IX(L, i + o[q][0], j + o[q][1], k + o[q][2])];
+  }
+};
+// the cell and its six face neighbours, and with two edge neighbours more
+#define HC_NB7 {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}
+
 template <typename T, int TI, int TJ, int NF, class Body>
 __device__ inline void adj_march(AdjRing<T, TI, TJ, NF> &sh, const Layout &L, T *const *dst, int i0, int j0, int ka,
                                  int kb, Body &&body) {
@@ -277,8 +296,7 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs
   T *const dst1[3] = {lam.u, lam.v, lam.w};
   T *const dst2[6] = {lam.um, lam.vm, lam.wm, lam.ss, lam.ssv, lam.rhop};
   T *const *dst = PART == 1 ? dst1 : dst2;
-  const RView<T> U{f.u, L}, V{f.v, L}, W{f.w, L}, UM{f.um, L}, VM{f.vm, L}, WM{f.wm, L}, SS{ss0, L}, SV{ssv0, L},
-      RH{f.rhop, L};
+  const RView<T> RH{f.rhop, L};
   typedef LAcc<T, R::SI, R::S> A;
   // the ring of each accumulated field; the other part's accumulators are
   // never called (their terms are compiled out) and point at ring 0
@@ -293,15 +311,52 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs
       F3(lam.fy, i, j, k) = (T)0;
       F3(lam.fz, i, j, k) = (T)0;
     }
-    if (lx != (T)0)
-      adj_forces_x<T, PART>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphau, lx, au, av, aw, aum,
-                            avm, awm, ass, assv, gf);
-    if (ly != (T)0)
-      adj_forces_y<T, PART>(L, i, j, k, s, ph, fs, U, V, W, UM, VM, WM, SS, SV, azzmin, alphav, ly, au, av, aw, aum,
-                            avm, awm, ass, assv, gf);
-    if (lz != (T)0)
-      adj_forces_z<T, PART>(L, i, j, k, s, ph, U, V, W, UM, VM, WM, SS, SV, RH, azzmin, alphae0, alphaw, lz, au, av, aw,
-                            aum, avm, awm, ass, assv, arh);
+    auto body = [&](const auto &u, const auto &v, const auto &w, const auto &um, const auto &vm, const auto &wm,
+                    const auto &ss, const auto &sv) {
+      if (lx != (T)0)
+        adj_forces_x<T, PART>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphau, lx, au, av, aw, aum,
+                              avm, awm, ass, assv, gf);
+      if (ly != (T)0)
+        adj_forces_y<T, PART>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphav, ly, au, av, aw, aum,
+                              avm, awm, ass, assv, gf);
+      if (lz != (T)0)
+        adj_forces_z<T, PART>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, sv, RH, azzmin, alphae0, alphaw, lz, au, av,
+                              aw, aum, avm, awm, ass, assv, arh);
+    };
+    // the inputs the part's three components read, loaded before their
+    // zero tests so that a cell's loads are in flight together; not in the
+    // FP64 diffusion part, whose 49 values would take its registers from
+    // 143 to 189 and its waves per SIMD from 3 to 2 (it reads them as the
+    // formulas need them)
+    if constexpr (PART == 2 && sizeof(T) == 8) {
+      const RView<T> U{f.u, L}, V{f.v, L}, W{f.w, L}, UM{f.um, L}, VM{f.vm, L}, WM{f.wm, L}, SS{ss0, L},
+          SV{ssv0, L};
+      body(U, V, W, UM, VM, WM, SS, SV);
+    } else {
+      NbAt<T> u(i, j, k), v(i, j, k), w(i, j, k), um(i, j, k), vm(i, j, k), wm(i, j, k), ss(i, j, k), sv(i, j, k);
+      if constexpr (PART == 1) {
+        static constexpr int ou[9][3] = {HC_NB7, {-1, 1, 0}, {-1, 0, 1}};
+        static constexpr int ov[9][3] = {HC_NB7, {1, -1, 0}, {0, -1, 1}};
+        static constexpr int ow[9][3] = {HC_NB7, {1, 0, -1}, {0, 1, -1}};
+        u.load(f.u, L, ou);
+        v.load(f.v, L, ov);
+        w.load(f.w, L, ow);
+      } else {
+        static constexpr int oum[9][3] = {HC_NB7, {-1, 1, 0}, {-1, 0, 1}};
+        static constexpr int ovm[9][3] = {HC_NB7, {1, -1, 0}, {0, -1, 1}};
+        static constexpr int owm[9][3] = {HC_NB7, {1, 0, -1}, {0, 1, -1}};
+        static constexpr int oss[9][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
+                                          {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {0, 0, 1}};
+        static constexpr int osv[13][3] = {HC_NB7,     {1, 0, 1},  {1, 0, -1}, {0, 1, 1},
+                                           {0, 1, -1}, {-1, 0, 1}, {0, -1, 1}};
+        um.load(f.um, L, oum);
+        vm.load(f.vm, L, ovm);
+        wm.load(f.wm, L, owm);
+        ss.load(ss0, L, oss);
+        sv.load(ssv0, L, osv);
+      }
+      body(u, v, w, um, vm, wm, ss, sv);
+    }
   });
   block_add(gf, gflow);
 }
@@ -717,8 +772,23 @@ static dim3 donor_grid(const Layout &L, int &kc) {
   return dim3((unsigned)nbp, (unsigned)((L.nz + 2 + kc - 1) / kc));
 }
 
+// values of a field at offsets (-1..1)^3 around a cell, loaded into
+// registers up front; a reader called with constant offsets after inlining
+// (entries never loaded are never read)
+template <typename T>
+struct Nb27 {
+  T v[27];
+  __device__ T operator()(int a, int b, int c) const { return v[(a + 1) + 3 * (b + 1) + 9 * (c + 1)]; }
+  __device__ void load(const T *p, const Layout &L, int i, int j, int k, int a, int b, int c) {
+    v[(a + 1) + 3 * (b + 1) + 9 * (c + 1)] = p[IX(L, i + a, j + b, k + c)];
+  }
+};
+
 // the antidiffusive velocities reversed on the faces of the interior cells
-// (ud, vd, wd at (i, j, k) with i, j, k >= 1) as a tile
+// (ud, vd, wd at (i, j, k) with i, j, k >= 1) as a tile.  The values the
+// three face formulas read (cc at 16 offsets, U, V, W at 6 each) are loaded
+// before their zero tests, so that a cell's loads are in flight together
+// rather than in three rounds, one per face.
 template <typename T>
 __global__ void __launch_bounds__(ATI *ATJ) k_adj_anti_tile(Layout L, const T *__restrict__ cc,
                                                             const T *__restrict__ U, const T *__restrict__ V,
@@ -732,12 +802,28 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_anti_tile(Layout L, const T *_
   T *const dst[4] = {lcc, lU, lV, lW};
   typedef LRel<T, R::SI, R::S> A;
   adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
-    const RelR<T> C{cc, L, i, j, k}, Uf{U, L, i, j, k}, Vf{V, L, i, j, k}, Wf{W, L, i, j, k};
+    const T lu = F3(lud, i, j, k), lv = F3(lvd, i, j, k), lw = F3(lwd, i, j, k);
+    Nb27<T> C, Uf, Vf, Wf;
+    // cc: the cell, its six face neighbours and nine edge neighbours
+    static constexpr int oc[16][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},
+                                      {0, 0, -1}, {1, 1, 0},  {1, -1, 0}, {-1, 1, 0}, {1, 0, -1}, {1, 0, 1},
+                                      {-1, 0, 1}, {0, 1, -1}, {0, 1, 1},  {0, -1, 1}};
+    static constexpr int ou[6][3] = {{0, 0, 0}, {0, 1, 0}, {-1, 1, 0}, {-1, 0, 0}, {0, 0, 1}, {-1, 0, 1}};
+    static constexpr int ov[6][3] = {{0, 0, 0}, {1, 0, 0}, {1, -1, 0}, {0, -1, 0}, {0, 0, 1}, {0, -1, 1}};
+    static constexpr int ow[6][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, -1}, {0, 0, -1}, {0, 1, 0}, {0, 1, -1}};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) C.load(cc, L, i, j, k, oc[q][0], oc[q][1], oc[q][2]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      Uf.load(U, L, i, j, k, ou[q][0], ou[q][1], ou[q][2]);
+      Vf.load(V, L, i, j, k, ov[q][0], ov[q][1], ov[q][2]);
+      Wf.load(W, L, i, j, k, ow[q][0], ow[q][1], ow[q][2]);
+    }
     const A ac{sh.a[0][0], i0 - 1, j0 - 1, i, j, k}, aU{sh.a[1][0], i0 - 1, j0 - 1, i, j, k},
         aV{sh.a[2][0], i0 - 1, j0 - 1, i, j, k}, aW{sh.a[3][0], i0 - 1, j0 - 1, i, j, k};
-    adj_anti_u<T>(C, Uf, Vf, Wf, F3(lud, i, j, k), ac, aU, aV, aW);
-    adj_anti_v<T>(C, Uf, Vf, Wf, F3(lvd, i, j, k), ac, aU, aV, aW);
-    adj_anti_w<T>(C, Uf, Vf, Wf, F3(lwd, i, j, k), ac, aU, aV, aW);
+    adj_anti_u<T>(C, Uf, Vf, Wf, lu, ac, aU, aV, aW);
+    adj_anti_v<T>(C, Uf, Vf, Wf, lv, ac, aU, aV, aW);
+    adj_anti_w<T>(C, Uf, Vf, Wf, lw, ac, aU, aV, aW);
   });
 }
 
