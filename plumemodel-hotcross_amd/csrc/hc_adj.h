@@ -62,19 +62,22 @@ HC_HD inline AdjConsts adj_consts(const StepArgs &s, const Phys &ph) {
   return c;
 }
 
-// PART selects the terms of a component: 1 advection and the Coriolis and
-// crossflow terms (the adjoints of u, v, w and d/d flow through fbx, fby),
-// 2 diffusion, buoyancy and sponge (the adjoints of um, vm, wm, ss, ssv,
-// rhop and d/d flow through the sponge's background), 3 both
-template <typename T, int PART = 3, class R, class A>
+// PART selects the terms of a component: bit 1 advection and the Coriolis
+// and crossflow terms (the adjoints of u, v, w and d/d flow through fbx,
+// fby), bit 2 or 4 diffusion, buoyancy and sponge, whose adjoints go to um,
+// vm, wm (accumulators of type AM, with d/d flow through the sponge's
+// background under bit 2) and to ss, ssv, rhop (type AS); a caller that
+// takes one group passes accumulators of the other that discard their
+// values, and the terms that only feed those drop out
+template <typename T, int PART = 7, class R, class A, class AM, class AS>
 HC_HD inline void adj_forces_x(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph,
                                const FlowSens &fs, const R &u, const R &v, const R &w, const R &um, const R &vm,
                                const R &wm, const R &ss, const R &ssv, const double *azzmin, const double *alphau,
-                               T lam, const A &au, const A &av, const A &aw, const A &aum, const A &avm,
-                               const A &awm, const A &ass, const A &assv, double &gflow) {
+                               T lam, const A &au, const A &av, const A &aw, const AM &aum, const AM &avm,
+                               const AM &awm, const AS &ass, const AS &assv, double &gflow) {
   (void)ssv; (void)azzmin;
   const AdjConsts c = adj_consts(s, ph);
-  if constexpr ((PART & 2) != 0) {
+  if constexpr ((PART & 6) != 0) {
     const T axu_p = ss(i + 1, j, k) + (T)c.axmin, axu_c = ss(i, j, k) + (T)c.axmin;
     const T ayu_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + (T)c.aymin;
     const T ayu_m = (T).25 * (ss(i, j - 1, k) + ss(i + 1, j - 1, k) + ss(i, j, k) + ss(i + 1, j, k)) + (T)c.aymin;
@@ -120,7 +123,7 @@ HC_HD inline void adj_forces_x(const Layout &L, int i, int j, int k, const StepA
     // sponge -0.5 alphau (um - ubkg_m)
     const double au_ = alphau[(i + 1) + (L.nx + 3) * j];
     lumc -= (T)(0.5 * au_) * lam;
-    gflow += 0.5 * au_ * (double)lam * fs.dubkg_m;
+    if constexpr ((PART & 2) != 0) gflow += 0.5 * au_ * (double)lam * fs.dubkg_m;
     aum(i, j, k, lumc);
   }
   if constexpr ((PART & 1) != 0) {
@@ -153,14 +156,14 @@ HC_HD inline void adj_forces_x(const Layout &L, int i, int j, int k, const StepA
   }
 }
 
-template <typename T, int PART = 3, class R, class A>
+template <typename T, int PART = 7, class R, class A, class AM, class AS>
 HC_HD inline void adj_forces_y(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph,
                                const FlowSens &fs, const R &u, const R &v, const R &w, const R &um, const R &vm,
                                const R &wm, const R &ss, const R &ssv, const double *azzmin, const double *alphav,
-                               T lam, const A &au, const A &av, const A &aw, const A &aum, const A &avm,
-                               const A &awm, const A &ass, const A &assv, double &gflow) {
+                               T lam, const A &au, const A &av, const A &aw, const AM &aum, const AM &avm,
+                               const AM &awm, const AS &ass, const AS &assv, double &gflow) {
   const AdjConsts c = adj_consts(s, ph);
-  if constexpr ((PART & 2) != 0) {
+  if constexpr ((PART & 6) != 0) {
     const T axv_c = (T).25 * (ss(i, j, k) + ss(i + 1, j, k) + ss(i, j + 1, k) + ss(i + 1, j + 1, k)) + (T)c.axmin;
     const T axv_m = (T).25 * (ss(i - 1, j, k) + ss(i, j, k) + ss(i - 1, j + 1, k) + ss(i, j + 1, k)) + (T)c.axmin;
     const T ayv_p = ss(i, j + 1, k) + (T)c.aymin, ayv_c = ss(i, j, k) + (T)c.aymin;
@@ -206,7 +209,7 @@ HC_HD inline void adj_forces_y(const Layout &L, int i, int j, int k, const StepA
     // sponge -0.5 alphav (vm - vbkg_m)
     const double av_ = alphav[i + (L.nx + 2) * (j + 1)];
     lvmc -= (T)(0.5 * av_) * lam;
-    gflow += 0.5 * av_ * (double)lam * fs.dvbkg_m;
+    if constexpr ((PART & 2) != 0) gflow += 0.5 * av_ * (double)lam * fs.dvbkg_m;
     avm(i, j, k, lvmc);
   }
   if constexpr ((PART & 1) != 0) {
@@ -239,16 +242,16 @@ HC_HD inline void adj_forces_y(const Layout &L, int i, int j, int k, const StepA
   }
 }
 
-template <typename T, int PART = 3, class R, class RR, class A>
+template <typename T, int PART = 7, class R, class RR, class A, class AM, class AS>
 HC_HD inline void adj_forces_z(const Layout &L, int i, int j, int k, const StepArgs &s, const Phys &ph,
                                const R &u, const R &v, const R &w, const R &um, const R &vm, const R &wm,
                                const R &ss, const R &ssv, const RR &rhop, const double *azzmin,
                                const double *alphae, const double *alphaw, T lam, const A &au, const A &av,
-                               const A &aw, const A &aum, const A &avm, const A &awm, const A &ass, const A &assv,
-                               const A &arhop) {
+                               const A &aw, const AM &aum, const AM &avm, const AM &awm, const AS &ass,
+                               const AS &assv, const AS &arhop) {
   (void)rhop;
   const AdjConsts c = adj_consts(s, ph);
-  if constexpr ((PART & 2) != 0) {
+  if constexpr ((PART & 6) != 0) {
     const T ahp = ss(i, j, k + 1) + (T)c.axmin, ahm = ss(i, j, k) + (T)c.axmin;
     const T zk = (T)azzmin[k];
     const T s0 = ssv(i, j, k + 1), s1 = ssv(i, j, k);

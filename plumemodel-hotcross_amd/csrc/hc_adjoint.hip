@@ -272,12 +272,18 @@ __global__ void __launch_bounds__(256) k_adj_forces(Layout L, Box3 b, StepArgs s
 }
 
 
-// FORCES, interior cells i, j, k >= 1 of all three components as a tile.
-// PART 1 takes the advection terms (rings of u, v, w), PART 2 the
-// diffusion, buoyancy and sponge terms (rings of um, vm, wm, ss, ssv, rhop)
-// and then clears the output adjoints both read; two launches with a third
-// and two thirds of the LDS of one that takes all terms keep more waves in
-// flight on each SIMD.
+// an accumulator that discards its values (the terms that only feed it
+// drop out of the formulas)
+struct NullAcc {
+  template <typename T>
+  __device__ void operator()(int, int, int, T) const {}
+};
+
+// FORCES, interior cells i, j, k >= 1 of all three components as a tile,
+// in three launches of three LDS rings each (24.6 KB per block instead of
+// 73.6 KB for all nine): PART 1 the advection terms (rings of u, v, w), 2
+// and 4 the diffusion, buoyancy and sponge terms into um, vm, wm and into
+// ss, ssv, rhop; the last launch clears the output adjoints all three read.
 template <typename T, int PART>
 __global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs s, Phys ph, FlowSens fs,
                                                               FieldPtrs<T> f, const T *__restrict__ ss0,
@@ -287,75 +293,78 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_forces_tile(Layout L, StepArgs
                                                               const double *__restrict__ alphav,
                                                               const double *__restrict__ alphae0,
                                                               const double *__restrict__ alphaw, double *gflow, int kc) {
-  static_assert(PART == 1 || PART == 2, "one part per launch");
-  constexpr int NF = PART == 1 ? 3 : 6;
-  typedef AdjRing<T, ATI, ATJ, NF> R;
+  static_assert(PART == 1 || PART == 2 || PART == 4, "one part per launch");
+  typedef AdjRing<T, ATI, ATJ, 3> R;
   __shared__ R sh;
   const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
   const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
   T *const dst1[3] = {lam.u, lam.v, lam.w};
-  T *const dst2[6] = {lam.um, lam.vm, lam.wm, lam.ss, lam.ssv, lam.rhop};
-  T *const *dst = PART == 1 ? dst1 : dst2;
+  T *const dst2[3] = {lam.um, lam.vm, lam.wm};
+  T *const dst4[3] = {lam.ss, lam.ssv, lam.rhop};
+  T *const *dst = PART == 1 ? dst1 : PART == 2 ? dst2 : dst4;
   const RView<T> RH{f.rhop, L};
   typedef LAcc<T, R::SI, R::S> A;
-  // the ring of each accumulated field; the other part's accumulators are
-  // never called (their terms are compiled out) and point at ring 0
-  auto ring = [&](int q1, int q2) { return A{sh.a[PART == 1 ? q1 : q2][0], i0 - 1, j0 - 1}; };
-  const A au = ring(0, 0), av = ring(1, 0), aw = ring(2, 0), aum = ring(0, 0), avm = ring(0, 1), awm = ring(0, 2),
-          ass = ring(0, 3), assv = ring(0, 4), arh = ring(0, 5);
+  const A a0{sh.a[0][0], i0 - 1, j0 - 1}, a1{sh.a[1][0], i0 - 1, j0 - 1}, a2{sh.a[2][0], i0 - 1, j0 - 1};
+  const NullAcc nul;
   double gf = 0.0;
   adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
     const T lx = F3(lam.fx, i, j, k), ly = F3(lam.fy, i, j, k), lz = F3(lam.fz, i, j, k);
-    if (PART == 2) {
+    if (PART == 4) {
       F3(lam.fx, i, j, k) = (T)0;  // consumed (FORCES overwrites them)
       F3(lam.fy, i, j, k) = (T)0;
       F3(lam.fz, i, j, k) = (T)0;
     }
-    auto body = [&](const auto &u, const auto &v, const auto &w, const auto &um, const auto &vm, const auto &wm,
-                    const auto &ss, const auto &sv) {
+    // the inputs the part reads, loaded before the zero tests of the three
+    // components so that a cell's loads are in flight together
+    NbAt<T> u(i, j, k), v(i, j, k), w(i, j, k), um(i, j, k), vm(i, j, k), wm(i, j, k), ss(i, j, k), sv(i, j, k);
+    if constexpr (PART == 1) {
+      static constexpr int ou[9][3] = {HC_NB7, {-1, 1, 0}, {-1, 0, 1}};
+      static constexpr int ov[9][3] = {HC_NB7, {1, -1, 0}, {0, -1, 1}};
+      static constexpr int ow[9][3] = {HC_NB7, {1, 0, -1}, {0, 1, -1}};
+      u.load(f.u, L, ou);
+      v.load(f.v, L, ov);
+      w.load(f.w, L, ow);
       if (lx != (T)0)
-        adj_forces_x<T, PART>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphau, lx, au, av, aw, aum,
-                              avm, awm, ass, assv, gf);
+        adj_forces_x<T, 1>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphau, lx, a0, a1, a2, nul,
+                           nul, nul, nul, nul, gf);
       if (ly != (T)0)
-        adj_forces_y<T, PART>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphav, ly, au, av, aw, aum,
-                              avm, awm, ass, assv, gf);
+        adj_forces_y<T, 1>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphav, ly, a0, a1, a2, nul,
+                           nul, nul, nul, nul, gf);
       if (lz != (T)0)
-        adj_forces_z<T, PART>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, sv, RH, azzmin, alphae0, alphaw, lz, au, av,
-                              aw, aum, avm, awm, ass, assv, arh);
-    };
-    // the inputs the part's three components read, loaded before their
-    // zero tests so that a cell's loads are in flight together; not in the
-    // FP64 diffusion part, whose 49 values would take its registers from
-    // 143 to 189 and its waves per SIMD from 3 to 2 (it reads them as the
-    // formulas need them)
-    if constexpr (PART == 2 && sizeof(T) == 8) {
-      const RView<T> U{f.u, L}, V{f.v, L}, W{f.w, L}, UM{f.um, L}, VM{f.vm, L}, WM{f.wm, L}, SS{ss0, L},
-          SV{ssv0, L};
-      body(U, V, W, UM, VM, WM, SS, SV);
-    } else {
-      NbAt<T> u(i, j, k), v(i, j, k), w(i, j, k), um(i, j, k), vm(i, j, k), wm(i, j, k), ss(i, j, k), sv(i, j, k);
-      if constexpr (PART == 1) {
-        static constexpr int ou[9][3] = {HC_NB7, {-1, 1, 0}, {-1, 0, 1}};
-        static constexpr int ov[9][3] = {HC_NB7, {1, -1, 0}, {0, -1, 1}};
-        static constexpr int ow[9][3] = {HC_NB7, {1, 0, -1}, {0, 1, -1}};
-        u.load(f.u, L, ou);
-        v.load(f.v, L, ov);
-        w.load(f.w, L, ow);
-      } else {
-        static constexpr int oum[9][3] = {HC_NB7, {-1, 1, 0}, {-1, 0, 1}};
-        static constexpr int ovm[9][3] = {HC_NB7, {1, -1, 0}, {0, -1, 1}};
-        static constexpr int owm[9][3] = {HC_NB7, {1, 0, -1}, {0, 1, -1}};
-        static constexpr int oss[9][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
-                                          {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {0, 0, 1}};
-        static constexpr int osv[13][3] = {HC_NB7,     {1, 0, 1},  {1, 0, -1}, {0, 1, 1},
-                                           {0, 1, -1}, {-1, 0, 1}, {0, -1, 1}};
-        um.load(f.um, L, oum);
-        vm.load(f.vm, L, ovm);
-        wm.load(f.wm, L, owm);
-        ss.load(ss0, L, oss);
-        sv.load(ssv0, L, osv);
-      }
-      body(u, v, w, um, vm, wm, ss, sv);
+        adj_forces_z<T, 1>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, sv, RH, azzmin, alphae0, alphaw, lz, a0, a1, a2,
+                           nul, nul, nul, nul, nul, nul);
+    } else if constexpr (PART == 2) {  // the coefficients: ss and ssv
+      static constexpr int oss[9][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
+                                        {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {0, 0, 1}};
+      static constexpr int osv[13][3] = {HC_NB7,     {1, 0, 1},  {1, 0, -1}, {0, 1, 1},
+                                         {0, 1, -1}, {-1, 0, 1}, {0, -1, 1}};
+      ss.load(ss0, L, oss);
+      sv.load(ssv0, L, osv);
+      if (lx != (T)0)
+        adj_forces_x<T, 2>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphau, lx, nul, nul, nul, a0,
+                           a1, a2, nul, nul, gf);
+      if (ly != (T)0)
+        adj_forces_y<T, 2>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphav, ly, nul, nul, nul, a0,
+                           a1, a2, nul, nul, gf);
+      if (lz != (T)0)
+        adj_forces_z<T, 2>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, sv, RH, azzmin, alphae0, alphaw, lz, nul, nul,
+                           nul, a0, a1, a2, nul, nul, nul);
+    } else {  // the differences of um, vm, wm
+      static constexpr int oum[9][3] = {HC_NB7, {-1, 1, 0}, {-1, 0, 1}};
+      static constexpr int ovm[9][3] = {HC_NB7, {1, -1, 0}, {0, -1, 1}};
+      static constexpr int owm[9][3] = {HC_NB7, {1, 0, -1}, {0, 1, -1}};
+      um.load(f.um, L, oum);
+      vm.load(f.vm, L, ovm);
+      wm.load(f.wm, L, owm);
+      if (lx != (T)0)
+        adj_forces_x<T, 4>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphau, lx, nul, nul, nul, nul,
+                           nul, nul, a0, a1, gf);
+      if (ly != (T)0)
+        adj_forces_y<T, 4>(L, i, j, k, s, ph, fs, u, v, w, um, vm, wm, ss, sv, azzmin, alphav, ly, nul, nul, nul, nul,
+                           nul, nul, a0, a1, gf);
+      if (lz != (T)0)
+        adj_forces_z<T, 4>(L, i, j, k, s, ph, u, v, w, um, vm, wm, ss, sv, RH, azzmin, alphae0, alphaw, lz, nul, nul,
+                           nul, nul, nul, nul, a0, a1, a2);
     }
   });
   block_add(gf, gflow);
@@ -1064,6 +1073,9 @@ static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
                      fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
                      ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc, kc);
   hipLaunchKernelGGL((k_adj_forces_tile<T, 2>), tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
+                     fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
+                     ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc, kc);
+  hipLaunchKernelGGL((k_adj_forces_tile<T, 4>), tg, dim3(ATI * ATJ), 0, ctx->st, L, s, ctx->P,
                      fs, ptrs<T>(ctx), ss0, ssv0, lam, ctx->aux.azzmin, ctx->aux.alphau, ctx->aux.alphav,
                      ctx->aux.alphae0, ctx->aux.alphaw, A->d_sc, kc);
   // lam.fx, fy, fz are nonzero only where FORCES writes (MOMENTUM's and
