@@ -173,7 +173,8 @@ struct NbAt {
   template <int N>
   __device__ void load(const T *p, const Layout &L, const int (&o)[N][3]) {
 #pragma unroll
-    for (int q = 0; q < N; ++q) v[(o[q][0] + 1) + 3 * (o[q][1] + 1) + 9 * (o[q][2] + 1)] = p[IX(L, i + o[q][0], j + o[q][1], k + o[q][2])];
+    for (int q = 0; q < N; ++q)
+      v[(o[q][0] + 1) + 3 * (o[q][1] + 1) + 9 * (o[q][2] + 1)] = p[IX(L, i + o[q][0], j + o[q][1], k + o[q][2])];
   }
 };
 // the cell and its six face neighbours, and with two edge neighbours more
