@@ -936,6 +936,12 @@ struct AdjCtx {
   AdjTracerScratch sc[2];
   hipStream_t st2 = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // the tape's copies of a window's states run on a stream of their own
+  // while the next step's FORCES runs (copy_pending until the context
+  // stream has waited for them)
+  hipStream_t stc = nullptr;
+  hipEvent_t evc[2] = {nullptr, nullptr};
+  bool copy_pending = false;
   void *g = nullptr;          // compact pressure work
   // ADJ_SLOTS partial sums each of d/d flow_scale, sum lam_ss*ss and
   // sum lam_ssv*ssv, then a device 0.0
@@ -967,6 +973,12 @@ void adj_free(hc_ctx *ctx) {
     hipStreamDestroy(A->st2);
   }
   for (hipEvent_t e : A->ev)
+    if (e) hipEventDestroy(e);
+  if (A->stc) {
+    hipStreamSynchronize(A->stc);
+    hipStreamDestroy(A->stc);
+  }
+  for (hipEvent_t e : A->evc)
     if (e) hipEventDestroy(e);
   for (void *p : A->lam) ps.push_back(p);
   for (void *p : A->luc) ps.push_back(p);
@@ -1509,7 +1521,45 @@ static void save_slot(hc_ctx *ctx, size_t slot, int nf = NTAPE) {
   for (int f = 0; f < nf; ++f)
     hipMemcpyAsync(slot_ptr(ctx, slot, f), ctx->f[TAPE_FIELDS[f]], ctx->L.n * ctx->esz, hipMemcpyDeviceToDevice, ctx->st);
 }
+// the context stream waits for the tape copies in flight
+static void copy_join(hc_ctx *ctx) {
+  AdjCtx *A = ctx->adj;
+  if (A && A->copy_pending) {
+    hipStreamWaitEvent(ctx->st, A->evc[1], 0);
+    A->copy_pending = false;
+  }
+}
+// save_slot on the copy stream, after the work queued so far on the context
+// stream; the fields it reads must not be written on the context stream
+// before copy_join (fwd_step joins before PRESSURE, which is the first of a
+// step's kernels to write a taped field).  Without the stream it copies in
+// order on the context stream.
+static void save_slot_async(hc_ctx *ctx, size_t slot, int nf = NTAPE) {
+  AdjCtx *A = ctx->adj;
+  if (!A->stc) {
+    bool ok = hipStreamCreateWithFlags(&A->stc, hipStreamNonBlocking) == hipSuccess;
+    for (hipEvent_t &e : A->evc) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      if (A->stc) hipStreamDestroy(A->stc);
+      for (hipEvent_t &e : A->evc)
+        if (e) hipEventDestroy(e), e = nullptr;
+      A->stc = nullptr;
+    }
+  }
+  if (!A->stc) {
+    save_slot(ctx, slot, nf);
+    return;
+  }
+  hipEventRecord(A->evc[0], ctx->st);
+  hipStreamWaitEvent(A->stc, A->evc[0], 0);
+  for (int f = 0; f < nf; ++f)
+    hipMemcpyAsync(slot_ptr(ctx, slot, f), ctx->f[TAPE_FIELDS[f]], ctx->L.n * ctx->esz, hipMemcpyDeviceToDevice, A->stc);
+  hipEventRecord(A->evc[1], A->stc);
+  A->copy_pending = true;
+}
 static void load_slot(hc_ctx *ctx, size_t slot) {
+  copy_join(ctx);
   for (int f = 0; f < NTAPE; ++f)
     hipMemcpyAsync(ctx->f[TAPE_FIELDS[f]], slot_ptr(ctx, slot, f), ctx->L.n * ctx->esz, hipMemcpyDeviceToDevice, ctx->st);
 }
@@ -1600,6 +1650,7 @@ static int fwd_step(hc_ctx *ctx, int m, int tq = -1) {
   const hc_params &P = ctx->P;
   const bool tape = tq >= 0 && ctx->adj && ctx->adj->ttape;
   if (!ctx->fuse) {
+    copy_join(ctx);
     step_head(ctx, m);
     if (P.toggle_t == 1) {
       if (tape) taped_tracer(ctx, HC_T, tracer_slot(ctx, tq, 0));
@@ -1619,7 +1670,8 @@ static int fwd_step(hc_ctx *ctx, int m, int tq = -1) {
   // rotation (the same values as the path above)
   hc_begin_step(ctx, m, ctx->time + P.dt);
   hc_body_force(ctx);
-  hc_forces(ctx);
+  hc_forces(ctx);  // reads the taped fields only: the previous step's tape copies may still run
+  copy_join(ctx);
   hc_pressure(ctx);
   for (int c = 0; c < 3; ++c) momentum_fused(ctx, c, true, P.dt);
   hc_viscosity3d(ctx);
@@ -1795,20 +1847,24 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
   const int alast = (nck - 1) * seg;
   for (int n = 0; n < nsteps; ++n) {
     if (n % seg == 0) {
-      save_slot(ctx, n / seg);
+      save_slot_async(ctx, n / seg);
       ck[n / seg] = host_state(ctx);
     }
     int rs;
     if (n >= alast && n < nsteps - 1) {
       rs = fwd_step(ctx, m0 + n, n - alast);
-      save_slot(ctx, tape0 + (n + 1 - alast), NTAPE_REV);
+      save_slot_async(ctx, tape0 + (n + 1 - alast), NTAPE_REV);
       tp[n + 1 - alast] = host_state(ctx);
     } else {
       rs = fwd_step(ctx, m0 + n);
     }
-    if (rs) return rs;
+    if (rs) {
+      copy_join(ctx);
+      return rs;
+    }
     if ((n + 1) % A->every == 0) probe_sample<T>(ctx, (size_t)((n + 1) / A->every - 1));
   }
+  copy_join(ctx);
   save_slot(ctx, nck);
   ck[nck] = host_state(ctx);
   // the last segment's first state, also when that segment is a single step
@@ -1841,10 +1897,14 @@ static int window_t(hc_ctx *ctx, int nsteps, int seg, const double *obs, double 
       tp[0] = ck[sg];
       for (int n = a; n < b - 1; ++n) {
         const int rs = fwd_step(ctx, m0 + n, n - a);
-        if (rs) return rs;
-        save_slot(ctx, tape0 + (n + 1 - a), NTAPE_REV);
+        if (rs) {
+          copy_join(ctx);
+          return rs;
+        }
+        save_slot_async(ctx, tape0 + (n + 1 - a), NTAPE_REV);
         tp[n + 1 - a] = host_state(ctx);
       }
+      copy_join(ctx);  // the adjoint steps read the tape
     }
     for (int n = b - 1; n >= a; --n) {
       FieldPtrs<T> lam = lptrs<T>(ctx);
