@@ -499,6 +499,18 @@ __global__ void __launch_bounds__(256) k_adj_visc_shell(Layout L, StepArgs s, Ph
 }
 
 // ---------------------------------------------------------------------------
+// up to eight whole fields set to zero in one launch (16-byte stores; a
+// padded field is a multiple of 64 bytes); a kernel, not a memset, so that
+// it is also safe inside a captured graph
+struct ZeroList {
+  uint4 *p[8];
+};
+__global__ void __launch_bounds__(256) k_zero_fields(ZeroList z, size_t n16) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < n16) z.p[blockIdx.y][e] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// ---------------------------------------------------------------------------
 // DENSITY, REFRESH, work velocities (elementwise)
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_density(Layout L, Box3 b, Phys ph, const T *__restrict__ t,
@@ -1061,6 +1073,13 @@ static FlowSens flow_sens(hc_ctx *ctx) {
 }
 
 static void zero(hc_ctx *ctx, void *p) { hipMemsetAsync(p, 0, ctx->L.n * ctx->esz, ctx->st); }
+// n <= 8 whole fields to zero, one launch on stream st
+static void zero_fields(hc_ctx *ctx, hipStream_t st, void *const *ps, int n) {
+  ZeroList z{};
+  for (int q = 0; q < n; ++q) z.p[q] = (uint4 *)ps[q];
+  const size_t n16 = ctx->L.n * ctx->esz / 16;
+  hipLaunchKernelGGL(k_zero_fields, dim3((unsigned)((n16 + 255) / 256), (unsigned)n), dim3(256), 0, st, z, n16);
+}
 
 static Box3 interior(const Layout &L) { return Box3{1, L.nx, 1, L.ny, 1, L.nz}; }
 
@@ -1097,7 +1116,7 @@ static void adj_forces_t(hc_ctx *ctx, const T *ss0, const T *ssv0) {
 }
 
 template <typename T>
-static void adj_momentum_t(hc_ctx *ctx, int comp) {
+static void adj_momentum_t(hc_ctx *ctx, int comp, bool clear = true) {
   AdjCtx *A = ctx->adj;
   const Layout &L = ctx->L;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -1114,7 +1133,7 @@ static void adj_momentum_t(hc_ctx *ctx, int comp) {
   const double d = comp == 0 ? s.dx : comp == 1 ? s.dy : s.dz;
   hipLaunchKernelGGL(k_adj_mom_p<T>, grid3(pb), dim3(64, 4), 0, ctx->st, L, pb, fb, comp, (T)(s.dt * s.leap / d),
                      (const T *)A->lamb, lam.p);
-  zero(ctx, lup);
+  if (clear) zero(ctx, lup);  // consumed (else the caller clears it)
 }
 
 template <typename T>
@@ -1218,8 +1237,8 @@ static void adj_tracer_t(hc_ctx *ctx, int field, const double *d_cbkg, double cm
   // adjoint buffers: lpre and the pass-2 face velocities' adjoints are
   // first added to by atomics; the donor gathers assign the others on their
   // first write (lc2, lud3 and lcnew in pass 3, lc1 in pass 2)
-  void *zs[] = {X.lpre, X.lud2[0], X.lud2[1], X.lud2[2]};
-  for (void *p : zs) hipMemsetAsync(p, 0, fb, st);
+  void *const zs[] = {X.lpre, X.lud2[0], X.lud2[1], X.lud2[2]};
+  zero_fields(ctx, st, zs, 4);
   T *lout = (T *)A->lam[field];
   const long nghost = 2L * (L.ny + 2) * (L.nz + 2) + 2L * L.nx * (L.nz + 2) + 2L * L.nx * L.ny;
   const unsigned nbc = (unsigned)((nghost + 255) / 256);
@@ -1776,9 +1795,14 @@ static void adj_step(hc_ctx *ctx, int m, const T *t1, const T *s1, const T *ss0,
   (void)nz;
   adj_courant_t<T>(ctx, P.dt);
   adj_viscosity_t<T>(ctx, (const T *)ctx->f[HC_SS], (const T *)ctx->f[HC_SSV]);
-  adj_momentum_t<T>(ctx, 2);
-  adj_momentum_t<T>(ctx, 1);
-  adj_momentum_t<T>(ctx, 0);
+  adj_momentum_t<T>(ctx, 2, false);
+  adj_momentum_t<T>(ctx, 1, false);
+  adj_momentum_t<T>(ctx, 0, false);
+  {
+    FieldPtrs<T> lam = lptrs<T>(ctx);
+    void *const ups[3] = {lam.up, lam.vp, lam.wp};  // consumed by the three
+    zero_fields(ctx, ctx->st, ups, 3);
+  }
   adj_pressure_t<T>(ctx);
   adj_forces_t<T>(ctx, ss0, ssv0);
 }
