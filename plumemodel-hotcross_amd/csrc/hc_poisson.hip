@@ -1105,47 +1105,58 @@ __global__ void __launch_bounds__(256) k_padj_rhs(Layout L, PadjConsts c, const 
   double gf = 0.0;
   if (j <= ny) {
     const T corr = (T)(dsum[0] / ((double)ny * nz));
-    auto LR = [&](int ii, int jj, int kk) -> T {
-      return (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny && kk >= 1 && kk <= nz) ? F3(pad, ii, jj, kk) - corr : (T)0;
-    };
+    // every value the face (i,j,k) needs is loaded first, unconditionally
+    // (all at box indices 0..n+1, inside the padded allocation), and the
+    // terms that do not exist are masked afterwards, so that the loads are
+    // in flight together.  The adjoint of p is taken at the interior cells
+    // only (zero at the ghosts); the boundary data's terms read the same
+    // four values: at i = 0 p(1) is p(i+1), at i = nx p(nx) is p(i), and
+    // the same in y and z.
+    const T p0 = F3(pad, i, j, k), px = F3(pad, i + 1, j, k), py = F3(pad, i, j + 1, k), pz = F3(pad, i, j, k + 1);
+    const T oum = F3(lum, i, j, k), ofx = F3(lfx, i, j, k), ovm = F3(lvm, i, j, k), ofy = F3(lfy, i, j, k);
+    const T owm = F3(lwm, i, j, k), ofz = F3(lfz, i, j, k), ow = F3(lw, i, j, k);
+    const T L0 = (i >= 1 && j >= 1 && k >= 1) ? p0 - corr : (T)0;
+    const T Lx = (i + 1 <= nx && j >= 1 && k >= 1) ? px - corr : (T)0;
+    const T Ly = (i >= 1 && j + 1 <= ny && k >= 1) ? py - corr : (T)0;
+    const T Lz = (i >= 1 && j >= 1 && k + 1 <= nz) ? pz - corr : (T)0;
     const double dlx = c.dlx, dly = c.dly, dlz = c.dlz;
     const T cm = (T)c.cm;
     if (j >= 1 && k >= 1) {
-      const T d = LR(i, j, k) - LR(i + 1, j, k);
-      F3(lum, i, j, k) += d * (T)c.rdx * cm;
+      const T d = L0 - Lx;
+      F3(lum, i, j, k) = oum + d * (T)c.rdx * cm;
       T f = d * (T)c.rdx;
       const double dbd = c.dbdu;
-      if (i == 0) { f += LR(1, j, k) * (T)dlx; gf += (double)LR(1, j, k) * dlx * dbd; }
-      if (i == nx) { f -= LR(nx, j, k) * (T)dlx; gf -= (double)LR(nx, j, k) * dlx * dbd; }
-      F3(lfx, i, j, k) += f;
+      if (i == 0) { f += Lx * (T)dlx; gf += (double)Lx * dlx * dbd; }
+      if (i == nx) { f -= L0 * (T)dlx; gf -= (double)L0 * dlx * dbd; }
+      F3(lfx, i, j, k) = ofx + f;
     }
     if (i >= 1 && k >= 1) {
-      const T d = LR(i, j, k) - LR(i, j + 1, k);
-      F3(lvm, i, j, k) += d * (T)c.rdy * cm;
+      const T d = L0 - Ly;
+      F3(lvm, i, j, k) = ovm + d * (T)c.rdy * cm;
       T f = d * (T)c.rdy;
       const double dbd = c.dbdv;
-      if (j == 0) { f += LR(i, 1, k) * (T)dly; gf += (double)LR(i, 1, k) * dly * dbd; }
-      if (j == ny) { f -= LR(i, ny, k) * (T)dly; gf -= (double)LR(i, ny, k) * dly * dbd; }
-      F3(lfy, i, j, k) += f;
+      if (j == 0) { f += Ly * (T)dly; gf += (double)Ly * dly * dbd; }
+      if (j == ny) { f -= L0 * (T)dly; gf -= (double)L0 * dly * dbd; }
+      F3(lfy, i, j, k) = ofy + f;
     }
     if (i >= 1 && j >= 1) {
-      const T d = LR(i, j, k) - LR(i, j, k + 1);
+      const T d = L0 - Lz;
       T wmv = d * (T)c.rdz * cm;
       T f = d * (T)c.rdz;
       if (k == 0) {
-        const T a = LR(i, j, 1) * (T)dlz;
+        const T a = Lz * (T)dlz;
         f += a;
-        F3(lw, i, j, 0) -= a * (T)c.rdt;
+        F3(lw, i, j, 0) = ow - a * (T)c.rdt;
         wmv += a * (T)c.rdt;
       }
       if (k == nz) {
-        const T a = LR(i, j, nz) * (T)dlz;
+        const T a = L0 * (T)dlz;
         f -= a;
-        F3(lw, i, j, nz) += a * (T)c.rdt;
+        F3(lw, i, j, nz) = ow + a * (T)c.rdt;
         wmv -= a * (T)c.rdt;
       }
-      F3(lwm, i, j, k) += wmv;
-      F3(lfz, i, j, k) += f;
+      F3(lwm, i, j, k) = owm + wmv;
+      F3(lfz, i, j, k) = ofz + f;
     }
   }
   __shared__ double sh[256];
