@@ -946,11 +946,11 @@ struct AdjCtx {
   // the T adjoint's scratch (and that of the per-subroutine entry), and the
   // S adjoint's when a window's step runs S on a stream of its own beside T
   AdjTracerScratch sc[2];
+  // the context's two side streams (ctx->tst): S's tracer work on st2, the
+  // tape's copies of a window's states on stc while the next step's FORCES
+  // runs (copy_pending until the context stream has waited for them)
   hipStream_t st2 = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
-  // the tape's copies of a window's states run on a stream of their own
-  // while the next step's FORCES runs (copy_pending until the context
-  // stream has waited for them)
   hipStream_t stc = nullptr;
   hipEvent_t evc[2] = {nullptr, nullptr};
   bool copy_pending = false;
@@ -980,16 +980,10 @@ void adj_free(hc_ctx *ctx) {
   if (!A) return;
   hipStreamSynchronize(ctx->st);
   std::vector<void *> ps;
-  if (A->st2) {
-    hipStreamSynchronize(A->st2);
-    hipStreamDestroy(A->st2);
-  }
+  if (A->st2) hipStreamSynchronize(A->st2);  // the context's streams: not destroyed here
+  if (A->stc) hipStreamSynchronize(A->stc);
   for (hipEvent_t e : A->ev)
     if (e) hipEventDestroy(e);
-  if (A->stc) {
-    hipStreamSynchronize(A->stc);
-    hipStreamDestroy(A->stc);
-  }
   for (hipEvent_t e : A->evc)
     if (e) hipEventDestroy(e);
   for (void *p : A->lam) ps.push_back(p);
@@ -1027,8 +1021,9 @@ static int adj_par_init(hc_ctx *ctx) {
   if (A->st2) return HC_OK;
   int r = scratch_alloc(ctx, A->sc[1]);
   if (r) return r;
-  HIPCHK(hipStreamCreateWithFlags(&A->st2, hipStreamNonBlocking));
+  if ((r = tracer_streams(ctx))) return r;
   for (hipEvent_t &e : A->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  A->st2 = ctx->tst[0];
   return HC_OK;
 }
 
@@ -1556,14 +1551,14 @@ static void copy_join(hc_ctx *ctx) {
 static void save_slot_async(hc_ctx *ctx, size_t slot, int nf = NTAPE) {
   AdjCtx *A = ctx->adj;
   if (!A->stc) {
-    bool ok = hipStreamCreateWithFlags(&A->stc, hipStreamNonBlocking) == hipSuccess;
+    bool ok = tracer_streams(ctx) == HC_OK;
     for (hipEvent_t &e : A->evc) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
+    if (ok) {
+      A->stc = ctx->tst[1];
+    } else {
       (void)hipGetLastError();
-      if (A->stc) hipStreamDestroy(A->stc);
       for (hipEvent_t &e : A->evc)
         if (e) hipEventDestroy(e), e = nullptr;
-      A->stc = nullptr;
     }
   }
   if (!A->stc) {

@@ -247,6 +247,7 @@ extern "C" int hc_create(const hc_params *prm, int precision, hc_ctx **out) {
 extern "C" int hc_destroy(hc_ctx *ctx) {
   if (!ctx) return HC_OK;
   if (ctx->st) hipStreamSynchronize(ctx->st);
+  adj_free(ctx);  // before the side streams it uses are destroyed
   for (int q = 0; q < HC_NFIELDS; ++q) hipFree(ctx->f[q]);
   void *ps[] = {ctx->tw.uc, ctx->tw.vc, ctx->tw.wc, ctx->tw.ud2, ctx->tw.vd2, ctx->tw.wd2, ctx->tw.c1, ctx->tw.c2, ctx->tw.budget_part,
                 ctx->tw.budget, ctx->aux.azzmin, ctx->aux.rho_bkg, ctx->aux.pgradx, ctx->aux.pgrady,
@@ -277,7 +278,6 @@ extern "C" int hc_destroy(hc_ctx *ctx) {
     if (p) hipFree(p);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->d_qmap) hipFree(ctx->d_qmap);
-  adj_free(ctx);
   stats_free(ctx);
   for (auto &p : ctx->phases) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
   if (ctx->st) hipStreamDestroy(ctx->st);
@@ -742,9 +742,22 @@ static int tracer(hc_ctx *ctx, int field, const double *d_cbkg, double cmin, dou
   return tracer_on(ctx, ctx->st, nullptr, field, d_cbkg, cmin, cmax, tau3, vent, vent_value, want_budget, c1);
 }
 
+// the two streams beside the context stream and their events: the S and C
+// tracers of the native loop run on them, an adjoint window's S tracer and
+// tape copies too (one set per context: with few hardware queues per
+// process, every further stream would share a queue with one of these)
+int tracer_streams(hc_ctx *ctx) {
+  if (ctx->tst[0] && ctx->tst[1]) return HC_OK;
+  for (int q = 0; q < 2; ++q)
+    if (!ctx->tst[q]) HIPCHK(hipStreamCreateWithFlags(&ctx->tst[q], hipStreamNonBlocking));
+  for (hipEvent_t &e : ctx->tev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return HC_OK;
+}
+
 // the streams, events and scratch of the concurrent S and C tracers
 static int par_tracers_init(hc_ctx *ctx) {
-  if (ctx->tst[0]) return HC_OK;
+  if (ctx->tw_par[1].budget) return tracer_streams(ctx);
   int rc = 0;
   const size_t fb = ctx->L.n * ctx->esz;
   for (int q = 0; q < 2; ++q) {
@@ -756,9 +769,7 @@ static int par_tracers_init(hc_ctx *ctx) {
     if (!(w.budget_part = (double *)dalloc(ctx, (size_t)3 * ctx->P.ny * ctx->P.nz * 8, &rc))) return rc;
     if (!(w.budget = (double *)dalloc(ctx, 64, &rc))) return rc;
   }
-  for (int q = 0; q < 2; ++q) HIPCHK(hipStreamCreateWithFlags(&ctx->tst[q], hipStreamNonBlocking));
-  for (hipEvent_t &e : ctx->tev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  return HC_OK;
+  return tracer_streams(ctx);
 }
 
 // pass 1 of the native loop's tracers fields[0..n) (T, S or C) in one

@@ -113,6 +113,8 @@ int native_tracer(hc_ctx *ctx, int field, void *c1 = nullptr);
 // the same on stream st with the corrective-pass scratch of tw (the
 // context's own when null)
 int native_tracer_on(hc_ctx *ctx, int field, void *c1, hipStream_t st, const TracerWork *tw);
+// creates ctx->tst[0..1] and ctx->tev if missing (HC_OK or an error code)
+int tracer_streams(hc_ctx *ctx);
 int native_pass1(hc_ctx *ctx, int n, const int *fields, void *const *dst);
 void *native_c1(hc_ctx *ctx, int q, int *rc);
 // MOMENTUM with the work velocity and REFRESH's filter fused in, and the
