@@ -180,6 +180,37 @@ struct NbAt {
 // the cell and its six face neighbours, and with two edge neighbours more
 #define HC_NB7 {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}
 
+// Accumulators in a thread's registers, 27 entries at the offsets (-1..1)^3
+// around a cell, indexed by constants after inlining: the contributions a
+// cell's formulas make to one field at one offset add up here and reach the
+// LDS ring as one atomic add per offset (RegRel: offsets relative to the
+// cell; RegAbs: absolute indices).  reg_flush adds the entries at the listed
+// offsets to a ring accumulator.
+template <typename T>
+struct RegRel {
+  T *v;
+  __device__ void operator()(int a, int b, int c, T x) const { v[(a + 1) + 3 * (b + 1) + 9 * (c + 1)] += x; }
+};
+template <typename T>
+struct RegAbs {
+  T *v;
+  int i, j, k;
+  __device__ void operator()(int a, int b, int c, T x) const {
+    v[(a - i + 1) + 3 * (b - j + 1) + 9 * (c - k + 1)] += x;
+  }
+};
+template <typename T, int N, class A>
+__device__ inline void reg_flush(const T *v, const int (&o)[N][3], const A &ring) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) ring(o[q][0], o[q][1], o[q][2], v[(o[q][0] + 1) + 3 * (o[q][1] + 1) + 9 * (o[q][2] + 1)]);
+}
+template <typename T, int N, class A>
+__device__ inline void reg_flush_at(const T *v, const int (&o)[N][3], const A &ring, int i, int j, int k) {
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    ring(i + o[q][0], j + o[q][1], k + o[q][2], v[(o[q][0] + 1) + 3 * (o[q][1] + 1) + 9 * (o[q][2] + 1)]);
+}
+
 template <typename T, int TI, int TJ, int NF, class Body>
 __device__ inline void adj_march(AdjRing<T, TI, TJ, NF> &sh, const Layout &L, T *const *dst, int i0, int j0, int ka,
                                  int kb, Body &&body) {
@@ -841,11 +872,20 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_anti_tile(Layout L, const T *_
       Vf.load(V, L, i, j, k, ov[q][0], ov[q][1], ov[q][2]);
       Wf.load(W, L, i, j, k, ow[q][0], ow[q][1], ow[q][2]);
     }
+    if (lu == (T)0 && lv == (T)0 && lw == (T)0) return;
+    // the three faces' contributions summed per field and offset in
+    // registers, then one LDS atomic per offset: 34 instead of 57 a cell
+    T rc[27] = {}, rU[27] = {}, rV[27] = {}, rW[27] = {};
+    const RegRel<T> gc{rc}, gU{rU}, gV{rV}, gW{rW};
+    adj_anti_u<T>(C, Uf, Vf, Wf, lu, gc, gU, gV, gW);
+    adj_anti_v<T>(C, Uf, Vf, Wf, lv, gc, gU, gV, gW);
+    adj_anti_w<T>(C, Uf, Vf, Wf, lw, gc, gU, gV, gW);
     const A ac{sh.a[0][0], i0 - 1, j0 - 1, i, j, k}, aU{sh.a[1][0], i0 - 1, j0 - 1, i, j, k},
         aV{sh.a[2][0], i0 - 1, j0 - 1, i, j, k}, aW{sh.a[3][0], i0 - 1, j0 - 1, i, j, k};
-    adj_anti_u<T>(C, Uf, Vf, Wf, lu, ac, aU, aV, aW);
-    adj_anti_v<T>(C, Uf, Vf, Wf, lv, ac, aU, aV, aW);
-    adj_anti_w<T>(C, Uf, Vf, Wf, lw, ac, aU, aV, aW);
+    reg_flush(rc, oc, ac);
+    reg_flush(rU, ou, aU);
+    reg_flush(rV, ov, aV);
+    reg_flush(rW, ow, aW);
   });
 }
 
@@ -868,7 +908,26 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_pass1_tile(Layout L, StepArgs 
       awc{sh.a[3][0], i0 - 1, j0 - 1}, ass{sh.a[4][0], i0 - 1, j0 - 1}, assv{sh.a[5][0], i0 - 1, j0 - 1};
   adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
     const T l = F3(lc1, i, j, k);
-    if (l != (T)0) adj_pass1<T>(L, s, ph, C, UC, VC, WC, SS, SV, azzmin, cbkg, i, j, k, l, ac, auc, avc, awc, ass, assv);
+    if (l == (T)0) return;
+    // the contributions summed per field and offset in registers, then one
+    // LDS atomic per offset: 21 instead of 29 a cell
+    T rc[27] = {}, ru[27] = {}, rv[27] = {}, rw[27] = {}, rs[27] = {}, rsv[27] = {};
+    const RegAbs<T> gc{rc, i, j, k}, gu{ru, i, j, k}, gv{rv, i, j, k}, gw{rw, i, j, k}, gs{rs, i, j, k},
+        gsv{rsv, i, j, k};
+    adj_pass1<T>(L, s, ph, C, UC, VC, WC, SS, SV, azzmin, cbkg, i, j, k, l, gc, gu, gv, gw, gs, gsv);
+    static constexpr int o7[7][3] = {HC_NB7};
+    static constexpr int o5[5][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+    static constexpr int oz[3][3] = {{0, 0, -1}, {0, 0, 0}, {0, 0, 1}};
+    static constexpr int ox[2][3] = {{-1, 0, 0}, {0, 0, 0}}, oy[2][3] = {{0, -1, 0}, {0, 0, 0}},
+                         ozw[2][3] = {{0, 0, -1}, {0, 0, 0}};
+    reg_flush_at(rc, o7, ac, i, j, k);
+    reg_flush_at(ru, ox, auc, i, j, k);
+    reg_flush_at(rv, oy, avc, i, j, k);
+    reg_flush_at(rw, ozw, awc, i, j, k);
+    reg_flush_at(rs, o5, ass, i, j, k);
+    // (at k = 1 the entry below and at k = nz the one above stay zero: pass
+    // 1 has no z face there, and the flush skips zero entries)
+    reg_flush_at(rsv, oz, assv, i, j, k);
   });
 }
 
