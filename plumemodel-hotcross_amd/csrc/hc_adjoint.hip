@@ -243,6 +243,69 @@ __device__ inline void adj_march(AdjRing<T, TI, TJ, NF> &sh, const Layout &L, T 
   flush(kb + 1);
 }
 
+// The same march with a ring depth per field: NP::np[f] planes, 3 for a
+// field the formula adds to at k-1, k and k+1, 2 for one it adds to at
+// k-1 and k, 1 for one it adds to at k only.  After plane k a 3- or
+// 2-plane field's plane k-1 is complete and a 1-plane field's plane k: each
+// is added to HBM then, in the same flush, and its slot cleared.  Fewer
+// planes mean less LDS per block and more blocks per CU.
+template <typename T, int TI, int TJ, int NF, class NP>
+struct AdjRingV {
+  static constexpr int SI = TI + 2, SJ = TJ + 2, S = SI * SJ, NT = TI * TJ;
+  static constexpr int base(int f) {
+    int o = 0;
+    for (int q = 0; q < f; ++q) o += NP::np[q];
+    return o * S;
+  }
+  static constexpr int TOT = base(NF);
+  double a[TOT];
+};
+template <typename T, int SI, int S, int NPL>
+struct LAccV {
+  double *ring;  // [NPL][S] of one field
+  int ib, jb;
+  __device__ void operator()(int i, int j, int k, T v) const {
+    adj_ladd(&ring[((unsigned)(k + 3) % (unsigned)NPL) * S + (i - ib) + SI * (j - jb)], v);  // k >= -3
+  }
+};
+template <typename T, int TI, int TJ, int NF, class NP, class Body>
+__device__ inline void adj_march_v(AdjRingV<T, TI, TJ, NF, NP> &sh, const Layout &L, T *const *dst, int i0, int j0,
+                                   int ka, int kb, Body &&body) {
+  typedef AdjRingV<T, TI, TJ, NF, NP> R;
+  constexpr int SI = R::SI, S = R::S, NT = R::NT;
+  const int tid = (int)threadIdx.x;
+  for (int e = tid; e < R::TOT; e += NT) sh.a[e] = 0.0;
+  __syncthreads();
+  const int i = i0 + tid % TI, j = j0 + tid / TI;
+  const bool own = i <= L.nx && j <= L.ny;
+  // after plane k (last: the planes kb and kb + 1 that only the deeper rings
+  // still hold)
+  auto flush = [&](int k, bool last) {
+    for (int e = tid; e < NF * S; e += NT) {
+      const int f = e / S, r = e - f * S;
+      const int np = NP::np[f];
+      int p = np == 1 ? k : k - 1;  // the plane of field f complete now
+      if (last) {
+        if (np == 1 || (np == 2 && k == kb + 1)) continue;
+        p = k;
+      }
+      double *cell = &sh.a[R::base(f) + (int)((unsigned)(p + 3) % (unsigned)np) * S + r];
+      const T v = (T)*cell;
+      const int ii = i0 - 1 + r % SI, jj = j0 - 1 + r / SI;
+      if (v != (T)0 && ii <= L.nx + 1 && jj <= L.ny + 1) adj_add(dst[f] + IX(L, ii, jj, p), v);
+      *cell = 0.0;
+    }
+  };
+  for (int k = ka; k <= kb; ++k) {
+    if (own) body(i, j, k);
+    __syncthreads();
+    flush(k, false);
+    __syncthreads();
+  }
+  flush(kb, true);
+  flush(kb + 1, true);
+}
+
 template <int TI, int TJ>
 inline dim3 adj_tile_grid(const Layout &L, int kc) {
   return dim3((unsigned)((L.nx + TI - 1) / TI), (unsigned)((L.ny + TJ - 1) / TJ), (unsigned)((L.nz + kc - 1) / kc));
@@ -889,6 +952,13 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_anti_tile(Layout L, const T *_
   });
 }
 
+// ring depths of pass 1's adjoint: the entry value and ss_v take
+// contributions at k-1, k and k+1, the z Courant number at k-1 and k, the x
+// and y ones and ss at k only
+struct Pass1Planes {
+  static constexpr int np[6] = {3, 1, 1, 2, 1, 3};
+};
+
 // pass 1 reversed, as a tile
 template <typename T>
 __global__ void __launch_bounds__(ATI *ATJ) k_adj_pass1_tile(Layout L, StepArgs s, Phys ph, FieldPtrs<T> f,
@@ -897,16 +967,17 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_pass1_tile(Layout L, StepArgs 
                                                              const double *__restrict__ azzmin,
                                                              const double *__restrict__ cbkg, const T *__restrict__ lc1,
                                                              T *lentry, T *luc, T *lvc, T *lwc, T *lss, T *lssv, int kc) {
-  typedef AdjRing<T, ATI, ATJ, 6> R;
+  typedef AdjRingV<T, ATI, ATJ, 6, Pass1Planes> R;
   __shared__ R sh;
   const int i0 = 1 + (int)blockIdx.x * ATI, j0 = 1 + (int)blockIdx.y * ATJ;
   const int ka = 1 + (int)blockIdx.z * kc, kb = min(L.nz, ka + kc - 1);
   T *const dst[6] = {lentry, luc, lvc, lwc, lss, lssv};
   const RView<T> C{c, L}, UC{uc, L}, VC{vc, L}, WC{wc, L}, SS{f.ss, L}, SV{f.ssv, L};
-  typedef LAcc<T, R::SI, R::S> A;
-  const A ac{sh.a[0][0], i0 - 1, j0 - 1}, auc{sh.a[1][0], i0 - 1, j0 - 1}, avc{sh.a[2][0], i0 - 1, j0 - 1},
-      awc{sh.a[3][0], i0 - 1, j0 - 1}, ass{sh.a[4][0], i0 - 1, j0 - 1}, assv{sh.a[5][0], i0 - 1, j0 - 1};
-  adj_march(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
+  const LAccV<T, R::SI, R::S, 3> ac{&sh.a[R::base(0)], i0 - 1, j0 - 1}, assv{&sh.a[R::base(5)], i0 - 1, j0 - 1};
+  const LAccV<T, R::SI, R::S, 1> auc{&sh.a[R::base(1)], i0 - 1, j0 - 1}, avc{&sh.a[R::base(2)], i0 - 1, j0 - 1},
+      ass{&sh.a[R::base(4)], i0 - 1, j0 - 1};
+  const LAccV<T, R::SI, R::S, 2> awc{&sh.a[R::base(3)], i0 - 1, j0 - 1};
+  adj_march_v(sh, L, dst, i0, j0, ka, kb, [&](int i, int j, int k) {
     const T l = F3(lc1, i, j, k);
     if (l == (T)0) return;
     // the contributions summed per field and offset in registers, then one
