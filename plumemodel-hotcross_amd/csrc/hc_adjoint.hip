@@ -243,26 +243,40 @@ __device__ inline void adj_march(AdjRing<T, TI, TJ, NF> &sh, const Layout &L, T 
   flush(kb + 1);
 }
 
-// The same march with a ring depth per field: NP::np[f] planes, 3 for a
-// field the formula adds to at k-1, k and k+1, 2 for one it adds to at
-// k-1 and k, 1 for one it adds to at k only.  After plane k a 3- or
-// 2-plane field's plane k-1 is complete and a 1-plane field's plane k: each
-// is added to HBM then, in the same flush, and its slot cleared.  Fewer
-// planes mean less LDS per block and more blocks per CU.
+// The same march with a ring depth per field: field f takes contributions
+// at planes k + lo .. k + hi of the plane k being processed (lo = -1 or 0,
+// hi = 0 or 1; NP::lo[f], NP::hi[f]), so its ring holds hi - lo + 1 planes.
+// After plane k its plane k + lo is complete: it is added to HBM in that
+// plane's flush and its slot cleared; after the last plane kb the planes
+// kb + lo + 1 .. kb + hi follow.  Shallower rings mean less LDS per block and
+// more blocks per CU.
 template <typename T, int TI, int TJ, int NF, class NP>
 struct AdjRingV {
   static constexpr int SI = TI + 2, SJ = TJ + 2, S = SI * SJ, NT = TI * TJ;
+  static constexpr int depth(int f) { return NP::hi[f] - NP::lo[f] + 1; }
   static constexpr int base(int f) {
     int o = 0;
-    for (int q = 0; q < f; ++q) o += NP::np[q];
+    for (int q = 0; q < f; ++q) o += depth(q);
     return o * S;
   }
   static constexpr int TOT = base(NF);
+  // the same for a field index known only at run time: a chain of selects
+  // between constants (no loop, no table in memory)
+  struct Of {
+    int np, lo, base;
+  };
+  static __device__ Of of(int f) {
+    Of o{depth(0), NP::lo[0], 0};
+#pragma unroll
+    for (int q = 1; q < NF; ++q)
+      if (f == q) o = Of{depth(q), NP::lo[q], base(q)};
+    return o;
+  }
   double a[TOT];
 };
 template <typename T, int SI, int S, int NPL>
 struct LAccV {
-  double *ring;  // [NPL][S] of one field
+  double *ring;  // [NPL][S] of one field, plane p in slot (p + 3) mod NPL
   int ib, jb;
   __device__ void operator()(int i, int j, int k, T v) const {
     adj_ladd(&ring[((unsigned)(k + 3) % (unsigned)NPL) * S + (i - ib) + SI * (j - jb)], v);  // k >= -3
@@ -278,18 +292,15 @@ __device__ inline void adj_march_v(AdjRingV<T, TI, TJ, NF, NP> &sh, const Layout
   __syncthreads();
   const int i = i0 + tid % TI, j = j0 + tid / TI;
   const bool own = i <= L.nx && j <= L.ny;
-  // after plane k (last: the planes kb and kb + 1 that only the deeper rings
-  // still hold)
-  auto flush = [&](int k, bool last) {
+  // plane k + lo of every field (d = 0), or after the last plane the
+  // plane kb + lo + d of the fields with hi - lo >= d (d = 1, 2)
+  auto flush = [&](int k, int d) {
     for (int e = tid; e < NF * S; e += NT) {
       const int f = e / S, r = e - f * S;
-      const int np = NP::np[f];
-      int p = np == 1 ? k : k - 1;  // the plane of field f complete now
-      if (last) {
-        if (np == 1 || (np == 2 && k == kb + 1)) continue;
-        p = k;
-      }
-      double *cell = &sh.a[R::base(f) + (int)((unsigned)(p + 3) % (unsigned)np) * S + r];
+      const typename R::Of o = R::of(f);
+      if (d >= o.np) continue;
+      const int p = k + o.lo + d;
+      double *cell = &sh.a[o.base + (int)((unsigned)(p + 3) % (unsigned)o.np) * S + r];
       const T v = (T)*cell;
       const int ii = i0 - 1 + r % SI, jj = j0 - 1 + r / SI;
       if (v != (T)0 && ii <= L.nx + 1 && jj <= L.ny + 1) adj_add(dst[f] + IX(L, ii, jj, p), v);
@@ -299,11 +310,11 @@ __device__ inline void adj_march_v(AdjRingV<T, TI, TJ, NF, NP> &sh, const Layout
   for (int k = ka; k <= kb; ++k) {
     if (own) body(i, j, k);
     __syncthreads();
-    flush(k, false);
+    flush(k, 0);
     __syncthreads();
   }
-  flush(kb, true);
-  flush(kb + 1, true);
+  flush(kb, 1);
+  flush(kb, 2);
 }
 
 template <int TI, int TJ>
@@ -956,7 +967,7 @@ __global__ void __launch_bounds__(ATI *ATJ) k_adj_anti_tile(Layout L, const T *_
 // contributions at k-1, k and k+1, the z Courant number at k-1 and k, the x
 // and y ones and ss at k only
 struct Pass1Planes {
-  static constexpr int np[6] = {3, 1, 1, 2, 1, 3};
+  static constexpr int lo[6] = {-1, 0, 0, -1, 0, -1}, hi[6] = {1, 0, 0, 0, 0, 1};
 };
 
 // pass 1 reversed, as a tile
