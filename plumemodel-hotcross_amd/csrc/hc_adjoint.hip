@@ -13,15 +13,17 @@
 // work velocities, DENSITY) are gathers without atomics.
 //
 // MPDATA's non-smooth parts follow the forward branch: the limiter flags
-// of both corrective passes are recomputed from the forward values
-// (c1, c2 and the antidiffusive velocities, recomputed from the pass inputs
-// by the forward kernels) and held fixed; |U| differentiates to sign(U).
+// of both corrective passes are the ones the forward kernels formed (taped
+// with the passes' intermediates, or written by a forward recompute of the
+// step) and are held fixed; |U| differentiates to sign(U).
 //
 // The window driver (hc_adj_window) runs the forward with a checkpoint
 // every `seg` steps, evaluates a probe-misfit loss, then sweeps the segments
 // backwards: each segment is recomputed from its checkpoint into a tape of
-// its states in HBM, and its steps are reversed one by one (the step's
-// intermediates are recomputed from its taped state).
+// its states in HBM (the tracers' pass intermediates taped beside them), and
+// its steps are reversed one by one.  The S tracer's forward passes and its
+// adjoint run on a side stream beside T's, and the tape copies of each state
+// on the other side stream beside the next step's FORCES.
 #include <stdlib.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -131,7 +133,9 @@ inline dim3 grid3(const Box3 &b) {
 // field the formula feeds, over the tile plus a one-cell halo.  After plane
 // k, plane k-1 receives nothing more from this block: it is added to HBM
 // (one atomic per touched cell and field, instead of one per contribution)
-// and its ring slot cleared for plane k+2.
+// and its ring slot cleared for plane k+2.  Kernels whose formulas add to
+// one field at one offset several times per cell sum those contributions
+// in registers first (RegRel, RegAbs) and add each sum to the ring once.
 // The rings accumulate in FP64 in both forms: the LDS FP64 atomic add is
 // fast on gfx950, while the FP32 one halved the FP32 window's speed at 256^3
 // (measured: the same window with the ring adds made non-atomic ran in
