@@ -21,6 +21,7 @@ import pytest
 
 import hc_testlib as T
 import gpu_common as G
+import adjoint as AD  # on the path gpu_common sets
 from test_gpu_adjoint import EVERY, NSTEPS, PROBES, SCALE, SEG, base_case, window_case
 
 pytestmark = pytest.mark.gpu
@@ -248,4 +249,109 @@ def test_window_initial_state():
     assert abs(lo - loss) <= 1e-12 * loss, (lo, loss)
     rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
     print("window: loss %.10e  dL (tangent) %.15e  (adjoint) %.15e  rel %.1e" % (loss, lhs, rhs, rel), flush=True)
+    assert rel <= 1e-9, (lhs, rhs)
+
+
+def _ragged_window_case(seed=31, vel_noise=0.01):
+    """A 72 x 36 x 40 grid with les64's spacing and physics (the ragged grid
+    of the native-loop parity test): x and y extents that are no multiple of
+    the adjoint tiles (32 x 8 cells; the donor gathers take 256 columns a
+    block), 40 planes that split into several chunks per launch.  The start state is
+    hc_init_case's plus velocity noise and random viscosities, so that every
+    term of the step is active; the forcing ramps are cut as in the 16 x 12 x
+    10 window.  Returns the parameters, the profile inputs and the state."""
+    H = G.H
+    p = H.workload_params("les64")
+    p.nx, p.ny, p.nz = 72, 36, 40
+    p.zend = p.zstart + p.nz * p.dz
+    p.flow_ramp_hrs = 0.0
+    p.source_ramp_hrs = 1.0e-3
+    p.wramp_hrs = 1.0e-3
+    prm = T.Params.from_buffer_copy(bytes(p))
+    z, t, s = H.linear_stratification(p)
+    o = T.Oracle(prm)
+    o.init_profiles(z, t, s)
+    for c in ("density", "initialize", "sponge_maker"):
+        o.call(c)
+    st = {k: o.get(k) for k in ("t_bkg", "s_bkg", "rho_bkg", "azzmin", "alphau", "alphav", "alphae", "alphaw")}
+    ctx = H.Context(p, H.FP64)
+    try:
+        ctx.init_case(z, t, s, t_noise=1.0e-4, seed=seed)
+        for f in G.UPLOAD:
+            st[f] = ctx.download(f)
+    finally:
+        ctx.close()
+    rng = np.random.default_rng(seed)
+    for f in ("u", "v", "w", "um", "vm", "wm", "up", "vp", "wp"):
+        a = st[f].copy()
+        core = np.abs(a) < 1e30
+        a[core] += vel_noise * (rng.random(core.sum()) - 0.5)
+        st[f] = a
+    for f in ("ss", "ss_v"):
+        st[f] = 1e-4 * (1.0 + rng.random(st[f].shape))
+    return prm, (z, t, s), st
+
+
+def test_window_initial_state_multi_tile():
+    """The window's initial-state adjoint against the oracle's tangent, as in
+    test_window_initial_state, on the ragged multi-tile grid: every x, y and z
+    seam of the adjoint's tile kernels (LDS rings flushed by atomics at the
+    tile borders and chunk ends), of the donor gathers' plane marches and of
+    the pressure adjoint's transforms (lengths 72 and 36) lies inside the
+    region the probes' adjoints reach in 8 steps (25 probes 20 m above the
+    vent, 4 cells apart, and 3 on the tiles' x seam)."""
+    prm, (z, t, s), st = _ragged_window_case()
+    st = _snap(st)
+    prm.cs_h = prm.cs_v = 0.2
+    probes = AD.plane_probes(prm, n=5, spacing=4) + [(32, 18, 20), (33, 17, 21), (64, 9, 20)]
+    nsteps, seg, every = 8, 3, 2
+    ns = nsteps // every
+    print("ragged window: %d x %d x %d, %d steps, %d probes" % (prm.nx, prm.ny, prm.nz, nsteps, len(probes)),
+          flush=True)
+    ctx = G.context_from_state(prm, st)
+    try:
+        npatch = ctx.npatch()
+        noise = [ctx.source_noise(m) for m in range(nsteps)]
+        ctx.set_probes(probes, every, 1.0e4, 1.0)
+        ctx.set_controls(1.1 + 0.05 * np.sin(np.arange(npatch)), 0.9, 0.2)
+        _, obs, _, _ = ctx.adj_window(nsteps, seg, np.zeros((ns, len(probes), 2)), end_state=False)
+        ctx.set_controls(None, 1.0, 0.2)
+        loss, samp, _, _ = ctx.adj_window(nsteps, seg, obs, end_state=False)
+        ins = ("u", "v", "w", "um", "vm", "wm", "t", "s", "ss", "ss_v", "rhop")
+        lam0 = {k: ctx.adj_download(k) for k in ins}
+    finally:
+        ctx.close()
+    rng = np.random.default_rng(7)
+    d = {k: _direction(rng, st[k], SCALE[k]) for k in ins}
+    rhs = sum(float(np.sum(d[k] * lam0[k])) for k in ins)
+    print("ragged window: tangent of the oracle", flush=True)
+    o = T.OracleCS(prm)
+    o.init_profiles(z, t, s)
+    for c in ("density", "initialize", "sponge_maker"):
+        o.call(c)
+    for k in G.UPLOAD:
+        o.set(k, st[k])
+    o.set_time(-prm.dt, 0)
+    for k in ins:
+        o.set_tan(k, d[k])
+    bw, bt = o.bounds("w"), o.bounds("t")
+    wmax = np.abs(samp[:, :, 0]).max()
+    lhs, lo = 0.0, 0.0
+    for n in range(nsteps):
+        o.step(n, noise[n])
+        if (n + 1) % every == 0:
+            q = (n + 1) // every - 1
+            w, tt_, tw, tt = o.get("w"), o.get("t"), o.get_tan("w"), o.get_tan("t")
+            for pi, (i, j, k) in enumerate(probes):
+                iw = (i - bw[0][0], j - bw[1][0], k - bw[2][0])
+                it = (i - bt[0][0], j - bt[1][0], k - bt[2][0])
+                assert abs(w[iw] - samp[q, pi, 0]) <= 1e-9 * wmax, (q, pi, w[iw], samp[q, pi, 0])
+                assert abs(tt_[it] - samp[q, pi, 1]) <= 1e-9 * abs(tt_[it]), (q, pi, tt_[it], samp[q, pi, 1])
+                dw, dT = samp[q, pi, 0] - obs[q, pi, 0], samp[q, pi, 1] - obs[q, pi, 1]
+                lo += 0.5 * (1.0e4 * dw * dw + dT * dT)
+                lhs += 1.0e4 * dw * tw[iw] + dT * tt[it]
+    assert abs(lo - loss) <= 1e-12 * loss, (lo, loss)
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print("ragged window: loss %.10e  dL (tangent) %.15e  (adjoint) %.15e  rel %.1e" % (loss, lhs, rhs, rel),
+          flush=True)
     assert rel <= 1e-9, (lhs, rhs)
